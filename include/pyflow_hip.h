@@ -1,0 +1,254 @@
+/*
+ * libpyflow_hip.so — C-ABI of the MI355X (gfx950) kernels behind PyFlow's
+ * single-box time-step path.
+ *
+ * The reference (MacRTL/PyFlow-public) is pure Python + PyTorch; its "boundary"
+ * for this path is the set of Python classes listed beside each entry point
+ * below (paths relative to the reference's src/).  The Python package
+ * pyflow_amd binds these entry points with ctypes and keeps the reference's
+ * class names, signatures and in-place semantics.
+ *
+ * Conventions
+ *   - Every function returns 0 on success, a negative PF_E* code on error
+ *     (pf_last_error() then describes it), or, for solvers, a PF_SOLVE_*
+ *     status >= 0.
+ *   - Field arguments are raw device pointers.  "padded" fields are
+ *     (nx+4)*(ny+4)*(nz+4) contiguous arrays, z fastest, two overlap cells per
+ *     side, byte-compatible with state_P.var (data/state.py:123).  "interior"
+ *     fields are nx*ny*nz contiguous arrays (the reference's rhs_u,
+ *     source_P).  Element type is the box's dtype.
+ *   - Work is enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ *     and is stream-ordered.  Functions that return host scalars (solvers,
+ *     statistics) synchronise on the stream before returning, like the
+ *     reference's .cpu() reductions.
+ *   - Every kernel that writes a padded field also fills its overlap cells
+ *     with periodic images, as state_P.update_border does.
+ *   - No entry point allocates device memory except pf_box_create; all
+ *     workspace belongs to the box.  A box may be used by one host thread
+ *     and one stream at a time.
+ */
+#ifndef PYFLOW_HIP_H
+#define PYFLOW_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_ABI_VERSION 1
+
+/* dtypes */
+#define PF_F64 0
+#define PF_F32 1
+
+/* error codes */
+#define PF_EINVAL  (-1)
+#define PF_EHIP    (-2)
+#define PF_ENOMEM  (-3)
+
+/* solver status codes (returned by the Poisson entry points) */
+#define PF_SOLVE_CONVERGED   0   /* max|r|/max|r0| <= tol                    */
+#define PF_SOLVE_MAXIT       1   /* iteration cap reached                    */
+#define PF_SOLVE_BREAKDOWN_A 2   /* <v, r_hat> == 0  (pressure.py:247-248)   */
+#define PF_SOLVE_BREAKDOWN_B 3   /* <t, t> == 0      (pressure.py:266-267)   */
+
+/* field layouts */
+#define PF_LAYOUT_PADDED   0
+#define PF_LAYOUT_INTERIOR 1
+
+/* pf_state_op operations */
+#define PF_EW_ZAXPY     0   /* out = (a*X) + Y                  state.py:157-165 */
+#define PF_EW_COPY      1   /* out = X                          state.py:168-195 */
+#define PF_EW_SUBSCALED 2   /* out = out - a*X (one FMA)        state.py:198-211 */
+
+/* pf_metric_pair kinds */
+#define PF_PAIR_INTERP 0    /* dst = coef*(a + b) */
+#define PF_PAIR_DIFF   1    /* dst = (a - b)*coef */
+
+/* velocity-correction solver selectors */
+#define PF_POISSON_BICGSTAB 0
+#define PF_POISSON_JACOBI   1
+
+typedef struct pf_box pf_box;
+
+/* ABI version and last error (thread-local text). */
+int pf_abi_version(void);
+const char* pf_last_error(void);
+
+/*
+ * Create / destroy a box: a triply periodic uniform grid nx*ny*nz with
+ * spacings dx, dy, dz and element type dtype, plus all device workspace.
+ * Replaces the single-rank parts of parallel.decomp (library/parallel.py:118-284),
+ * geometry.uniform (geometry/geometry.py:76-116) and the workspace allocations of
+ * solver_bicgstab.__init__ (solver/pressure.py:184-190) and RK4
+ * (solver/advancer.py:63-73).
+ */
+int pf_box_create(pf_box** out, int nx, int ny, int nz,
+                  double dx, double dy, double dz, int dtype);
+int pf_box_destroy(pf_box* box);
+/* Device bytes held by the box (for admission control). */
+int64_t pf_box_device_bytes(const pf_box* box);
+/* Set the grid spacing after creation: parallel.decomp is built before the
+ * geometry in PyFlow.run (core/PyFlow.py:81-104), so the box learns dx, dy,
+ * dz from geometry.uniform. */
+int pf_box_set_spacing(pf_box* box, double dx, double dy, double dz);
+
+/*
+ * Periodic overlap fill of a padded field.
+ * Replaces state_P.update_border / decomp.communicate_border
+ * (data/state.py:148-150, library/parallel.py:301-414).
+ */
+int pf_fill_halo(pf_box* box, void* field, void* stream);
+
+/*
+ * Interior update of a padded field `out` (state_P methods ZAXPY, copy,
+ * update, vel_corr; data/state.py:157-211).  X and Y are padded or interior
+ * fields as given by their layout flags; with fill_halo != 0 the overlap
+ * cells receive periodic images (update_border), otherwise only the interior
+ * is written (state_P.copy).
+ */
+int pf_state_op(pf_box* box, int op, double a, void* out, const void* X, int x_layout,
+                const void* Y, int y_layout, int fill_halo, void* stream);
+
+/*
+ * Fused momentum RHS + time advance.
+ *   pf_rk4_step: the reference's four-stage advancer RK4.step
+ *     (solver/advancer.py:75-113) with rhs_NavierStokes.evaluate
+ *     (solver/velocity.py:221-383) fused into each stage.  u,v,w (padded) hold
+ *     u^n on entry and u* on exit; ut,vt,wt (padded) receive the last stage
+ *     state, as state_uTmp does.
+ *   pf_euler_step: Euler.step (solver/advancer.py:49-57).
+ *   mu, rho: constant molecular viscosity and density (domain.py:125-131).
+ *   sfs6: NULL, or 6 padded cell-centred stress fields in the order
+ *     FXX, FYY, FZZ, FXY, FXZ, FYZ (the 'tensor' closure contract,
+ *     solver/velocity.py:262-315).
+ */
+int pf_rk4_step(pf_box* box, void* u, void* v, void* w, void* ut, void* vt, void* wt,
+                double dt, double mu, double rho, const void* const* sfs6, void* stream);
+int pf_euler_step(pf_box* box, void* u, void* v, void* w,
+                  double dt, double mu, double rho, const void* const* sfs6, void* stream);
+
+/*
+ * One evaluation of the Navier-Stokes RHS into interior arrays ru, rv, rw.
+ * Replaces rhs_NavierStokes.evaluate (solver/velocity.py:221-383).
+ */
+int pf_rhs_ns(pf_box* box, const void* u, const void* v, const void* w,
+              double mu, double rho, const void* const* sfs6,
+              void* ru, void* rv, void* rw, void* stream);
+
+/*
+ * Interior cell-centred divergence of padded u, v, w into interior `divg`.
+ * Replaces metric_uniform.div_vel (metric/metric_staggered.py:438-453).
+ */
+int pf_div_vel(pf_box* box, const void* u, const void* v, const void* w, void* divg, void* stream);
+
+/*
+ * BiCGStab pressure solve.  Replaces solver_bicgstab.solve
+ * (solver/pressure.py:196-297): DP (padded) := solution of
+ * (-vol*L7) DP = -vol*(rho/dt)*source from DP = 0, p (padded) += DP, and
+ * `source` (interior) is scaled in place by rho/dt and then by -vol, as the
+ * reference does.  Stops when max|r|/max|r0| <= tol or after maxit
+ * iterations; breakdowns end the solve silently like the reference.
+ * Outputs (host pointers, each may be NULL): *out_res = the reference's
+ * returned max_res, *out_iters = iterations executed.
+ * Returns a PF_SOLVE_* status (>= 0) or an error (< 0).
+ */
+int pf_bicgstab_solve(pf_box* box, void* DP, void* p, void* source,
+                      double rho, double dt, double tol, int maxit,
+                      double* out_res, int* out_iters, void* stream);
+
+/*
+ * B (interior) = (-vol*L7) A for a padded A whose overlap is already filled.
+ * Replaces Pressure_Base.applyOperator (solver/pressure.py:54-84) after its
+ * communicate_border call; same nine-term summation order.
+ */
+int pf_apply_laplacian(pf_box* box, const void* A, void* B, void* stream);
+
+/*
+ * Jacobi pressure solve.  Replaces solver_jacobi.solve
+ * (solver/pressure.py:115-161), including its conventions: p is zeroed and
+ * iterated nit times, pOld holds iterate nit-1 on exit (the field the
+ * reference's velocity correction then uses), and the returned residual is
+ * the reference's diagnostic value.
+ */
+int pf_jacobi_solve(pf_box* box, void* pOld, void* p, void* source,
+                    double rho, double dt, int nit, double* out_res, void* stream);
+
+/*
+ * Velocity correction of Domain.velCorr (data/domain.py:373-402), fused:
+ * source := div(u*) (metric_uniform.div_vel), source scaled by src_scale
+ * (the solver's rho/simDt, pressure.py:125/205), solve for DP with the
+ * selected solver (PF_POISSON_BICGSTAB: also -vol scaling and p += DP;
+ * PF_POISSON_JACOBI: the reference's Jacobi conventions, DP = pOld), then
+ * u -= corr_scale * grad(DP) with corr_scale = the domain's simDt/rho,
+ * using the staggered gradient of metric_uniform.grad_P
+ * (metric_staggered.py:390-401) and state_P.vel_corr (data/state.py:198-211).
+ * u,v,w,DP,p padded; source interior.  `iters` is the BiCGStab cap or the
+ * Jacobi sweep count.  Returns a PF_SOLVE_* status or an error.
+ */
+int pf_vel_corr(pf_box* box, void* u, void* v, void* w, void* DP, void* p, void* source,
+                int solver, double src_scale, double corr_scale, double tol, int iters,
+                double* out_res, int* out_iters, void* stream);
+
+/*
+ * Per-step monitor values of PyFlow.run (core/PyFlow.py:378-395):
+ * out5 = {max|U|, max|V|, max|W|, TKE, max|div|} over the interior, with
+ * TKE = 0.5*sum(u^2+v^2+w^2)/N.  If divg is not NULL the interior divergence
+ * is also written there (the reference writes it into source_P).
+ */
+int pf_stats(pf_box* box, const void* u, const void* v, const void* w, void* divg,
+             double* out5, void* stream);
+
+/*
+ * Gradient closure ('tensor' model) of sfsmodel_gradient.residual_stress.update
+ * (sfsmodel/sfsmodel_gradient.py:70-148): writes the six padded cell-centred
+ * stress fields FXX, FYY, FZZ, FXY, FXZ, FYZ from padded u, v, w using the
+ * cell-centred gradients of metric_uniform.grad_vel_center
+ * (metric_staggered.py:215-268).
+ */
+int pf_sfs_gradient(pf_box* box, const void* u, const void* v, const void* w,
+                    void* const* sfs6, void* stream);
+
+/*
+ * metric_uniform building blocks (metric/metric_staggered.py:112-526) with
+ * the reference's slice semantics over whole padded arrays (overlap cells
+ * included, untouched cells left as they are):
+ *   pf_metric_pair: along `axis`, with upper = 1 the box [0, n-1) gets
+ *     OP(src[p+e], src[p]) (interp_u_xm/v_ym/w_zm, grad_vel_visc); with
+ *     upper = 0 the box [1, n) gets OP(src[p], src[p-e]) (interp_sc_x/y/z,
+ *     interp_vw_x, interp_uw_y, interp_uv_z, interp_uvwi_*, grad_P).
+ *     dst == src is allowed (interp_uvwi_*: reads a copy).
+ *   pf_metric_edge4: interp_sc_xy (plane 0), _xz (1), _yz (2).
+ *   pf_div_visc: rhs (interior) += flux divergence (metric_staggered.py:406-418).
+ *   pf_div_vel_over: cell-centre divergence over the padded array (:424-433).
+ *   pf_vel_conv: rhs (interior) -= ((A1*B1 - A0*B0)/h)*sign on padded var_i
+ *     arrays; diagonal = 1 for vel_conv_xx/yy/zz (cells i, i-1), 0 for
+ *     vel_conv_x/y/z (cells i+1, i) (:458-526).
+ */
+int pf_metric_pair(pf_box* box, int kind, int axis, int upper, void* dst, const void* src,
+                   double coef, void* stream);
+int pf_metric_edge4(pf_box* box, int plane, void* dst, const void* src, void* stream);
+int pf_div_visc(pf_box* box, const void* FX, const void* FY, const void* FZ, void* rhs,
+                void* stream);
+int pf_div_vel_over(pf_box* box, const void* u, const void* v, const void* w, void* divg_o,
+                    void* stream);
+int pf_vel_conv(pf_box* box, int axis, int diagonal, const void* A, const void* B, void* rhs,
+                double sign, void* stream);
+
+/*
+ * Measurement support (no counterpart in the reference): while enabled, every
+ * launch of the three BiCGStab kernels (0 = KA: p update + A p + <v,r_hat>;
+ * 1 = KB: s + A s + <s,t>,<t,t>; 2 = KC: x, r updates + <r,r_hat>, max|r|) is
+ * bracketed by a hipEvent pair on its stream; pf_profile_read returns the
+ * accumulated milliseconds and the number of timed launches that did work.
+ * Enabling resets the accumulators.  Creates events on first use.
+ */
+int pf_profile_enable(pf_box* box, int on);
+int pf_profile_read(pf_box* box, int kernel, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYFLOW_HIP_H */

@@ -1,0 +1,94 @@
+// Internal definition of the box handle and error plumbing shared by the
+// translation units of libpyflow_hip.so.  Not part of the public C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "pf_common.h"
+#include "../../include/pyflow_hip.h"
+
+namespace pf {
+
+// Device-resident control block of one BiCGStab solve.  Block 0 of the
+// Krylov kernels is the only writer of each field; every block reads the
+// fields written by earlier kernels (kernel boundaries order the accesses).
+struct KrylovCtl {
+  double rho1, rho2, alpha, omega;
+  double res0;      // max|r0|
+  double max_res;   // max|r_k| / max|r0| of the last completed iteration
+  double tol;
+  int iter;         // completed iterations
+  int maxit;
+  int done;        // set only by KA/KB, whose early returns are then uniform
+  int status;       // PF_SOLVE_* code
+  int halt;         // set by KC on a <t,t> breakdown; the next KA turns it into done
+};
+
+constexpr int kMaxPartSlots = 8;     // doubles per block per reduction
+constexpr int kNumPartBufs = 4;      // A, B, C, stats
+
+}  // namespace pf
+
+struct pf_box {
+  int nx, ny, nz;
+  double dx, dy, dz;
+  int dtype;          // PF_F64 / PF_F32
+  size_t esize;
+  int device;
+  pf::Geom g;
+  int nblk;           // blocks of the reducing kernels (fixed per box)
+  // Krylov vectors, interior layout
+  void* x;
+  void* r;
+  void* rh;
+  void* p[2];
+  void* v[2];
+  void* s;
+  void* t;
+  // RK workspace
+  void* acc[3];       // interior
+  void* ypong[3];     // padded
+  // reductions and control
+  double* part;       // kNumPartBufs * nblk * kMaxPartSlots
+  pf::KrylovCtl* ctl;
+  pf::KrylovCtl* ctl_host;   // pinned
+  double* stats_dev;         // 8 doubles
+  double* stats_host;        // pinned, 8 doubles
+  int last_iters;
+  int chunk;                 // iterations per launch batch between host checks
+  int64_t bytes;             // device bytes owned by the handle
+  // optional per-launch timing of the Krylov kernels (measurement only)
+  int prof_on;
+  int prof_cap;              // event pairs per kernel class
+  hipEvent_t* prof_ev;       // [3][prof_cap][2]
+  double prof_ms[3];         // accumulated milliseconds of KA, KB, KC
+  int64_t prof_n[3];         // timed launches that did work
+};
+
+namespace pf {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+int check_box(const pf_box* h);
+int launch_blocks(const Geom& g);   // grid size for flat one-cell-per-thread kernels
+
+inline double* part_buf(pf_box* h, int which) {
+  return h->part + (size_t)which * h->nblk * kMaxPartSlots;
+}
+
+}  // namespace pf
+
+#define PF_HIP(call)                                             \
+  do {                                                           \
+    hipError_t e_ = (call);                                      \
+    if (e_ != hipSuccess) return pf::check_hip(e_, #call);       \
+  } while (0)
+
+#define PF_LAUNCH_CHECK(what)                                    \
+  do {                                                           \
+    hipError_t e_ = hipGetLastError();                           \
+    if (e_ != hipSuccess) return pf::check_hip(e_, what);        \
+  } while (0)

@@ -1,0 +1,863 @@
+// Pressure Poisson path: divergence source, BiCGStab, Jacobi, projection
+// and the per-step monitor values.
+//
+// BiCGStab (solver/pressure.py:196-297) runs as three kernels per iteration,
+// one per global reduction of the recurrence:
+//   KA: p = (p - omega*v)*beta + r, v = A p,        partial <v, r_hat>
+//   KB: s = r - alpha*v,            t = A s,        partials <s,t>, <t,t>
+//   KC: x += alpha*p; x += omega*s; r = s - omega*t, partials <r,r_hat>, max|r|
+// The scalars never leave the device: every block of a kernel reduces the
+// previous kernel's per-block partials itself (same data, same order, same
+// bits in every block), and block 0 records them in the box's KrylovCtl.
+// KA also checks the previous iteration's convergence, so the host only
+// polls the control block once per batch of iterations.  Once a solve is
+// done every later kernel of the batch returns at once.
+//
+// The stencil of A is evaluated on the fly from p (resp. s) recomputed at the
+// six neighbours, so p and s are never re-read after being written in the
+// same kernel.  All vector arithmetic follows the reference's op order
+// (add_/sub_ with a scalar are single FMAs in torch, everything else rounds
+// per op); only the dot-product summation order differs from torch's CPU sum.
+#include <hip/hip_runtime.h>
+
+#include "pf_box.h"
+
+namespace pf {
+
+enum { kPartA = 0, kPartB = 1, kPartC = 2, kPartS = 3 };
+
+template <typename T>
+struct Lap {
+  T xm, x0, xp, ym, y0, yp, zm, z0, zp;
+};
+
+// Laplace coefficients as metric_uniform builds them (metric_staggered.py:85-102):
+// 0 + h^-2 (or -2.0*h^-1*h^-1) added in the field dtype, then scaled by -vol.
+template <typename T>
+Lap<T> make_lap(const pf_box* h) {
+  const double vol = h->dx * h->dy * h->dz;
+  const double inv[3] = {1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz};
+  T c[3][3];
+  for (int a = 0; a < 3; ++a) {
+    const double hi = inv[a];
+    const double vals[3] = {hi * hi, -2.0 * hi * hi, hi * hi};
+    for (int b = 0; b < 3; ++b) c[a][b] = (T)vals[b] * (T)(-1.0 * vol);
+  }
+  Lap<T> L;
+  L.xm = c[0][0]; L.x0 = c[0][1]; L.xp = c[0][2];
+  L.ym = c[1][0]; L.y0 = c[1][1]; L.yp = c[1][2];
+  L.zm = c[2][0]; L.z0 = c[2][1]; L.zp = c[2][2];
+  return L;
+}
+
+// B = sum in pressure.py:63-84 order: x-, x0, x+, y-, y0, y+, z-, z0, z+.
+template <typename T>
+__device__ __forceinline__ T lap7(const Lap<T>& L, T xm, T c, T xp, T ym, T yp, T zm, T zp) {
+  T b = (T)0 + xm * L.xm;
+  b = b + c * L.x0;
+  b = b + xp * L.xp;
+  b = b + ym * L.ym;
+  b = b + c * L.y0;
+  b = b + yp * L.yp;
+  b = b + zm * L.zm;
+  b = b + c * L.z0;
+  b = b + zp * L.zp;
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// Control block
+__global__ void ctl_init_kernel(KrylovCtl* ctl, double tol, int maxit) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl->rho1 = 1.0; ctl->rho2 = 1.0; ctl->alpha = 0.0; ctl->omega = 1.0;
+    ctl->res0 = 0.0; ctl->max_res = 0.0; ctl->tol = tol;
+    ctl->iter = 0; ctl->maxit = maxit; ctl->done = 0; ctl->status = PF_SOLVE_CONVERGED;
+    ctl->halt = 0;
+  }
+}
+
+__device__ __forceinline__ bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+// Block-uniform snapshot of the control block: thread 0 reads it once and
+// broadcasts through LDS, so that a write by block 0 of the same kernel can
+// never split a block between an early return and a __syncthreads().
+__device__ __forceinline__ KrylovCtl snapshot(const KrylovCtl* ctl) {
+  __shared__ KrylovCtl sc;
+  if (threadIdx.x == 0) sc = *ctl;
+  __syncthreads();
+  KrylovCtl c = sc;
+  __syncthreads();
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// Source and Krylov start: b = (src*(rho/dt))*(-vol) (pressure.py:205-208),
+// r = r_hat = b, x = p = v = 0 (pressure.py:202, 217-218).  With FROM_DIV the
+// source is first computed as div(u) (metric_staggered.py:438-453).
+template <typename T, bool FROM_DIV>
+__global__ void __launch_bounds__(kBlock) bicg_init_kernel(
+    Geom g, T* __restrict__ src, const T* __restrict__ U, const T* __restrict__ V,
+    const T* __restrict__ W, T dxi, T dyi, T dzi, T sc1, T sc2, T* __restrict__ r,
+    T* __restrict__ rh, T* __restrict__ x, T* __restrict__ p0, T* __restrict__ v0,
+    double* __restrict__ partC) {
+  double acc[2] = {0.0, 0.0};
+  const int ops[2] = {0, 1};
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    T b;
+    if (FROM_DIV) {
+      int i, j, k;
+      decode(g, idx, i, j, k);
+      const int64_t c = i * g.psx + j * g.psy + k;
+      b = (U[c + g.psx] - U[c]) * dxi;
+      b = b + ((V[c + g.psy] - V[c]) * dyi);
+      b = b + ((W[c + 1] - W[c]) * dzi);
+    } else {
+      b = src[idx];
+    }
+    b = b * sc1;
+    b = b * sc2;
+    src[idx] = b;
+    r[idx] = b;
+    rh[idx] = b;
+    x[idx] = (T)0;
+    p0[idx] = (T)0;
+    v0[idx] = (T)0;
+    acc[0] += (double)b * (double)b;
+    acc[1] = nmax(acc[1], fabs((double)b));
+  }
+  block_reduce<2>(acc, ops);
+  write_partials<2>(partC, acc);
+}
+
+// KA: convergence check of the previous iteration, then p and v = A p.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) bicg_ka_kernel(
+    Geom g, int nblk, KrylovCtl* __restrict__ ctl, const double* __restrict__ partC,
+    double* __restrict__ partA, const T* __restrict__ r, const T* __restrict__ rh,
+    const T* __restrict__ p_old, const T* __restrict__ v_old, T* __restrict__ p_new,
+    T* __restrict__ v_new, Lap<T> L) {
+  const KrylovCtl c0 = snapshot(ctl);
+  if (c0.done) return;
+  const int halt = c0.halt;
+  const int k = c0.iter;
+  const int maxit = c0.maxit;
+  const double tol = c0.tol;
+  const double alpha = c0.alpha, omega = c0.omega, rho2 = c0.rho2;
+  const double res0_prev = c0.res0;
+  if (halt) {
+    if (lead()) ctl->done = 1;
+    return;
+  }
+  double red[2];
+  const int ops[2] = {0, 1};
+  reduce_partials<2>(partC, nblk, red, ops);
+  const double rho1 = red[0];
+  if (k == 0) {
+    if (lead()) ctl->res0 = red[1];
+  } else {
+    const double max_res = red[1] / res0_prev;     // pressure.py:279-280
+    if (lead()) ctl->max_res = max_res;
+    const bool cap = k >= maxit;                    // pressure.py:283-287
+    const bool conv = max_res <= tol;
+    if (cap || conv) {
+      if (lead()) {
+        ctl->status = conv ? PF_SOLVE_CONVERGED : PF_SOLVE_MAXIT;
+        ctl->done = 1;
+      }
+      return;
+    }
+  }
+  const double beta = alpha * rho1 / (rho2 * omega); // pressure.py:233
+  if (lead()) ctl->rho1 = rho1;
+  const T tb = (T)beta, tmo = (T)(-omega);
+  double acc[1] = {0.0};
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, kk;
+    decode(g, idx, i, j, kk);
+    const Nbr n = nbr(g, i, j, kk);
+    // pressure.py:235-237: pp.sub_(omega, vv); pp.mul_(beta); pp.add_(res)
+#define PNEW(o) ((fma(v_old[idx + (o)], tmo, p_old[idx + (o)]) * tb) + r[idx + (o)])
+    const T pc = PNEW(0);
+    const T vn = lap7(L, PNEW(n.xm), pc, PNEW(n.xp), PNEW(n.ym), PNEW(n.yp), PNEW(n.zm), PNEW(n.zp));
+#undef PNEW
+    p_new[idx] = pc;
+    v_new[idx] = vn;
+    acc[0] += (double)vn * (double)rh[idx];
+  }
+  const int ops1[1] = {0};
+  block_reduce<1>(acc, ops1);
+  write_partials<1>(partA, acc);
+}
+
+// KB: alpha = rho1/<v,r_hat>, s = r - alpha*v, t = A s.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) bicg_kb_kernel(
+    Geom g, int nblk, KrylovCtl* __restrict__ ctl, const double* __restrict__ partA,
+    double* __restrict__ partB, const T* __restrict__ r, const T* __restrict__ v,
+    T* __restrict__ s, T* __restrict__ t, Lap<T> L) {
+  const KrylovCtl c0 = snapshot(ctl);
+  if (c0.done) return;
+  const double rho1 = c0.rho1;
+  double red[1];
+  const int ops[1] = {0};
+  reduce_partials<1>(partA, nblk, red, ops);
+  const double gamma = red[0];
+  if (gamma == 0.0) {                               // pressure.py:247-248
+    if (lead()) {
+      ctl->status = PF_SOLVE_BREAKDOWN_A;
+      ctl->done = 1;
+    }
+    return;
+  }
+  const double alpha = rho1 / gamma;                // pressure.py:249
+  if (lead()) ctl->alpha = alpha;
+  const T tma = (T)(-alpha);
+  double acc[2] = {0.0, 0.0};
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, kk;
+    decode(g, idx, i, j, kk);
+    const Nbr n = nbr(g, i, j, kk);
+#define SNEW(o) fma(v[idx + (o)], tma, r[idx + (o)])  // pressure.py:253-254
+    const T sc = SNEW(0);
+    const T tn = lap7(L, SNEW(n.xm), sc, SNEW(n.xp), SNEW(n.ym), SNEW(n.yp), SNEW(n.zm), SNEW(n.zp));
+#undef SNEW
+    s[idx] = sc;
+    t[idx] = tn;
+    acc[0] += (double)sc * (double)tn;
+    acc[1] += (double)tn * (double)tn;
+  }
+  const int ops2[2] = {0, 0};
+  block_reduce<2>(acc, ops2);
+  write_partials<2>(partB, acc);
+}
+
+// KC: omega = <s,t>/<t,t>, x += alpha*p, x += omega*s, r = s - omega*t.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) bicg_kc_kernel(
+    Geom g, int nblk, KrylovCtl* __restrict__ ctl, const double* __restrict__ partB,
+    double* __restrict__ partC, T* __restrict__ x, const T* __restrict__ p,
+    const T* __restrict__ s, const T* __restrict__ t, T* __restrict__ r,
+    const T* __restrict__ rh) {
+  const KrylovCtl c0 = snapshot(ctl);
+  if (c0.done) return;
+  const double alpha = c0.alpha;
+  const double rho1 = c0.rho1;
+  const int k = c0.iter;
+  double red[2];
+  const int ops[2] = {0, 0};
+  reduce_partials<2>(partB, nblk, red, ops);
+  const double buf1 = red[0], buf2 = red[1];
+  const T ta = (T)alpha;
+  if (buf2 == 0) {                                  // pressure.py:251, 266-267
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+         idx += gridDim.x * blockDim.x)
+      x[idx] = fma(p[idx], ta, x[idx]);
+    if (lead()) {
+      ctl->status = PF_SOLVE_BREAKDOWN_B;
+      ctl->halt = 1;
+    }
+    return;
+  }
+  const double omega = buf1 / buf2;                 // pressure.py:268
+  if (lead()) {
+    ctl->omega = omega;
+    ctl->rho2 = rho1;                               // pressure.py:276
+    ctl->iter = k + 1;
+  }
+  const T to = (T)omega, tmo = (T)(-omega);
+  double acc[2] = {0.0, 0.0};
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    const T sc = s[idx];
+    T xv = fma(p[idx], ta, x[idx]);                 // pressure.py:251
+    xv = fma(sc, to, xv);                           // pressure.py:270
+    x[idx] = xv;
+    const T rn = fma(t[idx], tmo, sc);              // pressure.py:274-275
+    r[idx] = rn;
+    acc[0] += (double)rn * (double)rh[idx];
+    acc[1] = nmax(acc[1], fabs((double)rn));
+  }
+  const int ops2[2] = {0, 1};
+  block_reduce<2>(acc, ops2);
+  write_partials<2>(partC, acc);
+}
+
+// DP (padded) := x with periodic overlap (pressure.py:293), then
+// p += DP over the whole padded array (pressure.py:294).
+template <typename T, bool ADD_P>
+__global__ void __launch_bounds__(kBlock) scatter_padded_kernel(Geom g, const T* __restrict__ x,
+                                                                 T* __restrict__ DP,
+                                                                 T* __restrict__ p) {
+  const int nxo = g.nx + 2 * kNover, nyo = g.ny + 2 * kNover, nzo = g.nz + 2 * kNover;
+  const int64_t total = (int64_t)nxo * nyo * nzo;
+  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < total;
+       a += (int64_t)gridDim.x * blockDim.x) {
+    const int ko = (int)(a % nzo);
+    const int64_t t1 = a / nzo;
+    const int jo = (int)(t1 % nyo);
+    const int io = (int)(t1 / nyo);
+    int i = io - kNover, j = jo - kNover, k = ko - kNover;
+    i = i < 0 ? i + g.nx : (i >= g.nx ? i - g.nx : i);
+    j = j < 0 ? j + g.ny : (j >= g.ny ? j - g.ny : j);
+    k = k < 0 ? k + g.nz : (k >= g.nz ? k - g.nz : k);
+    const T d = x[i * g.isx + j * g.isy + k];
+    DP[a] = d;
+    if (ADD_P) p[a] = p[a] + d;
+  }
+}
+
+// u -= (dt/rho) * grad_P(DP) on the interior (metric_staggered.py:390-401,
+// state.py:198-211); DP is the solver's interior-layout solution.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) project_kernel(Geom g, const T* __restrict__ x, T* U,
+                                                         T* V, T* W, T dxi, T dyi, T dzi,
+                                                         T neg_c) {
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(g, idx, i, j, k);
+    const Nbr n = nbr(g, i, j, k);
+    const T d = x[idx];
+    const int64_t c = pidx(g, i, j, k);
+    const T gx = (d - x[idx + n.xm]) * dxi;
+    const T gy = (d - x[idx + n.ym]) * dyi;
+    const T gz = (d - x[idx + n.zm]) * dzi;
+    store_padded(g, U, i, j, k, fma(gx, neg_c, U[c]));
+    store_padded(g, V, i, j, k, fma(gy, neg_c, V[c]));
+    store_padded(g, W, i, j, k, fma(gz, neg_c, W[c]));
+  }
+}
+
+// Same projection with DP given as a padded field (Jacobi's pOld).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) project_padded_kernel(Geom g, const T* __restrict__ D,
+                                                                T* U, T* V, T* W, T dxi, T dyi,
+                                                                T dzi, T neg_c) {
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(g, idx, i, j, k);
+    const int64_t c = pidx(g, i, j, k);
+    const T d = D[c];
+    const T gx = (d - D[c - g.psx]) * dxi;
+    const T gy = (d - D[c - g.psy]) * dyi;
+    const T gz = (d - D[c - 1]) * dzi;
+    store_padded(g, U, i, j, k, fma(gx, neg_c, U[c]));
+    store_padded(g, V, i, j, k, fma(gy, neg_c, V[c]));
+    store_padded(g, W, i, j, k, fma(gz, neg_c, W[c]));
+  }
+}
+
+// Monitor values: max|u|,|v|,|w|, sum(u^2+v^2+w^2), max|div| (+ div into divg).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) stats_kernel(Geom g, const T* __restrict__ U,
+                                                       const T* __restrict__ V,
+                                                       const T* __restrict__ W, T dxi, T dyi,
+                                                       T dzi, T* __restrict__ divg,
+                                                       double* __restrict__ part) {
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const int ops[5] = {1, 1, 1, 0, 1};
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(g, idx, i, j, k);
+    const int64_t c = i * g.psx + j * g.psy + k;
+    const T u = U[c], v = V[c], w = W[c];
+    acc[0] = nmax(acc[0], fabs((double)u));
+    acc[1] = nmax(acc[1], fabs((double)v));
+    acc[2] = nmax(acc[2], fabs((double)w));
+    acc[3] += (double)(((u * u) + (v * v)) + (w * w));   // PyFlow.py:388-390
+    T d = (U[c + g.psx] - u) * dxi;
+    d = d + ((V[c + g.psy] - v) * dyi);
+    d = d + ((W[c + 1] - w) * dzi);
+    if (divg) divg[idx] = d;
+    acc[4] = nmax(acc[4], fabs((double)d));
+  }
+  block_reduce<5>(acc, ops);
+  write_partials<5>(part, acc);
+}
+
+struct Ops8 {
+  int v[8];
+};
+
+template <int NV>
+__global__ void __launch_bounds__(kBlock) finalize_kernel(const double* __restrict__ part,
+                                                          int nblk, Ops8 o,
+                                                          double* __restrict__ out) {
+  int ops[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) ops[q] = o.v[q];
+  double v[NV];
+  reduce_partials<NV>(part, nblk, v, ops);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NV; ++q) out[q] = v[q];
+}
+
+// B (interior) = 7-point operator applied to padded A (pressure.py:63-84).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) apply_lap_kernel(Geom g, const T* __restrict__ A,
+                                                           T* __restrict__ B, Lap<T> L) {
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(g, idx, i, j, k);
+    const int64_t c = pidx(g, i, j, k);
+    B[idx] = lap7(L, A[c - g.psx], A[c], A[c + g.psx], A[c - g.psy], A[c + g.psy], A[c - 1], A[c + 1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi (pressure.py:115-161)
+// init: p = 0 (and the ping-pong partner), src *= rho/dt, max|src|, src *= dx^2
+template <typename T>
+__global__ void __launch_bounds__(kBlock) jacobi_init_kernel(Geom g, T* __restrict__ src,
+                                                             const T* __restrict__ U,
+                                                             const T* __restrict__ V,
+                                                             const T* __restrict__ W,
+                                                             int from_div, T dxi, T dyi, T dzi,
+                                                             T sc1, T dx2,
+                                                             double* __restrict__ part) {
+  double acc[1] = {0.0};
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    T b;
+    if (from_div) {
+      int i, j, k;
+      decode(g, idx, i, j, k);
+      const int64_t c = i * g.psx + j * g.psy + k;
+      b = (U[c + g.psx] - U[c]) * dxi;
+      b = b + ((V[c + g.psy] - V[c]) * dyi);
+      b = b + ((W[c + 1] - W[c]) * dzi);
+    } else {
+      b = src[idx];
+    }
+    b = b * sc1;
+    acc[0] = nmax(acc[0], fabs((double)b));
+    src[idx] = b * dx2;
+  }
+  const int ops[1] = {1};
+  block_reduce<1>(acc, ops);
+  write_partials<1>(part, acc);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) zero_padded_kernel(int64_t n, T* __restrict__ a) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    a[q] = (T)0;
+}
+
+// One sweep: out = ((((((-o(i+1) - o(i-1)) - o(j+1)) - o(j-1)) - o(k+1)) - o(k-1)) + src)*DInv
+template <typename T>
+__global__ void __launch_bounds__(kBlock) jacobi_sweep_kernel(Geom g, const T* __restrict__ O,
+                                                              const T* __restrict__ src,
+                                                              T* __restrict__ out, T dinv) {
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(g, idx, i, j, k);
+    const int64_t c = pidx(g, i, j, k);
+    T a = -O[c + g.psx];
+    a = a - O[c - g.psx];
+    a = a - O[c + g.psy];
+    a = a - O[c - g.psy];
+    a = a - O[c + 1];
+    a = a - O[c - 1];
+    a = a + src[idx];
+    store_padded(g, out, i, j, k, a * dinv);
+  }
+}
+
+// Final residual: src /= dx^2; res = -(A p) + src; max|res|.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) jacobi_resid_kernel(Geom g, const T* __restrict__ P,
+                                                              T* __restrict__ src, T dx2, Lap<T> L,
+                                                              double* __restrict__ part) {
+  double acc[1] = {0.0};
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(g, idx, i, j, k);
+    const int64_t c = pidx(g, i, j, k);
+    const T s = src[idx] / dx2;
+    src[idx] = s;
+    T b = lap7(L, P[c - g.psx], P[c], P[c + g.psx], P[c - g.psy], P[c + g.psy], P[c - 1], P[c + 1]);
+    b = b * (T)(-1.0);
+    b = b + s;
+    acc[0] = nmax(acc[0], fabs((double)b));
+  }
+  const int ops[1] = {1};
+  block_reduce<1>(acc, ops);
+  write_partials<1>(part, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Host drivers
+template <typename T>
+int finalize(pf_box* h, hipStream_t s, const double* part, int nv, const int* ops, double* out) {
+  Ops8 o;
+  for (int q = 0; q < 8; ++q) o.v[q] = q < nv ? ops[q] : 0;
+  switch (nv) {
+    case 1: hipLaunchKernelGGL(finalize_kernel<1>, dim3(1), dim3(kBlock), 0, s, part, h->nblk, o, h->stats_dev); break;
+    case 5: hipLaunchKernelGGL(finalize_kernel<5>, dim3(1), dim3(kBlock), 0, s, part, h->nblk, o, h->stats_dev); break;
+    default: return fail(PF_EINVAL, "finalize: unsupported width");
+  }
+  PF_LAUNCH_CHECK("finalize_kernel");
+  PF_HIP(hipMemcpyAsync(h->stats_host, h->stats_dev, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+  PF_HIP(hipStreamSynchronize(s));
+  for (int q = 0; q < nv; ++q) out[q] = h->stats_host[q];
+  return 0;
+}
+
+template <typename T>
+int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, const T* W,
+                  double src_scale, double tol, int maxit, double* out_res, int* out_iters) {
+  const Geom& g = h->g;
+  const int nb = h->nblk;
+  const double vol = h->dx * h->dy * h->dz;
+  const Lap<T> L = make_lap<T>(h);
+  T* x = (T*)h->x;
+  T* r = (T*)h->r;
+  T* rh = (T*)h->rh;
+  T* P[2] = {(T*)h->p[0], (T*)h->p[1]};
+  T* Vv[2] = {(T*)h->v[0], (T*)h->v[1]};
+  T* sv = (T*)h->s;
+  T* tv = (T*)h->t;
+  double* pA = part_buf(h, kPartA);
+  double* pB = part_buf(h, kPartB);
+  double* pC = part_buf(h, kPartC);
+
+  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(1), 0, s, h->ctl, tol, maxit);
+  PF_LAUNCH_CHECK("ctl_init_kernel");
+  const T dxi = (T)(1.0 / h->dx), dyi = (T)(1.0 / h->dy), dzi = (T)(1.0 / h->dz);
+  const T sc1 = (T)src_scale, sc2 = (T)(-1.0 * vol);
+  if (U)
+    hipLaunchKernelGGL((bicg_init_kernel<T, true>), dim3(nb), dim3(kBlock), 0, s, g, src,
+                       U + g.poff, V + g.poff, W + g.poff, dxi, dyi, dzi, sc1, sc2, r, rh, x,
+                       P[0], Vv[0], pC);
+  else
+    hipLaunchKernelGGL((bicg_init_kernel<T, false>), dim3(nb), dim3(kBlock), 0, s, g, src,
+                       (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, dxi, dyi, dzi,
+                       sc1, sc2, r, rh, x, P[0], Vv[0], pC);
+  PF_LAUNCH_CHECK("bicg_init_kernel");
+
+  // KA of iteration 0, then batches of (KB, KC, KA) until the device says done.
+  // With profiling on, each launch is bracketed by an event pair; pairs are
+  // read back after the batch's host sync and only launches that did work
+  // (iteration index below the completed count) are accumulated.
+  struct Rec {
+    int kind, iter, slot;
+  };
+  Rec recs[3 * 1024 + 1];
+  int nrec = 0;
+  int slot[3] = {0, 0, 0};
+  auto launch = [&](int kind, int iter, auto&& go) {
+    const bool prof = h->prof_on && slot[kind] < h->prof_cap;
+    hipEvent_t* e = prof ? h->prof_ev + ((size_t)kind * h->prof_cap + slot[kind]) * 2 : nullptr;
+    if (prof) hipEventRecord(e[0], s);
+    go();
+    if (prof) {
+      hipEventRecord(e[1], s);
+      recs[nrec++] = Rec{kind, iter, slot[kind]++};
+    }
+  };
+  int it = 0;  // iterations whose KA has been launched
+  launch(0, 0, [&] {
+    hipLaunchKernelGGL(bicg_ka_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, nb, h->ctl, pC, pA, r,
+                       rh, P[0], Vv[0], P[1], Vv[1], L);
+  });
+  PF_LAUNCH_CHECK("bicg_ka_kernel");
+  const int cap = maxit < 0 ? 0 : maxit;
+  bool first = true;
+  for (;;) {
+    int m = h->chunk;
+    if (first && h->last_iters > m) m = h->last_iters - 1;
+    if (h->prof_on && m > h->prof_cap - 1) m = h->prof_cap - 1;
+    if (m > 1024) m = 1024;
+    first = false;
+    for (int q = 0; q < m; ++q) {
+      const int cur = (it + 1) & 1;   // buffers written by iteration `it`
+      launch(1, it, [&] {
+        hipLaunchKernelGGL(bicg_kb_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, nb, h->ctl, pA,
+                           pB, r, Vv[cur], sv, tv, L);
+      });
+      launch(2, it, [&] {
+        hipLaunchKernelGGL(bicg_kc_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, nb, h->ctl, pB,
+                           pC, x, P[cur], sv, tv, r, rh);
+      });
+      ++it;
+      const int nxt = (it + 1) & 1;
+      launch(0, it, [&] {
+        hipLaunchKernelGGL(bicg_ka_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, nb, h->ctl, pC,
+                           pA, r, rh, P[cur], Vv[cur], P[nxt], Vv[nxt], L);
+      });
+    }
+    PF_LAUNCH_CHECK("bicgstab batch");
+    PF_HIP(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
+    PF_HIP(hipStreamSynchronize(s));
+    const KrylovCtl cb = *h->ctl_host;
+    for (int q = 0; q < nrec; ++q) {
+      const Rec& rr = recs[q];
+      const bool worked = rr.kind == 0 ? (rr.iter < cb.iter || !cb.done) : rr.iter < cb.iter;
+      if (!worked) continue;
+      float ms = 0.f;
+      hipEvent_t* e = h->prof_ev + ((size_t)rr.kind * h->prof_cap + rr.slot) * 2;
+      if (hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess) {
+        h->prof_ms[rr.kind] += ms;
+        h->prof_n[rr.kind] += 1;
+      }
+    }
+    nrec = 0;
+    slot[0] = slot[1] = slot[2] = 0;
+    if (cb.done) break;
+    if (it > cap + 2)
+      return fail(PF_EHIP, "bicgstab: device control block did not finish within the iteration cap");
+  }
+  const KrylovCtl c = *h->ctl_host;
+  h->last_iters = c.iter;
+  if (out_res) *out_res = c.max_res;
+  if (out_iters) *out_iters = c.iter;
+  return c.status;
+}
+
+template <typename T>
+int scatter_dp(pf_box* h, hipStream_t s, T* DP, T* p) {
+  const Geom& g = h->g;
+  const int64_t total = (int64_t)(g.nx + 4) * (g.ny + 4) * (g.nz + 4);
+  int64_t nb = (total + kBlock - 1) / kBlock;
+  if (nb > 65536) nb = 65536;
+  if (p)
+    hipLaunchKernelGGL((scatter_padded_kernel<T, true>), dim3((int)nb), dim3(kBlock), 0, s, g,
+                       (const T*)h->x, DP, p);
+  else
+    hipLaunchKernelGGL((scatter_padded_kernel<T, false>), dim3((int)nb), dim3(kBlock), 0, s, g,
+                       (const T*)h->x, DP, p);
+  PF_LAUNCH_CHECK("scatter_padded_kernel");
+  return 0;
+}
+
+template <typename T>
+int jacobi_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, const T* W, T* pOld,
+                T* p, double src_scale, int nit, double* out_res) {
+  const Geom& g = h->g;
+  const int nb = h->nblk;
+  const int nbf = launch_blocks(g);
+  const T dxi = (T)(1.0 / h->dx), dyi = (T)(1.0 / h->dy), dzi = (T)(1.0 / h->dz);
+  const T dx2 = (T)(h->dx * h->dx);
+  double* pS = part_buf(h, kPartS);
+  const int64_t np = (int64_t)(g.nx + 4) * (g.ny + 4) * (g.nz + 4);
+  int64_t nbz = (np + kBlock - 1) / kBlock;
+  if (nbz > 65536) nbz = 65536;
+  // p = 0 (pressure.py:121); the sweep reading iterate 0 reads zeros.
+  T* bufs[2] = {p, pOld};
+  // iterate j lives in bufs[(nit - j) & 1] so that p = iterate nit, pOld = iterate nit-1
+  T* it0 = bufs[nit & 1];
+  hipLaunchKernelGGL(zero_padded_kernel<T>, dim3((int)nbz), dim3(kBlock), 0, s, np, p);
+  if (it0 != p) hipLaunchKernelGGL(zero_padded_kernel<T>, dim3((int)nbz), dim3(kBlock), 0, s, np, it0);
+  hipLaunchKernelGGL(jacobi_init_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, src,
+                     U ? U + g.poff : nullptr, V ? V + g.poff : nullptr, W ? W + g.poff : nullptr,
+                     U ? 1 : 0, dxi, dyi, dzi, (T)src_scale, dx2, pS);
+  PF_LAUNCH_CHECK("jacobi_init_kernel");
+  double res0;
+  const int opm[1] = {1};
+  if (int rc = finalize<T>(h, s, pS, 1, opm, &res0)) return rc;
+  const T dinv = (T)(-1.0 / 6.0);
+  for (int j = 1; j <= nit; ++j) {
+    const T* in = bufs[(nit - j + 1) & 1];
+    T* out = bufs[(nit - j) & 1];
+    hipLaunchKernelGGL(jacobi_sweep_kernel<T>, dim3(nbf), dim3(kBlock), 0, s, g, in, src, out, dinv);
+  }
+  PF_LAUNCH_CHECK("jacobi_sweep_kernel");
+  const Lap<T> L = make_lap<T>(h);
+  hipLaunchKernelGGL(jacobi_resid_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, (const T*)p + g.poff,
+                     src, dx2, L, pS);
+  PF_LAUNCH_CHECK("jacobi_resid_kernel");
+  double resP;
+  if (int rc = finalize<T>(h, s, pS, 1, opm, &resP)) return rc;
+  if (out_res) *out_res = resP / res0;
+  return 0;
+}
+
+template <typename T>
+int stats_impl(pf_box* h, hipStream_t s, const T* U, const T* V, const T* W, T* divg,
+               double* out5) {
+  const Geom& g = h->g;
+  double* pS = part_buf(h, kPartS);
+  hipLaunchKernelGGL(stats_kernel<T>, dim3(h->nblk), dim3(kBlock), 0, s, g, U + g.poff,
+                     V + g.poff, W + g.poff, (T)(1.0 / h->dx), (T)(1.0 / h->dy), (T)(1.0 / h->dz),
+                     divg, pS);
+  PF_LAUNCH_CHECK("stats_kernel");
+  const int ops[5] = {1, 1, 1, 0, 1};
+  double v[5];
+  if (int rc = finalize<T>(h, s, pS, 5, ops, v)) return rc;
+  const double npts = (double)g.ncell;
+  out5[0] = v[0]; out5[1] = v[1]; out5[2] = v[2];
+  out5[3] = v[3] / npts * 0.5;
+  out5[4] = v[4];
+  return 0;
+}
+
+template <typename T>
+int div_impl(pf_box* h, hipStream_t s, const T* U, const T* V, const T* W, T* divg) {
+  double out5[5];
+  // the stats kernel computes the divergence on the way; its reductions are cheap
+  return stats_impl<T>(h, s, U, V, W, divg, out5);
+}
+
+template <typename T>
+int vel_corr_impl(pf_box* h, hipStream_t s, T* U, T* V, T* W, T* DP, T* p, T* src, int solver,
+                  double src_scale, double corr_scale, double tol, int iters, double* out_res,
+                  int* out_iters) {
+  const Geom& g = h->g;
+  const int nbf = launch_blocks(g);
+  const T dxi = (T)(1.0 / h->dx), dyi = (T)(1.0 / h->dy), dzi = (T)(1.0 / h->dz);
+  const T neg_c = (T)(-corr_scale);
+  if (solver == PF_POISSON_BICGSTAB) {
+    int st = bicgstab_impl<T>(h, s, src, U, V, W, src_scale, tol, iters, out_res, out_iters);
+    if (st < 0) return st;
+    if (int rc = scatter_dp<T>(h, s, DP, p)) return rc;
+    hipLaunchKernelGGL(project_kernel<T>, dim3(nbf), dim3(kBlock), 0, s, g, (const T*)h->x, U, V,
+                       W, dxi, dyi, dzi, neg_c);
+    PF_LAUNCH_CHECK("project_kernel");
+    return st;
+  }
+  int rc = jacobi_impl<T>(h, s, src, U, V, W, DP, p, src_scale, iters, out_res);
+  if (rc) return rc;
+  if (out_iters) *out_iters = iters;
+  hipLaunchKernelGGL(project_padded_kernel<T>, dim3(nbf), dim3(kBlock), 0, s, g,
+                     (const T*)DP + g.poff, U, V, W, dxi, dyi, dzi, neg_c);
+  PF_LAUNCH_CHECK("project_padded_kernel");
+  return 0;
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+extern "C" {
+
+int pf_bicgstab_solve(pf_box* h, void* DP, void* p, void* source, double rho, double dt,
+                      double tol, int maxit, double* out_res, int* out_iters, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (!DP || !p || !source) return fail(PF_EINVAL, "pf_bicgstab_solve: NULL field pointer");
+  if (!(dt != 0.0)) return fail(PF_EINVAL, "pf_bicgstab_solve: dt must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  int st;
+  if (h->dtype == PF_F64) {
+    st = bicgstab_impl<double>(h, s, (double*)source, nullptr, nullptr, nullptr, rho / dt, tol,
+                               maxit, out_res, out_iters);
+    if (st >= 0) {
+      int rc = scatter_dp<double>(h, s, (double*)DP, (double*)p);
+      if (rc) return rc;
+    }
+  } else {
+    st = bicgstab_impl<float>(h, s, (float*)source, nullptr, nullptr, nullptr, rho / dt, tol,
+                              maxit, out_res, out_iters);
+    if (st >= 0) {
+      int rc = scatter_dp<float>(h, s, (float*)DP, (float*)p);
+      if (rc) return rc;
+    }
+  }
+  return st;
+}
+
+int pf_jacobi_solve(pf_box* h, void* pOld, void* p, void* source, double rho, double dt, int nit,
+                    double* out_res, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (!pOld || !p || !source) return fail(PF_EINVAL, "pf_jacobi_solve: NULL field pointer");
+  if (nit < 1) return fail(PF_EINVAL, "pf_jacobi_solve: nit must be >= 1");
+  if (!(dt != 0.0)) return fail(PF_EINVAL, "pf_jacobi_solve: dt must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == PF_F64)
+    return jacobi_impl<double>(h, s, (double*)source, nullptr, nullptr, nullptr, (double*)pOld,
+                               (double*)p, rho / dt, nit, out_res);
+  return jacobi_impl<float>(h, s, (float*)source, nullptr, nullptr, nullptr, (float*)pOld,
+                            (float*)p, rho / dt, nit, out_res);
+}
+
+int pf_vel_corr(pf_box* h, void* u, void* v, void* w, void* DP, void* p, void* source, int solver,
+                double src_scale, double corr_scale, double tol, int iters, double* out_res,
+                int* out_iters, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (!u || !v || !w || !DP || !p || !source)
+    return fail(PF_EINVAL, "pf_vel_corr: NULL field pointer");
+  if (solver != PF_POISSON_BICGSTAB && solver != PF_POISSON_JACOBI)
+    return fail(PF_EINVAL, "pf_vel_corr: unknown solver");
+  if (solver == PF_POISSON_JACOBI && iters < 1)
+    return fail(PF_EINVAL, "pf_vel_corr: Jacobi needs at least one sweep");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == PF_F64)
+    return vel_corr_impl<double>(h, s, (double*)u, (double*)v, (double*)w, (double*)DP,
+                                 (double*)p, (double*)source, solver, src_scale, corr_scale, tol,
+                                 iters, out_res, out_iters);
+  return vel_corr_impl<float>(h, s, (float*)u, (float*)v, (float*)w, (float*)DP, (float*)p,
+                              (float*)source, solver, src_scale, corr_scale, tol, iters, out_res,
+                              out_iters);
+}
+
+int pf_profile_enable(pf_box* h, int on) {
+  if (int rc = check_box(h)) return rc;
+  if (on && !h->prof_ev) {
+    const int cap = 1024;
+    h->prof_ev = new hipEvent_t[(size_t)3 * cap * 2];
+    for (int q = 0; q < 3 * cap * 2; ++q) PF_HIP(hipEventCreate(&h->prof_ev[q]));
+    h->prof_cap = cap;
+  }
+  h->prof_on = on ? 1 : 0;
+  for (int k = 0; k < 3; ++k) {
+    h->prof_ms[k] = 0.0;
+    h->prof_n[k] = 0;
+  }
+  return 0;
+}
+
+int pf_profile_read(pf_box* h, int kernel, double* total_ms, int64_t* launches) {
+  if (int rc = check_box(h)) return rc;
+  if (kernel < 0 || kernel > 2) return fail(PF_EINVAL, "pf_profile_read: kernel is 0 (KA), 1 (KB) or 2 (KC)");
+  if (total_ms) *total_ms = h->prof_ms[kernel];
+  if (launches) *launches = h->prof_n[kernel];
+  return 0;
+}
+
+int pf_apply_laplacian(pf_box* h, const void* A, void* B, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (!A || !B) return fail(PF_EINVAL, "pf_apply_laplacian: NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = launch_blocks(h->g);
+  if (h->dtype == PF_F64)
+    hipLaunchKernelGGL(apply_lap_kernel<double>, dim3(nb), dim3(kBlock), 0, s, h->g,
+                       (const double*)A, (double*)B, make_lap<double>(h));
+  else
+    hipLaunchKernelGGL(apply_lap_kernel<float>, dim3(nb), dim3(kBlock), 0, s, h->g,
+                       (const float*)A, (float*)B, make_lap<float>(h));
+  PF_LAUNCH_CHECK("apply_lap_kernel");
+  return 0;
+}
+
+int pf_stats(pf_box* h, const void* u, const void* v, const void* w, void* divg, double* out5,
+             void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (!u || !v || !w || !out5) return fail(PF_EINVAL, "pf_stats: NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == PF_F64)
+    return stats_impl<double>(h, s, (const double*)u, (const double*)v, (const double*)w,
+                              (double*)divg, out5);
+  return stats_impl<float>(h, s, (const float*)u, (const float*)v, (const float*)w, (float*)divg,
+                           out5);
+}
+
+int pf_div_vel(pf_box* h, const void* u, const void* v, const void* w, void* divg, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (!u || !v || !w || !divg) return fail(PF_EINVAL, "pf_div_vel: NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == PF_F64)
+    return div_impl<double>(h, s, (const double*)u, (const double*)v, (const double*)w,
+                            (double*)divg);
+  return div_impl<float>(h, s, (const float*)u, (const float*)v, (const float*)w, (float*)divg);
+}
+
+}  // extern "C"

@@ -1,0 +1,357 @@
+// Fused Navier-Stokes right-hand side and the RK4 / Euler stage updates.
+//
+// One thread owns one interior cell (i,j,k) and produces the u-, v- and
+// w-momentum RHS there (u at face x_i, v at y_j, w at z_k) straight from the
+// 33 padded velocity values around it, instead of the reference's ~220
+// whole-array passes (solver/velocity.py:221-383).  The arithmetic follows the
+// reference's op order exactly: every torch op rounds on its own, and the
+// reference's add_/sub_ with a scalar (metric_staggered.py:411-418) is one
+// fused multiply-add, so the fluxes use separate ops and div_visc uses fma().
+// The file is built with -ffp-contract=off so the compiler adds no other
+// contractions.
+#include <hip/hip_runtime.h>
+
+#include "pf_box.h"
+
+namespace pf {
+
+template <typename T>
+struct StepCoef {
+  T dxi, dyi, dzi;
+  T c2;               // 2*VISC/rho          (velocity.py:250)
+  T exy, exz, eyz;    // interp_sc_**(VISC)/rho  (velocity.py:252-260)
+  T half_dt;          // 0.5*simDt           (advancer.py:81)
+  T dt6;              // simDt/6.0           (advancer.py:100)
+  T dt;               // simDt               (advancer.py:55)
+};
+
+// Scalars rounded exactly as the reference's tensor ops round them: VISC_P is
+// ones*mu in the field dtype (domain.py:130-131); edge viscosities are
+// 0.5*0.5*(((V+V)+V)+V) (metric_staggered.py:127-144) divided by rho.
+template <typename T>
+StepCoef<T> make_step_coef(const pf_box* h, double mu, double rho, double dt) {
+  StepCoef<T> c;
+  c.dxi = (T)(1.0 / h->dx);
+  c.dyi = (T)(1.0 / h->dy);
+  c.dzi = (T)(1.0 / h->dz);
+  const T V = (T)mu;
+  const T R = (T)rho;
+  c.c2 = ((T)2.0 * V) / R;
+  const T edge = (T)0.25 * (((V + V) + V) + V);
+  c.exy = edge / R;
+  c.exz = edge / R;
+  c.eyz = edge / R;
+  c.half_dt = (T)(0.5 * dt);
+  c.dt6 = (T)(dt / 6.0);
+  c.dt = (T)dt;
+  return c;
+}
+
+template <typename T>
+struct Sfs {
+  const T* f[6];      // FXX, FYY, FZZ, FXY, FXZ, FYZ at the interior origin, or all NULL
+};
+
+// 0.25*(((F(0)+F(-a))+F(-b))+F(-a-b)): interp_sc_xy/xz/yz at an edge.
+template <typename T>
+__device__ __forceinline__ T edge4(const T* __restrict__ F, int64_t c, int64_t sa, int64_t sb) {
+  return (T)0.25 * (((F[c] + F[c - sa]) + F[c - sb]) + F[c - sa - sb]);
+}
+
+// RHS at interior cell (i,j,k); U,V,W point at the padded interior origin and
+// c is the padded offset of the cell.
+template <typename T, bool SFS>
+__device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __restrict__ V,
+                                         const T* __restrict__ W, int64_t c, int64_t sx,
+                                         int64_t sy, const StepCoef<T>& k, const Sfs<T>& sfs,
+                                         T& ru, T& rv, T& rw) {
+  const T dxi = k.dxi, dyi = k.dyi, dzi = k.dzi;
+  const T half = (T)0.5;
+#define LD(A, di, dj, dk) A[c + (di) * sx + (dj) * sy + (dk)]
+  const T u000 = LD(U, 0, 0, 0), up00 = LD(U, 1, 0, 0), um00 = LD(U, -1, 0, 0);
+  const T u0p0 = LD(U, 0, 1, 0), u0m0 = LD(U, 0, -1, 0), u00p = LD(U, 0, 0, 1), u00m = LD(U, 0, 0, -1);
+  const T ump0 = LD(U, -1, 1, 0), upm0 = LD(U, 1, -1, 0), um0p = LD(U, -1, 0, 1), up0m = LD(U, 1, 0, -1);
+  const T v000 = LD(V, 0, 0, 0), vp00 = LD(V, 1, 0, 0), vm00 = LD(V, -1, 0, 0);
+  const T v0p0 = LD(V, 0, 1, 0), v0m0 = LD(V, 0, -1, 0), v00p = LD(V, 0, 0, 1), v00m = LD(V, 0, 0, -1);
+  const T vmp0 = LD(V, -1, 1, 0), vpm0 = LD(V, 1, -1, 0), v0mp = LD(V, 0, -1, 1), v0pm = LD(V, 0, 1, -1);
+  const T w000 = LD(W, 0, 0, 0), wp00 = LD(W, 1, 0, 0), wm00 = LD(W, -1, 0, 0);
+  const T w0p0 = LD(W, 0, 1, 0), w0m0 = LD(W, 0, -1, 0), w00p = LD(W, 0, 0, 1), w00m = LD(W, 0, 0, -1);
+  const T wm0p = LD(W, -1, 0, 1), wp0m = LD(W, 1, 0, -1), w0mp = LD(W, 0, -1, 1), w0pm = LD(W, 0, 1, -1);
+#undef LD
+
+  // div_vel_over / 3 at this cell and its three lower neighbours
+  // (metric_staggered.py:424-433, velocity.py:240-241)
+  const T third = (T)3.0;
+  const T D_c = ((((up00 - u000) * dxi) + ((v0p0 - v000) * dyi)) + ((w00p - w000) * dzi)) / third;
+  const T D_xm = ((((u000 - um00) * dxi) + ((vmp0 - vm00) * dyi)) + ((wm0p - wm00) * dzi)) / third;
+  const T D_ym = ((((upm0 - u0m0) * dxi) + ((v000 - v0m0) * dyi)) + ((w0mp - w0m0) * dzi)) / third;
+  const T D_zm = ((((up0m - u00m) * dxi) + ((v0pm - v00m) * dyi)) + ((w000 - w00m) * dzi)) / third;
+
+  // ---- u: viscous fluxes (velocity.py:248-269) ----
+  T FX_c = (((up00 - u000) * dxi) - D_c) * k.c2;
+  T FX_m = (((u000 - um00) * dxi) - D_xm) * k.c2;
+  T FY_c = (((u0p0 - u000) * dyi) + ((vp00 - v000) * dxi)) * k.exy;
+  T FY_m = (((u000 - u0m0) * dyi) + ((vpm0 - v0m0) * dxi)) * k.exy;
+  T FZ_c = (((u00p - u000) * dzi) + ((wp00 - w000) * dxi)) * k.exz;
+  T FZ_m = (((u000 - u00m) * dzi) + ((wp0m - w00m) * dxi)) * k.exz;
+  if (SFS) {
+    FX_c = FX_c + sfs.f[0][c];
+    FX_m = FX_m + sfs.f[0][c - sx];
+    FY_c = FY_c + edge4(sfs.f[3], c, sx, sy);
+    FY_m = FY_m + edge4(sfs.f[3], c - sy, sx, sy);
+    FZ_c = FZ_c + edge4(sfs.f[4], c, sx, (int64_t)1);
+    FZ_m = FZ_m + edge4(sfs.f[4], c - 1, sx, (int64_t)1);
+  }
+  T r = fma(FX_c, dxi, (T)0);
+  r = fma(FX_m, -dxi, r);
+  r = fma(FY_c, dyi, r);
+  r = fma(FY_m, -dyi, r);
+  r = fma(FZ_c, dzi, r);
+  r = fma(FZ_m, -dzi, r);
+  // ---- u: advection (velocity.py:322-331, metric_staggered.py:458-526) ----
+  {
+    const T a = half * (up00 + u000), b = half * (u000 + um00);
+    r = r - ((a * a - b * b) * dxi);
+    const T uy_p = half * (u0p0 + u000), vx_p = half * (v0p0 + vmp0);
+    const T uy_c = half * (u000 + u0m0), vx_c = half * (v000 + vm00);
+    r = r - ((uy_p * vx_p - uy_c * vx_c) * dyi);
+    const T uz_p = half * (u00p + u000), wx_p = half * (w00p + wm0p);
+    const T uz_c = half * (u000 + u00m), wx_c = half * (w000 + wm00);
+    r = r - ((uz_p * wx_p - uz_c * wx_c) * dzi);
+  }
+  ru = r;
+
+  // ---- v: viscous fluxes (velocity.py:272-293) ----
+  FX_c = (((vp00 - v000) * dxi) + ((u0p0 - u000) * dyi)) * k.exy;
+  FX_m = (((v000 - vm00) * dxi) + ((ump0 - um00) * dyi)) * k.exy;
+  FY_c = (((v0p0 - v000) * dyi) - D_c) * k.c2;
+  FY_m = (((v000 - v0m0) * dyi) - D_ym) * k.c2;
+  FZ_c = (((v00p - v000) * dzi) + ((w0p0 - w000) * dyi)) * k.eyz;
+  FZ_m = (((v000 - v00m) * dzi) + ((w0pm - w00m) * dyi)) * k.eyz;
+  if (SFS) {
+    FX_c = FX_c + edge4(sfs.f[3], c, sx, sy);
+    FX_m = FX_m + edge4(sfs.f[3], c - sx, sx, sy);
+    FY_c = FY_c + sfs.f[1][c];
+    FY_m = FY_m + sfs.f[1][c - sy];
+    FZ_c = FZ_c + edge4(sfs.f[5], c, sy, (int64_t)1);
+    FZ_m = FZ_m + edge4(sfs.f[5], c - 1, sy, (int64_t)1);
+  }
+  r = fma(FX_c, dxi, (T)0);
+  r = fma(FX_m, -dxi, r);
+  r = fma(FY_c, dyi, r);
+  r = fma(FY_m, -dyi, r);
+  r = fma(FZ_c, dzi, r);
+  r = fma(FZ_m, -dzi, r);
+  // ---- v: advection (velocity.py:334-343) ----
+  {
+    const T vx_p1 = half * (vp00 + v000), uy_p1 = half * (up00 + upm0);
+    const T vx_c = half * (v000 + vm00), uy_c = half * (u000 + u0m0);
+    r = r - ((vx_p1 * uy_p1 - vx_c * uy_c) * dxi);
+    const T a = half * (v0p0 + v000), b = half * (v000 + v0m0);
+    r = r - ((a * a - b * b) * dyi);
+    const T vz_p = half * (v00p + v000), wy_p = half * (w00p + w0mp);
+    const T vz_c = half * (v000 + v00m), wy_c = half * (w000 + w0m0);
+    r = r - ((vz_p * wy_p - vz_c * wy_c) * dzi);
+  }
+  rv = r;
+
+  // ---- w: viscous fluxes (velocity.py:296-317) ----
+  FX_c = (((wp00 - w000) * dxi) + ((u00p - u000) * dzi)) * k.exz;
+  FX_m = (((w000 - wm00) * dxi) + ((um0p - um00) * dzi)) * k.exz;
+  FY_c = (((w0p0 - w000) * dyi) + ((v00p - v000) * dzi)) * k.eyz;
+  FY_m = (((w000 - w0m0) * dyi) + ((v0mp - v0m0) * dzi)) * k.eyz;
+  FZ_c = (((w00p - w000) * dzi) - D_c) * k.c2;
+  FZ_m = (((w000 - w00m) * dzi) - D_zm) * k.c2;
+  if (SFS) {
+    FX_c = FX_c + edge4(sfs.f[4], c, sx, (int64_t)1);
+    FX_m = FX_m + edge4(sfs.f[4], c - sx, sx, (int64_t)1);
+    FY_c = FY_c + edge4(sfs.f[5], c, sy, (int64_t)1);
+    FY_m = FY_m + edge4(sfs.f[5], c - sy, sy, (int64_t)1);
+    FZ_c = FZ_c + sfs.f[2][c];
+    FZ_m = FZ_m + sfs.f[2][c - 1];
+  }
+  r = fma(FX_c, dxi, (T)0);
+  r = fma(FX_m, -dxi, r);
+  r = fma(FY_c, dyi, r);
+  r = fma(FY_m, -dyi, r);
+  r = fma(FZ_c, dzi, r);
+  r = fma(FZ_m, -dzi, r);
+  // ---- w: advection (velocity.py:346-355) ----
+  {
+    const T wx_p1 = half * (wp00 + w000), uz_p1 = half * (up00 + up0m);
+    const T wx_c = half * (w000 + wm00), uz_c = half * (u000 + u00m);
+    r = r - ((wx_p1 * uz_p1 - wx_c * uz_c) * dxi);
+    const T wy_p1 = half * (w0p0 + w000), vz_p1 = half * (v0p0 + v0pm);
+    const T wy_c = half * (w000 + w0m0), vz_c = half * (v000 + v00m);
+    r = r - ((wy_p1 * vz_p1 - wy_c * vz_c) * dyi);
+    const T a = half * (w00p + w000), b = half * (w000 + w00m);
+    r = r - ((a * a - b * b) * dzi);
+  }
+  rw = r;
+}
+
+// Stage kinds.
+enum { kStage1 = 1, kStage2 = 2, kStage3 = 3, kStage4 = 4, kEuler = 0, kRhsOnly = 5 };
+
+// One RK stage (or Euler step, or a bare RHS evaluation).
+//   Y     : stage input state (padded, interior origin)
+//   Un    : u^n (padded, interior origin)
+//   acc   : running k1 + 2k2 + 2k3 (interior)
+//   Yout  : next stage state (padded), or the Euler result, or the RHS (interior)
+//   Uout  : u* for stage 4 (padded; same arrays as Un, read and written at the own cell only)
+template <typename T, int KIND, bool SFS>
+__global__ void __launch_bounds__(kBlock) rk_stage_kernel(Geom g, CVec3<T> Y, CVec3<T> Un,
+                                                          Vec3<T> acc, Vec3<T> Yout, Vec3<T> Uout,
+                                                          StepCoef<T> k, Sfs<T> sfs) {
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, kk;
+    decode(g, idx, i, j, kk);
+    const int64_t c = i * g.psx + j * g.psy + kk;
+    T r[3];
+    rhs_cell<T, SFS>(Y.a[0], Y.a[1], Y.a[2], c, g.psx, g.psy, k, sfs, r[0], r[1], r[2]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (KIND == kRhsOnly) {
+        Yout.a[q][idx] = r[q];
+      } else if (KIND == kEuler) {
+        const T un = Un.a[q][c];
+        store_padded(g, Yout.a[q] - g.poff, i, j, kk, un + (r[q] * k.dt));
+      } else if (KIND == kStage4) {
+        const T un = Un.a[q][c];
+        const T s = acc.a[q][idx] + r[q];
+        store_padded(g, Uout.a[q] - g.poff, i, j, kk, un + (k.dt6 * s));
+      } else {
+        const T un = Un.a[q][c];
+        if (KIND == kStage1)
+          acc.a[q][idx] = r[q];
+        else
+          acc.a[q][idx] = acc.a[q][idx] + ((T)2.0 * r[q]);
+        store_padded(g, Yout.a[q] - g.poff, i, j, kk, (k.half_dt * r[q]) + un);
+      }
+    }
+  }
+}
+
+template <typename T, int KIND>
+int launch_stage(pf_box* h, hipStream_t s, const void* const Y[3], const void* const Un[3],
+                 void* const Yout[3], void* const Uout[3], const StepCoef<T>& k,
+                 const void* const* sfs6) {
+  const Geom& g = h->g;
+  CVec3<T> y, un;
+  Vec3<T> acc, yo, uo;
+  for (int q = 0; q < 3; ++q) {
+    y.a[q] = (const T*)Y[q] + g.poff;
+    un.a[q] = Un && Un[q] ? (const T*)Un[q] + g.poff : nullptr;
+    acc.a[q] = (T*)h->acc[q];
+    if (KIND == kRhsOnly)
+      yo.a[q] = (T*)Yout[q];
+    else
+      yo.a[q] = Yout && Yout[q] ? (T*)Yout[q] + g.poff : nullptr;
+    uo.a[q] = Uout && Uout[q] ? (T*)Uout[q] + g.poff : nullptr;
+  }
+  Sfs<T> sf;
+  for (int m = 0; m < 6; ++m) sf.f[m] = sfs6 ? (const T*)sfs6[m] + g.poff : nullptr;
+  const int nb = launch_blocks(g);
+  if (sfs6)
+    hipLaunchKernelGGL((rk_stage_kernel<T, KIND, true>), dim3(nb), dim3(kBlock), 0, s, g, y, un,
+                       acc, yo, uo, k, sf);
+  else
+    hipLaunchKernelGGL((rk_stage_kernel<T, KIND, false>), dim3(nb), dim3(kBlock), 0, s, g, y, un,
+                       acc, yo, uo, k, sf);
+  PF_LAUNCH_CHECK("rk_stage_kernel");
+  return 0;
+}
+
+template <typename T>
+int rk4_impl(pf_box* h, void* const U[3], void* const Ut[3], double dt, double mu, double rho,
+             const void* const* sfs6, hipStream_t s) {
+  const StepCoef<T> k = make_step_coef<T>(h, mu, rho, dt);
+  // Stage states alternate between the caller's ut/vt/wt and the box's
+  // second buffer so that the last stage state lands in ut/vt/wt
+  // (advancer.py:93-96 leaves Y4 in state_uTmp).
+  void* const A[3] = {Ut[0], Ut[1], Ut[2]};
+  void* const B[3] = {h->ypong[0], h->ypong[1], h->ypong[2]};
+  const void* const Uc[3] = {U[0], U[1], U[2]};
+  const void* const Ac[3] = {A[0], A[1], A[2]};
+  const void* const Bc[3] = {B[0], B[1], B[2]};
+  int rc;
+  if ((rc = launch_stage<T, kStage1>(h, s, Uc, Uc, A, nullptr, k, sfs6))) return rc;
+  if ((rc = launch_stage<T, kStage2>(h, s, Ac, Uc, B, nullptr, k, sfs6))) return rc;
+  if ((rc = launch_stage<T, kStage3>(h, s, Bc, Uc, A, nullptr, k, sfs6))) return rc;
+  if ((rc = launch_stage<T, kStage4>(h, s, Ac, Uc, nullptr, U, k, sfs6))) return rc;
+  return 0;
+}
+
+template <typename T>
+int euler_impl(pf_box* h, void* const U[3], double dt, double mu, double rho,
+               const void* const* sfs6, hipStream_t s) {
+  const StepCoef<T> k = make_step_coef<T>(h, mu, rho, dt);
+  void* const B[3] = {h->ypong[0], h->ypong[1], h->ypong[2]};
+  const void* const Uc[3] = {U[0], U[1], U[2]};
+  int rc = launch_stage<T, kEuler>(h, s, Uc, Uc, B, nullptr, k, sfs6);
+  if (rc) return rc;
+  const size_t np = (size_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4) * sizeof(T);
+  for (int q = 0; q < 3; ++q) PF_HIP(hipMemcpyAsync(U[q], B[q], np, hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+static int check_ptrs(const void* const* p, int n, const char* what) {
+  for (int i = 0; i < n; ++i)
+    if (!p[i]) return fail(PF_EINVAL, std::string(what) + ": NULL field pointer");
+  return 0;
+}
+
+extern "C" {
+
+int pf_rk4_step(pf_box* h, void* u, void* v, void* w, void* ut, void* vt, void* wt, double dt,
+                double mu, double rho, const void* const* sfs6, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  void* U[3] = {u, v, w};
+  void* Ut[3] = {ut, vt, wt};
+  if (int rc = check_ptrs((const void* const*)U, 3, "pf_rk4_step")) return rc;
+  if (int rc = check_ptrs((const void* const*)Ut, 3, "pf_rk4_step")) return rc;
+  if (sfs6)
+    if (int rc = check_ptrs(sfs6, 6, "pf_rk4_step(sfs6)")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_rk4_step: rho must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  return h->dtype == PF_F64 ? rk4_impl<double>(h, U, Ut, dt, mu, rho, sfs6, s)
+                            : rk4_impl<float>(h, U, Ut, dt, mu, rho, sfs6, s);
+}
+
+int pf_euler_step(pf_box* h, void* u, void* v, void* w, double dt, double mu, double rho,
+                  const void* const* sfs6, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  void* U[3] = {u, v, w};
+  if (int rc = check_ptrs((const void* const*)U, 3, "pf_euler_step")) return rc;
+  if (sfs6)
+    if (int rc = check_ptrs(sfs6, 6, "pf_euler_step(sfs6)")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_euler_step: rho must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  return h->dtype == PF_F64 ? euler_impl<double>(h, U, dt, mu, rho, sfs6, s)
+                            : euler_impl<float>(h, U, dt, mu, rho, sfs6, s);
+}
+
+int pf_rhs_ns(pf_box* h, const void* u, const void* v, const void* w, double mu, double rho,
+              const void* const* sfs6, void* ru, void* rv, void* rw, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  const void* U[3] = {u, v, w};
+  void* R[3] = {ru, rv, rw};
+  if (int rc = check_ptrs(U, 3, "pf_rhs_ns")) return rc;
+  if (int rc = check_ptrs((const void* const*)R, 3, "pf_rhs_ns")) return rc;
+  if (sfs6)
+    if (int rc = check_ptrs(sfs6, 6, "pf_rhs_ns(sfs6)")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_rhs_ns: rho must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == PF_F64)
+    return launch_stage<double, kRhsOnly>(h, s, U, nullptr, R, nullptr,
+                                          make_step_coef<double>(h, mu, rho, 0.0), sfs6);
+  return launch_stage<float, kRhsOnly>(h, s, U, nullptr, R, nullptr,
+                                       make_step_coef<float>(h, mu, rho, 0.0), sfs6);
+}
+
+}  // extern "C"

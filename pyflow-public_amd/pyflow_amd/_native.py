@@ -1,0 +1,282 @@
+"""ctypes binding of libpyflow_hip.so (declared in include/pyflow_hip.h).
+
+The library is required: importing the GPU classes without it, or without a
+ROCm device, raises instead of silently computing something else.
+"""
+
+import ctypes
+import os
+
+import torch
+
+from . import _build
+
+PF_F64 = 0
+PF_F32 = 1
+PF_POISSON_BICGSTAB = 0
+PF_POISSON_JACOBI = 1
+SOLVE_STATUS = {0: 'converged', 1: 'maxit', 2: 'breakdown <v,r_hat>=0', 3: 'breakdown <t,t>=0'}
+
+_lib = None
+
+# (name, restype, argtypes)
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_PP = ctypes.POINTER(ctypes.c_void_p)
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int)
+SIGNATURES = [
+    ('pf_abi_version', _I, []),
+    ('pf_last_error', ctypes.c_char_p, []),
+    ('pf_box_create', _I, [ctypes.POINTER(_P), _I, _I, _I, _D, _D, _D, _I]),
+    ('pf_box_destroy', _I, [_P]),
+    ('pf_box_device_bytes', ctypes.c_int64, [_P]),
+    ('pf_box_set_spacing', _I, [_P, _D, _D, _D]),
+    ('pf_fill_halo', _I, [_P, _P, _P]),
+    ('pf_rk4_step', _I, [_P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _PP, _P]),
+    ('pf_euler_step', _I, [_P, _P, _P, _P, _D, _D, _D, _PP, _P]),
+    ('pf_rhs_ns', _I, [_P, _P, _P, _P, _D, _D, _PP, _P, _P, _P, _P]),
+    ('pf_div_vel', _I, [_P, _P, _P, _P, _P, _P]),
+    ('pf_bicgstab_solve', _I, [_P, _P, _P, _P, _D, _D, _D, _I, _PD, _PI, _P]),
+    ('pf_jacobi_solve', _I, [_P, _P, _P, _P, _D, _D, _I, _PD, _P]),
+    ('pf_vel_corr', _I, [_P, _P, _P, _P, _P, _P, _P, _I, _D, _D, _D, _I, _PD, _PI, _P]),
+    ('pf_apply_laplacian', _I, [_P, _P, _P, _P]),
+    ('pf_state_op', _I, [_P, _I, _D, _P, _P, _I, _P, _I, _I, _P]),
+    ('pf_stats', _I, [_P, _P, _P, _P, _P, _PD, _P]),
+    ('pf_sfs_gradient', _I, [_P, _P, _P, _P, _PP, _P]),
+    ('pf_metric_pair', _I, [_P, _I, _I, _I, _P, _P, _D, _P]),
+    ('pf_metric_edge4', _I, [_P, _I, _P, _P, _P]),
+    ('pf_div_visc', _I, [_P, _P, _P, _P, _P, _P]),
+    ('pf_div_vel_over', _I, [_P, _P, _P, _P, _P, _P]),
+    ('pf_vel_conv', _I, [_P, _I, _I, _P, _P, _P, _D, _P]),
+    ('pf_profile_enable', _I, [_P, _I]),
+    ('pf_profile_read', _I, [_P, _I, _PD, ctypes.POINTER(ctypes.c_int64)]),
+]
+
+
+def lib_path():
+    return os.environ.get('PYFLOW_HIP_LIB', _build.LIB_PATH)
+
+
+def lib():
+    """Load the library once (after torch, so both share one HIP runtime)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise RuntimeError('libpyflow_hip.so is not built (expected at %s); run '
+                           '`python -c "import __graft_entry__ as g; g.build()"`' % path)
+    L = ctypes.CDLL(path)
+    for name, res, args in SIGNATURES:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().pf_last_error().decode(errors='replace')
+
+
+def check(rc, what):
+    if rc < 0:
+        raise RuntimeError('%s failed (%d): %s' % (what, rc, last_error()))
+    return rc
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError('pyflow_amd runs its kernels on a ROCm GPU; no GPU is visible')
+
+
+def stream_ptr(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def dtype_code(dtype):
+    if dtype == torch.float64:
+        return PF_F64
+    if dtype == torch.float32:
+        return PF_F32
+    raise ValueError('unsupported dtype %s (float64 or float32)' % dtype)
+
+
+class Box:
+    """Owner of one pf_box handle: grid sizes, spacing, dtype and all device
+    workspace for one periodic box."""
+
+    def __init__(self, nx, ny, nz, dx=1.0, dy=1.0, dz=1.0, dtype=torch.float64, device=None):
+        require_gpu()
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.dtype = dtype
+        self.device = torch.device('cuda', torch.cuda.current_device()) if device is None \
+            else torch.device(device)
+        self.padded_shape = (self.nx + 4, self.ny + 4, self.nz + 4)
+        self.interior_shape = (self.nx, self.ny, self.nz)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(lib().pf_box_create(ctypes.byref(h), self.nx, self.ny, self.nz,
+                                      float(dx), float(dy), float(dz), dtype_code(dtype)),
+                  'pf_box_create')
+        self.h = h
+        self.spacing = (float(dx), float(dy), float(dz))
+
+    def set_spacing(self, dx, dy, dz):
+        check(lib().pf_box_set_spacing(self.h, float(dx), float(dy), float(dz)),
+              'pf_box_set_spacing')
+        self.spacing = (float(dx), float(dy), float(dz))
+
+    @property
+    def device_bytes(self):
+        return int(lib().pf_box_device_bytes(self.h))
+
+    def close(self):
+        if getattr(self, 'h', None) is not None and self.h.value:
+            lib().pf_box_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- argument checking -------------------------------------------------
+    def _ptr(self, t, shape, name):
+        if not isinstance(t, torch.Tensor):
+            raise TypeError('%s must be a torch.Tensor' % name)
+        if t.device.type != 'cuda':
+            raise ValueError('%s must live on the GPU (got %s)' % (name, t.device))
+        if t.dtype != self.dtype:
+            raise ValueError('%s has dtype %s, box uses %s' % (name, t.dtype, self.dtype))
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError('%s has shape %s, expected %s' % (name, tuple(t.shape), tuple(shape)))
+        if not t.is_contiguous():
+            raise ValueError('%s must be contiguous' % name)
+        return ctypes.c_void_p(t.data_ptr())
+
+    def padded(self, t, name='field'):
+        return self._ptr(t, self.padded_shape, name)
+
+    def interior(self, t, name='field'):
+        return self._ptr(t, self.interior_shape, name)
+
+    def sfs_array(self, sfs6):
+        if sfs6 is None:
+            return None
+        arr = (ctypes.c_void_p * 6)(*[self.padded(f, 'sfs[%d]' % m).value for m, f in enumerate(sfs6)])
+        return ctypes.cast(arr, _PP), arr
+
+    def stream(self):
+        return stream_ptr(self.device)
+
+    # -- entry points --------------------------------------------------------
+    def fill_halo(self, a):
+        check(lib().pf_fill_halo(self.h, self.padded(a), self.stream()), 'pf_fill_halo')
+
+    def rk4_step(self, u, v, w, ut, vt, wt, dt, mu, rho, sfs6=None):
+        sa = self.sfs_array(sfs6)
+        check(lib().pf_rk4_step(self.h, self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
+                                self.padded(ut, 'ut'), self.padded(vt, 'vt'), self.padded(wt, 'wt'),
+                                float(dt), float(mu), float(rho), sa[0] if sa else None,
+                                self.stream()), 'pf_rk4_step')
+
+    def euler_step(self, u, v, w, dt, mu, rho, sfs6=None):
+        sa = self.sfs_array(sfs6)
+        check(lib().pf_euler_step(self.h, self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
+                                  float(dt), float(mu), float(rho), sa[0] if sa else None,
+                                  self.stream()), 'pf_euler_step')
+
+    def rhs_ns(self, u, v, w, mu, rho, ru, rv, rw, sfs6=None):
+        sa = self.sfs_array(sfs6)
+        check(lib().pf_rhs_ns(self.h, self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
+                              float(mu), float(rho), sa[0] if sa else None,
+                              self.interior(ru, 'rhs_u'), self.interior(rv, 'rhs_v'),
+                              self.interior(rw, 'rhs_w'), self.stream()), 'pf_rhs_ns')
+
+    def div_vel(self, u, v, w, divg):
+        check(lib().pf_div_vel(self.h, self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
+                               self.interior(divg, 'divg'), self.stream()), 'pf_div_vel')
+
+    def bicgstab(self, DP, p, source, rho, dt, tol, maxit):
+        res = ctypes.c_double(0.0)
+        its = ctypes.c_int(0)
+        st = check(lib().pf_bicgstab_solve(self.h, self.padded(DP, 'DP'), self.padded(p, 'p'),
+                                           self.interior(source, 'source'), float(rho), float(dt),
+                                           float(tol), int(maxit), ctypes.byref(res),
+                                           ctypes.byref(its), self.stream()), 'pf_bicgstab_solve')
+        return res.value, its.value, st
+
+    def jacobi(self, pOld, p, source, rho, dt, nit):
+        res = ctypes.c_double(0.0)
+        check(lib().pf_jacobi_solve(self.h, self.padded(pOld, 'pOld'), self.padded(p, 'p'),
+                                    self.interior(source, 'source'), float(rho), float(dt), int(nit),
+                                    ctypes.byref(res), self.stream()), 'pf_jacobi_solve')
+        return res.value
+
+    def vel_corr(self, u, v, w, DP, p, source, solver, src_scale, corr_scale, tol, iters):
+        res = ctypes.c_double(0.0)
+        its = ctypes.c_int(0)
+        st = check(lib().pf_vel_corr(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                     self.padded(w, 'w'), self.padded(DP, 'DP'), self.padded(p, 'p'),
+                                     self.interior(source, 'source'), int(solver), float(src_scale),
+                                     float(corr_scale), float(tol), int(iters), ctypes.byref(res),
+                                     ctypes.byref(its), self.stream()), 'pf_vel_corr')
+        return res.value, its.value, st
+
+    def apply_laplacian(self, A, B):
+        check(lib().pf_apply_laplacian(self.h, self.padded(A, 'A'), self.interior(B, 'B'),
+                                       self.stream()), 'pf_apply_laplacian')
+
+    def stats(self, u, v, w, divg=None):
+        out = (ctypes.c_double * 5)()
+        check(lib().pf_stats(self.h, self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
+                             self.interior(divg, 'divg') if divg is not None else None,
+                             out, self.stream()), 'pf_stats')
+        return tuple(out)
+
+    def metric_pair(self, kind, axis, upper, dst, src, coef):
+        check(lib().pf_metric_pair(self.h, kind, axis, 1 if upper else 0, self.padded(dst, 'dst'),
+                                   self.padded(src, 'src'), float(coef), self.stream()),
+              'pf_metric_pair')
+
+    def metric_edge4(self, plane, dst, src):
+        check(lib().pf_metric_edge4(self.h, plane, self.padded(dst, 'dst'), self.padded(src, 'src'),
+                                    self.stream()), 'pf_metric_edge4')
+
+    def div_visc(self, FX, FY, FZ, rhs):
+        check(lib().pf_div_visc(self.h, self.padded(FX, 'FX'), self.padded(FY, 'FY'),
+                                self.padded(FZ, 'FZ'), self.interior(rhs, 'rhs'), self.stream()),
+              'pf_div_visc')
+
+    def div_vel_over(self, u, v, w, divg_o):
+        check(lib().pf_div_vel_over(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                    self.padded(w, 'w'), self.padded(divg_o, 'divg_o'),
+                                    self.stream()), 'pf_div_vel_over')
+
+    def vel_conv(self, axis, diagonal, A, B, rhs, sign):
+        check(lib().pf_vel_conv(self.h, axis, 1 if diagonal else 0, self.padded(A, 'A'),
+                                self.padded(B, 'B'), self.interior(rhs, 'rhs'), float(sign),
+                                self.stream()), 'pf_vel_conv')
+
+    def profile_enable(self, on=True):
+        check(lib().pf_profile_enable(self.h, 1 if on else 0), 'pf_profile_enable')
+
+    def profile_read(self):
+        """{'KA': (ms_total, launches), 'KB': ..., 'KC': ...}"""
+        out = {}
+        for k, name in enumerate(('KA', 'KB', 'KC')):
+            ms = ctypes.c_double(0.0)
+            n = ctypes.c_int64(0)
+            check(lib().pf_profile_read(self.h, k, ctypes.byref(ms), ctypes.byref(n)), 'pf_profile_read')
+            out[name] = (ms.value, n.value)
+        return out
+
+    def sfs_gradient(self, u, v, w, sfs6):
+        arr = (ctypes.c_void_p * 6)(*[self.padded(f, 'sfs[%d]' % m).value for m, f in enumerate(sfs6)])
+        check(lib().pf_sfs_gradient(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                    self.padded(w, 'w'), ctypes.cast(arr, _PP), self.stream()),
+              'pf_sfs_gradient')

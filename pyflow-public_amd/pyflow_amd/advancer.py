@@ -1,0 +1,70 @@
+"""solver/advancer.py:41-113 on the GPU.
+
+With this package's rhs_NavierStokes the whole step is fused: each RK4 stage
+is one kernel that evaluates the RHS and applies the stage update, and the
+overlap cells are written by the same kernel.  Any other RHS class (duck
+typed with evaluate() and rhs_u/v/w) runs the reference's sequence with the
+state kernels.
+"""
+
+import torch
+
+from .velocity import rhs_NavierStokes
+
+
+def _fusable(r):
+    return isinstance(r, rhs_NavierStokes)
+
+
+class Euler:
+    def __init__(self, state_u, state_v, state_w, rhsClass):
+        self.state_u = state_u
+        self.state_v = state_v
+        self.state_w = state_w
+        self.rhsClass = rhsClass
+
+    def step(self, simDt):
+        """advancer.py:49-57"""
+        r = self.rhsClass.rhs1
+        if _fusable(r):
+            r.box.euler_step(self.state_u.var, self.state_v.var, self.state_w.var,
+                             simDt, r.mu(), r.rho, r.sfs_fields())
+            return
+        r.evaluate(self.state_u, self.state_v, self.state_w)
+        for st, k in ((self.state_u, r.rhs_u), (self.state_v, r.rhs_v), (self.state_w, r.rhs_w)):
+            st.ZAXPY(simDt, k, st.interior())
+
+
+class RK4:
+    def __init__(self, state_u, state_uTmp, state_v, state_vTmp, state_w, state_wTmp, rhsClass):
+        self.state_u = state_u
+        self.state_v = state_v
+        self.state_w = state_w
+        self.state_uTmp = state_uTmp
+        self.state_vTmp = state_vTmp
+        self.state_wTmp = state_wTmp
+        self.rhsClass = rhsClass
+
+    def step(self, simDt):
+        """advancer.py:75-113: k1 = R(u^n); Y_{s+1} = u^n + dt/2*k_s (s = 1..3,
+        stage 4 included); u = u^n + dt/6*(k1 + 2k2 + 2k3 + k4)."""
+        rc = self.rhsClass
+        if all(_fusable(getattr(rc, n)) for n in ('rhs1', 'rhs2', 'rhs3', 'rhs4')):
+            r = rc.rhs1
+            r.box.rk4_step(self.state_u.var, self.state_v.var, self.state_w.var,
+                           self.state_uTmp.var, self.state_vTmp.var, self.state_wTmp.var,
+                           simDt, r.mu(), r.rho, r.sfs_fields())
+            return
+        U = (self.state_u, self.state_v, self.state_w)
+        Tm = (self.state_uTmp, self.state_vTmp, self.state_wTmp)
+        rc.rhs1.evaluate(*U)
+        prev = rc.rhs1
+        for nxt in (rc.rhs2, rc.rhs3, rc.rhs4):
+            for t, u, k in zip(Tm, U, (prev.rhs_u, prev.rhs_v, prev.rhs_w)):
+                t.ZAXPY(0.5 * simDt, k, u.interior())
+            nxt.evaluate(*Tm)
+            prev = nxt
+        for i, u in enumerate(U):
+            ks = [getattr(getattr(rc, n), 'rhs_' + 'uvw'[i]) for n in ('rhs1', 'rhs2', 'rhs3', 'rhs4')]
+            acc = ks[0] + 2.0 * ks[1] + 2.0 * ks[2] + ks[3]
+            u.ZAXPY(simDt / 6.0, acc, u.interior())

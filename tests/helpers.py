@@ -1,0 +1,49 @@
+"""Shared helpers for the tests: golden fixtures, GPU box set-up, error norms."""
+
+import math
+import os
+import types
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, 'golden', 'reference_vectors.npz')
+
+
+def golden():
+    return np.load(GOLDEN)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b.ravel())
+    return float(np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0))
+
+
+def rel_max(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    m = np.abs(b).max()
+    return float(np.abs(a - b).max() / (m if m > 0 else 1.0))
+
+
+def gpu_env(shape, L=(2 * math.pi,) * 3, dtype=torch.float64):
+    """(decomp, geometry, metric) of the GPU package for a periodic box."""
+    from pyflow_amd import parallel, geometry, metric_staggered
+    IC = types.SimpleNamespace(dtypeTorch=dtype,
+                               dtypeNumpy=np.float64 if dtype == torch.float64 else np.float32,
+                               nproc_x=1, nproc_y=1, nproc_z=1)
+    cfg = types.SimpleNamespace(Nx=shape[0], Ny=shape[1], Nz=shape[2], isper=[1, 1, 1])
+    d = parallel.decomp(IC, cfg)
+    xg = np.linspace(0.0, L[0], shape[0] + 1)
+    yg = np.linspace(0.0, L[1], shape[1] + 1)
+    zg = np.linspace(0.0, L[2], shape[2] + 1)
+    g = geometry.uniform(xg, yg, zg, d)
+    m = metric_staggered.metric_uniform(g)
+    return d, g, m
+
+
+def cpu(t):
+    return t.detach().to('cpu', torch.float64).numpy()

@@ -1,0 +1,114 @@
+"""The CPU oracle against the reference's own outputs (golden vectors written
+by tests/golden/make_golden.py): every function must match bit for bit,
+since the oracle issues the same torch CPU ops in the same order."""
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import golden
+from oracle import pyflow_oracle as O
+
+
+@pytest.fixture(scope='module')
+def G():
+    torch.set_num_threads(1)
+    return golden()
+
+
+def grid(G):
+    shape = tuple(int(x) for x in G['shape'])
+    L = tuple(float(x) for x in G['L'])
+    return O.Grid(*shape, Lx=L[0], Ly=L[1], Lz=L[2])
+
+
+def fields(g, G):
+    return O.Field(g, G['u0']), O.Field(g, G['v0']), O.Field(g, G['w0'])
+
+
+def visc(g, G):
+    V = torch.ones(g.nxo, g.nyo, g.nzo, dtype=torch.float64)
+    V.mul_(float(G['mu']))
+    return V
+
+
+def test_rhs(G):
+    g = grid(G)
+    r = O.NavierStokesRHS(g, float(G['rho']), visc(g, G))
+    r.evaluate(*fields(g, G))
+    for c in 'uvw':
+        assert np.array_equal(getattr(r, 'rhs_' + c).numpy(), G['rhs_' + c])
+
+
+def test_gradient_closure_and_rhs(G):
+    g = grid(G)
+    fu, fv, fw = fields(g, G)
+    gm = O.GradientModel(g)
+    gm.update(fu, fv, fw)
+    for n, f in (('u', fu), ('v', fv), ('w', fw)):
+        for a in 'xyz':
+            assert np.array_equal(getattr(f, 'grad_' + a).numpy(), G['gc_%s_%s' % (n, a)])
+    for n in ('FXX', 'FYY', 'FZZ', 'FXY', 'FXZ', 'FYZ'):
+        assert np.array_equal(getattr(gm, n).numpy(), G['sfs_' + n])
+    r = O.NavierStokesRHS(g, float(G['rho']), visc(g, G), gm)
+    r.evaluate(*fields(g, G))
+    for c in 'uvw':
+        assert np.array_equal(getattr(r, 'rhs_' + c).numpy(), G['rhs_sfs_' + c])
+
+
+def test_rk4_and_euler(G):
+    g = grid(G)
+    dt = float(G['dt'])
+    fu, fv, fw = fields(g, G)
+    tu, tv, tw = fields(g, G)
+    O.rk4_step(fu, fv, fw, tu, tv, tw,
+               [O.NavierStokesRHS(g, float(G['rho']), visc(g, G)) for _ in range(4)], dt)
+    for c, f in zip('uvw', (fu, fv, fw)):
+        assert np.array_equal(f.var.numpy(), G['rk4_' + c])
+    fu, fv, fw = fields(g, G)
+    O.euler_step(fu, fv, fw, O.NavierStokesRHS(g, float(G['rho']), visc(g, G)), dt)
+    for c, f in zip('uvw', (fu, fv, fw)):
+        assert np.array_equal(f.var.numpy(), G['euler_' + c])
+
+
+def test_bicgstab_history(G):
+    g = grid(G)
+    fu, fv, fw = fields(g, G)
+    src = g.interior_zeros()
+    O.div_vel(g, fu, fv, fw, src)
+    assert np.array_equal(src.numpy(), G['div0'])
+    DP = O.Field(g, np.zeros(g.interior_zeros().shape))
+    p = O.Field(g, np.zeros(g.interior_zeros().shape))
+    bs = O.BiCGStab(g, float(G['rho']), float(G['dt']), 1e-11, 400)
+    res = bs.solve(DP, p, src)
+    assert res == float(G['bicg_res'])
+    hist = G['bicg_hist'][1:] / G['bicg_hist'][0]
+    assert np.array_equal(np.array(bs.history), hist)
+    assert np.array_equal(DP.var.numpy(), G['bicg_DP'])
+    assert np.array_equal(p.var.numpy(), G['bicg_p'])
+    assert np.array_equal(src.numpy(), G['bicg_source_after'])
+
+
+def test_jacobi(G):
+    g = grid(G)
+    DP = O.Field(g, np.zeros(g.interior_zeros().shape))
+    p = O.Field(g, np.zeros(g.interior_zeros().shape))
+    src = torch.tensor(G['div0'].copy())
+    res = O.Jacobi(g, float(G['rho']), float(G['dt']), 25).solve(DP, p, src)
+    assert res == float(G['jac_res'])
+    assert np.array_equal(DP.var.numpy(), G['jac_pOld'])
+    assert np.array_equal(p.var.numpy(), G['jac_p'])
+
+
+@pytest.mark.parametrize('tag,adv,solver,tol,nit', [
+    ('dom_rk4_bicg', 'RK4', 'bicgstab', 1e-12, 500),
+    ('dom_euler_jac', 'Euler', 'Jacobi', 0.0, 30)])
+def test_domain_steps(G, tag, adv, solver, tol, nit):
+    g = grid(G)
+    B = O.Box(g, G['u0'], G['v0'], G['w0'], mu=float(G['mu']), rho=float(G['rho']),
+              dt=float(G['dt']), advancer=adv, solver=solver, tol=tol, maxit=nit)
+    for _ in range(2):
+        B.forward_step()
+    for c in ('u', 'v', 'w', 'p', 'DP'):
+        assert np.array_equal(getattr(B, c).var.numpy(), G['%s_%s' % (tag, c)]), c
+    assert B.max_resP == float(G[tag + '_res'])
