@@ -101,7 +101,7 @@ def main():
     ap.add_argument('--tol', type=float, default=None)
     ap.add_argument('--maxit', type=int, default=3000)
     ap.add_argument('--seed', type=int, default=2024)
-    ap.add_argument('--cpu-steps', type=int, default=1)
+    ap.add_argument('--cpu-steps', type=int, default=12)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-profile', action='store_true')
     a = ap.parse_args()
@@ -143,7 +143,7 @@ def main():
                              'bytes_per_launch': wmul * w * ncell, 'GB/s': round(gbs, 1)}
         out['kernels'] = per
         if 'KA' in per:
-            out['roofline'] = {'kernel': 'bicg_ka_kernel', 'bound': 'hbm',
+            out['roofline'] = {'kernel': 'bicg_tile_kernel<KA>', 'bound': 'hbm',
                                'achieved': per['KA']['GB/s'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                                'frac': round(per['KA']['GB/s'] / HBM_PEAK_GBS, 4),
                                'traffic': None,

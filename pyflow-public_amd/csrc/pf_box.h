@@ -51,7 +51,8 @@ struct pf_box {
   void* acc[3];       // interior
   void* ypong[3];     // padded
   // reductions and control
-  double* part;       // kNumPartBufs * nblk * kMaxPartSlots
+  double* part;       // kNumPartBufs * part_stride
+  int64_t part_stride;  // doubles per partial buffer
   pf::KrylovCtl* ctl;
   pf::KrylovCtl* ctl_host;   // pinned
   double* stats_dev;         // 8 doubles
@@ -76,7 +77,7 @@ int check_box(const pf_box* h);
 int launch_blocks(const Geom& g);   // grid size for flat one-cell-per-thread kernels
 
 inline double* part_buf(pf_box* h, int which) {
-  return h->part + (size_t)which * h->nblk * kMaxPartSlots;
+  return h->part + (size_t)which * h->part_stride;
 }
 
 }  // namespace pf

@@ -20,6 +20,10 @@
 // per op); only the dot-product summation order differs from torch's CPU sum.
 #include <hip/hip_runtime.h>
 
+#include <hip/hip_ext.h>
+
+#include <cstdlib>
+
 #include "pf_box.h"
 
 namespace pf {
@@ -130,108 +134,215 @@ __global__ void __launch_bounds__(kBlock) bicg_init_kernel(
   write_partials<2>(partC, acc);
 }
 
-// KA: convergence check of the previous iteration, then p and v = A p.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) bicg_ka_kernel(
-    Geom g, int nblk, KrylovCtl* __restrict__ ctl, const double* __restrict__ partC,
-    double* __restrict__ partA, const T* __restrict__ r, const T* __restrict__ rh,
-    const T* __restrict__ p_old, const T* __restrict__ v_old, T* __restrict__ p_new,
-    T* __restrict__ v_new, Lap<T> L) {
+// ---------------------------------------------------------------------------
+// Tiled KA / KB.  Both apply the 7-point operator to a field that is itself
+// recomputed from three (KA) or two (KB) stored vectors.  A block owns a
+// TZ x TY tile of the (y,z) plane and marches along x over a chunk of planes:
+// each thread keeps its column's field at x-1, x, x+1 in registers, the
+// current plane's tile plus a one-cell y/z halo goes through LDS, and the
+// loads of plane x+1 (and of the next halo) are issued before the barrier of
+// plane x.  Cells of partial tiles wrap periodically, so the LDS plane always
+// holds valid neighbours; only their outputs are masked.
+struct TileGrid {
+  int tz, ty;             // tile extents (tz*ty == kBlock)
+  int ntz, nty, ncx;      // tiles along z, y and chunks along x
+  int xc;                 // planes per chunk
+  int nblk;               // ntz*nty*ncx
+};
+
+inline TileGrid make_tile_grid(const Geom& g) {
+  TileGrid t;
+  t.tz = g.nz >= 64 ? 64 : g.nz >= 32 ? 32 : g.nz >= 16 ? 16 : 8;
+  t.ty = kBlock / t.tz;
+  t.ntz = (g.nz + t.tz - 1) / t.tz;
+  t.nty = (g.ny + t.ty - 1) / t.ty;
+  const int tiles = t.ntz * t.nty;
+  int cx = 1024 / tiles;            // keeps tiles*chunks <= max(1024, tiles)
+  const int maxc = g.nx >= 4 ? g.nx / 4 : 1;
+  if (cx > maxc) cx = maxc;
+  if (cx < 1) cx = 1;
+  t.xc = (g.nx + cx - 1) / cx;
+  t.ncx = (g.nx + t.xc - 1) / t.xc;
+  t.nblk = tiles * t.ncx;
+  return t;
+}
+
+enum { kModeA = 0, kModeB = 1 };
+
+// Field recomputed at a cell: KA p = (p_old - omega v_old)*beta + r,
+// KB s = r - alpha v.
+template <typename T, int MODE>
+struct FieldOp {
+  const T* __restrict__ r;
+  const T* __restrict__ a;   // KA: p_old; KB: unused
+  const T* __restrict__ v;   // KA: v_old; KB: v
+  T c1, c2;                  // KA: (-omega, beta); KB: (-alpha, -)
+  __device__ __forceinline__ T at(int64_t q) const {
+    if (MODE == kModeA) return (fma(v[q], c1, a[q]) * c2) + r[q];
+    return fma(v[q], c1, r[q]);
+  }
+};
+
+template <typename T, int MODE, int TZ>
+__global__ void __launch_bounds__(kBlock) bicg_tile_kernel(
+    Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,
+    const double* __restrict__ part_in, double* __restrict__ part_out,
+    const T* __restrict__ r, const T* __restrict__ rh, const T* __restrict__ a_old,
+    const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L) {
+  constexpr int TY = kBlock / TZ;
   const KrylovCtl c0 = snapshot(ctl);
   if (c0.done) return;
-  const int halt = c0.halt;
-  const int k = c0.iter;
-  const int maxit = c0.maxit;
-  const double tol = c0.tol;
-  const double alpha = c0.alpha, omega = c0.omega, rho2 = c0.rho2;
-  const double res0_prev = c0.res0;
-  if (halt) {
-    if (lead()) ctl->done = 1;
-    return;
-  }
-  double red[2];
-  const int ops[2] = {0, 1};
-  reduce_partials<2>(partC, nblk, red, ops);
-  const double rho1 = red[0];
-  if (k == 0) {
-    if (lead()) ctl->res0 = red[1];
+  T c1, c2 = (T)0;
+  if (MODE == kModeA) {
+    if (c0.halt) {
+      if (lead()) ctl->done = 1;
+      return;
+    }
+    double red[2];
+    const int ops[2] = {0, 1};
+    reduce_partials<2>(part_in, nblk_in, red, ops);
+    const double rho1 = red[0];
+    if (c0.iter == 0) {
+      if (lead()) ctl->res0 = red[1];
+    } else {
+      const double max_res = red[1] / c0.res0;       // pressure.py:279-280
+      if (lead()) ctl->max_res = max_res;
+      const bool cap = c0.iter >= c0.maxit;           // pressure.py:283-287
+      const bool conv = max_res <= c0.tol;
+      if (cap || conv) {
+        if (lead()) {
+          ctl->status = conv ? PF_SOLVE_CONVERGED : PF_SOLVE_MAXIT;
+          ctl->done = 1;
+        }
+        return;
+      }
+    }
+    const double beta = c0.alpha * rho1 / (c0.rho2 * c0.omega);   // pressure.py:233
+    if (lead()) ctl->rho1 = rho1;
+    c1 = (T)(-c0.omega);
+    c2 = (T)beta;
   } else {
-    const double max_res = red[1] / res0_prev;     // pressure.py:279-280
-    if (lead()) ctl->max_res = max_res;
-    const bool cap = k >= maxit;                    // pressure.py:283-287
-    const bool conv = max_res <= tol;
-    if (cap || conv) {
+    double red[1];
+    const int ops[1] = {0};
+    reduce_partials<1>(part_in, nblk_in, red, ops);
+    const double gamma = red[0];
+    if (gamma == 0.0) {                                // pressure.py:247-248
       if (lead()) {
-        ctl->status = conv ? PF_SOLVE_CONVERGED : PF_SOLVE_MAXIT;
+        ctl->status = PF_SOLVE_BREAKDOWN_A;
         ctl->done = 1;
       }
       return;
     }
+    const double alpha = c0.rho1 / gamma;              // pressure.py:249
+    if (lead()) ctl->alpha = alpha;
+    c1 = (T)(-alpha);
   }
-  const double beta = alpha * rho1 / (rho2 * omega); // pressure.py:233
-  if (lead()) ctl->rho1 = rho1;
-  const T tb = (T)beta, tmo = (T)(-omega);
-  double acc[1] = {0.0};
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
-       idx += gridDim.x * blockDim.x) {
-    int i, j, kk;
-    decode(g, idx, i, j, kk);
-    const Nbr n = nbr(g, i, j, kk);
-    // pressure.py:235-237: pp.sub_(omega, vv); pp.mul_(beta); pp.add_(res)
-#define PNEW(o) ((fma(v_old[idx + (o)], tmo, p_old[idx + (o)]) * tb) + r[idx + (o)])
-    const T pc = PNEW(0);
-    const T vn = lap7(L, PNEW(n.xm), pc, PNEW(n.xp), PNEW(n.ym), PNEW(n.yp), PNEW(n.zm), PNEW(n.zp));
-#undef PNEW
-    p_new[idx] = pc;
-    v_new[idx] = vn;
-    acc[0] += (double)vn * (double)rh[idx];
+  const FieldOp<T, MODE> F{r, a_old, v_in, c1, c2};
+
+  // XCD-aware block order: the hardware deals consecutive block ids round
+  // robin over 8 XCDs; renumber so each XCD gets a contiguous run of tiles
+  // (y fastest, so y-neighbour tiles share an L2).
+  const int nb = gridDim.x;
+  int b = blockIdx.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  const int ty_i = b % tg.nty;
+  const int rest = b / tg.nty;
+  const int tz_i = rest % tg.ntz;
+  const int cx_i = rest / tg.ntz;
+  const int z0 = tz_i * TZ, y0 = ty_i * TY, x0 = cx_i * tg.xc;
+  const int x1 = min(x0 + tg.xc, g.nx);
+
+  const int t = threadIdx.x;
+  const int lz = t % TZ, ly = t / TZ;
+  const int zr = z0 + lz, yr = y0 + ly;
+  const bool active = zr < g.nz && yr < g.ny;
+  const int z = zr % g.nz, y = yr % g.ny;
+  const int64_t col = (int64_t)y * g.isy + z;
+
+  // halo assignment: 2*TZ y-halo cells, then 2*TY z-halo cells
+  int hz = -1, hy = -1, hslot_y = 0, hslot_z = 0;
+  bool is_halo = false;
+  if (t < 2 * TZ) {
+    is_halo = true;
+    hslot_z = t % TZ + 1;
+    hslot_y = (t < TZ) ? 0 : TY + 1;
+    hz = (z0 + t % TZ) % g.nz;
+    hy = (t < TZ) ? (y0 - 1 + g.ny) % g.ny : (y0 + TY) % g.ny;
+  } else if (t < 2 * TZ + 2 * TY) {
+    is_halo = true;
+    const int u = t - 2 * TZ;
+    hslot_y = u % TY + 1;
+    hslot_z = (u < TY) ? 0 : TZ + 1;
+    hy = (y0 + u % TY) % g.ny;
+    hz = (u < TY) ? (z0 - 1 + g.nz) % g.nz : (z0 + TZ) % g.nz;
   }
-  const int ops1[1] = {0};
-  block_reduce<1>(acc, ops1);
-  write_partials<1>(partA, acc);
+  const int64_t hcol = is_halo ? (int64_t)hy * g.isy + hz : 0;
+
+  __shared__ T plane[TY + 2][TZ + 2];
+  auto xoff = [&](int x) -> int64_t {
+    x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
+    return (int64_t)x * g.isx;
+  };
+  T fm = F.at(xoff(x0 - 1) + col);
+  T fc = F.at(xoff(x0) + col);
+  T hv = is_halo ? F.at(xoff(x0) + hcol) : (T)0;
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int x = x0; x < x1; ++x) {
+    const int64_t xo = (int64_t)x * g.isx;
+    // next plane: own column and next halo (issued before the barrier)
+    const T fp = F.at(xoff(x + 1) + col);
+    const T hn = (is_halo && x + 1 < x1) ? F.at(xoff(x + 1) + hcol) : (T)0;
+    const T rhc = (MODE == kModeA && active) ? rh[xo + col] : (T)0;
+    plane[ly + 1][lz + 1] = fc;
+    if (is_halo) plane[hslot_y][hslot_z] = hv;
+    __syncthreads();
+    if (active) {
+      const T fym = plane[ly][lz + 1], fyp = plane[ly + 2][lz + 1];
+      const T fzm = plane[ly + 1][lz], fzp = plane[ly + 1][lz + 2];
+      const T af = lap7(L, fm, fc, fp, fym, fyp, fzm, fzp);
+      f_out[xo + col] = fc;
+      af_out[xo + col] = af;
+      if (MODE == kModeA) {
+        acc0 += (double)af * (double)rhc;
+      } else {
+        acc0 += (double)fc * (double)af;
+        acc1 += (double)af * (double)af;
+      }
+    }
+    __syncthreads();
+    fm = fc;
+    fc = fp;
+    hv = hn;
+  }
+  if (MODE == kModeA) {
+    double accv[1] = {acc0};
+    const int ops1[1] = {0};
+    block_reduce<1>(accv, ops1);
+    write_partials<1>(part_out, accv);
+  } else {
+    double accv[2] = {acc0, acc1};
+    const int ops2[2] = {0, 0};
+    block_reduce<2>(accv, ops2);
+    write_partials<2>(part_out, accv);
+  }
 }
 
-// KB: alpha = rho1/<v,r_hat>, s = r - alpha*v, t = A s.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) bicg_kb_kernel(
-    Geom g, int nblk, KrylovCtl* __restrict__ ctl, const double* __restrict__ partA,
-    double* __restrict__ partB, const T* __restrict__ r, const T* __restrict__ v,
-    T* __restrict__ s, T* __restrict__ t, Lap<T> L) {
-  const KrylovCtl c0 = snapshot(ctl);
-  if (c0.done) return;
-  const double rho1 = c0.rho1;
-  double red[1];
-  const int ops[1] = {0};
-  reduce_partials<1>(partA, nblk, red, ops);
-  const double gamma = red[0];
-  if (gamma == 0.0) {                               // pressure.py:247-248
-    if (lead()) {
-      ctl->status = PF_SOLVE_BREAKDOWN_A;
-      ctl->done = 1;
-    }
-    return;
+template <typename T, int MODE>
+void launch_tile(const TileGrid& tg, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const Geom& g,
+                 int nblk_in, KrylovCtl* ctl, const double* pin, double* pout, const T* r,
+                 const T* rh, const T* a_old, const T* v_in, T* f_out, T* af_out,
+                 const Lap<T>& L) {
+#define PF_TILE(TZ)                                                                          \
+  hipExtLaunchKernelGGL((bicg_tile_kernel<T, MODE, TZ>), dim3(tg.nblk), dim3(kBlock), 0, s, e0, \
+                        e1, 0, g, tg, nblk_in, ctl, pin, pout, r, rh, a_old, v_in, f_out,       \
+                        af_out, L)
+  switch (tg.tz) {
+    case 64: PF_TILE(64); break;
+    case 32: PF_TILE(32); break;
+    case 16: PF_TILE(16); break;
+    default: PF_TILE(8); break;
   }
-  const double alpha = rho1 / gamma;                // pressure.py:249
-  if (lead()) ctl->alpha = alpha;
-  const T tma = (T)(-alpha);
-  double acc[2] = {0.0, 0.0};
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
-       idx += gridDim.x * blockDim.x) {
-    int i, j, kk;
-    decode(g, idx, i, j, kk);
-    const Nbr n = nbr(g, i, j, kk);
-#define SNEW(o) fma(v[idx + (o)], tma, r[idx + (o)])  // pressure.py:253-254
-    const T sc = SNEW(0);
-    const T tn = lap7(L, SNEW(n.xm), sc, SNEW(n.xp), SNEW(n.ym), SNEW(n.yp), SNEW(n.zm), SNEW(n.zp));
-#undef SNEW
-    s[idx] = sc;
-    t[idx] = tn;
-    acc[0] += (double)sc * (double)tn;
-    acc[1] += (double)tn * (double)tn;
-  }
-  const int ops2[2] = {0, 0};
-  block_reduce<2>(acc, ops2);
-  write_partials<2>(partB, acc);
+#undef PF_TILE
 }
 
 // KC: omega = <s,t>/<t,t>, x += alpha*p, x += omega*s, r = s - omega*t.
@@ -546,9 +657,10 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
   PF_LAUNCH_CHECK("bicg_init_kernel");
 
   // KA of iteration 0, then batches of (KB, KC, KA) until the device says done.
-  // With profiling on, each launch is bracketed by an event pair; pairs are
-  // read back after the batch's host sync and only launches that did work
-  // (iteration index below the completed count) are accumulated.
+  // With profiling on, each launch carries a start/stop event pair recorded
+  // by the dispatch itself (hipExtLaunchKernelGGL); pairs are read back after
+  // the batch's host sync and only launches that did work (iteration index
+  // below the completed count) are accumulated.
   struct Rec {
     int kind, iter, slot;
   };
@@ -558,19 +670,18 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
   auto launch = [&](int kind, int iter, auto&& go) {
     const bool prof = h->prof_on && slot[kind] < h->prof_cap;
     hipEvent_t* e = prof ? h->prof_ev + ((size_t)kind * h->prof_cap + slot[kind]) * 2 : nullptr;
-    if (prof) hipEventRecord(e[0], s);
-    go();
-    if (prof) {
-      hipEventRecord(e[1], s);
-      recs[nrec++] = Rec{kind, iter, slot[kind]++};
-    }
+    go(prof ? e[0] : nullptr, prof ? e[1] : nullptr);
+    if (prof) recs[nrec++] = Rec{kind, iter, slot[kind]++};
   };
   int it = 0;  // iterations whose KA has been launched
-  launch(0, 0, [&] {
-    hipLaunchKernelGGL(bicg_ka_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, nb, h->ctl, pC, pA, r,
-                       rh, P[0], Vv[0], P[1], Vv[1], L);
-  });
-  PF_LAUNCH_CHECK("bicg_ka_kernel");
+  const TileGrid tg = make_tile_grid(g);
+  const int nbA = tg.nblk;   // blocks writing partials A and B
+  auto ka = [&](int old, int nw, hipEvent_t e0, hipEvent_t e1) {
+    launch_tile<T, kModeA>(tg, s, e0, e1, g, nb, h->ctl, pC, pA, r, rh, P[old], Vv[old], P[nw],
+                           Vv[nw], L);
+  };
+  launch(0, 0, [&](hipEvent_t e0, hipEvent_t e1) { ka(0, 1, e0, e1); });
+  PF_LAUNCH_CHECK("bicg_tile_kernel");
   const int cap = maxit < 0 ? 0 : maxit;
   bool first = true;
   for (;;) {
@@ -581,20 +692,17 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
     first = false;
     for (int q = 0; q < m; ++q) {
       const int cur = (it + 1) & 1;   // buffers written by iteration `it`
-      launch(1, it, [&] {
-        hipLaunchKernelGGL(bicg_kb_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, nb, h->ctl, pA,
-                           pB, r, Vv[cur], sv, tv, L);
+      launch(1, it, [&](hipEvent_t e0, hipEvent_t e1) {
+        launch_tile<T, kModeB>(tg, s, e0, e1, g, nbA, h->ctl, pA, pB, r, rh, (const T*)nullptr,
+                               Vv[cur], sv, tv, L);
       });
-      launch(2, it, [&] {
-        hipLaunchKernelGGL(bicg_kc_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, nb, h->ctl, pB,
-                           pC, x, P[cur], sv, tv, r, rh);
+      launch(2, it, [&](hipEvent_t e0, hipEvent_t e1) {
+        hipExtLaunchKernelGGL(bicg_kc_kernel<T>, dim3(nb), dim3(kBlock), 0, s, e0, e1, 0, g, nbA,
+                              h->ctl, pB, pC, x, P[cur], sv, tv, r, rh);
       });
       ++it;
       const int nxt = (it + 1) & 1;
-      launch(0, it, [&] {
-        hipLaunchKernelGGL(bicg_ka_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, nb, h->ctl, pC,
-                           pA, r, rh, P[cur], Vv[cur], P[nxt], Vv[nxt], L);
-      });
+      launch(0, it, [&](hipEvent_t e0, hipEvent_t e1) { ka(cur, nxt, e0, e1); });
     }
     PF_LAUNCH_CHECK("bicgstab batch");
     PF_HIP(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
@@ -673,8 +781,8 @@ int jacobi_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, const 
   }
   PF_LAUNCH_CHECK("jacobi_sweep_kernel");
   const Lap<T> L = make_lap<T>(h);
-  hipLaunchKernelGGL(jacobi_resid_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, (const T*)p + g.poff,
-                     src, dx2, L, pS);
+  hipLaunchKernelGGL(jacobi_resid_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, (const T*)p, src,
+                     dx2, L, pS);
   PF_LAUNCH_CHECK("jacobi_resid_kernel");
   double resP;
   if (int rc = finalize<T>(h, s, pS, 1, opm, &resP)) return rc;
@@ -728,8 +836,8 @@ int vel_corr_impl(pf_box* h, hipStream_t s, T* U, T* V, T* W, T* DP, T* p, T* sr
   int rc = jacobi_impl<T>(h, s, src, U, V, W, DP, p, src_scale, iters, out_res);
   if (rc) return rc;
   if (out_iters) *out_iters = iters;
-  hipLaunchKernelGGL(project_padded_kernel<T>, dim3(nbf), dim3(kBlock), 0, s, g,
-                     (const T*)DP + g.poff, U, V, W, dxi, dyi, dzi, neg_c);
+  hipLaunchKernelGGL(project_padded_kernel<T>, dim3(nbf), dim3(kBlock), 0, s, g, (const T*)DP, U,
+                     V, W, dxi, dyi, dzi, neg_c);
   PF_LAUNCH_CHECK("project_padded_kernel");
   return 0;
 }

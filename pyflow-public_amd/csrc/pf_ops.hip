@@ -311,11 +311,11 @@ int pf_vel_conv(pf_box* h, int axis, int diagonal, const void* A, const void* B,
   const double hinv = axis == 0 ? 1.0 / h->dx : (axis == 1 ? 1.0 / h->dy : 1.0 / h->dz);
   if (h->dtype == PF_F64)
     hipLaunchKernelGGL(vel_conv_kernel<double>, dim3(launch_blocks(g)), dim3(kBlock), 0, s, g, o1,
-                       o0, (const double*)A + g.poff, (const double*)B + g.poff, (double*)rhs,
+                       o0, (const double*)A, (const double*)B, (double*)rhs,
                        hinv, sign);
   else
     hipLaunchKernelGGL(vel_conv_kernel<float>, dim3(launch_blocks(g)), dim3(kBlock), 0, s, g, o1,
-                       o0, (const float*)A + g.poff, (const float*)B + g.poff, (float*)rhs,
+                       o0, (const float*)A, (const float*)B, (float*)rhs,
                        (float)hinv, (float)sign);
   PF_LAUNCH_CHECK("vel_conv_kernel");
   return 0;

@@ -87,6 +87,14 @@ int pf_box_create(pf_box** out, int nx, int ny, int nz, double dx, double dy, do
   int64_t nb = (h->g.ncell + kBlock - 1) / kBlock;
   h->nblk = (int)(nb < 1024 ? nb : 1024);
   h->chunk = 16;
+  {
+    // the tiled Krylov kernels may use more blocks than the flat ones
+    const int tz = nz >= 64 ? 64 : nz >= 32 ? 32 : nz >= 16 ? 16 : 8;
+    const int ty = kBlock / tz;
+    const int64_t tiles = (int64_t)((nz + tz - 1) / tz) * ((ny + ty - 1) / ty);
+    const int64_t mx = tiles > 1024 ? tiles : 1024;   // >= every reducing grid
+    h->part_stride = mx * kMaxPartSlots;
+  }
 
   const size_t ni = (size_t)h->g.ncell * h->esize;
   const size_t np = (size_t)(nx + 4) * (ny + 4) * (nz + 4) * h->esize;
@@ -104,7 +112,7 @@ int pf_box_create(pf_box** out, int nx, int ny, int nz, double dx, double dy, do
   for (void** p : ivecs)
     if (!rc) rc = alloc(p, ni);
   for (int c = 0; c < 3 && !rc; ++c) rc = alloc(&h->ypong[c], np);
-  if (!rc) rc = alloc((void**)&h->part, sizeof(double) * kNumPartBufs * h->nblk * kMaxPartSlots);
+  if (!rc) rc = alloc((void**)&h->part, sizeof(double) * kNumPartBufs * h->part_stride);
   if (!rc) rc = alloc((void**)&h->ctl, sizeof(KrylovCtl));
   if (!rc) rc = alloc((void**)&h->stats_dev, sizeof(double) * 8);
   if (!rc) {

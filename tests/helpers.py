@@ -29,6 +29,16 @@ def rel_max(a, b):
     return float(np.abs(a - b).max() / (m if m > 0 else 1.0))
 
 
+def rel_l2_mean_free(a, b, padded=True):
+    """rel_l2 after removing each field's interior mean: the periodic pressure
+    is defined up to a constant (the 7-point operator annihilates constants),
+    so solvers that differ only in rounding can differ in that constant."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    core = (slice(2, -2),) * 3 if padded else (slice(None),) * 3
+    return rel_l2(a - a[core].mean(), b - b[core].mean())
+
+
 def gpu_env(shape, L=(2 * math.pi,) * 3, dtype=torch.float64):
     """(decomp, geometry, metric) of the GPU package for a periodic box."""
     from pyflow_amd import parallel, geometry, metric_staggered

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import golden, gpu_env, cpu, rel_l2, rel_max
+from helpers import golden, gpu_env, cpu, rel_l2, rel_max, rel_l2_mean_free
 
 pytestmark = pytest.mark.gpu
 
@@ -126,8 +126,8 @@ def test_bicgstab_matches_reference(G):
     n_ref = len(G['bicg_hist']) - 1
     assert abs(solver.iterations - n_ref) <= 1, (solver.iterations, n_ref)
     assert res <= 1e-11
-    assert rel_l2(cpu(DP.var), G['bicg_DP']) < 1e-9
-    assert rel_l2(cpu(p.var), G['bicg_p']) < 1e-9
+    assert rel_l2_mean_free(cpu(DP.var), G['bicg_DP']) < 1e-9
+    assert rel_l2_mean_free(cpu(p.var), G['bicg_p']) < 1e-9
     # the source is rescaled in place exactly like the reference
     assert np.array_equal(cpu(src), G['bicg_source_after'])
 
@@ -155,7 +155,8 @@ def test_jacobi_bit_exact(G):
     ('dom_euler_jac', 'Euler', 'Jacobi', 0.0, 30)])
 def test_domain_forward_step(G, tag, adv, solver, tol, nit):
     """Two Domain.forwardStep calls.  With Jacobi everything is bit-exact;
-    with BiCGStab (tolerance 1e-12) fields agree to 1e-10 relative L2."""
+    with BiCGStab (tolerance 1e-12) velocities agree to 1e-12 and the
+    pressures, up to their free constant, to 1e-10 relative L2."""
     import types
     from pyflow_amd import domain
     shape, d, g, m = setup(G)
@@ -178,5 +179,5 @@ def test_domain_forward_step(G, tag, adv, solver, tol, nit):
         for c in ('u', 'v', 'w'):
             assert rel_l2(got[c], G['%s_%s' % (tag, c)]) < 1e-12, c
         for c in ('p', 'DP'):
-            assert rel_l2(got[c], G['%s_%s' % (tag, c)]) < 1e-10, c
+            assert rel_l2_mean_free(got[c], G['%s_%s' % (tag, c)]) < 1e-10, c
         assert D.max_resP <= 1e-12
