@@ -9,8 +9,9 @@ PyFlow.setup.  A step is one full time step of that loop.
 
 Prints one JSON line.  `value` is Mcell-steps/s of the timed steps; the
 roofline object is for the dominant kernel (KA of BiCGStab) with its average
-launch duration measured with hipEvent pairs around every launch on the
-launch stream during the timed steps; `cpu_baseline` times the CPU oracle (a
+launch duration measured during the timed steps with start/stop hipEvents
+recorded by the dispatch of every 32nd launch of each Krylov kernel (sampling
+keeps the events' own cost out of `value`); `cpu_baseline` times the CPU oracle (a
 torch-CPU FP64 restatement of the reference's op sequence) on a bounded
 sample of the same workload on this host.
 """
@@ -57,7 +58,7 @@ def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True):
         D.stats()
     box = decomp.box
     if profile:
-        box.profile_enable(True)
+        box.profile_enable(profile)
     torch.cuda.synchronize()
     its = []
     t = time.perf_counter()
@@ -104,6 +105,8 @@ def main():
     ap.add_argument('--cpu-steps', type=int, default=12)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-profile', action='store_true')
+    ap.add_argument('--profile-period', type=int, default=32,
+                    help='time every n-th launch of each Krylov kernel with dispatch events')
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU: this build runs one periodic box per GPU and has no '
@@ -113,7 +116,8 @@ def main():
     dtype = torch.float64 if a.dtype == 'f64' else torch.float32
     w = 8 if a.dtype == 'f64' else 4
     tol = a.tol if a.tol is not None else (1e-10 if a.dtype == 'f64' else 1e-8)
-    r = gpu_run(a.N, dtype, a.nu, tol, a.maxit, a.steps, a.warmup, a.seed, not a.no_profile)
+    r = gpu_run(a.N, dtype, a.nu, tol, a.maxit, a.steps, a.warmup, a.seed,
+                0 if a.no_profile else a.profile_period)
     ncell = a.N ** 3
     K = a.steps
     value = ncell * K / r['wall'] / 1e6

@@ -237,14 +237,28 @@ int pf_vel_conv(pf_box* box, int axis, int diagonal, const void* A, const void* 
                 double sign, void* stream);
 
 /*
- * Measurement support (no counterpart in the reference): while enabled, every
- * launch of the three BiCGStab kernels (0 = KA: p update + A p + <v,r_hat>;
- * 1 = KB: s + A s + <s,t>,<t,t>; 2 = KC: x, r updates + <r,r_hat>, max|r|) is
- * bracketed by a hipEvent pair on its stream; pf_profile_read returns the
- * accumulated milliseconds and the number of timed launches that did work.
+ * Closure MLP with the same 'tensor' contract (no counterpart in the
+ * reference, whose ML closure is a 'source' model): the nine gradients of
+ * grad_vel_center -> Linear(9,20) -> tanh -> Linear(20,20) -> tanh ->
+ * Linear(20,6), scaled by dx^2/12, into the six stress fields.  GEMMs on
+ * MFMA with BF16 operands and FP32 accumulation.  `params` is a device FP32
+ * vector of pf_sfs_mlp_param_count() values: W1[20][9] b1[20] W2[20][20]
+ * b2[20] W3[6][20] b3[6].
+ */
+int pf_sfs_mlp(pf_box* box, const void* u, const void* v, const void* w, const float* params,
+               void* const* sfs6, void* stream);
+int pf_sfs_mlp_param_count(void);
+
+/*
+ * Measurement support (no counterpart in the reference): with period > 0,
+ * every period-th launch of each of the three BiCGStab kernels (0 = KA:
+ * p update + A p + <v,r_hat>; 1 = KB: s + A s + <s,t>,<t,t>; 2 = KC: x, r
+ * updates + <r,r_hat>, max|r|) carries a start/stop hipEvent pair recorded by
+ * the dispatch itself; pf_profile_read returns the accumulated milliseconds
+ * and the number of timed launches that did work.  period = 0 turns it off.
  * Enabling resets the accumulators.  Creates events on first use.
  */
-int pf_profile_enable(pf_box* box, int on);
+int pf_profile_enable(pf_box* box, int period);
 int pf_profile_read(pf_box* box, int kernel, double* total_ms, int64_t* launches);
 
 #ifdef __cplusplus

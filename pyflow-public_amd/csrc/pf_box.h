@@ -59,9 +59,11 @@ struct pf_box {
   double* stats_host;        // pinned, 8 doubles
   int last_iters;
   int chunk;                 // iterations per launch batch between host checks
+  int tile_rows;             // block rows of the tiled Krylov kernels: 256*tile_rows threads
   int64_t bytes;             // device bytes owned by the handle
   // optional per-launch timing of the Krylov kernels (measurement only)
-  int prof_on;
+  int prof_on;               // 0 = off, else time every prof_on-th launch
+  int64_t prof_seq[3];       // launches seen per kernel kind while profiling
   int prof_cap;              // event pairs per kernel class
   hipEvent_t* prof_ev;       // [3][prof_cap][2]
   double prof_ms[3];         // accumulated milliseconds of KA, KB, KC

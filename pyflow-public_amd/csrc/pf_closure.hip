@@ -1,63 +1,14 @@
 // Closures with the 'tensor' contract: six cell-centred stress fields
 // FXX, FYY, FZZ, FXY, FXZ, FYZ from the nine cell-centred velocity gradients
-// of metric_uniform.grad_vel_center (metric/metric_staggered.py:215-268).
-//
-// The gradients are evaluated over the padded index range exactly as the
-// reference's slice sequence leaves them, including one reference artefact:
-// for an off-diagonal gradient d/dx_d of a component interpolated along
-// another axis, interp_uvwi_* writes var_i[1:] only, so at overlap index 0
-// of axis d the "lower" interpolant is the stale cell value c[0] and the
-// gradient there is (0.5*(c[1]+c[0]) - c[0]) / h_d rather than a periodic
-// image.  The interior RHS reads those cells through the edge interpolation
-// of FXY/FXZ/FYZ (velocity.py:264-315), so the artefact is reproduced.
-// Cells at the last overlap index (n+3), whose reference values depend on
-// the arrays' history and are never read, receive the periodic image.
+// of metric_uniform.grad_vel_center (pf_features.h).  Cells at the last
+// overlap index (n+3), whose reference values depend on the arrays' history
+// and are never read, receive the periodic image.
 #include <hip/hip_runtime.h>
 
 #include "pf_box.h"
+#include "pf_features.h"
 
 namespace pf {
-
-// Nine gradients at padded cell (a,b,k) [all in 0..n+2], padded base pointers.
-// G[c][d]: component c (u,v,w) differentiated along d (x,y,z).
-template <typename T>
-__device__ __forceinline__ void grad_center(const T* __restrict__ U, const T* __restrict__ V,
-                                            const T* __restrict__ W, int64_t sx, int64_t sy,
-                                            int a, int b, int k, T dxi, T dyi, T dzi,
-                                            T (&G)[3][3]) {
-  const int64_t c0 = a * sx + b * sy + k;
-  const T half = (T)0.5;
-  const T* F[3] = {U, V, W};
-  const int64_t st[3] = {sx, sy, (int64_t)1};
-  const int pos[3] = {a, b, k};
-  const T hinv[3] = {dxi, dyi, dzi};
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const T* f = F[c];
-    const int64_t sc = st[c];   // interpolation axis of component c
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (d == c) {
-        // direct staggered difference: grad[:-1] = (var[1:] - var[:-1]) * h
-        G[c][d] = (f[c0 + sc] - f[c0]) * hinv[d];
-      } else {
-        const int64_t sd = st[d];
-        // c(m) = 0.5*(f[m + e_c] + f[m]) at offsets 0 and +1 (and -1) along d
-        const T cm0 = half * (f[c0 + sc] + f[c0]);
-        const T cp1 = half * (f[c0 + sd + sc] + f[c0 + sd]);
-        T e0;
-        if (pos[d] == 0) {
-          e0 = cm0;   // stale: interp_uvwi_* leaves var_i[0] untouched
-        } else {
-          const T cm1 = half * (f[c0 - sd + sc] + f[c0 - sd]);
-          e0 = half * (cm0 + cm1);
-        }
-        const T e1 = half * (cp1 + cm0);
-        G[c][d] = (e1 - e0) * hinv[d];
-      }
-    }
-  }
-}
 
 // Gradient-model stresses (sfsmodel_gradient.py:77-148), one padded cell per thread.
 template <typename T>

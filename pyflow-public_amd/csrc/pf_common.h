@@ -144,9 +144,9 @@ __device__ __forceinline__ double wave_max(double v) {
 
 // Reduce NV values per thread over the block (op per slot: 0 = sum, 1 = max).
 // Result valid in every thread.  Uses NV*(kBlock/kWave) doubles of LDS.
-template <int NV>
+template <int NV, int NT = kBlock>
 __device__ __forceinline__ void block_reduce(double (&v)[NV], const int (&op)[NV]) {
-  __shared__ double sh[NV][kBlock / kWave];
+  __shared__ double sh[NV][NT / kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
 #pragma unroll
@@ -160,26 +160,26 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], const int (&op)[NV
   for (int q = 0; q < NV; ++q) {
     double a = sh[q][0];
 #pragma unroll
-    for (int w = 1; w < kBlock / kWave; ++w) a = op[q] ? nmax(a, sh[q][w]) : a + sh[q][w];
+    for (int w = 1; w < NT / kWave; ++w) a = op[q] ? nmax(a, sh[q][w]) : a + sh[q][w];
     v[q] = a;
   }
   __syncthreads();
 }
 
 // Reduce NV partial slots written by nblk blocks (layout part[b*NV + q]).
-template <int NV>
+template <int NV, int NT = kBlock>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nblk,
                                                 double (&v)[NV], const int (&op)[NV]) {
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kBlock) {
+  for (int b = threadIdx.x; b < nblk; b += NT) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       double x = part[(int64_t)b * NV + q];
       v[q] = op[q] ? nmax(v[q], x) : v[q] + x;
     }
   }
-  block_reduce<NV>(v, op);
+  block_reduce<NV, NT>(v, op);
 }
 
 template <int NV>

@@ -150,10 +150,10 @@ struct TileGrid {
   int nblk;               // ntz*nty*ncx
 };
 
-inline TileGrid make_tile_grid(const Geom& g) {
+inline TileGrid make_tile_grid(const Geom& g, int rows = 1) {
   TileGrid t;
   t.tz = g.nz >= 64 ? 64 : g.nz >= 32 ? 32 : g.nz >= 16 ? 16 : 8;
-  t.ty = kBlock / t.tz;
+  t.ty = rows * kBlock / t.tz;
   t.ntz = (g.nz + t.tz - 1) / t.tz;
   t.nty = (g.ny + t.ty - 1) / t.ty;
   const int tiles = t.ntz * t.nty;
@@ -183,13 +183,13 @@ struct FieldOp {
   }
 };
 
-template <typename T, int MODE, int TZ>
-__global__ void __launch_bounds__(kBlock) bicg_tile_kernel(
+template <typename T, int MODE, int TZ, int TY>
+__global__ void __launch_bounds__(TZ * TY) bicg_tile_kernel(
     Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,
     const double* __restrict__ part_in, double* __restrict__ part_out,
     const T* __restrict__ r, const T* __restrict__ rh, const T* __restrict__ a_old,
     const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L) {
-  constexpr int TY = kBlock / TZ;
+  constexpr int NT = TZ * TY;
   const KrylovCtl c0 = snapshot(ctl);
   if (c0.done) return;
   T c1, c2 = (T)0;
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(kBlock) bicg_tile_kernel(
     }
     double red[2];
     const int ops[2] = {0, 1};
-    reduce_partials<2>(part_in, nblk_in, red, ops);
+    reduce_partials<2, NT>(part_in, nblk_in, red, ops);
     const double rho1 = red[0];
     if (c0.iter == 0) {
       if (lead()) ctl->res0 = red[1];
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(kBlock) bicg_tile_kernel(
   } else {
     double red[1];
     const int ops[1] = {0};
-    reduce_partials<1>(part_in, nblk_in, red, ops);
+    reduce_partials<1, NT>(part_in, nblk_in, red, ops);
     const double gamma = red[0];
     if (gamma == 0.0) {                                // pressure.py:247-248
       if (lead()) {
@@ -317,12 +317,12 @@ __global__ void __launch_bounds__(kBlock) bicg_tile_kernel(
   if (MODE == kModeA) {
     double accv[1] = {acc0};
     const int ops1[1] = {0};
-    block_reduce<1>(accv, ops1);
+    block_reduce<1, NT>(accv, ops1);
     write_partials<1>(part_out, accv);
   } else {
     double accv[2] = {acc0, acc1};
     const int ops2[2] = {0, 0};
-    block_reduce<2>(accv, ops2);
+    block_reduce<2, NT>(accv, ops2);
     write_partials<2>(part_out, accv);
   }
 }
@@ -332,15 +332,20 @@ void launch_tile(const TileGrid& tg, hipStream_t s, hipEvent_t e0, hipEvent_t e1
                  int nblk_in, KrylovCtl* ctl, const double* pin, double* pout, const T* r,
                  const T* rh, const T* a_old, const T* v_in, T* f_out, T* af_out,
                  const Lap<T>& L) {
-#define PF_TILE(TZ)                                                                          \
-  hipExtLaunchKernelGGL((bicg_tile_kernel<T, MODE, TZ>), dim3(tg.nblk), dim3(kBlock), 0, s, e0, \
-                        e1, 0, g, tg, nblk_in, ctl, pin, pout, r, rh, a_old, v_in, f_out,       \
+#define PF_TILE(TZ, TY)                                                                      \
+  hipExtLaunchKernelGGL((bicg_tile_kernel<T, MODE, TZ, TY>), dim3(tg.nblk), dim3(TZ * TY), 0, s,  \
+                        e0, e1, 0, g, tg, nblk_in, ctl, pin, pout, r, rh, a_old, v_in, f_out,    \
                         af_out, L)
-  switch (tg.tz) {
-    case 64: PF_TILE(64); break;
-    case 32: PF_TILE(32); break;
-    case 16: PF_TILE(16); break;
-    default: PF_TILE(8); break;
+  const int key = tg.tz * 1000 + tg.ty;
+  switch (key) {
+    case 64004: PF_TILE(64, 4); break;
+    case 64008: PF_TILE(64, 8); break;
+    case 32008: PF_TILE(32, 8); break;
+    case 32016: PF_TILE(32, 16); break;
+    case 16016: PF_TILE(16, 16); break;
+    case 16032: PF_TILE(16, 32); break;
+    case 8032: PF_TILE(8, 32); break;
+    default: PF_TILE(8, 64); break;
   }
 #undef PF_TILE
 }
@@ -668,13 +673,15 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
   int nrec = 0;
   int slot[3] = {0, 0, 0};
   auto launch = [&](int kind, int iter, auto&& go) {
-    const bool prof = h->prof_on && slot[kind] < h->prof_cap;
+    // every prof_on-th launch of each kind carries the event pair
+    const bool prof = h->prof_on && (h->prof_seq[kind]++ % h->prof_on) == 0 &&
+                      slot[kind] < h->prof_cap;
     hipEvent_t* e = prof ? h->prof_ev + ((size_t)kind * h->prof_cap + slot[kind]) * 2 : nullptr;
     go(prof ? e[0] : nullptr, prof ? e[1] : nullptr);
     if (prof) recs[nrec++] = Rec{kind, iter, slot[kind]++};
   };
   int it = 0;  // iterations whose KA has been launched
-  const TileGrid tg = make_tile_grid(g);
+  const TileGrid tg = make_tile_grid(g, h->tile_rows);
   const int nbA = tg.nblk;   // blocks writing partials A and B
   auto ka = [&](int old, int nw, hipEvent_t e0, hipEvent_t e1) {
     launch_tile<T, kModeA>(tg, s, e0, e1, g, nb, h->ctl, pC, pA, r, rh, P[old], Vv[old], P[nw],
@@ -915,7 +922,8 @@ int pf_profile_enable(pf_box* h, int on) {
     for (int q = 0; q < 3 * cap * 2; ++q) PF_HIP(hipEventCreate(&h->prof_ev[q]));
     h->prof_cap = cap;
   }
-  h->prof_on = on ? 1 : 0;
+  h->prof_on = on > 0 ? on : 0;
+  for (int k = 0; k < 3; ++k) h->prof_seq[k] = 0;
   for (int k = 0; k < 3; ++k) {
     h->prof_ms[k] = 0.0;
     h->prof_n[k] = 0;

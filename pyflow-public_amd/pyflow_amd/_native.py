@@ -50,6 +50,8 @@ SIGNATURES = [
     ('pf_div_visc', _I, [_P, _P, _P, _P, _P, _P]),
     ('pf_div_vel_over', _I, [_P, _P, _P, _P, _P, _P]),
     ('pf_vel_conv', _I, [_P, _I, _I, _P, _P, _P, _D, _P]),
+    ('pf_sfs_mlp', _I, [_P, _P, _P, _P, _P, _PP, _P]),
+    ('pf_sfs_mlp_param_count', _I, []),
     ('pf_profile_enable', _I, [_P, _I]),
     ('pf_profile_read', _I, [_P, _I, _PD, ctypes.POINTER(ctypes.c_int64)]),
 ]
@@ -262,8 +264,9 @@ class Box:
                                 self.padded(B, 'B'), self.interior(rhs, 'rhs'), float(sign),
                                 self.stream()), 'pf_vel_conv')
 
-    def profile_enable(self, on=True):
-        check(lib().pf_profile_enable(self.h, 1 if on else 0), 'pf_profile_enable')
+    def profile_enable(self, period=1):
+        """Time every `period`-th launch of each Krylov kernel (0/False = off)."""
+        check(lib().pf_profile_enable(self.h, int(period)), 'pf_profile_enable')
 
     def profile_read(self):
         """{'KA': (ms_total, launches), 'KB': ..., 'KC': ...}"""
@@ -274,6 +277,16 @@ class Box:
             check(lib().pf_profile_read(self.h, k, ctypes.byref(ms), ctypes.byref(n)), 'pf_profile_read')
             out[name] = (ms.value, n.value)
         return out
+
+    def sfs_mlp(self, u, v, w, params, sfs6):
+        if params.dtype != torch.float32 or params.device.type != 'cuda' or not params.is_contiguous():
+            raise ValueError('MLP parameters must be a contiguous float32 GPU tensor')
+        if params.numel() != lib().pf_sfs_mlp_param_count():
+            raise ValueError('wrong number of MLP parameters')
+        arr = (ctypes.c_void_p * 6)(*[self.padded(f, 'sfs[%d]' % m).value for m, f in enumerate(sfs6)])
+        check(lib().pf_sfs_mlp(self.h, self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
+                               ctypes.c_void_p(params.data_ptr()), ctypes.cast(arr, _PP),
+                               self.stream()), 'pf_sfs_mlp')
 
     def sfs_gradient(self, u, v, w, sfs6):
         arr = (ctypes.c_void_p * 6)(*[self.padded(f, 'sfs[%d]' % m).value for m, f in enumerate(sfs6)])
