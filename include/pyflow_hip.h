@@ -248,6 +248,84 @@ int pf_vel_conv(pf_box* box, int axis, int diagonal, const void* A, const void* 
 int pf_sfs_mlp(pf_box* box, const void* u, const void* v, const void* w, const float* params,
                void* const* sfs6, void* stream);
 int pf_sfs_mlp_param_count(void);
+/* The same MLP in FP32 arithmetic (the precision its reverse pass
+ * differentiates); same arguments as pf_sfs_mlp. */
+int pf_sfs_mlp_f32(pf_box* box, const void* u, const void* v, const void* w, const float* params,
+                   void* const* sfs6, void* stream);
+
+/*
+ * Adjoint step of closure training (replaces Domain.adjointStep,
+ * data/domain.py:409-438, with rhs_adjPredictor.evaluate,
+ * solver/adjoint.py:94-298, and adj_conv_*, metric/metric_staggered.py:529-800).
+ * u, v, w: the forward state (padded) the adjoint is linearised about, held
+ * fixed; ua, va, wa: the adjoint state (padded).  The adjoint RHS is the
+ * viscous operator of the forward RHS applied to the adjoint field plus the
+ * 15 convective transposes, bit-for-bit in the reference's op order, plus an
+ * optional closure term delta3 (padded, added last).
+ *   pf_rhs_adj: RHS into ru, rv, rw (interior arrays).
+ *   pf_adj_rk4_step / pf_adj_euler_step: the reference's RK4 / Euler on the
+ *     adjoint RHS (advancer.py:49-113); uta..wta receive the last stage state.
+ *     closure = -1 (none), 0 (gradient model) or 1 (MLP with `params`): the
+ *     closure term of every stage is recomputed from that stage's adjoint
+ *     state with pf_adj_closure_term; then work (pf_adj_work_bytes bytes) and
+ *     delta3 (three padded scratch arrays) are required and dparams (746
+ *     doubles, may be NULL) accumulates the MLP parameter gradient of every
+ *     stage (the reference accumulates its closure's parameter gradients
+ *     over every adjoint RHS evaluation the same way, adjoint.py:266-270).
+ *   pf_adj_closure_term: delta = (d tau/du)^T (d R/d tau)^T a for a 'tensor'
+ *     closure tau at the forward state (the reference's closure term,
+ *     adjoint.py:241-286, is this transpose for its 'source' closure),
+ *     folded onto the interior with periodic overlap values.
+ *   pf_adj_conv: one adj_conv_* method on padded arrays: for s in {0, e_s2},
+ *     rhs -= ((a[m - e_d + s] - a[m + s]) * vi[m + s + vsh]) * coef with
+ *     vsh = -e_d when centre != 0 (adj_conv_xx/yy/zz) and 0 otherwise.
+ */
+int pf_rhs_adj(pf_box* box, const void* u, const void* v, const void* w, const void* ua,
+               const void* va, const void* wa, double mu, double rho, const void* const* delta3,
+               void* ru, void* rv, void* rw, void* stream);
+int pf_adj_rk4_step(pf_box* box, const void* u, const void* v, const void* w, void* ua, void* va,
+                    void* wa, void* uta, void* vta, void* wta, double dt, double mu, double rho,
+                    int closure, const float* params, void* work, void* const* delta3,
+                    double* dparams, void* stream);
+int pf_adj_euler_step(pf_box* box, const void* u, const void* v, const void* w, void* ua,
+                      void* va, void* wa, double dt, double mu, double rho, int closure,
+                      const float* params, void* work, void* const* delta3, double* dparams,
+                      void* stream);
+size_t pf_adj_work_bytes(const pf_box* box);
+int pf_adj_closure_term(pf_box* box, int kind, const void* u, const void* v, const void* w,
+                        const float* params, const void* ua, const void* va, const void* wa,
+                        void* work, void* const* delta3, double* dparams, void* stream);
+int pf_adj_conv(pf_box* box, int d_axis, int s2_axis, int centre, const void* a, const void* vi,
+                void* rhs, double coef, void* stream);
+
+/*
+ * Reverse-mode (vector-Jacobian) kernels of the forward step, for exact
+ * gradients through it (no counterpart in the reference, which differentiates
+ * only its closure, by torch autograd, adjoint.py:241-286).  Adjoint fields
+ * are padded arrays with periodic overlap values; every pass is a per-cell
+ * local reverse followed by a fixed-order gather (no atomics), so results
+ * are deterministic.
+ *   pf_rhs_vjp: for lambda = (lu, lv, lw), out += J_u(R)(u)^T lambda (R the
+ *     RHS without closure) and, when ku..kw are given, kprev += hcoef * the
+ *     same (the RK stage chain).  work: 33 * nx*ny*nz elements.
+ *   pf_sfs_vjp: taubar (6 padded) = (d R/d tau)^T lambda, or += when
+ *     accumulate != 0.
+ *   pf_fold_halo: transpose of the periodic overlap fill, in place.
+ *   pf_closure_vjp: f = (d tau/du)^T taubar for kind 0 (gradient model) or 1
+ *     (MLP, FP32), before folding; gbar9: 9 padded arrays of feature
+ *     adjoints; dparams (+=, 746 doubles) with wpart
+ *     (pf_closure_vjp_waves(box) * 746 doubles) for the MLP parameters.
+ */
+int pf_rhs_vjp(pf_box* box, const void* u, const void* v, const void* w, double mu, double rho,
+               const void* lu, const void* lv, const void* lw, void* work, void* ou, void* ov,
+               void* ow, void* ku, void* kv, void* kw, double hcoef, void* stream);
+int pf_sfs_vjp(pf_box* box, const void* lu, const void* lv, const void* lw, void* const* taubar6,
+               int accumulate, void* stream);
+int pf_fold_halo(pf_box* box, void* field, void* stream);
+int pf_closure_vjp_waves(const pf_box* box);
+int pf_closure_vjp(pf_box* box, int kind, const void* u, const void* v, const void* w,
+                   const float* params, void* const* taubar6, void* gbar9, void* fu, void* fv,
+                   void* fw, double* dparams, double* wpart, void* stream);
 
 /*
  * Measurement support (no counterpart in the reference): with period > 0,

@@ -448,10 +448,29 @@ class NavierStokesRHS:
 
     def evaluate(self, fu, fv, fw):
         g = self.g
+        sfs = self.sfs
+        self.rhs_u.zero_(); self.rhs_v.zero_(); self.rhs_w.zero_()   # :227-229
+        self.viscous(fu, fv, fw, sfs is not None and sfs.modelType == 'tensor')
+
+        # advection: :322-355
+        interp_u_xm(fu); vel_conv_xx(g, fu, fu, self.rhs_u)
+        interp_uw_y(fu); interp_vw_x(fv); vel_conv_y(g, fu, fv, self.rhs_u)
+        interp_uv_z(fu); interp_vw_x(fw); vel_conv_z(g, fu, fw, self.rhs_u)
+
+        interp_vw_x(fv); interp_uw_y(fu); vel_conv_x(g, fv, fu, self.rhs_v)
+        interp_v_ym(fv); vel_conv_yy(g, fv, fv, self.rhs_v)
+        interp_uv_z(fv); interp_uw_y(fw); vel_conv_z(g, fv, fw, self.rhs_v)
+
+        interp_vw_x(fw); interp_uv_z(fu); vel_conv_x(g, fw, fu, self.rhs_w)
+        interp_uw_y(fw); interp_uv_z(fv); vel_conv_y(g, fw, fv, self.rhs_w)
+        interp_w_zm(fw); vel_conv_zz(g, fw, fw, self.rhs_w)
+
+    def viscous(self, fu, fv, fw, tensor):
+        """Viscous (and tensor-closure) flux divergence accumulated into
+        rhs_u/v/w, velocity.py:235-317."""
+        g = self.g
         rho, VISC, sfs = self.rho, self.VISC, self.sfs
         FX, FY, FZ, SC = self.FX, self.FY, self.FZ, self.interp_SC
-        tensor = sfs is not None and sfs.modelType == 'tensor'
-        self.rhs_u.zero_(); self.rhs_v.zero_(); self.rhs_w.zero_()   # :227-229
 
         grad_vel_visc(g, fu); grad_vel_visc(g, fv); grad_vel_visc(g, fw)   # :235-237
         div_vel_over(g, fu, fv, fw, self.div_vel)                        # :240
@@ -492,19 +511,6 @@ class NavierStokesRHS:
             interp_sc_yz(g, sfs.FYZ, SC); FY.add_(SC)
             FZ.add_(sfs.FZZ)
         div_visc(g, FX, FY, FZ, self.rhs_w)
-
-        # advection: :322-355
-        interp_u_xm(fu); vel_conv_xx(g, fu, fu, self.rhs_u)
-        interp_uw_y(fu); interp_vw_x(fv); vel_conv_y(g, fu, fv, self.rhs_u)
-        interp_uv_z(fu); interp_vw_x(fw); vel_conv_z(g, fu, fw, self.rhs_u)
-
-        interp_vw_x(fv); interp_uw_y(fu); vel_conv_x(g, fv, fu, self.rhs_v)
-        interp_v_ym(fv); vel_conv_yy(g, fv, fv, self.rhs_v)
-        interp_uv_z(fv); interp_uw_y(fw); vel_conv_z(g, fv, fw, self.rhs_v)
-
-        interp_vw_x(fw); interp_uv_z(fu); vel_conv_x(g, fw, fu, self.rhs_w)
-        interp_uw_y(fw); interp_uv_z(fv); vel_conv_y(g, fw, fv, self.rhs_w)
-        interp_w_zm(fw); vel_conv_zz(g, fw, fw, self.rhs_w)
 
 
 # ---------------------------------------------------------------------------

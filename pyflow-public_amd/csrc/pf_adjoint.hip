@@ -228,7 +228,7 @@ __device__ __forceinline__ T lam_at(const Geom& g, const T* L, int a, int b, int
   return L[pidx(g, i, j, k)];
 }
 
-template <typename T>
+template <typename T, bool ACC>
 __global__ void __launch_bounds__(kBlock) sfs_vjp_kernel(Geom g, CVec3<T> L, Coef<T> k,
                                                          Vec6<T> tb) {
   const int nxo = g.nx + 2 * kNover, nyo = g.ny + 2 * kNover, nzo = g.nz + 2 * kNover;
@@ -255,12 +255,15 @@ __global__ void __launch_bounds__(kBlock) sfs_vjp_kernel(Geom g, CVec3<T> L, Coe
       return k.dzi * (lv(x, y, z) - lv(x, y, z + 1)) + k.dyi * (lw(x, y, z) - lw(x, y + 1, z));
     };
     const T q4 = (T)0.25;
-    tb.a[0][q] += k.dxi * (lu(a, b, e) - lu(a + 1, b, e));
-    tb.a[1][q] += k.dyi * (lv(a, b, e) - lv(a, b + 1, e));
-    tb.a[2][q] += k.dzi * (lw(a, b, e) - lw(a, b, e + 1));
-    tb.a[3][q] += q4 * (exy(a, b, e) + exy(a + 1, b, e) + exy(a, b + 1, e) + exy(a + 1, b + 1, e));
-    tb.a[4][q] += q4 * (exz(a, b, e) + exz(a + 1, b, e) + exz(a, b, e + 1) + exz(a + 1, b, e + 1));
-    tb.a[5][q] += q4 * (eyz(a, b, e) + eyz(a, b + 1, e) + eyz(a, b, e + 1) + eyz(a, b + 1, e + 1));
+    T t[6];
+    t[0] = k.dxi * (lu(a, b, e) - lu(a + 1, b, e));
+    t[1] = k.dyi * (lv(a, b, e) - lv(a, b + 1, e));
+    t[2] = k.dzi * (lw(a, b, e) - lw(a, b, e + 1));
+    t[3] = q4 * (exy(a, b, e) + exy(a + 1, b, e) + exy(a, b + 1, e) + exy(a + 1, b + 1, e));
+    t[4] = q4 * (exz(a, b, e) + exz(a + 1, b, e) + exz(a, b, e + 1) + exz(a + 1, b, e + 1));
+    t[5] = q4 * (eyz(a, b, e) + eyz(a, b + 1, e) + eyz(a, b, e + 1) + eyz(a, b + 1, e + 1));
+#pragma unroll
+    for (int m = 0; m < 6; ++m) tb.a[m][q] = ACC ? tb.a[m][q] + t[m] : t[m];
   }
 }
 
@@ -323,7 +326,7 @@ int pf_rhs_vjp(pf_box* h, const void* u, const void* v, const void* w, double mu
 }
 
 int pf_sfs_vjp(pf_box* h, const void* lu, const void* lv, const void* lw, void* const* taubar6,
-               void* stream) {
+               int accumulate, void* stream) {
   if (int rc = check_box(h)) return rc;
   if (!lu || !lv || !lw || !taubar6) return fail(PF_EINVAL, "pf_sfs_vjp: NULL pointer");
   for (int m = 0; m < 6; ++m)
@@ -338,8 +341,12 @@ int pf_sfs_vjp(pf_box* h, const void* lu, const void* lv, const void* lw, void* 
     CVec3<T> L = {{(const T*)lu, (const T*)lv, (const T*)lw}};
     Vec6<T> tb;
     for (int m = 0; m < 6; ++m) tb.a[m] = (T*)taubar6[m];
-    hipLaunchKernelGGL(sfs_vjp_kernel<T>, dim3((int)nb), dim3(kBlock), 0, s, g, L,
-                       make_coef<T>(h, 0.0, 1.0), tb);
+    if (accumulate)
+      hipLaunchKernelGGL((sfs_vjp_kernel<T, true>), dim3((int)nb), dim3(kBlock), 0, s, g, L,
+                         make_coef<T>(h, 0.0, 1.0), tb);
+    else
+      hipLaunchKernelGGL((sfs_vjp_kernel<T, false>), dim3((int)nb), dim3(kBlock), 0, s, g, L,
+                         make_coef<T>(h, 0.0, 1.0), tb);
     PF_LAUNCH_CHECK("sfs_vjp_kernel");
     return 0;
   };
@@ -426,7 +433,7 @@ __global__ void __launch_bounds__(kBlock) mlp_f32_kernel(Geom g, const T* __rest
 // Per padded cell: feature adjoints Gbar (9 arrays) from the stress adjoints;
 // for the MLP also the parameter-gradient partials of this wave.
 // KIND 0: gradient model (sfsmodel_gradient.py:77-148); 1: MLP (FP32).
-template <typename T, int KIND>
+template <typename T, int KIND, bool PGRAD>
 __global__ void __launch_bounds__(64) closure_vjp_local_kernel(
     Geom g, const T* __restrict__ U, const T* __restrict__ V, const T* __restrict__ W, T dxi,
     T dyi, T dzi, T cst, Vec6<T> tb, const float* __restrict__ P, T* __restrict__ gbar,
@@ -490,7 +497,7 @@ __global__ void __launch_bounds__(64) closure_vjp_local_kernel(
       for (int c = 0; c < 3; ++c)
         for (int d = 0; d < 3; ++d) gbar[(int64_t)(3 * c + d) * total + q] = Gb[c][d];
     }
-    if (KIND == 1) {
+    if (KIND == 1 && PGRAD) {
       // parameter gradients: wave sums of per-cell outer products
       for (int p = 0; p < kNp; ++p) {
         float v;
@@ -505,7 +512,7 @@ __global__ void __launch_bounds__(64) closure_vjp_local_kernel(
       }
     }
   }
-  if (KIND == 1) {
+  if (KIND == 1 && PGRAD) {
     for (int s = 0; s < (kNp + 63) / 64; ++s) {
       const int p = s * 64 + lane;
       if (p < kNp) wpart[(int64_t)blockIdx.x * kNp + p] = acc[s];
@@ -594,6 +601,76 @@ __global__ void __launch_bounds__(kBlock) closure_vjp_gather_kernel(Geom g, cons
 
 }  // namespace pf
 
+namespace pf {
+
+static int closure_vjp_impl(pf_box* h, int kind, const void* const U[3], const float* params,
+                            void* const* taubar6, void* gbar9, void* const F[3], double* dparams,
+                            double* wpart, hipStream_t s) {
+  const Geom& g = h->g;
+  const int nw = pf_closure_vjp_waves(h);
+  const int64_t total = (int64_t)(g.nx + 4) * (g.ny + 4) * (g.nz + 4);
+  int64_t nb = (total + kBlock - 1) / kBlock;
+  if (nb > 65536) nb = 65536;
+  auto run = [&](auto zero) -> int {
+    using T = decltype(zero);
+    Vec6<T> tb;
+    for (int m = 0; m < 6; ++m) tb.a[m] = (T*)taubar6[m];
+    const T dxi = (T)(1.0 / h->dx), dyi = (T)(1.0 / h->dy), dzi = (T)(1.0 / h->dz);
+    const T cst = (T)(h->dx * h->dx / 12.0);
+    const T* u = (const T*)U[0];
+    const T* v = (const T*)U[1];
+    const T* w = (const T*)U[2];
+    if (kind == 0)
+      hipLaunchKernelGGL((closure_vjp_local_kernel<T, 0, false>), dim3(nw), dim3(64), 0, s, g, u,
+                         v, w, dxi, dyi, dzi, cst, tb, params, (T*)gbar9, wpart);
+    else if (dparams)
+      hipLaunchKernelGGL((closure_vjp_local_kernel<T, 1, true>), dim3(nw), dim3(64), 0, s, g, u,
+                         v, w, dxi, dyi, dzi, cst, tb, params, (T*)gbar9, wpart);
+    else
+      hipLaunchKernelGGL((closure_vjp_local_kernel<T, 1, false>), dim3(nw), dim3(64), 0, s, g, u,
+                         v, w, dxi, dyi, dzi, cst, tb, params, (T*)gbar9, wpart);
+    PF_LAUNCH_CHECK("closure_vjp_local_kernel");
+    if (kind == 1 && dparams) {
+      hipLaunchKernelGGL(mlp_param_reduce_kernel, dim3((kNp + kBlock - 1) / kBlock), dim3(kBlock),
+                         0, s, wpart, nw, dparams);
+      PF_LAUNCH_CHECK("mlp_param_reduce_kernel");
+    }
+    Vec3<T> fb = {{(T*)F[0], (T*)F[1], (T*)F[2]}};
+    hipLaunchKernelGGL(closure_vjp_gather_kernel<T>, dim3((int)nb), dim3(kBlock), 0, s, g,
+                       (const T*)gbar9, dxi, dyi, dzi, fb);
+    PF_LAUNCH_CHECK("closure_vjp_gather_kernel");
+    return 0;
+  };
+  return h->dtype == PF_F64 ? run(0.0) : run(0.0f);
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t adj_work_bytes(const pf_box* h) {
+  const size_t np = (size_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4) * h->esize;
+  return 15 * align256(np) + align256((size_t)pf_closure_vjp_waves(h) * kNp * sizeof(double));
+}
+
+int adj_closure_term(pf_box* h, int kind, const void* const U[3], const float* params,
+                     const void* const A[3], void* work, void* const D[3], double* dparams,
+                     hipStream_t s) {
+  const size_t np = align256((size_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4) * h->esize);
+  char* base = (char*)work;
+  void* tb[6];
+  for (int m = 0; m < 6; ++m) tb[m] = base + m * np;
+  void* gbar = base + 6 * np;   // 9 padded arrays
+  double* wpart = (double*)(base + 15 * np);
+  if (int rc = pf_sfs_vjp(h, A[0], A[1], A[2], tb, 0, s)) return rc;
+  if (int rc = closure_vjp_impl(h, kind, U, params, tb, gbar, D, kind == 1 ? dparams : nullptr,
+                                wpart, s))
+    return rc;
+  for (int q = 0; q < 3; ++q)
+    if (int rc = pf_fold_halo(h, D[q], s)) return rc;
+  return 0;
+}
+
+}  // namespace pf
+
 extern "C" {
 
 int pf_sfs_mlp_f32(pf_box* h, const void* u, const void* v, const void* w, const float* params,
@@ -634,41 +711,29 @@ int pf_closure_vjp(pf_box* h, int kind, const void* u, const void* v, const void
   if (kind != 0 && kind != 1) return fail(PF_EINVAL, "pf_closure_vjp: kind is 0 (gradient) or 1 (MLP)");
   if (!u || !v || !w || !taubar6 || !gbar9 || !fu || !fv || !fw)
     return fail(PF_EINVAL, "pf_closure_vjp: NULL pointer");
-  if (kind == 1 && (!params || !dparams || !wpart))
-    return fail(PF_EINVAL, "pf_closure_vjp: the MLP needs params, dparams and wpart");
-  hipStream_t s = (hipStream_t)stream;
-  const Geom& g = h->g;
-  const int nw = pf_closure_vjp_waves(h);
-  const int64_t total = (int64_t)(g.nx + 4) * (g.ny + 4) * (g.nz + 4);
-  int64_t nb = (total + kBlock - 1) / kBlock;
-  if (nb > 65536) nb = 65536;
-  auto run = [&](auto zero) -> int {
-    using T = decltype(zero);
-    Vec6<T> tb;
-    for (int m = 0; m < 6; ++m) tb.a[m] = (T*)taubar6[m];
-    const T dxi = (T)(1.0 / h->dx), dyi = (T)(1.0 / h->dy), dzi = (T)(1.0 / h->dz);
-    const T cst = (T)(h->dx * h->dx / 12.0);
-    if (kind == 0)
-      hipLaunchKernelGGL((closure_vjp_local_kernel<T, 0>), dim3(nw), dim3(64), 0, s, g,
-                         (const T*)u, (const T*)v, (const T*)w, dxi, dyi, dzi, cst, tb, params,
-                         (T*)gbar9, wpart);
-    else
-      hipLaunchKernelGGL((closure_vjp_local_kernel<T, 1>), dim3(nw), dim3(64), 0, s, g,
-                         (const T*)u, (const T*)v, (const T*)w, dxi, dyi, dzi, cst, tb, params,
-                         (T*)gbar9, wpart);
-    PF_LAUNCH_CHECK("closure_vjp_local_kernel");
-    if (kind == 1) {
-      hipLaunchKernelGGL(mlp_param_reduce_kernel, dim3((kNp + kBlock - 1) / kBlock), dim3(kBlock),
-                         0, s, wpart, nw, dparams);
-      PF_LAUNCH_CHECK("mlp_param_reduce_kernel");
-    }
-    Vec3<T> fb = {{(T*)fu, (T*)fv, (T*)fw}};
-    hipLaunchKernelGGL(closure_vjp_gather_kernel<T>, dim3((int)nb), dim3(kBlock), 0, s, g,
-                       (const T*)gbar9, dxi, dyi, dzi, fb);
-    PF_LAUNCH_CHECK("closure_vjp_gather_kernel");
-    return 0;
-  };
-  return h->dtype == PF_F64 ? run(0.0) : run(0.0f);
+  if (kind == 1 && !params) return fail(PF_EINVAL, "pf_closure_vjp: the MLP needs params");
+  if (dparams && !wpart) return fail(PF_EINVAL, "pf_closure_vjp: dparams needs wpart");
+  const void* U[3] = {u, v, w};
+  void* F[3] = {fu, fv, fw};
+  return closure_vjp_impl(h, kind, U, params, taubar6, gbar9, F, kind == 1 ? dparams : nullptr,
+                          wpart, (hipStream_t)stream);
+}
+
+size_t pf_adj_work_bytes(const pf_box* h) { return h ? adj_work_bytes(h) : 0; }
+
+int pf_adj_closure_term(pf_box* h, int kind, const void* u, const void* v, const void* w,
+                        const float* params, const void* ua, const void* va, const void* wa,
+                        void* work, void* const* delta3, double* dparams, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (kind != 0 && kind != 1) return fail(PF_EINVAL, "pf_adj_closure_term: kind is 0 or 1");
+  if (!u || !v || !w || !ua || !va || !wa || !work || !delta3 || !delta3[0] || !delta3[1] ||
+      !delta3[2])
+    return fail(PF_EINVAL, "pf_adj_closure_term: NULL pointer");
+  if (kind == 1 && !params) return fail(PF_EINVAL, "pf_adj_closure_term: the MLP needs params");
+  const void* U[3] = {u, v, w};
+  const void* A[3] = {ua, va, wa};
+  void* D[3] = {delta3[0], delta3[1], delta3[2]};
+  return adj_closure_term(h, kind, U, params, A, work, D, dparams, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -78,6 +78,15 @@ int check_hip(hipError_t e, const char* what);
 int check_box(const pf_box* h);
 int launch_blocks(const Geom& g);   // grid size for flat one-cell-per-thread kernels
 
+// Closure term of the adjoint RHS (pf_adjoint.hip): for the adjoint state A
+// at the forward state U, D = (d tau/du)^T (d R/d tau)^T A, folded onto the
+// interior and refreshed in the overlap; dparams (may be NULL) accumulates
+// the MLP parameter gradient.  work holds adj_work_bytes(h) bytes.
+int adj_closure_term(pf_box* h, int kind, const void* const U[3], const float* params,
+                     const void* const A[3], void* work, void* const D[3], double* dparams,
+                     hipStream_t s);
+size_t adj_work_bytes(const pf_box* h);
+
 inline double* part_buf(pf_box* h, int which) {
   return h->part + (size_t)which * h->part_stride;
 }

@@ -175,6 +175,25 @@ __global__ void __launch_bounds__(kBlock) vel_conv_kernel(Geom g, int64_t o1, in
   }
 }
 
+// adj_conv_* (metric_staggered.py:529-800): for s in {0, s2}
+//   rhs -= ((a[m - d + s] - a[m + s]) * vi[m + s + vsh]) * coef
+template <typename T>
+__global__ void __launch_bounds__(kBlock) adj_conv_kernel(Geom g, int64_t d, int64_t s2,
+                                                          int64_t vsh, const T* __restrict__ a,
+                                                          const T* __restrict__ vi,
+                                                          T* __restrict__ rhs, T coef) {
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(g, idx, i, j, k);
+    const int64_t c = pidx(g, i, j, k);
+    T r = rhs[idx];
+    r = r - (((a[c - d] - a[c]) * vi[c + vsh]) * coef);
+    r = r - (((a[c - d + s2] - a[c + s2]) * vi[c + s2 + vsh]) * coef);
+    rhs[idx] = r;
+  }
+}
+
 static PBox full_box(const pf_box* h) {
   PBox b;
   b.n[0] = h->nx + 4; b.n[1] = h->ny + 4; b.n[2] = h->nz + 4;
@@ -318,6 +337,26 @@ int pf_vel_conv(pf_box* h, int axis, int diagonal, const void* A, const void* B,
                        o0, (const float*)A, (const float*)B, (float*)rhs,
                        (float)hinv, (float)sign);
   PF_LAUNCH_CHECK("vel_conv_kernel");
+  return 0;
+}
+
+int pf_adj_conv(pf_box* h, int d_axis, int s2_axis, int centre, const void* a, const void* vi,
+                void* rhs, double coef, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (!a || !vi || !rhs) return fail(PF_EINVAL, "pf_adj_conv: NULL field");
+  if (d_axis < 0 || d_axis > 2 || s2_axis < 0 || s2_axis > 2)
+    return fail(PF_EINVAL, "pf_adj_conv: bad axis");
+  hipStream_t s = (hipStream_t)stream;
+  const Geom& g = h->g;
+  const int64_t st[3] = {g.psx, g.psy, 1};
+  const int64_t d = st[d_axis], s2 = st[s2_axis], vsh = centre ? -d : 0;
+  if (h->dtype == PF_F64)
+    hipLaunchKernelGGL(adj_conv_kernel<double>, dim3(launch_blocks(g)), dim3(kBlock), 0, s, g, d,
+                       s2, vsh, (const double*)a, (const double*)vi, (double*)rhs, coef);
+  else
+    hipLaunchKernelGGL(adj_conv_kernel<float>, dim3(launch_blocks(g)), dim3(kBlock), 0, s, g, d,
+                       s2, vsh, (const float*)a, (const float*)vi, (float*)rhs, (float)coef);
+  PF_LAUNCH_CHECK("adj_conv_kernel");
   return 0;
 }
 

@@ -59,8 +59,9 @@ __device__ __forceinline__ T edge4(const T* __restrict__ F, int64_t c, int64_t s
 }
 
 // RHS at interior cell (i,j,k); U,V,W point at the padded interior origin and
-// c is the padded offset of the cell.
-template <typename T, bool SFS>
+// c is the padded offset of the cell.  ADV = false leaves out the advective
+// terms (the adjoint RHS applies the viscous operator alone to its field).
+template <typename T, bool SFS, bool ADV = true>
 __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __restrict__ V,
                                          const T* __restrict__ W, int64_t c, int64_t sx,
                                          int64_t sy, const StepCoef<T>& k, const Sfs<T>& sfs,
@@ -109,7 +110,7 @@ __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __res
   r = fma(FZ_c, dzi, r);
   r = fma(FZ_m, -dzi, r);
   // ---- u: advection (velocity.py:322-331, metric_staggered.py:458-526) ----
-  {
+  if (ADV) {
     const T a = half * (up00 + u000), b = half * (u000 + um00);
     r = r - ((a * a - b * b) * dxi);
     const T uy_p = half * (u0p0 + u000), vx_p = half * (v0p0 + vmp0);
@@ -143,7 +144,7 @@ __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __res
   r = fma(FZ_c, dzi, r);
   r = fma(FZ_m, -dzi, r);
   // ---- v: advection (velocity.py:334-343) ----
-  {
+  if (ADV) {
     const T vx_p1 = half * (vp00 + v000), uy_p1 = half * (up00 + upm0);
     const T vx_c = half * (v000 + vm00), uy_c = half * (u000 + u0m0);
     r = r - ((vx_p1 * uy_p1 - vx_c * uy_c) * dxi);
@@ -177,7 +178,7 @@ __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __res
   r = fma(FZ_c, dzi, r);
   r = fma(FZ_m, -dzi, r);
   // ---- w: advection (velocity.py:346-355) ----
-  {
+  if (ADV) {
     const T wx_p1 = half * (wp00 + w000), uz_p1 = half * (up00 + up0m);
     const T wx_c = half * (w000 + wm00), uz_c = half * (u000 + u00m);
     r = r - ((wx_p1 * uz_p1 - wx_c * uz_c) * dxi);
@@ -193,12 +194,40 @@ __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __res
 // Stage kinds.
 enum { kStage1 = 1, kStage2 = 2, kStage3 = 3, kStage4 = 4, kEuler = 0, kRhsOnly = 5 };
 
-// One RK stage (or Euler step, or a bare RHS evaluation).
-//   Y     : stage input state (padded, interior origin)
+// Stage update of one cell from its RHS r (advancer.py:49-113):
 //   Un    : u^n (padded, interior origin)
 //   acc   : running k1 + 2k2 + 2k3 (interior)
 //   Yout  : next stage state (padded), or the Euler result, or the RHS (interior)
 //   Uout  : u* for stage 4 (padded; same arrays as Un, read and written at the own cell only)
+template <typename T, int KIND>
+__device__ __forceinline__ void stage_update(const Geom& g, uint32_t idx, int i, int j, int kk,
+                                             int64_t c, const T (&r)[3], const CVec3<T>& Un,
+                                             const Vec3<T>& acc, const Vec3<T>& Yout,
+                                             const Vec3<T>& Uout, const StepCoef<T>& k) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (KIND == kRhsOnly) {
+      Yout.a[q][idx] = r[q];
+    } else if (KIND == kEuler) {
+      const T un = Un.a[q][c];
+      store_padded(g, Yout.a[q] - g.poff, i, j, kk, un + (r[q] * k.dt));
+    } else if (KIND == kStage4) {
+      const T un = Un.a[q][c];
+      const T s = acc.a[q][idx] + r[q];
+      store_padded(g, Uout.a[q] - g.poff, i, j, kk, un + (k.dt6 * s));
+    } else {
+      const T un = Un.a[q][c];
+      if (KIND == kStage1)
+        acc.a[q][idx] = r[q];
+      else
+        acc.a[q][idx] = acc.a[q][idx] + ((T)2.0 * r[q]);
+      store_padded(g, Yout.a[q] - g.poff, i, j, kk, (k.half_dt * r[q]) + un);
+    }
+  }
+}
+
+// One RK stage (or Euler step, or a bare RHS evaluation) of the forward RHS;
+// Y is the stage input state (padded, interior origin).
 template <typename T, int KIND, bool SFS>
 __global__ void __launch_bounds__(kBlock) rk_stage_kernel(Geom g, CVec3<T> Y, CVec3<T> Un,
                                                           Vec3<T> acc, Vec3<T> Yout, Vec3<T> Uout,
@@ -210,26 +239,124 @@ __global__ void __launch_bounds__(kBlock) rk_stage_kernel(Geom g, CVec3<T> Y, CV
     const int64_t c = i * g.psx + j * g.psy + kk;
     T r[3];
     rhs_cell<T, SFS>(Y.a[0], Y.a[1], Y.a[2], c, g.psx, g.psy, k, sfs, r[0], r[1], r[2]);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      if (KIND == kRhsOnly) {
-        Yout.a[q][idx] = r[q];
-      } else if (KIND == kEuler) {
-        const T un = Un.a[q][c];
-        store_padded(g, Yout.a[q] - g.poff, i, j, kk, un + (r[q] * k.dt));
-      } else if (KIND == kStage4) {
-        const T un = Un.a[q][c];
-        const T s = acc.a[q][idx] + r[q];
-        store_padded(g, Uout.a[q] - g.poff, i, j, kk, un + (k.dt6 * s));
-      } else {
-        const T un = Un.a[q][c];
-        if (KIND == kStage1)
-          acc.a[q][idx] = r[q];
-        else
-          acc.a[q][idx] = acc.a[q][idx] + ((T)2.0 * r[q]);
-        store_padded(g, Yout.a[q] - g.poff, i, j, kk, (k.half_dt * r[q]) + un);
-      }
-    }
+    stage_update<T, KIND>(g, idx, i, j, kk, c, r, Un, acc, Yout, Uout, k);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint RHS of the reference's adjoint step (solver/adjoint.py:94-298):
+// the viscous operator of the forward RHS applied to the adjoint field, then
+// the 15 discrete-exact convective transposes (metric_staggered.py:529-800)
+// with the forward state's interpolants, in the reference's order.  Each
+// transpose is two subtractions  r -= ((a[m - e_d + s] - a[m + s]) * vi[m + s + vsh]) * coef
+// for s = 0 and s = s2.
+
+// Interpolants of the forward state (metric_staggered.py:149-188), at the
+// padded offset o:  kind 0 = cell centre along the own axis,
+// 1 = the lower edge value 0.5*(f[o] + f[o - e]) along axis stride e.
+template <typename T>
+__device__ __forceinline__ T centre(const T* __restrict__ F, int64_t o, int64_t e) {
+  return (T)0.5 * (F[o + e] + F[o]);
+}
+template <typename T>
+__device__ __forceinline__ T lower(const T* __restrict__ F, int64_t o, int64_t e) {
+  return (T)0.5 * (F[o] + F[o - e]);
+}
+
+// One adj_conv_* method at cell c: a = adjoint component, F = forward
+// component whose interpolant vi is taken along stride ei (CENTRE selects
+// the cell-centre interpolant), d = stride of the difference, s2 = shift
+// of the second subtraction, vsh = shift of vi.
+template <typename T, bool CENTRE>
+__device__ __forceinline__ T adj_pair(T r, const T* __restrict__ a, const T* __restrict__ F,
+                                      int64_t c, int64_t ei, int64_t d, int64_t s2, int64_t vsh,
+                                      T coef) {
+  {
+    const int64_t o = c + vsh;
+    const T vi = CENTRE ? centre(F, o, ei) : lower(F, o, ei);
+    r = r - (((a[c - d] - a[c]) * vi) * coef);
+  }
+  {
+    const int64_t o = c + s2 + vsh;
+    const T vi = CENTRE ? centre(F, o, ei) : lower(F, o, ei);
+    r = r - (((a[c - d + s2] - a[c + s2]) * vi) * coef);
+  }
+  return r;
+}
+
+template <typename T>
+struct AdjCoef {
+  T dxi, dyi, dzi;      // grad_x/y/z
+  T hdx, hdy, hdz;      // 0.5*grad_x etc. as the reference forms them (a Python float)
+};
+
+template <typename T>
+AdjCoef<T> make_adj_coef(const pf_box* h) {
+  AdjCoef<T> a;
+  a.dxi = (T)(1.0 / h->dx);
+  a.dyi = (T)(1.0 / h->dy);
+  a.dzi = (T)(1.0 / h->dz);
+  a.hdx = (T)(0.5 * (1.0 / h->dx));
+  a.hdy = (T)(0.5 * (1.0 / h->dy));
+  a.hdz = (T)(0.5 * (1.0 / h->dz));
+  return a;
+}
+
+// Adjoint RHS at interior cell c.  A* = adjoint fields, U* = forward fields
+// (both padded, interior origin); D = optional closure term (padded).
+template <typename T, bool DELTA>
+__device__ __forceinline__ void adj_rhs_cell(const T* __restrict__ Au, const T* __restrict__ Av,
+                                             const T* __restrict__ Aw, const T* __restrict__ Uu,
+                                             const T* __restrict__ Uv, const T* __restrict__ Uw,
+                                             int64_t c, int64_t sx, int64_t sy,
+                                             const StepCoef<T>& k, const AdjCoef<T>& a,
+                                             const CVec3<T>& D, T (&r)[3]) {
+  const Sfs<T> none = {{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
+  rhs_cell<T, false, false>(Au, Av, Aw, c, sx, sy, k, none, r[0], r[1], r[2]);
+  const int64_t z = 1;
+  T ru = r[0], rv = r[1], rw = r[2];
+  // u (adjoint.py:172-192)
+  ru = adj_pair<T, true>(ru, Au, Uu, c, sx, sx, sx, -sx, a.dxi);    // 1a xx, Ubar
+  ru = adj_pair<T, false>(ru, Au, Uv, c, sx, sy, sy, 0, a.hdy);     // 1b y,  v^x
+  ru = adj_pair<T, false>(ru, Au, Uw, c, sx, z, z, 0, a.hdz);       // 1c z,  w^x
+  ru = adj_pair<T, false>(ru, Av, Uv, c, sx, sx, sy, 0, a.hdx);     // 1d,    v^x
+  ru = adj_pair<T, false>(ru, Aw, Uw, c, sx, sx, z, 0, a.hdx);      // 1e,    w^x
+  // v (adjoint.py:194-214)
+  rv = adj_pair<T, false>(rv, Av, Uu, c, sy, sx, sx, 0, a.hdx);     // 2a x,  u^y
+  rv = adj_pair<T, true>(rv, Av, Uv, c, sy, sy, sy, -sy, a.dyi);    // 2b yy, Vbar
+  rv = adj_pair<T, false>(rv, Av, Uw, c, sy, z, z, 0, a.hdz);       // 2c z,  w^y
+  rv = adj_pair<T, false>(rv, Au, Uu, c, sy, sy, sx, 0, a.hdy);     // 2d,    u^y
+  rv = adj_pair<T, false>(rv, Aw, Uw, c, sy, sy, z, 0, a.hdy);      // 2e,    w^y
+  // w (adjoint.py:216-236)
+  rw = adj_pair<T, false>(rw, Aw, Uu, c, z, sx, sx, 0, a.hdx);      // 3a x,  u^z
+  rw = adj_pair<T, false>(rw, Aw, Uv, c, z, sy, sy, 0, a.hdy);      // 3b y,  v^z
+  rw = adj_pair<T, true>(rw, Aw, Uw, c, z, z, z, -z, a.dzi);        // 3c zz, Wbar
+  rw = adj_pair<T, false>(rw, Au, Uu, c, z, z, sx, 0, a.hdz);       // 3d,    u^z
+  rw = adj_pair<T, false>(rw, Av, Uv, c, z, z, sy, 0, a.hdz);       // 3e,    v^z
+  if (DELTA) {
+    ru = ru + D.a[0][c];
+    rv = rv + D.a[1][c];
+    rw = rw + D.a[2][c];
+  }
+  r[0] = ru;
+  r[1] = rv;
+  r[2] = rw;
+}
+
+template <typename T, int KIND, bool DELTA>
+__global__ void __launch_bounds__(kBlock) adj_stage_kernel(Geom g, CVec3<T> Y, CVec3<T> Uf,
+                                                           CVec3<T> D, CVec3<T> Un, Vec3<T> acc,
+                                                           Vec3<T> Yout, Vec3<T> Uout,
+                                                           StepCoef<T> k, AdjCoef<T> a) {
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, kk;
+    decode(g, idx, i, j, kk);
+    const int64_t c = i * g.psx + j * g.psy + kk;
+    T r[3];
+    adj_rhs_cell<T, DELTA>(Y.a[0], Y.a[1], Y.a[2], Uf.a[0], Uf.a[1], Uf.a[2], c, g.psx, g.psy, k,
+                           a, D, r);
+    stage_update<T, KIND>(g, idx, i, j, kk, c, r, Un, acc, Yout, Uout, k);
   }
 }
 
@@ -296,6 +423,97 @@ int euler_impl(pf_box* h, void* const U[3], double dt, double mu, double rho,
   return 0;
 }
 
+template <typename T, int KIND>
+int launch_adj_stage(pf_box* h, hipStream_t s, const void* const Y[3], const void* const Uf[3],
+                     const void* const* D, const void* const Un[3], void* const Yout[3],
+                     void* const Uout[3], const StepCoef<T>& k) {
+  const Geom& g = h->g;
+  CVec3<T> y, uf, d, un;
+  Vec3<T> acc, yo, uo;
+  for (int q = 0; q < 3; ++q) {
+    y.a[q] = (const T*)Y[q] + g.poff;
+    uf.a[q] = (const T*)Uf[q] + g.poff;
+    d.a[q] = D ? (const T*)D[q] + g.poff : nullptr;
+    un.a[q] = Un && Un[q] ? (const T*)Un[q] + g.poff : nullptr;
+    acc.a[q] = (T*)h->acc[q];
+    if (KIND == kRhsOnly)
+      yo.a[q] = (T*)Yout[q];
+    else
+      yo.a[q] = Yout && Yout[q] ? (T*)Yout[q] + g.poff : nullptr;
+    uo.a[q] = Uout && Uout[q] ? (T*)Uout[q] + g.poff : nullptr;
+  }
+  const AdjCoef<T> a = make_adj_coef<T>(h);
+  const int nb = launch_blocks(g);
+  if (D)
+    hipLaunchKernelGGL((adj_stage_kernel<T, KIND, true>), dim3(nb), dim3(kBlock), 0, s, g, y, uf,
+                       d, un, acc, yo, uo, k, a);
+  else
+    hipLaunchKernelGGL((adj_stage_kernel<T, KIND, false>), dim3(nb), dim3(kBlock), 0, s, g, y, uf,
+                       d, un, acc, yo, uo, k, a);
+  PF_LAUNCH_CHECK("adj_stage_kernel");
+  return 0;
+}
+
+// Closure term of one adjoint stage, when the step carries a closure.
+struct AdjClosure {
+  int kind;              // -1 none, 0 gradient model, 1 MLP
+  const float* params;
+  void* work;
+  void* delta[3];
+  double* dparams;
+};
+
+static int adj_term(pf_box* h, const AdjClosure& cl, const void* const U[3],
+                    const void* const Y[3], hipStream_t s, const void* const** D) {
+  if (cl.kind < 0) {
+    *D = nullptr;
+    return 0;
+  }
+  if (int rc = adj_closure_term(h, cl.kind, U, cl.params, Y, cl.work, cl.delta, cl.dparams, s))
+    return rc;
+  *D = (const void* const*)cl.delta;
+  return 0;
+}
+
+// Adjoint RK4 (advancer.py:75-113 on the adjoint RHS): the same stage
+// structure as the forward step, the forward state U held fixed.
+template <typename T>
+int adj_rk4_impl(pf_box* h, const void* const U[3], void* const Ua[3], void* const Ut[3],
+                 double dt, double mu, double rho, const AdjClosure& cl, hipStream_t s) {
+  const StepCoef<T> k = make_step_coef<T>(h, mu, rho, dt);
+  void* const A[3] = {Ut[0], Ut[1], Ut[2]};
+  void* const B[3] = {h->ypong[0], h->ypong[1], h->ypong[2]};
+  const void* const Uc[3] = {Ua[0], Ua[1], Ua[2]};
+  const void* const Ac[3] = {A[0], A[1], A[2]};
+  const void* const Bc[3] = {B[0], B[1], B[2]};
+  const void* const* D;
+  int rc;
+  if ((rc = adj_term(h, cl, U, Uc, s, &D))) return rc;
+  if ((rc = launch_adj_stage<T, kStage1>(h, s, Uc, U, D, Uc, A, nullptr, k))) return rc;
+  if ((rc = adj_term(h, cl, U, Ac, s, &D))) return rc;
+  if ((rc = launch_adj_stage<T, kStage2>(h, s, Ac, U, D, Uc, B, nullptr, k))) return rc;
+  if ((rc = adj_term(h, cl, U, Bc, s, &D))) return rc;
+  if ((rc = launch_adj_stage<T, kStage3>(h, s, Bc, U, D, Uc, A, nullptr, k))) return rc;
+  if ((rc = adj_term(h, cl, U, Ac, s, &D))) return rc;
+  if ((rc = launch_adj_stage<T, kStage4>(h, s, Ac, U, D, Uc, nullptr, Ua, k))) return rc;
+  return 0;
+}
+
+template <typename T>
+int adj_euler_impl(pf_box* h, const void* const U[3], void* const Ua[3], double dt, double mu,
+                   double rho, const AdjClosure& cl, hipStream_t s) {
+  const StepCoef<T> k = make_step_coef<T>(h, mu, rho, dt);
+  void* const B[3] = {h->ypong[0], h->ypong[1], h->ypong[2]};
+  const void* const Uc[3] = {Ua[0], Ua[1], Ua[2]};
+  const void* const* D;
+  int rc;
+  if ((rc = adj_term(h, cl, U, Uc, s, &D))) return rc;
+  if ((rc = launch_adj_stage<T, kEuler>(h, s, Uc, U, D, Uc, B, nullptr, k))) return rc;
+  const size_t np = (size_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4) * sizeof(T);
+  for (int q = 0; q < 3; ++q) PF_HIP(hipMemcpyAsync(Ua[q], B[q], np, hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
 }  // namespace pf
 
 using namespace pf;
@@ -352,6 +570,83 @@ int pf_rhs_ns(pf_box* h, const void* u, const void* v, const void* w, double mu,
                                           make_step_coef<double>(h, mu, rho, 0.0), sfs6);
   return launch_stage<float, kRhsOnly>(h, s, U, nullptr, R, nullptr,
                                        make_step_coef<float>(h, mu, rho, 0.0), sfs6);
+}
+
+
+static int adj_closure_args(pf_box* h, int kind, const float* params, void* work,
+                            void* const* delta3, double* dparams, AdjClosure* cl,
+                            const char* what) {
+  cl->kind = kind;
+  cl->params = params;
+  cl->work = work;
+  cl->dparams = dparams;
+  for (int q = 0; q < 3; ++q) cl->delta[q] = delta3 ? delta3[q] : nullptr;
+  if (kind < 0) return 0;
+  if (kind > 1) return fail(PF_EINVAL, std::string(what) + ": closure kind is -1, 0 or 1");
+  if (!work || !delta3 || !delta3[0] || !delta3[1] || !delta3[2])
+    return fail(PF_EINVAL, std::string(what) + ": a closure needs work and delta3");
+  if (kind == 1 && !params) return fail(PF_EINVAL, std::string(what) + ": the MLP needs params");
+  return 0;
+}
+
+int pf_rhs_adj(pf_box* h, const void* u, const void* v, const void* w, const void* ua,
+               const void* va, const void* wa, double mu, double rho, const void* const* delta3,
+               void* ru, void* rv, void* rw, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  const void* U[3] = {u, v, w};
+  const void* A[3] = {ua, va, wa};
+  void* R[3] = {ru, rv, rw};
+  if (int rc = check_ptrs(U, 3, "pf_rhs_adj")) return rc;
+  if (int rc = check_ptrs(A, 3, "pf_rhs_adj")) return rc;
+  if (int rc = check_ptrs((const void* const*)R, 3, "pf_rhs_adj")) return rc;
+  if (delta3)
+    if (int rc = check_ptrs(delta3, 3, "pf_rhs_adj(delta3)")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_rhs_adj: rho must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == PF_F64)
+    return launch_adj_stage<double, kRhsOnly>(h, s, A, U, delta3, nullptr, R, nullptr,
+                                              make_step_coef<double>(h, mu, rho, 0.0));
+  return launch_adj_stage<float, kRhsOnly>(h, s, A, U, delta3, nullptr, R, nullptr,
+                                           make_step_coef<float>(h, mu, rho, 0.0));
+}
+
+int pf_adj_rk4_step(pf_box* h, const void* u, const void* v, const void* w, void* ua, void* va,
+                    void* wa, void* uta, void* vta, void* wta, double dt, double mu, double rho,
+                    int closure, const float* params, void* work, void* const* delta3,
+                    double* dparams, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  const void* U[3] = {u, v, w};
+  void* A[3] = {ua, va, wa};
+  void* At[3] = {uta, vta, wta};
+  if (int rc = check_ptrs(U, 3, "pf_adj_rk4_step")) return rc;
+  if (int rc = check_ptrs((const void* const*)A, 3, "pf_adj_rk4_step")) return rc;
+  if (int rc = check_ptrs((const void* const*)At, 3, "pf_adj_rk4_step")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_adj_rk4_step: rho must be nonzero");
+  AdjClosure cl;
+  if (int rc = adj_closure_args(h, closure, params, work, delta3, dparams, &cl, "pf_adj_rk4_step"))
+    return rc;
+  hipStream_t s = (hipStream_t)stream;
+  return h->dtype == PF_F64 ? adj_rk4_impl<double>(h, U, A, At, dt, mu, rho, cl, s)
+                            : adj_rk4_impl<float>(h, U, A, At, dt, mu, rho, cl, s);
+}
+
+int pf_adj_euler_step(pf_box* h, const void* u, const void* v, const void* w, void* ua, void* va,
+                      void* wa, double dt, double mu, double rho, int closure,
+                      const float* params, void* work, void* const* delta3, double* dparams,
+                      void* stream) {
+  if (int rc = check_box(h)) return rc;
+  const void* U[3] = {u, v, w};
+  void* A[3] = {ua, va, wa};
+  if (int rc = check_ptrs(U, 3, "pf_adj_euler_step")) return rc;
+  if (int rc = check_ptrs((const void* const*)A, 3, "pf_adj_euler_step")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_adj_euler_step: rho must be nonzero");
+  AdjClosure cl;
+  if (int rc = adj_closure_args(h, closure, params, work, delta3, dparams, &cl,
+                                "pf_adj_euler_step"))
+    return rc;
+  hipStream_t s = (hipStream_t)stream;
+  return h->dtype == PF_F64 ? adj_euler_impl<double>(h, U, A, dt, mu, rho, cl, s)
+                            : adj_euler_impl<float>(h, U, A, dt, mu, rho, cl, s);
 }
 
 }  // extern "C"

@@ -1,7 +1,9 @@
-"""core/PyFlow.py:69-464: the forward run loop on the GPU.
+"""core/PyFlow.py:69-662: the run loop on the GPU.
 
 run(inputConfig) keeps the reference's signature and behaviour for forward
-runs (set-up, per-step monitor line on stdout and in
+runs and for the adjoint-verification protocol (checkpointed windows of
+numCheckpointIt steps, adjoint steps back through them; set-up, per-step
+monitor line on stdout and in
 monitor/velocity_<caseName>.txt, NGA restart output every numItDataOut
 steps when fNameOut is set).  simulate(inputConfig) is the same loop
 returning the Domain and the per-step monitor values for programmatic use.
@@ -70,6 +72,8 @@ def simulate(inputConfig, verbose=True):
         s.update_border()
     _write_restart(inputConfig, D, simTime)
     viscNu = inputConfig.mu / inputConfig.rho
+    adjointTraining = bool(getattr(inputConfig, 'adjointTraining', False))
+    adjointVerification = bool(getattr(inputConfig, 'adjointVerification', False))
 
     def cfl(st):
         c = max((st['maxU'] / geometry.dx, st['maxV'] / geometry.dy, st['maxW'] / geometry.dz)) * simDt
@@ -78,6 +82,7 @@ def simulate(inputConfig, verbose=True):
 
     st = D.stats()
     history = [dict(step=0, time=simTime, CFL=cfl(st), **st)]
+    adj_history = []
     caseName = getattr(inputConfig, 'caseName', 'PyFlow')
     monitor = None
     if getattr(inputConfig, 'writeMonitor', True):
@@ -91,31 +96,96 @@ def simulate(inputConfig, verbose=True):
     t0 = time.time()
     itCount = 0
     numItDataOut = getattr(inputConfig, 'numItDataOut', 0) or 0
-    while simTime < stopTime and itCount < inputConfig.numIt:
-        D.forwardStep(simDt)
-        itCount += 1
-        simTime += simDt
-        st = D.stats()
-        rec = dict(step=itCount, time=simTime, CFL=cfl(st), **st)
-        rec['iters'] = getattr(D.poisson, 'iterations', None) if D.doPressure else None
-        history.append(rec)
-        line = LINE.format(itCount, simTime, rec['CFL'], st['maxU'], st['maxV'], st['maxW'],
-                           st['TKE'], st['maxDiv'], st['max_resP'])
+    # PyFlow.py:312-324: adjoint training splits the run into checkpointed windows
+    if adjointTraining:
+        numStepsOuter = inputConfig.numIt // D.numCheckpointIt
+        numStepsInner = D.numCheckpointIt
+    else:
+        numStepsOuter = 1
+        numStepsInner = inputConfig.numIt
+    new_obj = 0.0
+    new_adj = 0.0
+    npts = float(geometry.Nx * geometry.Ny * geometry.Nz)
+    nn = geometry.imin_ + geometry.nx_ // 2
+    for _outer in range(numStepsOuter):
+        itCountInner = 0
+        simTimeCheckpoint = simTime
+        if adjointTraining:
+            D.checkpoint(0)
+        # forward inner loop (PyFlow.py:352-425)
+        while simTime < stopTime and itCountInner < numStepsInner:
+            D.forwardStep(simDt)
+            if adjointTraining:
+                D.checkpoint(itCountInner + 1)
+            itCountInner += 1
+            itCount += 1
+            simTime += simDt
+            st = D.stats()
+            rec = dict(step=itCount, time=simTime, CFL=cfl(st), **st)
+            rec['iters'] = getattr(D.poisson, 'iterations', None) if D.doPressure else None
+            history.append(rec)
+            line = LINE.format(itCount, simTime, rec['CFL'], st['maxU'], st['maxV'], st['maxW'],
+                               st['TKE'], st['maxDiv'], st['max_resP'])
+            if verbose:
+                print(line)
+            if monitor:
+                monitor.write(line + "\n")
+            if numItDataOut and itCount % numItDataOut == 0:
+                _write_restart(inputConfig, D, simTime)
+        if adjointVerification:
+            # PyFlow.py:476-477.  The objective is evaluated on every
+            # verification run: the reference evaluates it only when the
+            # adjoint runs too and returns 0.0 for the perturbed forward runs
+            # of its own verification script.
+            new_obj = float(((D.state_u_P.interior() - 0.0) ** 2).mean().item())
+        if not adjointTraining:
+            continue
+        # adjoint inner loop (PyFlow.py:471-607)
+        if not adjointVerification:
+            raise NotImplementedError('adjoint training against target data files is not part '
+                                      'of this build; set adjointVerification')
+        D.state_u_adj_P.var.copy_(2.0 * (D.state_u_P.var - 0.0))      # PyFlow.py:480-482
+        D.state_v_adj_P.var.copy_(0.0 * (D.state_v_P.var - 0.0))
+        D.state_w_adj_P.var.copy_(0.0 * (D.state_w_P.var - 0.0))
+        for a in (D.state_u_adj_P, D.state_v_adj_P, D.state_w_adj_P):  # PyFlow.py:528-530
+            a.var.div_(npts)
         if verbose:
-            print(line)
-        if monitor:
-            monitor.write(line + "\n")
-        if numItDataOut and itCount % numItDataOut == 0:
-            _write_restart(inputConfig, D, simTime)
+            print('Starting adjoint iteration')
+        itCountInnerUp = 0
+        while itCountInner > 0:
+            D.restore(itCountInner - 1)                                # PyFlow.py:548-550
+            D.adjointStep(simDt)
+            itCountInner -= 1
+            itCountInnerUp += 1
+            simTime -= simDt
+            A = D.data_adj_CPU
+            maxU, maxV, maxW = A.absmax(0), A.absmax(1), A.absmax(2)
+            maxCFL = max((maxU / geometry.dx, maxV / geometry.dy, maxW / geometry.dz)) * simDt
+            adj_history.append(dict(step=itCount - itCountInnerUp, time=simTime, CFL=maxCFL,
+                                    maxU=maxU, maxV=maxV, maxW=maxW, max_resP=D.max_resP))
+            if verbose:
+                print("  Adj {:6d}   {:8.3E}   {:8.3E}   {:8.3E}   {:8.3E}   {:8.3E}   {:9s}   "
+                      "{:9s}    {:8.3E}".format(itCount - itCountInnerUp, simTime, maxCFL, maxU,
+                                                maxV, maxW, ' ', ' ', D.max_resP))
+        new_adj = float(D.state_u_adj_P.var[nn, nn, nn].item())        # PyFlow.py:580-582
+        D.restore(numStepsInner)                                       # PyFlow.py:600-606
+        simTime = simTimeCheckpoint + numStepsInner * simDt
     wall = time.time() - t0
     if monitor:
         monitor.close()
-    return Result(D, geometry, history, wall)
+    res = Result(D, geometry, history, wall)
+    res.adj_history = adj_history
+    res.objective = new_obj
+    res.adjoint = new_adj
+    return res
 
 
 def run(inputConfig):
-    """core/PyFlow.py:69: forward run, returns None like the reference."""
-    simulate(inputConfig)
+    """core/PyFlow.py:69: returns (objective, adjoint value at the perturbed
+    cell) for an adjoint-verification run (PyFlow.py:659-662), else None."""
+    r = simulate(inputConfig)
+    if getattr(inputConfig, 'adjointVerification', False):
+        return r.objective, r.adjoint
     return None
 
 

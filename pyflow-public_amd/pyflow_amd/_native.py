@@ -52,6 +52,19 @@ SIGNATURES = [
     ('pf_vel_conv', _I, [_P, _I, _I, _P, _P, _P, _D, _P]),
     ('pf_sfs_mlp', _I, [_P, _P, _P, _P, _P, _PP, _P]),
     ('pf_sfs_mlp_param_count', _I, []),
+    ('pf_sfs_mlp_f32', _I, [_P, _P, _P, _P, _P, _PP, _P]),
+    ('pf_rhs_adj', _I, [_P, _P, _P, _P, _P, _P, _P, _D, _D, _PP, _P, _P, _P, _P]),
+    ('pf_adj_rk4_step', _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _I, _P, _P, _PP,
+                             _PD, _P]),
+    ('pf_adj_euler_step', _I, [_P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _I, _P, _P, _PP, _PD, _P]),
+    ('pf_adj_work_bytes', ctypes.c_size_t, [_P]),
+    ('pf_adj_closure_term', _I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _PP, _PD, _P]),
+    ('pf_adj_conv', _I, [_P, _I, _I, _I, _P, _P, _P, _D, _P]),
+    ('pf_rhs_vjp', _I, [_P, _P, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P]),
+    ('pf_sfs_vjp', _I, [_P, _P, _P, _P, _PP, _I, _P]),
+    ('pf_fold_halo', _I, [_P, _P, _P]),
+    ('pf_closure_vjp_waves', _I, [_P]),
+    ('pf_closure_vjp', _I, [_P, _I, _P, _P, _P, _P, _PP, _P, _P, _P, _P, _PD, _PD, _P]),
     ('pf_profile_enable', _I, [_P, _I]),
     ('pf_profile_read', _I, [_P, _I, _PD, ctypes.POINTER(ctypes.c_int64)]),
 ]
@@ -293,3 +306,128 @@ class Box:
         check(lib().pf_sfs_gradient(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
                                     self.padded(w, 'w'), ctypes.cast(arr, _PP), self.stream()),
               'pf_sfs_gradient')
+
+    # -- adjoint step and reverse-mode kernels ---------------------------------
+    def _params(self, params):
+        if params is None:
+            return None
+        if params.dtype != torch.float32 or params.device.type != 'cuda' or not params.is_contiguous():
+            raise ValueError('MLP parameters must be a contiguous float32 GPU tensor')
+        if params.numel() != lib().pf_sfs_mlp_param_count():
+            raise ValueError('wrong number of MLP parameters')
+        return ctypes.c_void_p(params.data_ptr())
+
+    def _ptrs(self, fields, name):
+        arr = (ctypes.c_void_p * len(fields))(
+            *[self.padded(f, '%s[%d]' % (name, m)).value for m, f in enumerate(fields)])
+        return ctypes.cast(arr, _PP), arr
+
+    @staticmethod
+    def _dparams(d):
+        if d is None:
+            return None
+        if d.dtype != torch.float64 or d.device.type != 'cuda' or not d.is_contiguous() or \
+                d.numel() != lib().pf_sfs_mlp_param_count():
+            raise ValueError('dparams must be a contiguous float64 GPU tensor of the MLP size')
+        return ctypes.cast(ctypes.c_void_p(d.data_ptr()), _PD)
+
+    def sfs_mlp_f32(self, u, v, w, params, sfs6):
+        pa = self._ptrs(sfs6, 'sfs')
+        check(lib().pf_sfs_mlp_f32(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                   self.padded(w, 'w'), self._params(params), pa[0], self.stream()),
+              'pf_sfs_mlp_f32')
+
+    def adj_work_bytes(self):
+        return int(lib().pf_adj_work_bytes(self.h))
+
+    def rhs_adj(self, u, v, w, ua, va, wa, mu, rho, ru, rv, rw, delta3=None):
+        da = self._ptrs(delta3, 'delta') if delta3 is not None else None
+        check(lib().pf_rhs_adj(self.h, self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
+                               self.padded(ua, 'ua'), self.padded(va, 'va'), self.padded(wa, 'wa'),
+                               float(mu), float(rho), da[0] if da else None,
+                               self.interior(ru, 'rhs_u'), self.interior(rv, 'rhs_v'),
+                               self.interior(rw, 'rhs_w'), self.stream()), 'pf_rhs_adj')
+
+    def adj_closure_term(self, kind, u, v, w, params, ua, va, wa, work, delta3, dparams=None):
+        da = self._ptrs(delta3, 'delta')
+        check(lib().pf_adj_closure_term(self.h, int(kind), self.padded(u, 'u'), self.padded(v, 'v'),
+                                        self.padded(w, 'w'), self._params(params),
+                                        self.padded(ua, 'ua'), self.padded(va, 'va'),
+                                        self.padded(wa, 'wa'), self._work(work), da[0],
+                                        self._dparams(dparams), self.stream()),
+              'pf_adj_closure_term')
+
+    def _work(self, work):
+        if work is None:
+            return None
+        if work.device.type != 'cuda' or work.numel() * work.element_size() < self.adj_work_bytes():
+            raise ValueError('adjoint work buffer must be a GPU tensor of >= %d bytes'
+                             % self.adj_work_bytes())
+        return ctypes.c_void_p(work.data_ptr())
+
+    def adj_rk4_step(self, u, v, w, ua, va, wa, uta, vta, wta, dt, mu, rho, closure=-1,
+                     params=None, work=None, delta3=None, dparams=None):
+        da = self._ptrs(delta3, 'delta') if delta3 is not None else None
+        check(lib().pf_adj_rk4_step(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                    self.padded(w, 'w'), self.padded(ua, 'ua'), self.padded(va, 'va'),
+                                    self.padded(wa, 'wa'), self.padded(uta, 'uta'),
+                                    self.padded(vta, 'vta'), self.padded(wta, 'wta'), float(dt),
+                                    float(mu), float(rho), int(closure), self._params(params),
+                                    self._work(work), da[0] if da else None,
+                                    self._dparams(dparams), self.stream()), 'pf_adj_rk4_step')
+
+    def adj_euler_step(self, u, v, w, ua, va, wa, dt, mu, rho, closure=-1, params=None, work=None,
+                       delta3=None, dparams=None):
+        da = self._ptrs(delta3, 'delta') if delta3 is not None else None
+        check(lib().pf_adj_euler_step(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                      self.padded(w, 'w'), self.padded(ua, 'ua'),
+                                      self.padded(va, 'va'), self.padded(wa, 'wa'), float(dt),
+                                      float(mu), float(rho), int(closure), self._params(params),
+                                      self._work(work), da[0] if da else None,
+                                      self._dparams(dparams), self.stream()), 'pf_adj_euler_step')
+
+    def adj_conv(self, d_axis, s2_axis, centre, a, vi, rhs, coef):
+        check(lib().pf_adj_conv(self.h, int(d_axis), int(s2_axis), 1 if centre else 0,
+                                self.padded(a, 'a'), self.padded(vi, 'vi'), self.interior(rhs, 'rhs'),
+                                float(coef), self.stream()), 'pf_adj_conv')
+
+    def rhs_vjp(self, u, v, w, mu, rho, lam3, work, out3, kprev3=None, hcoef=0.0):
+        if work.device.type != 'cuda' or work.dtype != self.dtype or \
+                work.numel() < 33 * self.nx * self.ny * self.nz:
+            raise ValueError('rhs_vjp work must hold 33*nx*ny*nz elements of the box dtype')
+        k = [self.padded(t, 'kprev') for t in kprev3] if kprev3 is not None else [None] * 3
+        check(lib().pf_rhs_vjp(self.h, self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
+                               float(mu), float(rho), self.padded(lam3[0], 'lu'),
+                               self.padded(lam3[1], 'lv'), self.padded(lam3[2], 'lw'),
+                               ctypes.c_void_p(work.data_ptr()), self.padded(out3[0], 'ou'),
+                               self.padded(out3[1], 'ov'), self.padded(out3[2], 'ow'),
+                               k[0], k[1], k[2], float(hcoef), self.stream()), 'pf_rhs_vjp')
+
+    def sfs_vjp(self, lam3, taubar6, accumulate=True):
+        ta = self._ptrs(taubar6, 'taubar')
+        check(lib().pf_sfs_vjp(self.h, self.padded(lam3[0], 'lu'), self.padded(lam3[1], 'lv'),
+                               self.padded(lam3[2], 'lw'), ta[0], 1 if accumulate else 0,
+                               self.stream()), 'pf_sfs_vjp')
+
+    def fold_halo(self, a):
+        check(lib().pf_fold_halo(self.h, self.padded(a, 'field'), self.stream()), 'pf_fold_halo')
+
+    def closure_vjp_waves(self):
+        return int(lib().pf_closure_vjp_waves(self.h))
+
+    def closure_vjp(self, kind, u, v, w, params, taubar6, gbar9, f3, dparams=None, wpart=None):
+        ta = self._ptrs(taubar6, 'taubar')
+        if gbar9.dtype != self.dtype or gbar9.numel() != 9 * self.padded_shape[0] * \
+                self.padded_shape[1] * self.padded_shape[2] or not gbar9.is_contiguous():
+            raise ValueError('gbar9 must hold 9 padded arrays of the box dtype')
+        wp = None
+        if wpart is not None:
+            if wpart.dtype != torch.float64 or wpart.numel() < self.closure_vjp_waves() * \
+                    lib().pf_sfs_mlp_param_count():
+                raise ValueError('wpart must hold closure_vjp_waves()*746 float64 values')
+            wp = ctypes.cast(ctypes.c_void_p(wpart.data_ptr()), _PD)
+        check(lib().pf_closure_vjp(self.h, int(kind), self.padded(u, 'u'), self.padded(v, 'v'),
+                                   self.padded(w, 'w'), self._params(params), ta[0],
+                                   ctypes.c_void_p(gbar9.data_ptr()), self.padded(f3[0], 'fu'),
+                                   self.padded(f3[1], 'fv'), self.padded(f3[2], 'fw'),
+                                   self._dparams(dparams), wp, self.stream()), 'pf_closure_vjp')

@@ -1,0 +1,113 @@
+"""solver/adjoint.py:37-298 on the GPU: the RHS of the reference's adjoint
+step (closure training).
+
+rhs_adjPredictor.evaluate is one fused HIP kernel (pf_rhs_adj): the viscous
+operator of the forward RHS applied to the adjoint field plus the 15
+discrete-exact convective transposes (metric_staggered.py:529-800) with the
+forward state's interpolants, in the reference's op order.  The adjoint
+advancer of this package (advancer.RK4 / Euler with an AdjointRHS) fuses
+the RHS into its stage kernels (pf_adj_rk4_step / pf_adj_euler_step).
+
+Closure term (adjoint.py:241-286).  The reference's term is the transpose
+of its 'source' ML closure, obtained by torch autograd.  For the 'tensor'
+closures of this package (gradient model, MLP) the term is the exact
+transpose of the closure's share of the forward RHS,
+delta = (d tau/du)^T (d R/d tau)^T u_adj, from hand-written reverse kernels
+(pf_adj_closure_term), recomputed for every stage's adjoint state like the
+reference's.  For the MLP each evaluation also accumulates the parameter
+gradient d(sum tau . taubar)/d theta into sfsmodel.grad (the reference's
+g.backward() accumulates its model's parameter gradients the same way);
+the closure's finalize() scales it by +simDt (the reference scales by
+-simDt because its closure enters the RHS with a minus sign).
+"""
+
+import torch
+
+from .velocity import constant_viscosity
+
+
+def closure_kind(sfsmodel):
+    """-1 (no closure term), 0 (gradient model) or 1 (MLP)."""
+    if sfsmodel is None or getattr(sfsmodel, 'modelType', None) != 'tensor':
+        if getattr(sfsmodel, 'modelType', None) == 'source':
+            raise NotImplementedError("'source' closures (sfsmodel_ML) are not part of this build")
+        return -1
+    return 1 if hasattr(sfsmodel, 'params') else 0
+
+
+class AdjointScratch:
+    """Device work of the closure term: reverse-pass workspace and the three
+    padded delta arrays, allocated once per box."""
+
+    def __init__(self, box, decomp):
+        self.work = torch.empty(box.adj_work_bytes(), dtype=torch.uint8, device=decomp.device)
+        shape = (decomp.nxo_, decomp.nyo_, decomp.nzo_)
+        self.delta = [torch.zeros(shape, dtype=decomp.prec, device=decomp.device) for _ in range(3)]
+
+
+class rhs_adjPredictor:
+    def __init__(self, decomp, metric, rho, VISC, sfsmodel, state_u, state_v, state_w,
+                 scratch=None):
+        self.decomp = decomp
+        self.box = decomp.box
+        self.metric = metric
+        self.rho = rho
+        self.VISC = VISC
+        self.sfsmodel = sfsmodel
+        self.state_u = state_u
+        self.state_v = state_v
+        self.state_w = state_w
+        self.kind = closure_kind(sfsmodel) if getattr(sfsmodel, 'use', True) else -1
+        self.scratch = scratch
+        shape = (decomp.nx_, decomp.ny_, decomp.nz_)
+        self.rhs_u = torch.zeros(shape, dtype=decomp.prec, device=decomp.device)
+        self.rhs_v = torch.zeros(shape, dtype=decomp.prec, device=decomp.device)
+        self.rhs_w = torch.zeros(shape, dtype=decomp.prec, device=decomp.device)
+
+    def mu(self):
+        mu = constant_viscosity(self.VISC)
+        if mu is None:
+            raise NotImplementedError('a spatially varying VISC field is not supported by the '
+                                      'fused adjoint kernels')
+        return mu
+
+    def closure_args(self):
+        """(kind, params, work, delta3, dparams) for the fused kernels."""
+        if self.kind < 0:
+            return -1, None, None, None, None
+        if self.scratch is None:
+            self.scratch = AdjointScratch(self.box, self.decomp)
+        s = self.sfsmodel
+        params = getattr(s, 'params', None) if self.kind == 1 else None
+        dparams = s.param_grad() if self.kind == 1 and hasattr(s, 'param_grad') else None
+        return self.kind, params, self.scratch.work, self.scratch.delta, dparams
+
+    def evaluate(self, state_u_adj, state_v_adj, state_w_adj):
+        """adjoint.py:100-298"""
+        kind, params, work, delta, dparams = self.closure_args()
+        U = (self.state_u.var, self.state_v.var, self.state_w.var)
+        A = (state_u_adj.var, state_v_adj.var, state_w_adj.var)
+        if kind >= 0:
+            self.box.adj_closure_term(kind, *U, params, *A, work, delta, dparams)
+        self.box.rhs_adj(*U, *A, self.mu(), self.rho, self.rhs_u, self.rhs_v, self.rhs_w,
+                         delta if kind >= 0 else None)
+
+
+class AdjointRHS:
+    """adjoint.py:40-50: one rhs_adjPredictor per stage (sharing the closure
+    scratch)."""
+
+    def __init__(self, inputConfig, decomp, metric, rho, VISC, sfsmodel, state_u, state_v,
+                 state_w):
+        self.rhs1 = rhs_adjPredictor(decomp, metric, rho, VISC, sfsmodel, state_u, state_v, state_w)
+        if inputConfig.advancerName[:-1] == 'RK':
+            sc = None
+            if self.rhs1.kind >= 0:
+                sc = AdjointScratch(decomp.box, decomp)
+                self.rhs1.scratch = sc
+            self.rhs2 = rhs_adjPredictor(decomp, metric, rho, VISC, sfsmodel, state_u, state_v,
+                                         state_w, sc)
+            self.rhs3 = rhs_adjPredictor(decomp, metric, rho, VISC, sfsmodel, state_u, state_v,
+                                         state_w, sc)
+            self.rhs4 = rhs_adjPredictor(decomp, metric, rho, VISC, sfsmodel, state_u, state_v,
+                                         state_w, sc)

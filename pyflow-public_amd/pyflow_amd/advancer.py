@@ -1,19 +1,25 @@
 """solver/advancer.py:41-113 on the GPU.
 
-With this package's rhs_NavierStokes the whole step is fused: each RK4 stage
-is one kernel that evaluates the RHS and applies the stage update, and the
-overlap cells are written by the same kernel.  Any other RHS class (duck
+With this package's rhs_NavierStokes (or, for the adjoint step,
+rhs_adjPredictor) the whole step is fused: each RK4 stage is one kernel
+that evaluates the RHS and applies the stage update, and the overlap cells
+are written by the same kernel.  Any other RHS class (duck
 typed with evaluate() and rhs_u/v/w) runs the reference's sequence with the
 state kernels.
 """
 
 import torch
 
+from .adjoint import rhs_adjPredictor
 from .velocity import rhs_NavierStokes
 
 
 def _fusable(r):
     return isinstance(r, rhs_NavierStokes)
+
+
+def _adjoint(r):
+    return isinstance(r, rhs_adjPredictor)
 
 
 class Euler:
@@ -26,6 +32,12 @@ class Euler:
     def step(self, simDt):
         """advancer.py:49-57"""
         r = self.rhsClass.rhs1
+        if _adjoint(r):
+            kind, params, work, delta, dparams = r.closure_args()
+            r.box.adj_euler_step(r.state_u.var, r.state_v.var, r.state_w.var, self.state_u.var,
+                                 self.state_v.var, self.state_w.var, simDt, r.mu(), r.rho, kind,
+                                 params, work, delta, dparams)
+            return
         if _fusable(r):
             r.box.euler_step(self.state_u.var, self.state_v.var, self.state_w.var,
                              simDt, r.mu(), r.rho, r.sfs_fields())
@@ -49,6 +61,14 @@ class RK4:
         """advancer.py:75-113: k1 = R(u^n); Y_{s+1} = u^n + dt/2*k_s (s = 1..3,
         stage 4 included); u = u^n + dt/6*(k1 + 2k2 + 2k3 + k4)."""
         rc = self.rhsClass
+        if all(_adjoint(getattr(rc, n)) for n in ('rhs1', 'rhs2', 'rhs3', 'rhs4')):
+            r = rc.rhs1
+            kind, params, work, delta, dparams = r.closure_args()
+            r.box.adj_rk4_step(r.state_u.var, r.state_v.var, r.state_w.var, self.state_u.var,
+                               self.state_v.var, self.state_w.var, self.state_uTmp.var,
+                               self.state_vTmp.var, self.state_wTmp.var, simDt, r.mu(), r.rho,
+                               kind, params, work, delta, dparams)
+            return
         if all(_fusable(getattr(rc, n)) for n in ('rhs1', 'rhs2', 'rhs3', 'rhs4')):
             r = rc.rhs1
             r.box.rk4_step(self.state_u.var, self.state_v.var, self.state_w.var,

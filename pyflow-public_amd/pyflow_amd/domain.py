@@ -1,4 +1,4 @@
-"""data/domain.py:56-402 on the GPU.
+"""data/domain.py:56-438 on the GPU.
 
 Domain.forwardStep runs the closure kernel (for 'tensor' closures), the
 fused advancer and the fused velocity correction: per RK4 step that is four
@@ -10,6 +10,7 @@ import numpy as np
 import torch
 
 from . import _native
+from . import adjoint
 from . import advancer
 from . import pressure
 from . import sfsmodel_gradient
@@ -58,9 +59,36 @@ class Domain:
         self.VISC_P.mul_(self.mu)
         self.VISC_P.pf_const_mu = float(self.mu)
 
-        if getattr(inputConfig, 'adjointTraining', False):
-            raise NotImplementedError('adjoint training of the reference ML closure is not part '
-                                      'of this build')
+        # domain.py:135-191: adjoint state, temporaries and checkpoints
+        adjointTraining = getattr(inputConfig, 'adjointTraining', False)
+        self.numCheckpointIt = getattr(inputConfig, 'numCheckpointIt', None)
+        self.adjointTraining = bool(adjointTraining)
+        if adjointTraining:
+            if self.numCheckpointIt is None:
+                raise Exception("\nAdjoint training requires specification of numCheckpointIt")
+            if inputConfig.equationMode != 'NS':
+                raise Exception("\nAdjoint training requires Navier-Stokes solver\n")
+            zeros = np.zeros((nx_, ny_, nz_))
+            self.state_u_adj_P = state.state_P(decomp, zeros)
+            self.state_v_adj_P = state.state_P(decomp, zeros)
+            self.state_w_adj_P = state.state_P(decomp, zeros)
+            self.adjoint_data_all = (self.state_u_adj_P, self.state_v_adj_P, self.state_w_adj_P)
+            self.data_adj_CPU = state.data_all_CPU(decomp, startTime, simDt, names[0:3],
+                                                   self.adjoint_data_all)
+            if inputConfig.advancerName[:-1] == 'RK':
+                self.state_uTmp_adj_P = state.state_P(decomp, zeros)
+                self.state_vTmp_adj_P = state.state_P(decomp, zeros)
+                self.state_wTmp_adj_P = state.state_P(decomp, zeros)
+            else:
+                self.state_uTmp_adj_P = self.state_vTmp_adj_P = self.state_wTmp_adj_P = None
+            # Checkpoints: stored checkpoint-major so each one is a contiguous
+            # padded array; check_*_P are views with the reference's index
+            # order [nxo, nyo, nzo, numCheckpointIt+1].
+            K = self.numCheckpointIt + 1
+            self._check = [torch.zeros(K, nxo_, nyo_, nzo_, dtype=prec, device=decomp.device)
+                           for _ in range(3)]
+            self.check_u_P, self.check_v_P, self.check_w_P = (c.permute(1, 2, 3, 0)
+                                                              for c in self._check)
 
         # domain.py:231-251
         self.use_SFSmodel = True
@@ -93,6 +121,21 @@ class Domain:
         else:
             raise Exception('\n Advancer type ' + inputConfig.advancerName +
                             ' unknown! Options are Euler and RK4.')
+
+        # domain.py:263-288
+        if adjointTraining:
+            self.adjointRHS = adjoint.AdjointRHS(inputConfig, decomp, metric, self.rho,
+                                                 self.VISC_P, self.sfsmodel if self.use_SFSmodel
+                                                 else None, self.state_u_P, self.state_v_P,
+                                                 self.state_w_P)
+            if inputConfig.advancerName == 'Euler':
+                self.adjointAdvancer = advancer.Euler(self.state_u_adj_P, self.state_v_adj_P,
+                                                      self.state_w_adj_P, self.adjointRHS)
+            else:
+                self.adjointAdvancer = advancer.RK4(self.state_u_adj_P, self.state_uTmp_adj_P,
+                                                    self.state_v_adj_P, self.state_vTmp_adj_P,
+                                                    self.state_w_adj_P, self.state_wTmp_adj_P,
+                                                    self.adjointRHS)
 
         self.doPressure = False
         self.max_resP = 0.0
@@ -149,6 +192,38 @@ class Domain:
                                              P.Num_pressure_iterations)
         self.max_resP = res
         return res
+
+    def checkpoint(self, k):
+        """Store the forward velocity as checkpoint k (PyFlow.py:333-336, 364-367)."""
+        for c, st in zip(self._check, (self.state_u_P, self.state_v_P, self.state_w_P)):
+            c[k].copy_(st.var)
+
+    def restore(self, k):
+        """Load checkpoint k into the forward velocity (PyFlow.py:548-550, 601-603)."""
+        for c, st in zip(self._check, (self.state_u_P, self.state_v_P, self.state_w_P)):
+            st.var.copy_(c[k])
+
+    def adjointStep(self, simDt):
+        """domain.py:409-438: adjoint projection (the forward correction with
+        +simDt/rho on the adjoint field, one fused call), then the adjoint
+        advancer with the forward state held where the caller left it."""
+        if self.doPressure:
+            P = self.poisson
+            A = (self.state_u_adj_P.var, self.state_v_adj_P.var, self.state_w_adj_P.var)
+            if P.kind == _native.PF_POISSON_BICGSTAB:
+                res, its, st = self.box.vel_corr(*A, self.state_DP_P.var, self.state_p_P.var,
+                                                 self.source_P, P.kind, P.rho / P.simDt,
+                                                 +simDt / self.rho, P.min_residual,
+                                                 P.max_iterations)
+                P.iterations = its
+                P.status = _native.SOLVE_STATUS.get(st, st)
+            else:
+                res, its, st = self.box.vel_corr(*A, self.state_DP_P.var, self.state_p_P.var,
+                                                 self.source_P, P.kind, P.rho / P.simDt,
+                                                 +simDt / self.rho, 0.0,
+                                                 P.Num_pressure_iterations)
+            self.max_resP = res
+        self.adjointAdvancer.step(simDt)
 
     def stats(self):
         """Monitor values of core/PyFlow.py:378-395 in one kernel; the

@@ -167,3 +167,46 @@ class metric_uniform:
 
     def vel_conv_z(self, state_u, state_w, grad_z, sign=1.0):
         self.box.vel_conv(2, False, state_u.var_i, state_w.var_i, grad_z, sign)
+
+    # ---- adjoint convective fluxes (metric_staggered.py:529-800) ----
+    # Each is two subtractions, for s = 0 and s = e_s2:
+    #   rhs -= ((a[m - e_d + s] - a[m + s]) * vi[m + s + vsh]) * coef
+    # with vi the forward state's var_i left by the preceding interp_* call.
+    def _adj(self, d, s2, centre, a, f, rhs, coef):
+        self.box.adj_conv(d, s2, centre, a.var, f.var_i, rhs, coef)
+
+    def adj_conv_xx(self, state_u_adj, state_u, rhs_u):
+        self._adj(0, 0, True, state_u_adj, state_u, rhs_u, self.grad_x)
+
+    def adj_conv_yy(self, state_v_adj, state_v, rhs_v):
+        self._adj(1, 1, True, state_v_adj, state_v, rhs_v, self.grad_y)
+
+    def adj_conv_zz(self, state_w_adj, state_w, rhs_w):
+        self._adj(2, 2, True, state_w_adj, state_w, rhs_w, self.grad_z)
+
+    def adj_conv_x(self, state_v_adj, state_u, rhs_v):
+        self._adj(0, 0, False, state_v_adj, state_u, rhs_v, 0.5 * self.grad_x)
+
+    def adj_conv_y(self, state_u_adj, state_v, rhs_u):
+        self._adj(1, 1, False, state_u_adj, state_v, rhs_u, 0.5 * self.grad_y)
+
+    def adj_conv_z(self, state_u_adj, state_w, rhs_u):
+        self._adj(2, 2, False, state_u_adj, state_w, rhs_u, 0.5 * self.grad_z)
+
+    def adj_conv_1d(self, state_v_adj, state_v, rhs_u):
+        self._adj(0, 1, False, state_v_adj, state_v, rhs_u, 0.5 * self.grad_x)
+
+    def adj_conv_1e(self, state_w_adj, state_w, rhs_u):
+        self._adj(0, 2, False, state_w_adj, state_w, rhs_u, 0.5 * self.grad_x)
+
+    def adj_conv_2d(self, state_u_adj, state_u, rhs_v):
+        self._adj(1, 0, False, state_u_adj, state_u, rhs_v, 0.5 * self.grad_y)
+
+    def adj_conv_2e(self, state_w_adj, state_w, rhs_v):
+        self._adj(1, 2, False, state_w_adj, state_w, rhs_v, 0.5 * self.grad_y)
+
+    def adj_conv_3d(self, state_u_adj, state_u, rhs_w):
+        self._adj(2, 0, False, state_u_adj, state_u, rhs_w, 0.5 * self.grad_z)
+
+    def adj_conv_3e(self, state_v_adj, state_v, rhs_w):
+        self._adj(2, 1, False, state_v_adj, state_v, rhs_w, 0.5 * self.grad_z)

@@ -12,7 +12,11 @@ functions that no closed form or textbook routine pins on its own:
 * solver_bicgstab.solve with its residual history (solver/pressure.py:196-297);
 * solver_jacobi.solve (solver/pressure.py:115-161);
 * Domain.forwardStep with RK4 + BiCGStab and Euler + Jacobi
-  (data/domain.py:342-402).
+  (data/domain.py:342-402);
+* rhs_adjPredictor.evaluate (solver/adjoint.py:94-298) and Domain.adjointStep
+  (data/domain.py:409-438) with RK4 + BiCGStab and Euler + Jacobi, the
+  closure term fed by a stand-in 'source' closure whose stresses are zero
+  (the reference's ML closure architecture is not part of the build).
 
 The reference's MPI layer is replaced by a single-rank periodic stub with the
 semantics of library/parallel.py:301-414 on a periodic Cartesian communicator
@@ -65,6 +69,18 @@ class StubDecomp:
         A[:, 0:no].copy_(A[:, n2 - 2 * no:n2 - no].clone())
         A[:, :, n3 - no:n3].copy_(A[:, :, no:2 * no].clone())
         A[:, :, 0:no].copy_(A[:, :, n3 - 2 * no:n3 - no].clone())
+
+
+class ZeroSourceClosure:
+    """A 'source' closure whose outputs GX, GY, GZ are 0*input: it keeps
+    the autograd graph of rhs_adjPredictor's closure term (adjoint.py:241-286)
+    intact and contributes zeros."""
+    modelType = 'source'
+
+    def update(self, u_V, v_V, w_V, requires_grad=True):
+        self.GX = (0.0 * u_V)[..., None]
+        self.GY = (0.0 * v_V)[..., None]
+        self.GZ = (0.0 * w_V)[..., None]
 
 
 class StubComms:
@@ -129,6 +145,7 @@ def main():
     import sfsmodel_gradient
     import sfsmodel_smagorinsky
     import domain
+    import adjoint
 
     torch.set_num_threads(1)
     prec = torch.float64
@@ -249,6 +266,42 @@ def main():
         out[tag + '_p'] = D.state_p_P.var.numpy().copy()
         out[tag + '_DP'] = D.state_DP_P.var.numpy().copy()
         out[tag + '_res'] = float(D.max_resP)
+
+    # 8. Adjoint RHS and adjoint steps at the forward state u0
+    a0, b0, c0 = seeded_velocity(shape, L, 4321)
+    out['adj_u0'], out['adj_v0'], out['adj_w0'] = a0, b0, c0
+    zsrc = ZeroSourceClosure()
+    su, sv, sw = states(u0, v0, w0)
+    au, av, aw = states(a0, b0, c0)
+    ar = adjoint.rhs_adjPredictor(d, metric, rho, VISC, zsrc, su, sv, sw)
+    ar.evaluate(au, av, aw)
+    out['adjrhs_u'], out['adjrhs_v'], out['adjrhs_w'] = (
+        ar.rhs_u.numpy().copy(), ar.rhs_v.numpy().copy(), ar.rhs_w.numpy().copy())
+    for tag, advn, solv, tol, nit in (('adj_rk4_bicg', 'RK4', 'bicgstab', 1e-12, 500),
+                                      ('adj_euler_jac', 'Euler', 'Jacobi', 0.0, 30)):
+        cfg = types.SimpleNamespace(advancerName=advn)
+        su, sv, sw = states(u0, v0, w0)
+        au, av, aw = states(a0, b0, c0)
+        tu, tv, tw = states(a0, b0, c0)
+        arhs = adjoint.AdjointRHS(cfg, d, metric, rho, VISC, zsrc, su, sv, sw)
+        if advn == 'RK4':
+            adv = advancer.RK4(au, tu, av, tv, aw, tw, arhs)
+            poisson = pressure.solver_bicgstab(comms, d, metric, geo, rho, dt, tol, nit)
+        else:
+            adv = advancer.Euler(au, av, aw, arhs)
+            poisson = pressure.solver_jacobi(comms, d, metric, geo, rho, dt, nit)
+        ns = types.SimpleNamespace(
+            doPressure=True, metric=metric, state_u_adj_P=au, state_v_adj_P=av,
+            state_w_adj_P=aw, source_P=torch.zeros(shape, dtype=prec), poisson=poisson,
+            state_DP_P=state.state_P(d, zeros.copy()), state_p_P=state.state_P(d, zeros.copy()),
+            rho=rho, adjointAdvancer=adv, max_resP=0.0)
+        domain.Domain.adjointStep(ns, dt)
+        out[tag + '_u'] = au.var.numpy().copy()
+        out[tag + '_v'] = av.var.numpy().copy()
+        out[tag + '_w'] = aw.var.numpy().copy()
+        out[tag + '_DP'] = ns.state_DP_P.var.numpy().copy()
+        out[tag + '_p'] = ns.state_p_P.var.numpy().copy()
+        out[tag + '_res'] = float(ns.max_resP)
 
     np.savez_compressed(OUT, **out)
     print('wrote', OUT, 'keys:', len(out))
