@@ -275,3 +275,85 @@ def adjoint_verification_run(g, u, v, w, nsteps, perturbation=0.0, adjoint=True,
         b.u.var.copy_(cu); b.v.var.copy_(cv); b.w.var.copy_(cw)
         A.adjoint_step()
     return obj, A.au.var[nn, nn, nn].item()
+
+
+# ---------------------------------------------------------------------------
+# Differentiable forward step (the checker of the exact reverse pass)
+# ---------------------------------------------------------------------------
+def fft_project(g, u, v, w):
+    """u - G L^-1 D u on interior arrays, with D = div_vel (forward
+    differences, metric_staggered.py:438-453), G = grad_P (backward
+    differences, :390-401) and L = D G the 7-point Laplacian, solved exactly
+    through its Fourier symbol -(4/h^2) sin^2(k h/2) per axis (the mean of
+    D u is zero).  This is the projection of Domain.velCorr
+    (domain.py:373-402) with an exact Poisson solve, differentiable by
+    autograd."""
+    dxi, dyi, dzi = g.dxi, g.dyi, g.dzi
+    D = ((torch.roll(u, -1, 0) - u) * dxi + (torch.roll(v, -1, 1) - v) * dyi
+         + (torch.roll(w, -1, 2) - w) * dzi)
+    def sym(n, hi):
+        k = 2.0 * torch.pi * torch.fft.fftfreq(n, dtype=torch.float64)
+        return -4.0 * hi * hi * torch.sin(0.5 * k) ** 2
+    L = (sym(g.nx, dxi)[:, None, None] + sym(g.ny, dyi)[None, :, None]
+         + sym(g.nz, dzi)[None, None, :])
+    L[0, 0, 0] = 1.0
+    ph = torch.fft.fftn(D) / L
+    ph[0, 0, 0] = 0.0
+    phi = torch.fft.ifftn(ph).real
+    return (u - (phi - torch.roll(phi, 1, 0)) * dxi,
+            v - (phi - torch.roll(phi, 1, 1)) * dyi,
+            w - (phi - torch.roll(phi, 1, 2)) * dzi)
+
+
+def functional_step(g, u, v, w, mu, rho, dt, closure=None, project=True):
+    """One Domain.forwardStep (domain.py:342-402) as a differentiable
+    function of the interior velocity: closure stresses from u^n (a callable
+    as for closure_adjoint_term, or None), the reference's RK4
+    (advancer.py:75-113, stage 4 from u^n + dt/2*k3), then the projection
+    with an exact solve.  Returns interior arrays."""
+    VISC = torch.ones(g.nxo, g.nyo, g.nzo, dtype=g.dtype) * mu
+
+    def fields(x):
+        F = [O.Field(g) for _ in range(3)]
+        for f, t in zip(F, x):
+            f.var = _periodic_padded(t)
+        return F
+
+    class _T:
+        modelType = 'tensor'
+    sfs = None
+    if closure is not None:
+        sfs = _T()
+        with torch.autograd.graph.saved_tensors_hooks(_pack, _unpack):
+            taus = closure(*fields((u, v, w)))
+        for n, s in zip(('FXX', 'FYY', 'FZZ', 'FXY', 'FXZ', 'FYZ'), taus):
+            setattr(sfs, n, s)
+
+    def R(x):
+        r = O.NavierStokesRHS(g, rho, VISC, sfs)
+        r.evaluate(*fields(x))
+        return r.rhs_u, r.rhs_v, r.rhs_w
+
+    U = (u, v, w)
+    with torch.autograd.graph.saved_tensors_hooks(_pack, _unpack):
+        return _rk4_project(g, U, R, dt, project)
+
+
+# The restated RHS reuses its var_i / flux buffers in place, as the reference
+# does; autograd must see each saved operand as it was when it was used.
+def _pack(t):
+    return t.clone()
+
+
+def _unpack(t):
+    return t
+
+
+def _rk4_project(g, U, R, dt, project):
+    k1 = R(U)
+    k2 = R(tuple(0.5 * dt * k + x for k, x in zip(k1, U)))
+    k3 = R(tuple(0.5 * dt * k + x for k, x in zip(k2, U)))
+    k4 = R(tuple(0.5 * dt * k + x for k, x in zip(k3, U)))
+    us = tuple(x + dt / 6.0 * (a + 2.0 * b + 2.0 * c + d)
+               for x, a, b, c, d in zip(U, k1, k2, k3, k4))
+    return fft_project(g, *us) if project else us
