@@ -183,11 +183,27 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ part,
 }
 
 template <int NV>
-__device__ __forceinline__ void write_partials(double* __restrict__ part, const double (&v)[NV]) {
+__device__ __forceinline__ void write_partials(double* __restrict__ part, const double (&v)[NV],
+                                               int bid) {
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) part[(int64_t)blockIdx.x * NV + q] = v[q];
+    for (int q = 0; q < NV; ++q) part[(int64_t)bid * NV + q] = v[q];
   }
 }
+
+template <int NV>
+__device__ __forceinline__ void write_partials(double* __restrict__ part, const double (&v)[NV]) {
+  write_partials<NV>(part, v, (int)blockIdx.x);
+}
+
+// The block of a launch that works on one box: its index among the box's
+// blocks and their number.  A single-box launch uses (blockIdx.x, gridDim.x);
+// a batched launch over several boxes gives each box the same decomposition
+// it has alone, so per-box results do not depend on the batch.
+struct Blk {
+  int id, n;
+};
+
+__device__ __forceinline__ Blk this_blk() { return Blk{(int)blockIdx.x, (int)gridDim.x}; }
 
 }  // namespace pf

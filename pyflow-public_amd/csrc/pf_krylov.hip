@@ -24,50 +24,11 @@
 
 #include <cstdlib>
 
-#include "pf_box.h"
+#include "pf_kernels.h"
 
 namespace pf {
 
 enum { kPartA = 0, kPartB = 1, kPartC = 2, kPartS = 3 };
-
-template <typename T>
-struct Lap {
-  T xm, x0, xp, ym, y0, yp, zm, z0, zp;
-};
-
-// Laplace coefficients as metric_uniform builds them (metric_staggered.py:85-102):
-// 0 + h^-2 (or -2.0*h^-1*h^-1) added in the field dtype, then scaled by -vol.
-template <typename T>
-Lap<T> make_lap(const pf_box* h) {
-  const double vol = h->dx * h->dy * h->dz;
-  const double inv[3] = {1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz};
-  T c[3][3];
-  for (int a = 0; a < 3; ++a) {
-    const double hi = inv[a];
-    const double vals[3] = {hi * hi, -2.0 * hi * hi, hi * hi};
-    for (int b = 0; b < 3; ++b) c[a][b] = (T)vals[b] * (T)(-1.0 * vol);
-  }
-  Lap<T> L;
-  L.xm = c[0][0]; L.x0 = c[0][1]; L.xp = c[0][2];
-  L.ym = c[1][0]; L.y0 = c[1][1]; L.yp = c[1][2];
-  L.zm = c[2][0]; L.z0 = c[2][1]; L.zp = c[2][2];
-  return L;
-}
-
-// B = sum in pressure.py:63-84 order: x-, x0, x+, y-, y0, y+, z-, z0, z+.
-template <typename T>
-__device__ __forceinline__ T lap7(const Lap<T>& L, T xm, T c, T xp, T ym, T yp, T zm, T zp) {
-  T b = (T)0 + xm * L.xm;
-  b = b + c * L.x0;
-  b = b + xp * L.xp;
-  b = b + ym * L.ym;
-  b = b + c * L.y0;
-  b = b + yp * L.yp;
-  b = b + zm * L.zm;
-  b = b + c * L.z0;
-  b = b + zp * L.zp;
-  return b;
-}
 
 // ---------------------------------------------------------------------------
 // Control block
@@ -80,7 +41,7 @@ __global__ void ctl_init_kernel(KrylovCtl* ctl, double tol, int maxit) {
   }
 }
 
-__device__ __forceinline__ bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
+__device__ __forceinline__ bool lead(const Blk& b) { return b.id == 0 && threadIdx.x == 0; }
 
 // Block-uniform snapshot of the control block: thread 0 reads it once and
 // broadcasts through LDS, so that a write by block 0 of the same kernel can
@@ -99,15 +60,15 @@ __device__ __forceinline__ KrylovCtl snapshot(const KrylovCtl* ctl) {
 // r = r_hat = b, x = p = v = 0 (pressure.py:202, 217-218).  With FROM_DIV the
 // source is first computed as div(u) (metric_staggered.py:438-453).
 template <typename T, bool FROM_DIV>
-__global__ void __launch_bounds__(kBlock) bicg_init_kernel(
+__device__ __forceinline__ void bicg_init_kernel_body(
     Geom g, T* __restrict__ src, const T* __restrict__ U, const T* __restrict__ V,
     const T* __restrict__ W, T dxi, T dyi, T dzi, T sc1, T sc2, T* __restrict__ r,
     T* __restrict__ rh, T* __restrict__ x, T* __restrict__ p0, T* __restrict__ v0,
-    double* __restrict__ partC) {
+    double* __restrict__ partC, Blk blk) {
   double acc[2] = {0.0, 0.0};
   const int ops[2] = {0, 1};
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
-       idx += gridDim.x * blockDim.x) {
+  for (uint32_t idx = blk.id * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += blk.n * blockDim.x) {
     T b;
     if (FROM_DIV) {
       int i, j, k;
@@ -131,7 +92,16 @@ __global__ void __launch_bounds__(kBlock) bicg_init_kernel(
     acc[1] = nmax(acc[1], fabs((double)b));
   }
   block_reduce<2>(acc, ops);
-  write_partials<2>(partC, acc);
+  write_partials<2>(partC, acc, blk.id);
+}
+
+template <typename T, bool FROM_DIV>
+__global__ void __launch_bounds__(kBlock) bicg_init_kernel(
+    Geom g, T* __restrict__ src, const T* __restrict__ U, const T* __restrict__ V,
+    const T* __restrict__ W, T dxi, T dyi, T dzi, T sc1, T sc2, T* __restrict__ r,
+    T* __restrict__ rh, T* __restrict__ x, T* __restrict__ p0, T* __restrict__ v0,
+    double* __restrict__ partC) {
+  bicg_init_kernel_body<T, FROM_DIV>(g, src, U, V, W, dxi, dyi, dzi, sc1, sc2, r, rh, x, p0, v0, partC, this_blk());
 }
 
 // ---------------------------------------------------------------------------
@@ -143,30 +113,6 @@ __global__ void __launch_bounds__(kBlock) bicg_init_kernel(
 // loads of plane x+1 (and of the next halo) are issued before the barrier of
 // plane x.  Cells of partial tiles wrap periodically, so the LDS plane always
 // holds valid neighbours; only their outputs are masked.
-struct TileGrid {
-  int tz, ty;             // tile extents (tz*ty == kBlock)
-  int ntz, nty, ncx;      // tiles along z, y and chunks along x
-  int xc;                 // planes per chunk
-  int nblk;               // ntz*nty*ncx
-};
-
-inline TileGrid make_tile_grid(const Geom& g, int rows = 1) {
-  TileGrid t;
-  t.tz = g.nz >= 64 ? 64 : g.nz >= 32 ? 32 : g.nz >= 16 ? 16 : 8;
-  t.ty = rows * kBlock / t.tz;
-  t.ntz = (g.nz + t.tz - 1) / t.tz;
-  t.nty = (g.ny + t.ty - 1) / t.ty;
-  const int tiles = t.ntz * t.nty;
-  int cx = 1024 / tiles;            // keeps tiles*chunks <= max(1024, tiles)
-  const int maxc = g.nx >= 4 ? g.nx / 4 : 1;
-  if (cx > maxc) cx = maxc;
-  if (cx < 1) cx = 1;
-  t.xc = (g.nx + cx - 1) / cx;
-  t.ncx = (g.nx + t.xc - 1) / t.xc;
-  t.nblk = tiles * t.ncx;
-  return t;
-}
-
 enum { kModeA = 0, kModeB = 1 };
 
 // Field recomputed at a cell: KA p = (p_old - omega v_old)*beta + r,
@@ -184,18 +130,18 @@ struct FieldOp {
 };
 
 template <typename T, int MODE, int TZ, int TY>
-__global__ void __launch_bounds__(TZ * TY) bicg_tile_kernel(
+__device__ __forceinline__ void bicg_tile_kernel_body(
     Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,
     const double* __restrict__ part_in, double* __restrict__ part_out,
     const T* __restrict__ r, const T* __restrict__ rh, const T* __restrict__ a_old,
-    const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L) {
+    const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L, Blk blk) {
   constexpr int NT = TZ * TY;
   const KrylovCtl c0 = snapshot(ctl);
   if (c0.done) return;
   T c1, c2 = (T)0;
   if (MODE == kModeA) {
     if (c0.halt) {
-      if (lead()) ctl->done = 1;
+      if (lead(blk)) ctl->done = 1;
       return;
     }
     double red[2];
@@ -203,14 +149,14 @@ __global__ void __launch_bounds__(TZ * TY) bicg_tile_kernel(
     reduce_partials<2, NT>(part_in, nblk_in, red, ops);
     const double rho1 = red[0];
     if (c0.iter == 0) {
-      if (lead()) ctl->res0 = red[1];
+      if (lead(blk)) ctl->res0 = red[1];
     } else {
       const double max_res = red[1] / c0.res0;       // pressure.py:279-280
-      if (lead()) ctl->max_res = max_res;
+      if (lead(blk)) ctl->max_res = max_res;
       const bool cap = c0.iter >= c0.maxit;           // pressure.py:283-287
       const bool conv = max_res <= c0.tol;
       if (cap || conv) {
-        if (lead()) {
+        if (lead(blk)) {
           ctl->status = conv ? PF_SOLVE_CONVERGED : PF_SOLVE_MAXIT;
           ctl->done = 1;
         }
@@ -218,7 +164,7 @@ __global__ void __launch_bounds__(TZ * TY) bicg_tile_kernel(
       }
     }
     const double beta = c0.alpha * rho1 / (c0.rho2 * c0.omega);   // pressure.py:233
-    if (lead()) ctl->rho1 = rho1;
+    if (lead(blk)) ctl->rho1 = rho1;
     c1 = (T)(-c0.omega);
     c2 = (T)beta;
   } else {
@@ -227,14 +173,14 @@ __global__ void __launch_bounds__(TZ * TY) bicg_tile_kernel(
     reduce_partials<1, NT>(part_in, nblk_in, red, ops);
     const double gamma = red[0];
     if (gamma == 0.0) {                                // pressure.py:247-248
-      if (lead()) {
+      if (lead(blk)) {
         ctl->status = PF_SOLVE_BREAKDOWN_A;
         ctl->done = 1;
       }
       return;
     }
     const double alpha = c0.rho1 / gamma;              // pressure.py:249
-    if (lead()) ctl->alpha = alpha;
+    if (lead(blk)) ctl->alpha = alpha;
     c1 = (T)(-alpha);
   }
   const FieldOp<T, MODE> F{r, a_old, v_in, c1, c2};
@@ -242,8 +188,8 @@ __global__ void __launch_bounds__(TZ * TY) bicg_tile_kernel(
   // XCD-aware block order: the hardware deals consecutive block ids round
   // robin over 8 XCDs; renumber so each XCD gets a contiguous run of tiles
   // (y fastest, so y-neighbour tiles share an L2).
-  const int nb = gridDim.x;
-  int b = blockIdx.x;
+  const int nb = blk.n;
+  int b = blk.id;
   if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
   const int ty_i = b % tg.nty;
   const int rest = b / tg.nty;
@@ -318,13 +264,22 @@ __global__ void __launch_bounds__(TZ * TY) bicg_tile_kernel(
     double accv[1] = {acc0};
     const int ops1[1] = {0};
     block_reduce<1, NT>(accv, ops1);
-    write_partials<1>(part_out, accv);
+    write_partials<1>(part_out, accv, blk.id);
   } else {
     double accv[2] = {acc0, acc1};
     const int ops2[2] = {0, 0};
     block_reduce<2, NT>(accv, ops2);
-    write_partials<2>(part_out, accv);
+    write_partials<2>(part_out, accv, blk.id);
   }
+}
+
+template <typename T, int MODE, int TZ, int TY>
+__global__ void __launch_bounds__(TZ * TY) bicg_tile_kernel(
+    Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,
+    const double* __restrict__ part_in, double* __restrict__ part_out,
+    const T* __restrict__ r, const T* __restrict__ rh, const T* __restrict__ a_old,
+    const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L) {
+  bicg_tile_kernel_body<T, MODE, TZ, TY>(g, tg, nblk_in, ctl, part_in, part_out, r, rh, a_old, v_in, f_out, af_out, L, this_blk());
 }
 
 template <typename T, int MODE>
@@ -352,11 +307,11 @@ void launch_tile(const TileGrid& tg, hipStream_t s, hipEvent_t e0, hipEvent_t e1
 
 // KC: omega = <s,t>/<t,t>, x += alpha*p, x += omega*s, r = s - omega*t.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) bicg_kc_kernel(
+__device__ __forceinline__ void bicg_kc_kernel_body(
     Geom g, int nblk, KrylovCtl* __restrict__ ctl, const double* __restrict__ partB,
     double* __restrict__ partC, T* __restrict__ x, const T* __restrict__ p,
     const T* __restrict__ s, const T* __restrict__ t, T* __restrict__ r,
-    const T* __restrict__ rh) {
+    const T* __restrict__ rh, Blk blk) {
   const KrylovCtl c0 = snapshot(ctl);
   if (c0.done) return;
   const double alpha = c0.alpha;
@@ -368,25 +323,25 @@ __global__ void __launch_bounds__(kBlock) bicg_kc_kernel(
   const double buf1 = red[0], buf2 = red[1];
   const T ta = (T)alpha;
   if (buf2 == 0) {                                  // pressure.py:251, 266-267
-    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
-         idx += gridDim.x * blockDim.x)
+    for (uint32_t idx = blk.id * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+         idx += blk.n * blockDim.x)
       x[idx] = fma(p[idx], ta, x[idx]);
-    if (lead()) {
+    if (lead(blk)) {
       ctl->status = PF_SOLVE_BREAKDOWN_B;
       ctl->halt = 1;
     }
     return;
   }
   const double omega = buf1 / buf2;                 // pressure.py:268
-  if (lead()) {
+  if (lead(blk)) {
     ctl->omega = omega;
     ctl->rho2 = rho1;                               // pressure.py:276
     ctl->iter = k + 1;
   }
   const T to = (T)omega, tmo = (T)(-omega);
   double acc[2] = {0.0, 0.0};
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
-       idx += gridDim.x * blockDim.x) {
+  for (uint32_t idx = blk.id * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += blk.n * blockDim.x) {
     const T sc = s[idx];
     T xv = fma(p[idx], ta, x[idx]);                 // pressure.py:251
     xv = fma(sc, to, xv);                           // pressure.py:270
@@ -398,19 +353,28 @@ __global__ void __launch_bounds__(kBlock) bicg_kc_kernel(
   }
   const int ops2[2] = {0, 1};
   block_reduce<2>(acc, ops2);
-  write_partials<2>(partC, acc);
+  write_partials<2>(partC, acc, blk.id);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) bicg_kc_kernel(
+    Geom g, int nblk, KrylovCtl* __restrict__ ctl, const double* __restrict__ partB,
+    double* __restrict__ partC, T* __restrict__ x, const T* __restrict__ p,
+    const T* __restrict__ s, const T* __restrict__ t, T* __restrict__ r,
+    const T* __restrict__ rh) {
+  bicg_kc_kernel_body<T>(g, nblk, ctl, partB, partC, x, p, s, t, r, rh, this_blk());
 }
 
 // DP (padded) := x with periodic overlap (pressure.py:293), then
 // p += DP over the whole padded array (pressure.py:294).
 template <typename T, bool ADD_P>
-__global__ void __launch_bounds__(kBlock) scatter_padded_kernel(Geom g, const T* __restrict__ x,
+__device__ __forceinline__ void scatter_padded_kernel_body(Geom g, const T* __restrict__ x,
                                                                  T* __restrict__ DP,
-                                                                 T* __restrict__ p) {
+                                                                 T* __restrict__ p, Blk blk) {
   const int nxo = g.nx + 2 * kNover, nyo = g.ny + 2 * kNover, nzo = g.nz + 2 * kNover;
   const int64_t total = (int64_t)nxo * nyo * nzo;
-  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < total;
-       a += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t a = blk.id * (int64_t)blockDim.x + threadIdx.x; a < total;
+       a += (int64_t)blk.n * blockDim.x) {
     const int ko = (int)(a % nzo);
     const int64_t t1 = a / nzo;
     const int jo = (int)(t1 % nyo);
@@ -425,14 +389,21 @@ __global__ void __launch_bounds__(kBlock) scatter_padded_kernel(Geom g, const T*
   }
 }
 
+template <typename T, bool ADD_P>
+__global__ void __launch_bounds__(kBlock) scatter_padded_kernel(Geom g, const T* __restrict__ x,
+                                                                 T* __restrict__ DP,
+                                                                 T* __restrict__ p) {
+  scatter_padded_kernel_body<T, ADD_P>(g, x, DP, p, this_blk());
+}
+
 // u -= (dt/rho) * grad_P(DP) on the interior (metric_staggered.py:390-401,
 // state.py:198-211); DP is the solver's interior-layout solution.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) project_kernel(Geom g, const T* __restrict__ x, T* U,
+__device__ __forceinline__ void project_kernel_body(Geom g, const T* __restrict__ x, T* U,
                                                          T* V, T* W, T dxi, T dyi, T dzi,
-                                                         T neg_c) {
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
-       idx += gridDim.x * blockDim.x) {
+                                                         T neg_c, Blk blk) {
+  for (uint32_t idx = blk.id * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += blk.n * blockDim.x) {
     int i, j, k;
     decode(g, idx, i, j, k);
     const Nbr n = nbr(g, i, j, k);
@@ -445,6 +416,13 @@ __global__ void __launch_bounds__(kBlock) project_kernel(Geom g, const T* __rest
     store_padded(g, V, i, j, k, fma(gy, neg_c, V[c]));
     store_padded(g, W, i, j, k, fma(gz, neg_c, W[c]));
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) project_kernel(Geom g, const T* __restrict__ x, T* U,
+                                                         T* V, T* W, T dxi, T dyi, T dzi,
+                                                         T neg_c) {
+  project_kernel_body<T>(g, x, U, V, W, dxi, dyi, dzi, neg_c, this_blk());
 }
 
 // Same projection with DP given as a padded field (Jacobi's pOld).
@@ -469,15 +447,15 @@ __global__ void __launch_bounds__(kBlock) project_padded_kernel(Geom g, const T*
 
 // Monitor values: max|u|,|v|,|w|, sum(u^2+v^2+w^2), max|div| (+ div into divg).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) stats_kernel(Geom g, const T* __restrict__ U,
+__device__ __forceinline__ void stats_kernel_body(Geom g, const T* __restrict__ U,
                                                        const T* __restrict__ V,
                                                        const T* __restrict__ W, T dxi, T dyi,
                                                        T dzi, T* __restrict__ divg,
-                                                       double* __restrict__ part) {
+                                                       double* __restrict__ part, Blk blk) {
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const int ops[5] = {1, 1, 1, 0, 1};
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
-       idx += gridDim.x * blockDim.x) {
+  for (uint32_t idx = blk.id * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += blk.n * blockDim.x) {
     int i, j, k;
     decode(g, idx, i, j, k);
     const int64_t c = i * g.psx + j * g.psy + k;
@@ -493,7 +471,16 @@ __global__ void __launch_bounds__(kBlock) stats_kernel(Geom g, const T* __restri
     acc[4] = nmax(acc[4], fabs((double)d));
   }
   block_reduce<5>(acc, ops);
-  write_partials<5>(part, acc);
+  write_partials<5>(part, acc, blk.id);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) stats_kernel(Geom g, const T* __restrict__ U,
+                                                       const T* __restrict__ V,
+                                                       const T* __restrict__ W, T dxi, T dyi,
+                                                       T dzi, T* __restrict__ divg,
+                                                       double* __restrict__ part) {
+  stats_kernel_body<T>(g, U, V, W, dxi, dyi, dzi, divg, part, this_blk());
 }
 
 struct Ops8 {
@@ -609,6 +596,163 @@ __global__ void __launch_bounds__(kBlock) jacobi_resid_kernel(Geom g, const T* _
   const int ops[1] = {1};
   block_reduce<1>(acc, ops);
   write_partials<1>(part, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: every box of the list runs the single-box body with its
+// own block decomposition (see BoxDesc).
+
+#define PF_BATCH_BOX                                                \
+  const int bi_ = find_box(L, (int)blockIdx.x);                     \
+  const BoxDesc<T>& d = D[L.box[bi_]];                              \
+  const Blk blk{(int)blockIdx.x - L.begin[bi_], L.begin[bi_ + 1] - L.begin[bi_]}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_init_kernel(const BoxDesc<T>* __restrict__ D,
+                                                            BatchList L) {
+  PF_BATCH_BOX;
+  const int64_t o = d.g.poff;
+  bicg_init_kernel_body<T, true>(d.g, d.src, d.u[0] + o, d.u[1] + o, d.u[2] + o, d.dxi, d.dyi,
+                                 d.dzi, d.sc1, d.sc2, d.r, d.rh, d.x, d.pk[0], d.vk[0], d.pC,
+                                 blk);
+}
+
+// KA of iteration j (parity = j & 1): p[1-parity] from p[parity], v[parity];
+// KB of iteration j: s, t from r and v[1-parity].
+template <typename T, int MODE, int TZ, int TY>
+__global__ void __launch_bounds__(TZ * TY) batch_tile_kernel(const BoxDesc<T>* __restrict__ D,
+                                                             BatchList L, int parity) {
+  PF_BATCH_BOX;
+  if (MODE == kModeA) {
+    const int old = parity, nw = 1 - parity;
+    bicg_tile_kernel_body<T, kModeA, TZ, TY>(d.g, d.tg, d.nblk, d.ctl, d.pC, d.pA, d.r, d.rh,
+                                             d.pk[old], d.vk[old], d.pk[nw], d.vk[nw], d.L, blk);
+  } else {
+    const int cur = 1 - parity;
+    bicg_tile_kernel_body<T, kModeB, TZ, TY>(d.g, d.tg, d.tg.nblk, d.ctl, d.pA, d.pB, d.r, d.rh,
+                                             (const T*)nullptr, d.vk[cur], d.s, d.t, d.L, blk);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_kc_kernel(const BoxDesc<T>* __restrict__ D,
+                                                          BatchList L, int parity) {
+  PF_BATCH_BOX;
+  const int cur = 1 - parity;
+  bicg_kc_kernel_body<T>(d.g, d.tg.nblk, d.ctl, d.pB, d.pC, d.x, d.pk[cur], d.s, d.t, d.r, d.rh,
+                         blk);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_scatter_kernel(const BoxDesc<T>* __restrict__ D,
+                                                               BatchList L) {
+  PF_BATCH_BOX;
+  scatter_padded_kernel_body<T, true>(d.g, d.x, d.DP, d.p, blk);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_project_kernel(const BoxDesc<T>* __restrict__ D,
+                                                               BatchList L) {
+  PF_BATCH_BOX;
+  project_kernel_body<T>(d.g, d.x, d.u[0], d.u[1], d.u[2], d.dxi, d.dyi, d.dzi, d.neg_c, blk);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_stats_kernel(const BoxDesc<T>* __restrict__ D,
+                                                             BatchList L) {
+  PF_BATCH_BOX;
+  const int64_t o = d.g.poff;
+  stats_kernel_body<T>(d.g, d.u[0] + o, d.u[1] + o, d.u[2] + o, d.dxi, d.dyi, d.dzi, (T*)nullptr,
+                       d.pS, blk);
+}
+#undef PF_BATCH_BOX
+
+// Per-box control: kBatchCtlInit (one thread per box), kBatchCtlCount
+// (boxes whose solve is done, one block), kBatchFinalize (one block per box:
+// the monitor values and the solver's outcome into d.out).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_ctl_kernel(const BoxDesc<T>* __restrict__ D,
+                                                           int n, int what, int* __restrict__ done) {
+  if (what == kBatchCtlInit) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    KrylovCtl* c = D[b].ctl;
+    c->rho1 = 1.0; c->rho2 = 1.0; c->alpha = 0.0; c->omega = 1.0;
+    c->res0 = 0.0; c->max_res = 0.0; c->tol = D[b].tol;
+    c->iter = 0; c->maxit = D[b].maxit; c->done = 0; c->status = PF_SOLVE_CONVERGED;
+    c->halt = 0;
+  } else if (what == kBatchCtlCount) {
+    double cnt[1] = {0.0};
+    for (int b = threadIdx.x; b < n; b += blockDim.x) cnt[0] += D[b].ctl->done ? 1.0 : 0.0;
+    const int ops[1] = {0};
+    block_reduce<1>(cnt, ops);
+    if (threadIdx.x == 0) done[0] = (int)cnt[0];
+  } else {
+    const BoxDesc<T>& d = D[blockIdx.x];
+    const int ops[5] = {1, 1, 1, 0, 1};
+    double v[5];
+    reduce_partials<5>(d.pS, d.nblk, v, ops);
+    if (threadIdx.x == 0) {
+      for (int q = 0; q < 5; ++q) d.out[q] = v[q];
+      d.out[5] = d.ctl->max_res;
+      d.out[6] = (double)d.ctl->iter;
+      d.out[7] = (double)d.ctl->status;
+    }
+  }
+}
+
+int batch_krylov_launch(int dtype, int what, int parity, const void* desc, const BatchList& L,
+                        int nblocks, int tz, int ty, hipStream_t s) {
+  if (nblocks <= 0) return 0;
+  auto go = [&](auto zero) -> int {
+    using T = decltype(zero);
+    const BoxDesc<T>* D = (const BoxDesc<T>*)desc;
+    const dim3 gb(nblocks), tb(kBlock);
+    switch (what) {
+      case kBatchInit: hipLaunchKernelGGL(batch_init_kernel<T>, gb, tb, 0, s, D, L); break;
+      case kBatchKC: hipLaunchKernelGGL(batch_kc_kernel<T>, gb, tb, 0, s, D, L, parity); break;
+      case kBatchScatter: hipLaunchKernelGGL(batch_scatter_kernel<T>, gb, tb, 0, s, D, L); break;
+      case kBatchProject: hipLaunchKernelGGL(batch_project_kernel<T>, gb, tb, 0, s, D, L); break;
+      case kBatchStats: hipLaunchKernelGGL(batch_stats_kernel<T>, gb, tb, 0, s, D, L); break;
+      case kBatchKA:
+      case kBatchKB: {
+#define PF_BTILE(TZ, TY)                                                                       \
+  if (what == kBatchKA)                                                                        \
+    hipLaunchKernelGGL((batch_tile_kernel<T, kModeA, TZ, TY>), gb, dim3(TZ * TY), 0, s, D, L,  \
+                       parity);                                                                \
+  else                                                                                         \
+    hipLaunchKernelGGL((batch_tile_kernel<T, kModeB, TZ, TY>), gb, dim3(TZ * TY), 0, s, D, L,  \
+                       parity)
+        switch (tz * 1000 + ty) {
+          case 64004: PF_BTILE(64, 4); break;
+          case 32008: PF_BTILE(32, 8); break;
+          case 16016: PF_BTILE(16, 16); break;
+          case 8032: PF_BTILE(8, 32); break;
+          default: return fail(PF_EINVAL, "batch_krylov_launch: unsupported tile shape");
+        }
+#undef PF_BTILE
+        break;
+      }
+      default: return fail(PF_EINVAL, "batch_krylov_launch: unknown kernel");
+    }
+    PF_LAUNCH_CHECK("batch Krylov kernel");
+    return 0;
+  };
+  return dtype == PF_F64 ? go(0.0) : go(0.0f);
+}
+
+int batch_ctl_launch(int dtype, int what, const void* desc, int n, int* done, hipStream_t s) {
+  if (n <= 0) return 0;
+  auto go = [&](auto zero) -> int {
+    using T = decltype(zero);
+    const BoxDesc<T>* D = (const BoxDesc<T>*)desc;
+    const int nb = what == kBatchCtlInit ? (n + kBlock - 1) / kBlock
+                                         : (what == kBatchCtlCount ? 1 : n);
+    hipLaunchKernelGGL(batch_ctl_kernel<T>, dim3(nb), dim3(kBlock), 0, s, D, n, what, done);
+    PF_LAUNCH_CHECK("batch_ctl_kernel");
+    return 0;
+  };
+  return dtype == PF_F64 ? go(0.0) : go(0.0f);
 }
 
 // ---------------------------------------------------------------------------

@@ -6,7 +6,7 @@
 #include <cstring>
 #include <string>
 
-#include "pf_box.h"
+#include "pf_kernels.h"
 
 namespace pf {
 
@@ -93,9 +93,7 @@ int pf_box_create(pf_box** out, int nx, int ny, int nz, double dx, double dy, do
   h->tile_rows = 1;
   {
     // the tiled Krylov kernels may use more blocks than the flat ones
-    const int tz = nz >= 64 ? 64 : nz >= 32 ? 32 : nz >= 16 ? 16 : 8;
-    const int ty = kBlock / tz;
-    const int64_t tiles = (int64_t)((nz + tz - 1) / tz) * ((ny + ty - 1) / ty);
+    const int64_t tiles = make_tile_grid(h->g, h->tile_rows).nblk;
     const int64_t mx = tiles > 1024 ? tiles : 1024;   // >= every reducing grid
     h->part_stride = mx * kMaxPartSlots;
   }

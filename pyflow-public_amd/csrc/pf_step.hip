@@ -11,41 +11,9 @@
 // contractions.
 #include <hip/hip_runtime.h>
 
-#include "pf_box.h"
+#include "pf_kernels.h"
 
 namespace pf {
-
-template <typename T>
-struct StepCoef {
-  T dxi, dyi, dzi;
-  T c2;               // 2*VISC/rho          (velocity.py:250)
-  T exy, exz, eyz;    // interp_sc_**(VISC)/rho  (velocity.py:252-260)
-  T half_dt;          // 0.5*simDt           (advancer.py:81)
-  T dt6;              // simDt/6.0           (advancer.py:100)
-  T dt;               // simDt               (advancer.py:55)
-};
-
-// Scalars rounded exactly as the reference's tensor ops round them: VISC_P is
-// ones*mu in the field dtype (domain.py:130-131); edge viscosities are
-// 0.5*0.5*(((V+V)+V)+V) (metric_staggered.py:127-144) divided by rho.
-template <typename T>
-StepCoef<T> make_step_coef(const pf_box* h, double mu, double rho, double dt) {
-  StepCoef<T> c;
-  c.dxi = (T)(1.0 / h->dx);
-  c.dyi = (T)(1.0 / h->dy);
-  c.dzi = (T)(1.0 / h->dz);
-  const T V = (T)mu;
-  const T R = (T)rho;
-  c.c2 = ((T)2.0 * V) / R;
-  const T edge = (T)0.25 * (((V + V) + V) + V);
-  c.exy = edge / R;
-  c.exz = edge / R;
-  c.eyz = edge / R;
-  c.half_dt = (T)(0.5 * dt);
-  c.dt6 = (T)(dt / 6.0);
-  c.dt = (T)dt;
-  return c;
-}
 
 template <typename T>
 struct Sfs {
@@ -229,11 +197,12 @@ __device__ __forceinline__ void stage_update(const Geom& g, uint32_t idx, int i,
 // One RK stage (or Euler step, or a bare RHS evaluation) of the forward RHS;
 // Y is the stage input state (padded, interior origin).
 template <typename T, int KIND, bool SFS>
-__global__ void __launch_bounds__(kBlock) rk_stage_kernel(Geom g, CVec3<T> Y, CVec3<T> Un,
-                                                          Vec3<T> acc, Vec3<T> Yout, Vec3<T> Uout,
-                                                          StepCoef<T> k, Sfs<T> sfs) {
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
-       idx += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void rk_stage_body(const Geom& g, const CVec3<T>& Y, const CVec3<T>& Un,
+                                              const Vec3<T>& acc, const Vec3<T>& Yout,
+                                              const Vec3<T>& Uout, const StepCoef<T>& k,
+                                              const Sfs<T>& sfs, Blk blk) {
+  for (uint32_t idx = blk.id * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += blk.n * blockDim.x) {
     int i, j, kk;
     decode(g, idx, i, j, kk);
     const int64_t c = i * g.psx + j * g.psy + kk;
@@ -241,6 +210,13 @@ __global__ void __launch_bounds__(kBlock) rk_stage_kernel(Geom g, CVec3<T> Y, CV
     rhs_cell<T, SFS>(Y.a[0], Y.a[1], Y.a[2], c, g.psx, g.psy, k, sfs, r[0], r[1], r[2]);
     stage_update<T, KIND>(g, idx, i, j, kk, c, r, Un, acc, Yout, Uout, k);
   }
+}
+
+template <typename T, int KIND, bool SFS>
+__global__ void __launch_bounds__(kBlock) rk_stage_kernel(Geom g, CVec3<T> Y, CVec3<T> Un,
+                                                          Vec3<T> acc, Vec3<T> Yout, Vec3<T> Uout,
+                                                          StepCoef<T> k, Sfs<T> sfs) {
+  rk_stage_body<T, KIND, SFS>(g, Y, Un, acc, Yout, Uout, k, sfs, this_blk());
 }
 
 // ---------------------------------------------------------------------------
@@ -512,6 +488,49 @@ int adj_euler_impl(pf_box* h, const void* const U[3], void* const Ua[3], double 
   const size_t np = (size_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4) * sizeof(T);
   for (int q = 0; q < 3; ++q) PF_HIP(hipMemcpyAsync(Ua[q], B[q], np, hipMemcpyDeviceToDevice, s));
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batched RK4 stages: stage s of every box of the list (rk4_impl's buffer
+// routing: Y2 -> ut, Y3 -> box buffer, Y4 -> ut, u* -> u).
+template <typename T, int KIND>
+__global__ void __launch_bounds__(kBlock) batch_stage_kernel(const BoxDesc<T>* __restrict__ D,
+                                                             BatchList L) {
+  const int i = find_box(L, (int)blockIdx.x);
+  const BoxDesc<T>& d = D[L.box[i]];
+  const Blk blk{(int)blockIdx.x - L.begin[i], L.begin[i + 1] - L.begin[i]};
+  const int64_t o = d.g.poff;
+  CVec3<T> y, un;
+  Vec3<T> acc, yo, uo;
+  for (int q = 0; q < 3; ++q) {
+    T* const src = KIND == kStage1 ? d.u[q] : (KIND == kStage3 ? d.yp[q] : d.ut[q]);
+    y.a[q] = src + o;
+    un.a[q] = d.u[q] + o;
+    acc.a[q] = d.acc[q];
+    yo.a[q] = (KIND == kStage2 ? d.yp[q] : d.ut[q]) + o;
+    uo.a[q] = d.u[q] + o;
+  }
+  const Sfs<T> none = {{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
+  rk_stage_body<T, KIND, false>(d.g, y, un, acc, yo, uo, d.k, none, blk);
+}
+
+int batch_stage_launch(int dtype, int stage, const void* desc, const BatchList& L, int nblocks,
+                       hipStream_t s) {
+  if (nblocks <= 0) return 0;
+  auto go = [&](auto zero) -> int {
+    using T = decltype(zero);
+    const BoxDesc<T>* D = (const BoxDesc<T>*)desc;
+    switch (stage) {
+      case 1: hipLaunchKernelGGL((batch_stage_kernel<T, kStage1>), dim3(nblocks), dim3(kBlock), 0, s, D, L); break;
+      case 2: hipLaunchKernelGGL((batch_stage_kernel<T, kStage2>), dim3(nblocks), dim3(kBlock), 0, s, D, L); break;
+      case 3: hipLaunchKernelGGL((batch_stage_kernel<T, kStage3>), dim3(nblocks), dim3(kBlock), 0, s, D, L); break;
+      case 4: hipLaunchKernelGGL((batch_stage_kernel<T, kStage4>), dim3(nblocks), dim3(kBlock), 0, s, D, L); break;
+      default: return fail(PF_EINVAL, "batch_stage_launch: stage is 1..4");
+    }
+    PF_LAUNCH_CHECK("batch_stage_kernel");
+    return 0;
+  };
+  return dtype == PF_F64 ? go(0.0) : go(0.0f);
 }
 
 }  // namespace pf

@@ -65,6 +65,11 @@ SIGNATURES = [
     ('pf_fold_halo', _I, [_P, _P, _P]),
     ('pf_closure_vjp_waves', _I, [_P]),
     ('pf_closure_vjp', _I, [_P, _I, _P, _P, _P, _P, _PP, _P, _P, _P, _P, _PD, _PD, _P]),
+    ('pf_batch_create', _I, [ctypes.POINTER(_P), _I, _I]),
+    ('pf_batch_destroy', _I, [_P]),
+    ('pf_batch_set', _I, [_P, _I, _PP, _PP, _PP, _PD]),
+    ('pf_batch_step', _I, [_P, _PD, _PI, _P]),
+    ('pf_batch_chunk_iterations', _I, []),
     ('pf_profile_enable', _I, [_P, _I]),
     ('pf_profile_read', _I, [_P, _I, _PD, ctypes.POINTER(ctypes.c_int64)]),
 ]
@@ -431,3 +436,66 @@ class Box:
                                    ctypes.c_void_p(gbar9.data_ptr()), self.padded(f3[0], 'fu'),
                                    self.padded(f3[1], 'fv'), self.padded(f3[2], 'fw'),
                                    self._dparams(dparams), wp, self.stream()), 'pf_closure_vjp')
+
+
+class Batch:
+    """Owner of one pf_batch: batched, graph-captured time steps of many
+    resident boxes (all of one dtype).  set() takes, per box, the Box and its
+    fields; step() advances all of them by one time step."""
+
+    FIELDS = ('u', 'v', 'w', 'ut', 'vt', 'wt', 'DP', 'p')
+
+    def __init__(self, capacity, dtype=torch.float32):
+        require_gpu()
+        self.capacity = int(capacity)
+        self.dtype = dtype
+        h = ctypes.c_void_p()
+        check(lib().pf_batch_create(ctypes.byref(h), self.capacity, dtype_code(dtype)),
+              'pf_batch_create')
+        self.h = h
+        self.n = 0
+        self._keep = None
+
+    def close(self):
+        if getattr(self, 'h', None) is not None and self.h.value:
+            lib().pf_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, members):
+        """members: list of (Box, dict of the 8 padded fields + 'src', params)
+        with params = (dt, mu, rho, tol, maxit)."""
+        n = len(members)
+        if n > self.capacity:
+            raise ValueError('batch holds at most %d boxes' % self.capacity)
+        boxes = (ctypes.c_void_p * max(n, 1))()
+        fields = (ctypes.c_void_p * max(8 * n, 1))()
+        srcs = (ctypes.c_void_p * max(n, 1))()
+        prm = (ctypes.c_double * max(5 * n, 1))()
+        for i, (box, f, params) in enumerate(members):
+            if box.dtype != self.dtype:
+                raise ValueError('box dtype %s differs from the batch dtype %s' % (box.dtype, self.dtype))
+            boxes[i] = box.h.value
+            for q, name in enumerate(self.FIELDS):
+                fields[8 * i + q] = box.padded(f[name], name).value
+            srcs[i] = box.interior(f['src'], 'src').value
+            for q in range(5):
+                prm[5 * i + q] = float(params[q])
+        check(lib().pf_batch_set(self.h, n, ctypes.cast(boxes, _PP), ctypes.cast(fields, _PP),
+                                 ctypes.cast(srcs, _PP), ctypes.cast(prm, _PD)), 'pf_batch_set')
+        self.n = n
+        self._keep = members
+
+    def step(self, device=None):
+        """One time step of every box; returns (n x 8 list of monitor values,
+        graph replays of the Krylov chunk)."""
+        out = (ctypes.c_double * max(8 * self.n, 1))()
+        chunks = ctypes.c_int(0)
+        check(lib().pf_batch_step(self.h, ctypes.cast(out, _PD), ctypes.byref(chunks),
+                                  stream_ptr(device)), 'pf_batch_step')
+        return [list(out[8 * i:8 * i + 8]) for i in range(self.n)], chunks.value

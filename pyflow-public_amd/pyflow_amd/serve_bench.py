@@ -1,0 +1,76 @@
+"""Serving benchmark: a Poisson stream of decaying-turbulence requests served
+by the continuous-batching server, and the same requests run one after
+another through the plain single-box path.
+
+Workload: requests arrive at `--rate` per second (default 50); box size
+uniform over {32, 48, 64, 96, 128}^3, step count uniform in [10, 200],
+k0 in {2, 4, 8}, nu uniform in [0.002, 0.02], each with its own seed; FP32
+storage with FP64 Krylov reductions, BiCGStab to 1e-8; admission holds at
+most `--hbm-gb` (default 200) GB resident.
+
+Prints one JSON line: aggregate Mcell-steps/s and p50/p99 per-step latency
+of both runs, their ratio, and the number of requests whose final fields
+are bit-identical in the two runs.
+
+    python -m pyflow_amd.serve_bench --requests 48
+"""
+
+import argparse
+import json
+import sys
+
+import numpy as np
+import torch
+
+
+def main(argv=None):
+    from . import serving
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--requests', type=int, default=48)
+    ap.add_argument('--rate', type=float, default=50.0)
+    ap.add_argument('--seed', type=int, default=7)
+    ap.add_argument('--capacity', type=int, default=256)
+    ap.add_argument('--hbm-gb', type=float, default=200.0)
+    ap.add_argument('--tol', type=float, default=1e-8)
+    ap.add_argument('--dtype', choices=['f32', 'f64'], default='f32')
+    ap.add_argument('--no-sequential', action='store_true')
+    ap.add_argument('--no-batched', action='store_true')
+    a = ap.parse_args(argv)
+    dtype = torch.float32 if a.dtype == 'f32' else torch.float64
+    # warm both paths once (graph capture, kernel loads), untimed
+    warm = serving.make_requests(4, rate=1e9, steps=(2, 2), seed=a.seed + 1000, tol=a.tol)
+    serving.BatchServer(dtype=dtype, capacity=8).serve(warm, realtime=False)
+    warm = serving.make_requests(4, rate=1e9, steps=(2, 2), seed=a.seed + 1000, tol=a.tol)
+    serving.SequentialRunner(dtype=dtype).serve(warm, realtime=False)
+
+    out = {'workload': 'Poisson arrivals %.0f/s, N^3 in {32,48,64,96,128}, 10-200 steps, '
+                       'k0 in {2,4,8}, nu in [0.002,0.02], %s, BiCGStab %g'
+                       % (a.rate, a.dtype, a.tol)}
+    reqs = serving.make_requests(a.requests, rate=a.rate, seed=a.seed, tol=a.tol)
+    if not a.no_batched:
+        srv = serving.BatchServer(dtype=dtype, capacity=a.capacity,
+                                  hbm_cap=int(a.hbm_gb * serving.GB))
+        sb = srv.serve(reqs, keep_results=True, realtime=True)
+        sb['max_resident_gb'] = srv.max_resident / serving.GB
+        sb['mean_batch'] = float(np.mean(srv.batch_sizes))
+        sb['server_steps'] = len(srv.step_times)
+        out['batched'] = sb
+    if not a.no_sequential:
+        seq = serving.make_requests(a.requests, rate=a.rate, seed=a.seed, tol=a.tol)
+        ss = serving.SequentialRunner(dtype=dtype).serve(seq, keep_results=True, realtime=True)
+        out['sequential'] = ss
+        if not a.no_batched:
+            out['throughput_ratio'] = out['batched']['mcell_steps_per_s'] / ss['mcell_steps_per_s']
+            same = 0
+            for x, y in zip(reqs, seq):
+                if all(torch.equal(p, q) for p, q in zip(x.result, y.result)):
+                    same += 1
+            out['bit_identical_requests'] = '%d/%d' % (same, len(reqs))
+    for r in reqs:
+        r.result = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())
