@@ -330,31 +330,38 @@ int pf_closure_vjp(pf_box* box, int kind, const void* u, const void* v, const vo
 /*
  * Batched time steps of many resident boxes (the serving path; no
  * counterpart in the reference, which steps one box per process,
- * core/PyFlow.py:352-425).  A batch advances each of its boxes by one
- * Domain.forwardStep (RK4, BiCGStab velCorr, monitor values; closure-free)
- * per pf_batch_step, with every kernel launch covering all boxes and the
- * launches captured into hipGraphs that are replayed while the composition
- * is unchanged.  Each box keeps the block decomposition of its single-box
- * launches, so its results are bit-identical to pf_rk4_step + pf_vel_corr +
- * pf_stats on the same box, alone or in any mix.  All boxes of a batch
- * share its dtype; a box may be in one batch at a time (its workspace is
- * used by the batched kernels).
- *   pf_batch_set: the n boxes of the next steps.  fields: n x 8 device
- *     pointers (u, v, w, ut, vt, wt, DP, p; padded), sources: n interior
- *     arrays (left holding the scaled Poisson source), params: n x 5 values
- *     (dt, mu, rho, tol, maxit).  Synchronous; invalidates the graphs.
- *   pf_batch_step: one step of every box.  out (may be NULL): n x 8 doubles
- *     max|u|, max|v|, max|w|, sum(u^2+v^2+w^2), max|div u|, the solver's
- *     max residual, its iterations and status; out_chunks: graph replays of
- *     pf_batch_chunk_iterations() iterations each.  Ordered after the
- *     caller's work on `stream`; returns when the step is complete.
+ * core/PyFlow.py:352-425).  Each box of a batch is idle or inside a time
+ * step (one Domain.forwardStep: RK4, BiCGStab velCorr, monitor values;
+ * closure-free).  Every kernel launch covers all the boxes it concerns;
+ * the Krylov iterations run from a hipGraph of pf_batch_chunk_iterations()
+ * iterations over all resident boxes, captured once per composition.  Each
+ * box keeps the block decomposition of its single-box launches, so its
+ * results are bit-identical to pf_rk4_step + pf_vel_corr + pf_stats on the
+ * same box, alone or in any mix.  All boxes of a batch share its dtype; a
+ * box may be in one batch at a time (its workspace is used by the batched
+ * kernels).
+ *   pf_batch_set: the n resident boxes.  fields: n x 8 device pointers (u, v,
+ *     w, ut, vt, wt, DP, p; padded), sources: n interior arrays (left holding
+ *     the scaled Poisson source), params: n x 5 values (dt, mu, rho, tol,
+ *     maxit).  Boxes already resident keep their phase (a solve in progress
+ *     continues); new boxes join idle.  Synchronous; re-captures the graph.
+ *   pf_batch_advance: starts a step for every box with start[i] != 0 (it must
+ *     be idle), then runs Krylov chunks until at least one running box's solve
+ *     is done, finishes those boxes' steps and returns with finished[i] = 1
+ *     for them and, in out[8i..8i+8) (may be NULL): max|u|, max|v|, max|w|,
+ *     sum(u^2+v^2+w^2), max|div u|, the solver's max residual, iterations and
+ *     status.  Boxes whose solves take longer keep running across calls, so a
+ *     fast box starts its next step while slow ones iterate.  out_chunks:
+ *     graph replays in this call.  Ordered after the caller's work on
+ *     `stream`; returns when the finishing boxes' steps are complete.
  */
 typedef struct pf_batch pf_batch;
 int pf_batch_create(pf_batch** out, int capacity, int dtype);
 int pf_batch_destroy(pf_batch* batch);
 int pf_batch_set(pf_batch* batch, int n, pf_box* const* boxes, void* const* fields,
                  void* const* sources, const double* params);
-int pf_batch_step(pf_batch* batch, double* out, int* out_chunks, void* stream);
+int pf_batch_advance(pf_batch* batch, const int* start, int* finished, double* out,
+                     int* out_chunks, void* stream);
 int pf_batch_chunk_iterations(void);
 
 /*

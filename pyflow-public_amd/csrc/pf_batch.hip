@@ -1,24 +1,30 @@
-// Batched time steps of many independent boxes (the serving path).
+// Batched time steps of many independent boxes (the serving path), with
+// continuous batching at the granularity of Krylov iterations.
 //
-// A pf_batch holds the descriptors of up to `capacity` resident boxes and
-// advances all of them by one time step per pf_batch_step call: the RK4
-// stages, the Poisson source and Krylov start, BiCGStab iterations until
-// every box's solve is done, then the correction, the pressure update and
-// the monitor values.  Every kernel launch covers all boxes at once; each
-// box keeps the block decomposition of its single-box launch, so its
-// results are bit-identical alone and in any mix, and the same as the
-// single-box entry points'.
+// A pf_batch holds the descriptors of up to `capacity` resident boxes.  Each
+// box is idle or inside a time step.  pf_batch_advance
+//   1. starts a step for the boxes the caller names (one batched launch per
+//      kernel over just those boxes: RK4 stages, Poisson source and Krylov
+//      start, the first KA);
+//   2. replays the Krylov chunk graph - kChunk BiCGStab iterations over all
+//      resident boxes, ending with every box's done flag copied to pinned
+//      host memory - until at least one running box's solve is done;
+//   3. finishes the step of those boxes (batched correction, pressure update,
+//      monitor values) and returns which boxes finished.
+// A box whose solve ends early therefore starts its next step while slower
+// solves continue; no box waits for another.  Boxes that are idle or done
+// return at the top of every chunk kernel (their control block says so).
 //
-// The launches of one composition are captured once into three hipGraphs
-// (before the solve; one chunk of kChunk BiCGStab iterations ending with a
-// count of finished solves copied to pinned host memory; after the solve)
-// and replayed every step.  Boxes whose solve is done skip the rest of the
-// chunk on the device (their control block says so), so the graphs do not
-// depend on iteration counts.  Changing the composition (pf_batch_set)
-// re-captures them.
+// Every launch covers its boxes at once and each box keeps the block
+// decomposition of its single-box launch, so its results are bit-identical
+// alone, in any mix and through the single-box entry points.  The chunk
+// graph is captured once per composition (pf_batch_set) and replayed; it
+// does not depend on iteration counts.  Steps start at chunk boundaries,
+// which keeps every box's iteration parity aligned with the graph's.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 
 #include "pf_kernels.h"
@@ -31,13 +37,26 @@ struct List {
   std::vector<int> box, begin;
   int total = 0;
   int tz = 0, ty = 0;
-  int* dev = nullptr;        // box ids then begins, device copy
+  int* dev = nullptr;        // box ids, then begins (n + 1), on the device
   pf::BatchList view() const {
     pf::BatchList L;
     L.box = dev;
     L.begin = dev + box.size();
     L.n = (int)box.size();
     return L;
+  }
+};
+
+// Kernel-family lists of one set of boxes.
+struct Lists {
+  List flat, red, pad;
+  std::vector<List> tiles;
+  std::vector<int> ids;      // the boxes, in order
+  int* ids_dev = nullptr;
+  std::vector<List*> all() {
+    std::vector<List*> v = {&flat, &red, &pad};
+    for (auto& t : tiles) v.push_back(&t);
+    return v;
   }
 };
 
@@ -49,20 +68,20 @@ struct pf_batch {
   int n;
   size_t desc_size;
   void* desc;                 // device BoxDesc<T>[capacity]
+  std::vector<pf_box*> boxes;
+  std::vector<char> running;  // per box: inside a time step
   double* out_dev;            // 8 doubles per box
   double* out_host;           // pinned
-  int* done_dev;
-  int* done_host;             // pinned
+  int* flags_dev;             // done flag per box
+  int* flags_host;            // pinned
   hipStream_t st;
   hipEvent_t ev;
-  List flat, red, pad;
-  std::vector<List> tiles;
-  int* list_mem;
-  size_t list_cap;
-  hipGraphExec_t g_pre, g_chunk, g_post;
-  bool graphs;
+  Lists full, pre, post;
+  int* list_mem;              // device lists: full, then pre, then post regions
+  int* list_stage;            // pinned staging of the three regions
+  size_t list_cap;            // ints per region
+  hipGraphExec_t g_chunk;
   int maxit_max;
-  int last_chunks;            // chunk replays the previous step needed
 };
 
 using namespace pf;
@@ -121,118 +140,132 @@ void add(List& L, int box, int nblocks) {
   L.total += nblocks;
 }
 
-void destroy_graphs(pf_batch* b) {
-  if (b->g_pre) hipGraphExecDestroy(b->g_pre);
-  if (b->g_chunk) hipGraphExecDestroy(b->g_chunk);
-  if (b->g_post) hipGraphExecDestroy(b->g_post);
-  b->g_pre = b->g_chunk = b->g_post = nullptr;
-  b->graphs = false;
-}
-
-std::vector<List*> all_lists(pf_batch* b) {
-  std::vector<List*> v = {&b->flat, &b->red, &b->pad};
-  for (auto& t : b->tiles) v.push_back(&t);
-  return v;
-}
-
-int upload_lists(pf_batch* b) {
-  size_t need = 0;
-  for (List* L : all_lists(b)) need += L->box.size() * 2 + 1;
-  if (need > b->list_cap) {
-    if (b->list_mem) PF_HIP(hipFree(b->list_mem));
-    b->list_mem = nullptr;
-    b->list_cap = 0;
-    PF_HIP(hipMalloc((void**)&b->list_mem, need * sizeof(int)));
-    b->list_cap = need;
+// Lists of the boxes `ids` (indices into b->boxes).
+void build_lists(const pf_batch* b, const std::vector<int>& ids, Lists& out) {
+  out = Lists();
+  out.ids = ids;
+  for (int i : ids) {
+    const pf_box* h = b->boxes[i];
+    add(out.flat, i, launch_blocks(h->g));
+    add(out.red, i, h->nblk);
+    const int64_t np = (int64_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4);
+    const int64_t npb = (np + kBlock - 1) / kBlock;
+    add(out.pad, i, (int)(npb > 65536 ? 65536 : npb));
+    const TileGrid tg = make_tile_grid(h->g, h->tile_rows);
+    List* tl = nullptr;
+    for (auto& t : out.tiles)
+      if (t.tz == tg.tz && t.ty == tg.ty) tl = &t;
+    if (!tl) {
+      out.tiles.push_back(List());
+      tl = &out.tiles.back();
+      tl->tz = tg.tz;
+      tl->ty = tg.ty;
+    }
+    add(*tl, i, tg.nblk);
   }
-  std::vector<int> host;
-  host.reserve(need);
+}
+
+// Ints the lists of n boxes take: each family lists every box once with its
+// first block (2n + 1 per list; flat, red, pad and at most 4 tile shapes),
+// then the id list.
+size_t list_ints(int n) { return (size_t)7 * (2 * (size_t)n + 1) + (size_t)n; }
+
+// Serialise `L` into host[0..) and point its lists at dev[0..); returns ints used.
+size_t pack_lists(Lists& L, int* host, int* dev, size_t cap) {
   size_t off = 0;
-  for (List* L : all_lists(b)) {
-    L->dev = b->list_mem + off;
-    host.insert(host.end(), L->box.begin(), L->box.end());
-    host.insert(host.end(), L->begin.begin(), L->begin.end());
-    host.push_back(L->total);
-    off += L->box.size() * 2 + 1;
+  for (List* l : L.all()) {
+    const size_t need = 2 * l->box.size() + 1;
+    if (off + need > cap) return cap + 1;
+    l->dev = dev + off;
+    for (int v : l->box) host[off++] = v;
+    for (int v : l->begin) host[off++] = v;
+    host[off++] = l->total;
   }
-  if (!host.empty())
-    PF_HIP(hipMemcpy(b->list_mem, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (off + L.ids.size() > cap) return cap + 1;
+  L.ids_dev = dev + off;
+  for (int v : L.ids) host[off++] = v;
+  return off;
+}
+
+// Upload the lists of a set of boxes through the pinned staging area of
+// `region` (0: all boxes, 1: boxes starting a step, 2: boxes finishing one).
+int upload_lists(pf_batch* b, Lists& L, int region) {
+  int* host = b->list_stage + (size_t)region * b->list_cap;
+  int* dev = b->list_mem + (size_t)region * b->list_cap;
+  const size_t used = pack_lists(L, host, dev, b->list_cap);
+  if (used > b->list_cap) return fail(PF_EINVAL, "pf_batch: more than 4 tile shapes in one batch");
+  if (used) PF_HIP(hipMemcpyAsync(dev, host, used * sizeof(int), hipMemcpyHostToDevice, b->st));
   return 0;
 }
 
-int tiles_launch(pf_batch* b, int what, int parity) {
-  for (const List& t : b->tiles)
+int tiles_launch(pf_batch* b, Lists& L, int what, int parity) {
+  for (const List& t : L.tiles)
     if (int rc = batch_krylov_launch(b->dtype, what, parity, b->desc, t.view(), t.total, t.tz,
                                      t.ty, b->st))
       return rc;
   return 0;
 }
 
-int record_pre(pf_batch* b) {
-  if (int rc = batch_ctl_launch(b->dtype, kBatchCtlInit, b->desc, b->n, nullptr, b->st)) return rc;
+int launch_pre(pf_batch* b, Lists& L) {
+  const int m = (int)L.ids.size();
+  if (int rc = batch_ctl_launch(b->dtype, kBatchCtlInit, b->desc, L.ids_dev, m, nullptr, b->st))
+    return rc;
   for (int s = 1; s <= 4; ++s)
-    if (int rc = batch_stage_launch(b->dtype, s, b->desc, b->flat.view(), b->flat.total, b->st))
+    if (int rc = batch_stage_launch(b->dtype, s, b->desc, L.flat.view(), L.flat.total, b->st))
       return rc;
-  if (int rc = batch_krylov_launch(b->dtype, kBatchInit, 0, b->desc, b->red.view(), b->red.total,
+  if (int rc = batch_krylov_launch(b->dtype, kBatchInit, 0, b->desc, L.red.view(), L.red.total,
                                    0, 0, b->st))
     return rc;
-  return tiles_launch(b, kBatchKA, 0);
+  return tiles_launch(b, L, kBatchKA, 0);
+}
+
+int launch_post(pf_batch* b, Lists& L) {
+  const int m = (int)L.ids.size();
+  if (int rc = batch_krylov_launch(b->dtype, kBatchScatter, 0, b->desc, L.pad.view(),
+                                   L.pad.total, 0, 0, b->st))
+    return rc;
+  if (int rc = batch_krylov_launch(b->dtype, kBatchProject, 0, b->desc, L.flat.view(),
+                                   L.flat.total, 0, 0, b->st))
+    return rc;
+  if (int rc = batch_krylov_launch(b->dtype, kBatchStats, 0, b->desc, L.red.view(), L.red.total,
+                                   0, 0, b->st))
+    return rc;
+  return batch_ctl_launch(b->dtype, kBatchFinalize, b->desc, L.ids_dev, m, nullptr, b->st);
 }
 
 int record_chunk(pf_batch* b) {
+  Lists& L = b->full;
   for (int q = 0; q < kChunk; ++q) {
     const int par = q & 1;
-    if (int rc = tiles_launch(b, kBatchKB, par)) return rc;
-    if (int rc = batch_krylov_launch(b->dtype, kBatchKC, par, b->desc, b->red.view(),
-                                     b->red.total, 0, 0, b->st))
+    if (int rc = tiles_launch(b, L, kBatchKB, par)) return rc;
+    if (int rc = batch_krylov_launch(b->dtype, kBatchKC, par, b->desc, L.red.view(), L.red.total,
+                                     0, 0, b->st))
       return rc;
-    if (int rc = tiles_launch(b, kBatchKA, 1 - par)) return rc;
+    if (int rc = tiles_launch(b, L, kBatchKA, 1 - par)) return rc;
   }
-  if (int rc = batch_ctl_launch(b->dtype, kBatchCtlCount, b->desc, b->n, b->done_dev, b->st))
+  if (int rc = batch_ctl_launch(b->dtype, kBatchCtlFlags, b->desc, nullptr, b->n, b->flags_dev,
+                                b->st))
     return rc;
-  PF_HIP(hipMemcpyAsync(b->done_host, b->done_dev, sizeof(int), hipMemcpyDeviceToHost, b->st));
+  PF_HIP(hipMemcpyAsync(b->flags_host, b->flags_dev, sizeof(int) * b->n, hipMemcpyDeviceToHost,
+                        b->st));
   return 0;
 }
 
-int record_post(pf_batch* b) {
-  if (int rc = batch_krylov_launch(b->dtype, kBatchScatter, 0, b->desc, b->pad.view(),
-                                   b->pad.total, 0, 0, b->st))
-    return rc;
-  if (int rc = batch_krylov_launch(b->dtype, kBatchProject, 0, b->desc, b->flat.view(),
-                                   b->flat.total, 0, 0, b->st))
-    return rc;
-  if (int rc = batch_krylov_launch(b->dtype, kBatchStats, 0, b->desc, b->red.view(), b->red.total,
-                                   0, 0, b->st))
-    return rc;
-  if (int rc = batch_ctl_launch(b->dtype, kBatchFinalize, b->desc, b->n, nullptr, b->st))
-    return rc;
-  PF_HIP(hipMemcpyAsync(b->out_host, b->out_dev, sizeof(double) * 8 * b->n,
-                        hipMemcpyDeviceToHost, b->st));
-  return 0;
-}
-
-int capture(pf_batch* b, int (*rec)(pf_batch*), hipGraphExec_t* exec) {
+int capture_chunk(pf_batch* b) {
+  if (b->g_chunk) hipGraphExecDestroy(b->g_chunk);
+  b->g_chunk = nullptr;
   hipGraph_t g = nullptr;
   PF_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-  int rc = rec(b);
+  int rc = record_chunk(b);
   hipError_t e = hipStreamEndCapture(b->st, &g);
   if (rc) {
     if (g) hipGraphDestroy(g);
     return rc;
   }
   if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
-  e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&b->g_chunk, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
-  return 0;
-}
-
-int build_graphs(pf_batch* b) {
-  destroy_graphs(b);
-  if (int rc = capture(b, record_pre, &b->g_pre)) return rc;
-  if (int rc = capture(b, record_chunk, &b->g_chunk)) return rc;
-  if (int rc = capture(b, record_post, &b->g_post)) return rc;
-  b->graphs = true;
   return 0;
 }
 
@@ -253,21 +286,22 @@ int pf_batch_create(pf_batch** out, int capacity, int dtype) {
   b->desc = nullptr;
   b->out_dev = nullptr;
   b->out_host = nullptr;
-  b->done_dev = nullptr;
-  b->done_host = nullptr;
+  b->flags_dev = nullptr;
+  b->flags_host = nullptr;
   b->st = nullptr;
   b->ev = nullptr;
   b->list_mem = nullptr;
-  b->list_cap = 0;
-  b->g_pre = b->g_chunk = b->g_post = nullptr;
-  b->graphs = false;
+  b->list_stage = nullptr;
+  b->list_cap = list_ints(capacity);
+  b->g_chunk = nullptr;
   b->maxit_max = 0;
-  b->last_chunks = 1;
   hipError_t e = hipMalloc(&b->desc, b->desc_size * capacity);
   if (e == hipSuccess) e = hipMalloc((void**)&b->out_dev, sizeof(double) * 8 * capacity);
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->out_host, sizeof(double) * 8 * capacity);
-  if (e == hipSuccess) e = hipMalloc((void**)&b->done_dev, sizeof(int));
-  if (e == hipSuccess) e = hipHostMalloc((void**)&b->done_host, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->flags_dev, sizeof(int) * capacity);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->flags_host, sizeof(int) * capacity);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->list_mem, sizeof(int) * 3 * b->list_cap);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->list_stage, sizeof(int) * 3 * b->list_cap);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
   if (e != hipSuccess) {
@@ -281,13 +315,14 @@ int pf_batch_create(pf_batch** out, int capacity, int dtype) {
 int pf_batch_destroy(pf_batch* b) {
   if (!b) return 0;
   if (b->st) hipStreamSynchronize(b->st);
-  destroy_graphs(b);
+  if (b->g_chunk) hipGraphExecDestroy(b->g_chunk);
   if (b->desc) hipFree(b->desc);
   if (b->out_dev) hipFree(b->out_dev);
   if (b->out_host) hipHostFree(b->out_host);
-  if (b->done_dev) hipFree(b->done_dev);
-  if (b->done_host) hipHostFree(b->done_host);
+  if (b->flags_dev) hipFree(b->flags_dev);
+  if (b->flags_host) hipHostFree(b->flags_host);
   if (b->list_mem) hipFree(b->list_mem);
+  if (b->list_stage) hipHostFree(b->list_stage);
   if (b->ev) hipEventDestroy(b->ev);
   if (b->st) hipStreamDestroy(b->st);
   delete b;
@@ -301,17 +336,19 @@ int pf_batch_set(pf_batch* b, int n, pf_box* const* boxes, void* const* fields,
   if (n > 0 && (!boxes || !fields || !sources || !params))
     return fail(PF_EINVAL, "pf_batch_set: NULL array");
   PF_HIP(hipStreamSynchronize(b->st));
-  destroy_graphs(b);
-  b->flat = List();
-  b->red = List();
-  b->pad = List();
-  b->tiles.clear();
-  b->maxit_max = 0;
+  // boxes keep their phase across compositions (a running solve continues)
+  std::unordered_map<const pf_box*, char> was;
+  for (int i = 0; i < b->n; ++i) was[b->boxes[i]] = b->running[i];
   std::vector<char> host(b->desc_size * (n > 0 ? n : 1));
+  std::vector<pf_box*> bx(n);
+  std::vector<char> run(n, 0);
+  int maxit = 0;
   for (int i = 0; i < n; ++i) {
     pf_box* h = boxes[i];
     if (int rc = check_box(h)) return rc;
     if (h->dtype != b->dtype) return fail(PF_EINVAL, "pf_batch_set: box dtype differs from the batch");
+    for (int j = 0; j < i; ++j)
+      if (bx[j] == h) return fail(PF_EINVAL, "pf_batch_set: a box appears twice");
     void* const* f = fields + 8 * (size_t)i;
     for (int q = 0; q < 8; ++q)
       if (!f[q]) return fail(PF_EINVAL, "pf_batch_set: NULL field pointer");
@@ -327,57 +364,85 @@ int pf_batch_set(pf_batch* b, int n, pf_box* const* boxes, void* const* fields,
       BoxDesc<float> d = make_desc<float>(h, f, sources[i], prm, out);
       memcpy(host.data() + i * b->desc_size, &d, sizeof(d));
     }
-    if ((int)prm[4] > b->maxit_max) b->maxit_max = (int)prm[4];
-    add(b->flat, i, launch_blocks(h->g));
-    add(b->red, i, h->nblk);
-    const int64_t np = (int64_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4);
-    int64_t npb = (np + kBlock - 1) / kBlock;
-    add(b->pad, i, (int)(npb > 65536 ? 65536 : npb));
-    const TileGrid tg = make_tile_grid(h->g, h->tile_rows);
-    List* tl = nullptr;
-    for (auto& t : b->tiles)
-      if (t.tz == tg.tz && t.ty == tg.ty) tl = &t;
-    if (!tl) {
-      b->tiles.push_back(List());
-      tl = &b->tiles.back();
-      tl->tz = tg.tz;
-      tl->ty = tg.ty;
+    if ((int)prm[4] > maxit) maxit = (int)prm[4];
+    bx[i] = h;
+    auto it = was.find(h);
+    if (it != was.end()) {
+      run[i] = it->second;
+    } else {
+      // a box joins idle: mark its control block done so that the chunk
+      // kernels skip it until its first step starts
+      const int one = 1;
+      PF_HIP(hipMemcpy(&h->ctl->done, &one, sizeof(int), hipMemcpyHostToDevice));
     }
-    add(*tl, i, tg.nblk);
   }
+  b->boxes = bx;
+  b->running = run;
   b->n = n;
-  if (n > 0)
-    PF_HIP(hipMemcpy(b->desc, host.data(), b->desc_size * n, hipMemcpyHostToDevice));
-  return upload_lists(b);
+  b->maxit_max = maxit;
+  if (n > 0) PF_HIP(hipMemcpy(b->desc, host.data(), b->desc_size * n, hipMemcpyHostToDevice));
+  std::vector<int> all(n);
+  for (int i = 0; i < n; ++i) all[i] = i;
+  build_lists(b, all, b->full);
+  if (int rc = upload_lists(b, b->full, 0)) return rc;
+  PF_HIP(hipStreamSynchronize(b->st));
+  if (n > 0) return capture_chunk(b);
+  if (b->g_chunk) hipGraphExecDestroy(b->g_chunk);
+  b->g_chunk = nullptr;
+  return 0;
 }
 
-int pf_batch_step(pf_batch* b, double* out, int* out_chunks, void* stream) {
-  if (!b) return fail(PF_EINVAL, "pf_batch_step: NULL batch");
+int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, int* out_chunks,
+                     void* stream) {
+  if (!b) return fail(PF_EINVAL, "pf_batch_advance: NULL batch");
+  if (out_chunks) *out_chunks = 0;
+  for (int i = 0; i < b->n; ++i) {
+    if (finished) finished[i] = 0;
+    if (start && start[i] && b->running[i])
+      return fail(PF_EINVAL, "pf_batch_advance: a box was started while inside a step");
+  }
   if (b->n == 0) return 0;
-  if (!b->graphs)
-    if (int rc = build_graphs(b)) return rc;
-  // order after the caller's work on its stream
   PF_HIP(hipEventRecord(b->ev, (hipStream_t)stream));
   PF_HIP(hipStreamWaitEvent(b->st, b->ev, 0));
-  PF_HIP(hipGraphLaunch(b->g_pre, b->st));
+  std::vector<int> ids;
+  for (int i = 0; i < b->n; ++i)
+    if (start && start[i]) ids.push_back(i);
+  if (!ids.empty()) {
+    build_lists(b, ids, b->pre);
+    if (int rc = upload_lists(b, b->pre, 1)) return rc;
+    if (int rc = launch_pre(b, b->pre)) return rc;
+    for (int i : ids) b->running[i] = 1;
+  }
+  bool any = false;
+  for (int i = 0; i < b->n; ++i) any = any || b->running[i];
+  if (!any) {
+    PF_HIP(hipStreamSynchronize(b->st));
+    return 0;
+  }
   const int max_chunks = b->maxit_max / kChunk + 2;
   int chunks = 0;
-  // queue the chunks the last step needed, less one, before the first poll
-  int ahead = b->last_chunks - 1;
-  if (ahead < 1) ahead = 1;
+  std::vector<int> fin;
   for (;;) {
-    for (int q = 0; q < ahead; ++q) PF_HIP(hipGraphLaunch(b->g_chunk, b->st));
-    chunks += ahead;
-    ahead = 1;
+    PF_HIP(hipGraphLaunch(b->g_chunk, b->st));
+    ++chunks;
     PF_HIP(hipStreamSynchronize(b->st));
-    if (*b->done_host >= b->n) break;
+    for (int i = 0; i < b->n; ++i)
+      if (b->running[i] && b->flags_host[i]) fin.push_back(i);
+    if (!fin.empty()) break;
     if (chunks > max_chunks)
-      return fail(PF_EHIP, "pf_batch_step: solves did not finish within the iteration cap");
+      return fail(PF_EHIP, "pf_batch_advance: no solve finished within the iteration cap");
   }
-  b->last_chunks = chunks;
-  PF_HIP(hipGraphLaunch(b->g_post, b->st));
+  build_lists(b, fin, b->post);
+  if (int rc = upload_lists(b, b->post, 2)) return rc;
+  if (int rc = launch_post(b, b->post)) return rc;
+  PF_HIP(hipMemcpyAsync(b->out_host, b->out_dev, sizeof(double) * 8 * b->n,
+                        hipMemcpyDeviceToHost, b->st));
   PF_HIP(hipStreamSynchronize(b->st));
-  if (out) memcpy(out, b->out_host, sizeof(double) * 8 * b->n);
+  for (int i : fin) {
+    b->running[i] = 0;
+    if (finished) finished[i] = 1;
+    if (out) memcpy(out + 8 * (size_t)i, b->out_host + 8 * (size_t)i, sizeof(double) * 8);
+  }
   if (out_chunks) *out_chunks = chunks;
   return 0;
 }

@@ -167,10 +167,11 @@ int batch_stage_launch(int dtype, int stage, const void* desc, const BatchList& 
                        hipStream_t s);
 int batch_krylov_launch(int dtype, int what, int parity, const void* desc, const BatchList& L,
                         int nblocks, int tz, int ty, hipStream_t s);
-int batch_ctl_launch(int dtype, int what, const void* desc, int n, int* done, hipStream_t s);
+int batch_ctl_launch(int dtype, int what, const void* desc, const int* ids, int n, int* flags,
+                     hipStream_t s);
 
 enum { kBatchInit = 0, kBatchKA = 1, kBatchKB = 2, kBatchKC = 3, kBatchScatter = 4,
        kBatchProject = 5, kBatchStats = 6 };
-enum { kBatchCtlInit = 0, kBatchCtlCount = 1, kBatchFinalize = 2 };
+enum { kBatchCtlInit = 0, kBatchCtlFlags = 1, kBatchFinalize = 2 };
 
 }  // namespace pf

@@ -667,28 +667,18 @@ __global__ void __launch_bounds__(kBlock) batch_stats_kernel(const BoxDesc<T>* _
 }
 #undef PF_BATCH_BOX
 
-// Per-box control: kBatchCtlInit (one thread per box), kBatchCtlCount
-// (boxes whose solve is done, one block), kBatchFinalize (one block per box:
-// the monitor values and the solver's outcome into d.out).
+// Per-box control over the boxes ids[0..n) (all boxes 0..n when ids is
+// NULL): kBatchCtlInit resets their control blocks for a new solve (one
+// thread per box); kBatchCtlFlags writes each box's done flag to flags[i]
+// (one thread per box); kBatchFinalize writes the monitor values and the
+// solver's outcome into d.out (one block per box).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) batch_ctl_kernel(const BoxDesc<T>* __restrict__ D,
-                                                           int n, int what, int* __restrict__ done) {
-  if (what == kBatchCtlInit) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    KrylovCtl* c = D[b].ctl;
-    c->rho1 = 1.0; c->rho2 = 1.0; c->alpha = 0.0; c->omega = 1.0;
-    c->res0 = 0.0; c->max_res = 0.0; c->tol = D[b].tol;
-    c->iter = 0; c->maxit = D[b].maxit; c->done = 0; c->status = PF_SOLVE_CONVERGED;
-    c->halt = 0;
-  } else if (what == kBatchCtlCount) {
-    double cnt[1] = {0.0};
-    for (int b = threadIdx.x; b < n; b += blockDim.x) cnt[0] += D[b].ctl->done ? 1.0 : 0.0;
-    const int ops[1] = {0};
-    block_reduce<1>(cnt, ops);
-    if (threadIdx.x == 0) done[0] = (int)cnt[0];
-  } else {
-    const BoxDesc<T>& d = D[blockIdx.x];
+                                                           const int* __restrict__ ids, int n,
+                                                           int what, int* __restrict__ flags) {
+  if (what == kBatchFinalize) {
+    const int b = ids ? ids[blockIdx.x] : (int)blockIdx.x;
+    const BoxDesc<T>& d = D[b];
     const int ops[5] = {1, 1, 1, 0, 1};
     double v[5];
     reduce_partials<5>(d.pS, d.nblk, v, ops);
@@ -698,6 +688,19 @@ __global__ void __launch_bounds__(kBlock) batch_ctl_kernel(const BoxDesc<T>* __r
       d.out[6] = (double)d.ctl->iter;
       d.out[7] = (double)d.ctl->status;
     }
+    return;
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = ids ? ids[i] : i;
+  KrylovCtl* c = D[b].ctl;
+  if (what == kBatchCtlInit) {
+    c->rho1 = 1.0; c->rho2 = 1.0; c->alpha = 0.0; c->omega = 1.0;
+    c->res0 = 0.0; c->max_res = 0.0; c->tol = D[b].tol;
+    c->iter = 0; c->maxit = D[b].maxit; c->done = 0; c->status = PF_SOLVE_CONVERGED;
+    c->halt = 0;
+  } else {
+    flags[i] = c->done;
   }
 }
 
@@ -741,14 +744,14 @@ int batch_krylov_launch(int dtype, int what, int parity, const void* desc, const
   return dtype == PF_F64 ? go(0.0) : go(0.0f);
 }
 
-int batch_ctl_launch(int dtype, int what, const void* desc, int n, int* done, hipStream_t s) {
+int batch_ctl_launch(int dtype, int what, const void* desc, const int* ids, int n, int* flags,
+                     hipStream_t s) {
   if (n <= 0) return 0;
   auto go = [&](auto zero) -> int {
     using T = decltype(zero);
     const BoxDesc<T>* D = (const BoxDesc<T>*)desc;
-    const int nb = what == kBatchCtlInit ? (n + kBlock - 1) / kBlock
-                                         : (what == kBatchCtlCount ? 1 : n);
-    hipLaunchKernelGGL(batch_ctl_kernel<T>, dim3(nb), dim3(kBlock), 0, s, D, n, what, done);
+    const int nb = what == kBatchFinalize ? n : (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(batch_ctl_kernel<T>, dim3(nb), dim3(kBlock), 0, s, D, ids, n, what, flags);
     PF_LAUNCH_CHECK("batch_ctl_kernel");
     return 0;
   };

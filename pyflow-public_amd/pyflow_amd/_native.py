@@ -68,7 +68,7 @@ SIGNATURES = [
     ('pf_batch_create', _I, [ctypes.POINTER(_P), _I, _I]),
     ('pf_batch_destroy', _I, [_P]),
     ('pf_batch_set', _I, [_P, _I, _PP, _PP, _PP, _PD]),
-    ('pf_batch_step', _I, [_P, _PD, _PI, _P]),
+    ('pf_batch_advance', _I, [_P, _PI, _PI, _PD, _PI, _P]),
     ('pf_batch_chunk_iterations', _I, []),
     ('pf_profile_enable', _I, [_P, _I]),
     ('pf_profile_read', _I, [_P, _I, _PD, ctypes.POINTER(ctypes.c_int64)]),
@@ -491,11 +491,17 @@ class Batch:
         self.n = n
         self._keep = members
 
-    def step(self, device=None):
-        """One time step of every box; returns (n x 8 list of monitor values,
-        graph replays of the Krylov chunk)."""
-        out = (ctypes.c_double * max(8 * self.n, 1))()
+    def advance(self, start, device=None):
+        """Start a step for the boxes flagged in `start` (idle ones), run until
+        at least one running box finishes its step; returns (indices of the
+        boxes that finished, their monitor rows {i: [8 values]}, chunks)."""
+        n = self.n
+        st = (ctypes.c_int * max(n, 1))(*[1 if x else 0 for x in start])
+        fin = (ctypes.c_int * max(n, 1))()
+        out = (ctypes.c_double * max(8 * n, 1))()
         chunks = ctypes.c_int(0)
-        check(lib().pf_batch_step(self.h, ctypes.cast(out, _PD), ctypes.byref(chunks),
-                                  stream_ptr(device)), 'pf_batch_step')
-        return [list(out[8 * i:8 * i + 8]) for i in range(self.n)], chunks.value
+        check(lib().pf_batch_advance(self.h, ctypes.cast(st, _PI), ctypes.cast(fin, _PI),
+                                     ctypes.cast(out, _PD), ctypes.byref(chunks),
+                                     stream_ptr(device)), 'pf_batch_advance')
+        done = [i for i in range(n) if fin[i]]
+        return done, {i: list(out[8 * i:8 * i + 8]) for i in done}, chunks.value

@@ -5,8 +5,10 @@ another through the plain single-box path.
 Workload: requests arrive at `--rate` per second (default 50); box size
 uniform over {32, 48, 64, 96, 128}^3, step count uniform in [10, 200],
 k0 in {2, 4, 8}, nu uniform in [0.002, 0.02], each with its own seed; FP32
-storage with FP64 Krylov reductions, BiCGStab to 1e-8; admission holds at
-most `--hbm-gb` (default 200) GB resident.
+storage with FP64 Krylov reductions, BiCGStab to a relative L-inf residual
+of 1e-6 (FP32 storage does not reach 1e-8 on every 128^3 field: such solves
+stop at the iteration cap); admission holds at most `--hbm-gb` (default 200)
+GB resident.
 
 Prints one JSON line: aggregate Mcell-steps/s and p50/p99 per-step latency
 of both runs, their ratio, and the number of requests whose final fields
@@ -23,6 +25,11 @@ import numpy as np
 import torch
 
 
+def _same_bits(a, b):
+    """Equal bit patterns (NaNs included)."""
+    return torch.equal(a.contiguous().view(torch.uint8), b.contiguous().view(torch.uint8))
+
+
 def main(argv=None):
     from . import serving
     ap = argparse.ArgumentParser()
@@ -31,7 +38,7 @@ def main(argv=None):
     ap.add_argument('--seed', type=int, default=7)
     ap.add_argument('--capacity', type=int, default=256)
     ap.add_argument('--hbm-gb', type=float, default=200.0)
-    ap.add_argument('--tol', type=float, default=1e-8)
+    ap.add_argument('--tol', type=float, default=1e-6)
     ap.add_argument('--dtype', choices=['f32', 'f64'], default='f32')
     ap.add_argument('--no-sequential', action='store_true')
     ap.add_argument('--no-batched', action='store_true')
@@ -63,9 +70,11 @@ def main(argv=None):
             out['throughput_ratio'] = out['batched']['mcell_steps_per_s'] / ss['mcell_steps_per_s']
             same = 0
             for x, y in zip(reqs, seq):
-                if all(torch.equal(p, q) for p, q in zip(x.result, y.result)):
+                if all(_same_bits(p, q) for p, q in zip(x.result, y.result)):
                     same += 1
             out['bit_identical_requests'] = '%d/%d' % (same, len(reqs))
+        out['nonfinite_requests'] = sum(
+            1 for r in seq if not all(bool(torch.isfinite(t).all()) for t in r.result))
     for r in reqs:
         r.result = None
     print(json.dumps(out))

@@ -6,12 +6,14 @@ a number of time steps, fixed dt at CFL 0.3, the reference's forward step
 (RK4 + BiCGStab velCorr, core/PyFlow.py:352-425 / data/domain.py:342-402).
 
 The server keeps every admitted request resident in HBM and advances all of
-them together: each server step is one pf_batch_step (every kernel launch
-covers all resident boxes; the launches are replayed from hipGraphs captured
-for the current composition).  Requests that arrive while others run wait in
-a FIFO queue and are admitted at the next step boundary as soon as their
-HBM footprint fits under the admission limit and a batch slot is free;
-finished requests leave at the step boundary where their last step ends.
+them together with pf_batch_advance: every kernel launch covers all the
+boxes it concerns, the BiCGStab iterations of all resident boxes run from a
+hipGraph captured for the current composition, and each box starts its next
+time step as soon as its own solve is done (continuous batching at the
+granularity of Krylov iterations: no request waits for a slower one).
+Requests that arrive while others run wait in a FIFO queue and are admitted
+as soon as their HBM footprint fits under the admission limit and a batch
+slot is free; finished requests leave when their last step ends.
 Boxes and field tensors are pooled by size, so steady-state admission does
 not allocate.
 
@@ -223,7 +225,7 @@ class BatchServer:
         else all queued at once).  Returns a summary dict."""
         pending = sorted(requests, key=lambda r: r.arrival)
         queue = []
-        active = []          # (req, slot)
+        active = []          # [req, slot, inside a step, step start time]
         t0 = time.perf_counter()
         i = 0
         dirty = True
@@ -243,7 +245,7 @@ class BatchServer:
                 slot = self.pool.take(req.N)
                 slot.load(req)
                 req.admitted = time.perf_counter()
-                active.append((req, slot))
+                active.append([req, slot, False, None])
                 queue.pop(0)
                 dirty = True
             self.max_resident = max(self.max_resident, self.pool.bytes_resident)
@@ -252,24 +254,33 @@ class BatchServer:
                     time.sleep(max(0.0, min(0.01, pending[i].arrival - now)))
                 continue
             if dirty:
-                self.batch.set([(s.box, s.f, s.params(r)) for r, s in active])
+                self.batch.set([(e[1].box, e[1].f, e[1].params(e[0])) for e in active])
                 dirty = False
+            # every idle resident request starts its next step now
             ts = time.perf_counter()
-            rows, _ = self.batch.step(self.device)
-            dt_step = time.perf_counter() - ts
-            self.step_times.append(dt_step)
+            start = []
+            for e in active:
+                start.append(not e[2])
+                if not e[2]:
+                    e[2], e[3] = True, ts
+            fin, rows, _ = self.batch.advance(start, self.device)
+            te = time.perf_counter()
+            self.step_times.append(te - ts)
             self.batch_sizes.append(len(active))
             still = []
-            for (req, slot), row in zip(active, rows):
-                req.done_steps += 1
-                req.step_latency.append(dt_step)
-                _record(req, row, float(req.cells))
-                if req.done_steps >= req.steps:
-                    _finish(req, slot, keep_results)
-                    self.pool.give(slot)
-                    dirty = True
-                else:
-                    still.append((req, slot))
+            for idx, e in enumerate(active):
+                req, slot = e[0], e[1]
+                if idx in rows:
+                    e[2] = False
+                    req.done_steps += 1
+                    req.step_latency.append(te - e[3])
+                    _record(req, rows[idx], float(req.cells))
+                    if req.done_steps >= req.steps:
+                        _finish(req, slot, keep_results)
+                        self.pool.give(slot)
+                        dirty = True
+                        continue
+                still.append(e)
             active = still
         wall = time.perf_counter() - t0
         return summarize(requests, wall)
