@@ -13,7 +13,8 @@ launch duration measured during the timed steps with start/stop hipEvents
 recorded by the dispatch of every 32nd launch of each Krylov kernel (sampling
 keeps the events' own cost out of `value`); `cpu_baseline` times the CPU oracle (a
 torch-CPU FP64 restatement of the reference's op sequence) on a bounded
-sample of the same workload on this host.
+sample of the same workload on this host, and `oracle_error` is the relative
+L2 error of the GPU velocity against the oracle's after those steps.
 """
 
 import argparse
@@ -30,6 +31,13 @@ for p in (ROOT, os.path.join(ROOT, 'pyflow-public_amd')):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+
+# HBM-side bytes per KA launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on this bench's workload (one pass per counter), each scaled by
+# the counter's reading for a plain 512 MiB device copy (FETCH 0.500,
+# WRITE 1.000): (37437.5 KiB / 0.500 + 31468.5 KiB) per dispatch.  Keyed by
+# (grid, dtype); other workloads report null.
+KA_TRAFFIC_BYTES = {(128, 'f64'): int((37437.5 / 0.50003 + 31468.5) * 1024)}
 
 
 def make_config(N, dtype, nu, tol, maxit, seed):
@@ -75,7 +83,23 @@ def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True):
     return dict(wall=wall, its=its, prof=prof, dt=dt, stats=st, u0=(u0, v0, w0))
 
 
+def gpu_fields(N, dtype, nu, tol, maxit, steps, seed, dt):
+    """Interior velocity after `steps` steps from the initial field (for the
+    field error against the oracle)."""
+    import torch
+    from pyflow_amd import PyFlow
+    cfg = make_config(N, dtype, nu, tol, maxit, seed)
+    cfg.simDt = dt
+    comms, decomp, geometry, metric, D, t0 = PyFlow.setup(cfg)
+    for _ in range(steps):
+        D.forwardStep(dt)
+    torch.cuda.synchronize()
+    return [s.interior().double().cpu() for s in (D.state_u_P, D.state_v_P, D.state_w_P)]
+
+
 def cpu_baseline(u0, v0, w0, N, nu, dt, tol, maxit, steps=1):
+    """The CPU oracle (torch-CPU FP64) on `steps` steps of the same workload;
+    returns its throughput and final interior velocity."""
     import torch
     from oracle import pyflow_oracle as O
     cores = max(1, min(16, len(os.sched_getaffinity(0))))
@@ -88,7 +112,21 @@ def cpu_baseline(u0, v0, w0, N, nu, dt, tol, maxit, steps=1):
         B.stats()
     wall = time.perf_counter() - t
     return dict(value=N ** 3 * steps / wall / 1e6, wall=wall, cores=cores,
-                iters=B.poisson.iterations)
+                iters=B.poisson.iterations,
+                fields=[B.u.interior().clone(), B.v.interior().clone(), B.w.interior().clone()])
+
+
+def calibrate_copy(mib):
+    """A plain device copy of known size (the reference point for the HBM byte
+    counters): 5 copies of `mib` MiB."""
+    import torch
+    n = int(mib) << 20
+    a = torch.empty(n, dtype=torch.uint8, device='cuda')
+    b = torch.ones(n, dtype=torch.uint8, device='cuda')
+    for _ in range(5):
+        a.copy_(b)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_bytes': n, 'copies': 5}))
 
 
 def main():
@@ -102,7 +140,9 @@ def main():
     ap.add_argument('--tol', type=float, default=None)
     ap.add_argument('--maxit', type=int, default=3000)
     ap.add_argument('--seed', type=int, default=2024)
-    ap.add_argument('--cpu-steps', type=int, default=12)
+    ap.add_argument('--cpu-steps', type=int, default=10)
+    ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
+                    help='only run 5 device copies of MIB MiB (byte-counter reference) and exit')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-profile', action='store_true')
     ap.add_argument('--profile-period', type=int, default=32,
@@ -112,6 +152,9 @@ def main():
         print('bench.py measures one GPU: this build runs one periodic box per GPU and has no '
               'multi-GPU path (requested --gpus %d)' % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
     import torch
     dtype = torch.float64 if a.dtype == 'f64' else torch.float32
     w = 8 if a.dtype == 'f64' else 4
@@ -150,17 +193,25 @@ def main():
             out['roofline'] = {'kernel': 'bicg_tile_kernel<KA>', 'bound': 'hbm',
                                'achieved': per['KA']['GB/s'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                                'frac': round(per['KA']['GB/s'] / HBM_PEAK_GBS, 4),
-                               'traffic': None,
+                               'traffic': KA_TRAFFIC_BYTES.get((a.N, a.dtype)),
                                'bytes_per_launch': per['KA']['bytes_per_launch']}
     if not a.no_cpu:
         u0, v0, w0 = r['u0']
-        cb = cpu_baseline(u0, v0, w0, a.N, a.nu, r['dt'], tol, a.maxit, a.cpu_steps)
+        otol = 1e-10   # the oracle is the reference's FP64 path at the FP64 tolerance
+        cb = cpu_baseline(u0, v0, w0, a.N, a.nu, r['dt'], otol, a.maxit, a.cpu_steps)
         out['cpu_baseline'] = {'value': round(cb['value'], 5), 'unit': 'Mcell-steps/s',
                                'cores': cb['cores'], 'kind': 'port',
-                               'sample': '%d time step(s) of the same %d^3 FP64 workload from the '
-                                         'same initial field (CPU oracle, torch-CPU FP64, %d '
-                                         'threads; %d BiCGStab iterations, %.1f s)'
-                                         % (a.cpu_steps, a.N, cb['cores'], cb['iters'], cb['wall'])}
+                               'sample': '%d time step(s) of the same %d^3 workload in FP64 from '
+                                         'the same initial field (CPU oracle, torch-CPU FP64, %d '
+                                         'threads; %d BiCGStab iterations in its last step, '
+                                         '%.1f s)' % (a.cpu_steps, a.N, cb['cores'], cb['iters'],
+                                                      cb['wall'])}
+        # field error of the GPU path against the oracle after the same steps
+        gf = gpu_fields(a.N, dtype, a.nu, tol, a.maxit, a.cpu_steps, a.seed, r['dt'])
+        errs = [float(torch.linalg.norm(x - y) / torch.linalg.norm(y))
+                for x, y in zip(gf, cb['fields'])]
+        out['oracle_error'] = {'steps': a.cpu_steps, 'rel_l2_uvw': [float('%.3e' % e) for e in errs],
+                               'gpu_tol': tol, 'oracle_tol': otol}
     print(json.dumps(out))
 
 
