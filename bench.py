@@ -35,9 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM-side bytes per KA launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this bench's workload (one pass per counter), each scaled by
 # the counter's reading for a plain 512 MiB device copy (FETCH 0.500,
-# WRITE 1.000): (37437.5 KiB / 0.500 + 31468.5 KiB) per dispatch.  Keyed by
+# WRITE 1.000): (38081.5 KiB / 0.500 + 31987.1 KiB) per dispatch.  Keyed by
 # (grid, dtype); other workloads report null.
-KA_TRAFFIC_BYTES = {(128, 'f64'): int((37437.5 / 0.50003 + 31468.5) * 1024)}
+KA_TRAFFIC_BYTES = {(128, 'f64'): int((38081.5 / 0.50003 + 31987.1) * 1024)}
 
 
 def make_config(N, dtype, nu, tol, maxit, seed):
@@ -69,18 +69,21 @@ def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True):
         box.profile_enable(profile)
     torch.cuda.synchronize()
     its = []
+    status = []
     t = time.perf_counter()
     for _ in range(steps):
         D.forwardStep(dt)
         D.stats()
         its.append(D.poisson.iterations)
+        status.append(D.poisson.status)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t
     prof = box.profile_read() if profile else None
     if profile:
         box.profile_enable(False)
     st = D.stats()
-    return dict(wall=wall, its=its, prof=prof, dt=dt, stats=st, u0=(u0, v0, w0))
+    return dict(wall=wall, its=its, prof=prof, dt=dt, stats=st, u0=(u0, v0, w0),
+                res=D.max_resP, status={k: status.count(k) for k in set(status)})
 
 
 def gpu_fields(N, dtype, nu, tol, maxit, steps, seed, dt):
@@ -176,8 +179,11 @@ def main():
                                'Passot-Pouquet k0=4 u_rms=1, nu=%g, RK4, BiCGStab tol %g'
                                % (a.N, a.nu, tol),
                    'grid': a.N, 'nu': a.nu, 'tol': tol, 'dt': r['dt'],
-                   'bicgstab_iters_mean': round(kbar, 2), 'parallelism': 'single GPU'},
+                   'bicgstab_iters_mean': round(kbar, 2),
+                   'bicgstab_last_residual': float('%.3e' % r['res']),
+                   'bicgstab_status': r['status'], 'parallelism': 'single GPU'},
         'hbm_gbs_algorithmic': round(alg_bytes / r['wall'] / 1e9, 2),
+        'step_roofline_frac': round(alg_bytes / r['wall'] / 1e9 / HBM_PEAK_GBS, 4),
     }
     if r['prof']:
         per = {}

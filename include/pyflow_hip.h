@@ -52,6 +52,12 @@ extern "C" {
 #define PF_SOLVE_MAXIT       1   /* iteration cap reached                    */
 #define PF_SOLVE_BREAKDOWN_A 2   /* <v, r_hat> == 0  (pressure.py:247-248)   */
 #define PF_SOLVE_BREAKDOWN_B 3   /* <t, t> == 0      (pressure.py:266-267)   */
+/* rho2*omega == 0: beta (pressure.py:233) would divide by zero and every later
+ * iterate would be NaN; the solve stops with the last finite iterate.  This
+ * happens once <r, r_hat> has decayed to the rounding floor of the inner
+ * product and lands on exactly 0 (long solves, e.g. 512^3 beyond ~900
+ * iterations).                                                              */
+#define PF_SOLVE_BREAKDOWN_RHO 4
 
 /* field layouts */
 #define PF_LAYOUT_PADDED   0
@@ -375,6 +381,19 @@ int pf_batch_chunk_iterations(void);
  */
 int pf_profile_enable(pf_box* box, int period);
 int pf_profile_read(pf_box* box, int kernel, double* total_ms, int64_t* launches);
+
+/*
+ * Residual history of the BiCGStab solves of one box: the value the
+ * reference computes at pressure.py:279-280 after every iteration (and prints
+ * there when its commented-out line 289 is enabled).  pf_residual_history_enable
+ * allocates room for `capacity` iterations on the device (0 frees it); every
+ * later pf_bicgstab_solve / pf_vel_corr on the box then stores
+ * max|r_k|/max|r0| for k = 1 .. min(iterations, capacity), and
+ * pf_residual_history copies the last solve's values into out[0..*count).
+ * Synchronizes the stream.
+ */
+int pf_residual_history_enable(pf_box* box, int capacity);
+int pf_residual_history(pf_box* box, double* out, int capacity, int* count, void* stream);
 
 #ifdef __cplusplus
 }

@@ -567,7 +567,9 @@ def apply_operator(g, lap, A, B):
 class BiCGStab:
     """solver_bicgstab, pressure.py:164-297 (unpreconditioned, x0 = 0,
     r_hat = r0, stop on max|r|/max|r0| <= tol or iter >= max_iterations,
-    silent exit on the two breakdowns)."""
+    silent exit on the two breakdowns).  One deliberate difference: where
+    rho2*omega == 0 the reference divides by zero (numpy scalars: NaN from
+    then on), here the solve stops with the last finite iterate."""
 
     def __init__(self, g, rho, dt, tol, maxit):
         self.g, self.rho, self.dt = g, rho, dt
@@ -591,8 +593,14 @@ class BiCGStab:
         rho1 = 1.0; rho2 = 1.0; alpha = 0.0; omega = 1.0
         it = 0; done = False; max_res = 0.0
         self.history = []
+        self.status = 'ok'
         while not done:                                     # :229-290
             rho1 = torch.sum(self.res * self.res0).item()
+            if rho2 * omega == 0.0:
+                # the reference's numpy scalars give beta = x/0 here and NaN
+                # iterates from then on; stop at the last finite iterate
+                self.status = 'breakdown_rho'
+                break
             beta = alpha * rho1 / (rho2 * omega)
             pin = self.pp[inner]
             pin.sub_(self.vv, alpha=omega)

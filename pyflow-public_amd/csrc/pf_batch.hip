@@ -93,7 +93,7 @@ BoxDesc<T> make_desc(pf_box* h, void* const* f, void* src, const double* prm, do
   // f: u, v, w, ut, vt, wt, DP, p ; prm: dt, mu, rho, tol, maxit
   BoxDesc<T> d;
   d.g = h->g;
-  d.tg = make_tile_grid(h->g, h->tile_rows);
+  d.tg = make_tile_grid(h->g, (int)h->esize, h->tile_rows);
   d.nblk = h->nblk;
   for (int q = 0; q < 3; ++q) {
     d.u[q] = (T*)f[q];
@@ -151,7 +151,7 @@ void build_lists(const pf_batch* b, const std::vector<int>& ids, Lists& out) {
     const int64_t np = (int64_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4);
     const int64_t npb = (np + kBlock - 1) / kBlock;
     add(out.pad, i, (int)(npb > 65536 ? 65536 : npb));
-    const TileGrid tg = make_tile_grid(h->g, h->tile_rows);
+    const TileGrid tg = make_tile_grid(h->g, (int)h->esize, h->tile_rows);
     List* tl = nullptr;
     for (auto& t : out.tiles)
       if (t.tz == tg.tz && t.ty == tg.ty) tl = &t;
@@ -166,9 +166,9 @@ void build_lists(const pf_batch* b, const std::vector<int>& ids, Lists& out) {
 }
 
 // Ints the lists of n boxes take: each family lists every box once with its
-// first block (2n + 1 per list; flat, red, pad and at most 4 tile shapes),
+// first block (2n + 1 per list; flat, red, pad and at most 8 tile shapes),
 // then the id list.
-size_t list_ints(int n) { return (size_t)7 * (2 * (size_t)n + 1) + (size_t)n; }
+size_t list_ints(int n) { return (size_t)11 * (2 * (size_t)n + 1) + (size_t)n; }
 
 // Serialise `L` into host[0..) and point its lists at dev[0..); returns ints used.
 size_t pack_lists(Lists& L, int* host, int* dev, size_t cap) {
@@ -193,7 +193,7 @@ int upload_lists(pf_batch* b, Lists& L, int region) {
   int* host = b->list_stage + (size_t)region * b->list_cap;
   int* dev = b->list_mem + (size_t)region * b->list_cap;
   const size_t used = pack_lists(L, host, dev, b->list_cap);
-  if (used > b->list_cap) return fail(PF_EINVAL, "pf_batch: more than 4 tile shapes in one batch");
+  if (used > b->list_cap) return fail(PF_EINVAL, "pf_batch: more than 8 tile shapes in one batch");
   if (used) PF_HIP(hipMemcpyAsync(dev, host, used * sizeof(int), hipMemcpyHostToDevice, b->st));
   return 0;
 }

@@ -24,6 +24,8 @@ struct KrylovCtl {
   int done;        // set only by KA/KB, whose early returns are then uniform
   int status;       // PF_SOLVE_* code
   int halt;         // set by KC on a <t,t> breakdown; the next KA turns it into done
+  int hist_cap;     // entries of hist (0: no history kept)
+  double* hist;     // hist[k-1] = max|r_k| / max|r0| for k = 1 .. min(iter, hist_cap)
 };
 
 constexpr int kMaxPartSlots = 8;     // doubles per block per reduction
@@ -58,6 +60,8 @@ struct pf_box {
   double* stats_dev;         // 8 doubles
   double* stats_host;        // pinned, 8 doubles
   int last_iters;
+  double* hist;              // per-iteration relative residuals of the last solve (optional)
+  int hist_cap;              // entries of hist
   int chunk;                 // iterations per launch batch between host checks
   int tile_rows;             // block rows of the tiled Krylov kernels: 256*tile_rows threads
   int64_t bytes;             // device bytes owned by the handle

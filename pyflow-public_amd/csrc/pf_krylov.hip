@@ -32,12 +32,14 @@ enum { kPartA = 0, kPartB = 1, kPartC = 2, kPartS = 3 };
 
 // ---------------------------------------------------------------------------
 // Control block
-__global__ void ctl_init_kernel(KrylovCtl* ctl, double tol, int maxit) {
+__global__ void ctl_init_kernel(KrylovCtl* ctl, double tol, int maxit, double* hist,
+                                int hist_cap) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     ctl->rho1 = 1.0; ctl->rho2 = 1.0; ctl->alpha = 0.0; ctl->omega = 1.0;
     ctl->res0 = 0.0; ctl->max_res = 0.0; ctl->tol = tol;
     ctl->iter = 0; ctl->maxit = maxit; ctl->done = 0; ctl->status = PF_SOLVE_CONVERGED;
     ctl->halt = 0;
+    ctl->hist = hist; ctl->hist_cap = hist ? hist_cap : 0;
   }
 }
 
@@ -123,26 +125,38 @@ struct FieldOp {
   const T* __restrict__ a;   // KA: p_old; KB: unused
   const T* __restrict__ v;   // KA: v_old; KB: v
   T c1, c2;                  // KA: (-omega, beta); KB: (-alpha, -)
-  __device__ __forceinline__ T at(int64_t q) const {
-    if (MODE == kModeA) return (fma(v[q], c1, a[q]) * c2) + r[q];
-    return fma(v[q], c1, r[q]);
+  struct Raw {
+    T v, a, r;
+  };
+  // the stored values at a cell (loaded early, combined later)
+  __device__ __forceinline__ Raw load(int64_t q) const {
+    Raw w;
+    w.v = v[q];
+    w.a = MODE == kModeA ? a[q] : (T)0;
+    w.r = r[q];
+    return w;
   }
+  __device__ __forceinline__ T eval(const Raw& w) const {
+    if (MODE == kModeA) return (fma(w.v, c1, w.a) * c2) + w.r;
+    return fma(w.v, c1, w.r);
+  }
+  __device__ __forceinline__ T at(int64_t q) const { return eval(load(q)); }
 };
 
-template <typename T, int MODE, int TZ, int TY>
-__device__ __forceinline__ void bicg_tile_kernel_body(
-    Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,
-    const double* __restrict__ part_in, double* __restrict__ part_out,
-    const T* __restrict__ r, const T* __restrict__ rh, const T* __restrict__ a_old,
-    const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L, Blk blk) {
-  constexpr int NT = TZ * TY;
+// The scalars of KA / KB from the control block and the previous kernel's
+// partials; false when the block must return (solve done, converged,
+// capped or broken down).
+template <typename T, int MODE, int NT>
+__device__ __forceinline__ bool tile_scalars(KrylovCtl* __restrict__ ctl,
+                                             const double* __restrict__ part_in, int nblk_in,
+                                             const Blk& blk, T& c1, T& c2) {
   const KrylovCtl c0 = snapshot(ctl);
-  if (c0.done) return;
-  T c1, c2 = (T)0;
+  if (c0.done) return false;
+  c2 = (T)0;
   if (MODE == kModeA) {
     if (c0.halt) {
       if (lead(blk)) ctl->done = 1;
-      return;
+      return false;
     }
     double red[2];
     const int ops[2] = {0, 1};
@@ -152,7 +166,10 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
       if (lead(blk)) ctl->res0 = red[1];
     } else {
       const double max_res = red[1] / c0.res0;       // pressure.py:279-280
-      if (lead(blk)) ctl->max_res = max_res;
+      if (lead(blk)) {
+        ctl->max_res = max_res;
+        if (c0.iter <= c0.hist_cap) c0.hist[c0.iter - 1] = max_res;
+      }
       const bool cap = c0.iter >= c0.maxit;           // pressure.py:283-287
       const bool conv = max_res <= c0.tol;
       if (cap || conv) {
@@ -160,8 +177,16 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
           ctl->status = conv ? PF_SOLVE_CONVERGED : PF_SOLVE_MAXIT;
           ctl->done = 1;
         }
-        return;
+        return false;
       }
+    }
+    if (c0.rho2 * c0.omega == 0.0) {
+      // beta would be x/0: the reference's iterates turn NaN from here on
+      if (lead(blk)) {
+        ctl->status = PF_SOLVE_BREAKDOWN_RHO;
+        ctl->done = 1;
+      }
+      return false;
     }
     const double beta = c0.alpha * rho1 / (c0.rho2 * c0.omega);   // pressure.py:233
     if (lead(blk)) ctl->rho1 = rho1;
@@ -177,12 +202,24 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
         ctl->status = PF_SOLVE_BREAKDOWN_A;
         ctl->done = 1;
       }
-      return;
+      return false;
     }
     const double alpha = c0.rho1 / gamma;              // pressure.py:249
     if (lead(blk)) ctl->alpha = alpha;
     c1 = (T)(-alpha);
   }
+  return true;
+}
+
+template <typename T, int MODE, int TZ, int TY>
+__device__ __forceinline__ void bicg_tile_kernel_body(
+    Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,
+    const double* __restrict__ part_in, double* __restrict__ part_out,
+    const T* __restrict__ r, const T* __restrict__ rh, const T* __restrict__ a_old,
+    const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L, Blk blk) {
+  constexpr int NT = TZ * TY;
+  T c1, c2;
+  if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, c1, c2)) return;
   const FieldOp<T, MODE> F{r, a_old, v_in, c1, c2};
 
   // XCD-aware block order: the hardware deals consecutive block ids round
@@ -224,7 +261,11 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
   }
   const int64_t hcol = is_halo ? (int64_t)hy * g.isy + hz : 0;
 
-  __shared__ T plane[TY + 2][TZ + 2];
+  // Two LDS planes used alternately: plane x is written to buffer x & 1, so
+  // one barrier per plane suffices (a thread writing buffer (x+1) & 1 has
+  // passed the barrier of plane x, after every read of plane x-1).
+  __shared__ T plane[2][TY + 2][TZ + 2];
+  using Raw = typename FieldOp<T, MODE>::Raw;
   auto xoff = [&](int x) -> int64_t {
     x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
     return (int64_t)x * g.isx;
@@ -232,19 +273,27 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
   T fm = F.at(xoff(x0 - 1) + col);
   T fc = F.at(xoff(x0) + col);
   T hv = is_halo ? F.at(xoff(x0) + hcol) : (T)0;
+  // the raw loads of plane x+1 are in flight while plane x is computed, and
+  // those of plane x+2 are issued before plane x's barrier
+  Raw rn = F.load(xoff(x0 + 1) + col);
+  Raw hrn = {};
+  if (is_halo && x0 + 1 < x1) hrn = F.load(xoff(x0 + 1) + hcol);
   double acc0 = 0.0, acc1 = 0.0;
   for (int x = x0; x < x1; ++x) {
     const int64_t xo = (int64_t)x * g.isx;
-    // next plane: own column and next halo (issued before the barrier)
-    const T fp = F.at(xoff(x + 1) + col);
-    const T hn = (is_halo && x + 1 < x1) ? F.at(xoff(x + 1) + hcol) : (T)0;
+    Raw rnn = {}, hrnn = {};
+    if (x + 2 <= x1) rnn = F.load(xoff(x + 2) + col);
+    if (is_halo && x + 2 < x1) hrnn = F.load(xoff(x + 2) + hcol);
     const T rhc = (MODE == kModeA && active) ? rh[xo + col] : (T)0;
-    plane[ly + 1][lz + 1] = fc;
-    if (is_halo) plane[hslot_y][hslot_z] = hv;
+    const T fp = F.eval(rn);
+    const T hn = (is_halo && x + 1 < x1) ? F.eval(hrn) : (T)0;
+    T(*pl)[TZ + 2] = plane[x & 1];
+    pl[ly + 1][lz + 1] = fc;
+    if (is_halo) pl[hslot_y][hslot_z] = hv;
     __syncthreads();
     if (active) {
-      const T fym = plane[ly][lz + 1], fyp = plane[ly + 2][lz + 1];
-      const T fzm = plane[ly + 1][lz], fzp = plane[ly + 1][lz + 2];
+      const T fym = pl[ly][lz + 1], fyp = pl[ly + 2][lz + 1];
+      const T fzm = pl[ly + 1][lz], fzp = pl[ly + 1][lz + 2];
       const T af = lap7(L, fm, fc, fp, fym, fyp, fzm, fzp);
       f_out[xo + col] = fc;
       af_out[xo + col] = af;
@@ -255,10 +304,11 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
         acc1 += (double)af * (double)af;
       }
     }
-    __syncthreads();
     fm = fc;
     fc = fp;
     hv = hn;
+    rn = rnn;
+    hrn = hrnn;
   }
   if (MODE == kModeA) {
     double accv[1] = {acc0};
@@ -271,6 +321,172 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
     block_reduce<2, NT>(accv, ops2);
     write_partials<2>(part_out, accv, blk.id);
   }
+}
+
+// Two cells of a z row, loaded and stored with one vector access.
+template <typename T>
+struct alignas(2 * sizeof(T)) Pair {
+  T a, b;
+};
+
+// KA / KB with two z-neighbour cells per thread: a 256-thread block owns a
+// TZ x TY tile (TZ/2 threads per row, TY = 512/TZ rows), so the y halo is a
+// quarter of the tile instead of a half, and every load and store of the
+// interior columns is a 2-element vector access.  Used when TZ divides nz
+// (no periodic wrap inside a row).  Per-cell arithmetic is the scalar
+// kernel's; only the order of the per-thread partial sums differs.
+template <typename T, int MODE, int TZ, int TY>
+__device__ __forceinline__ void bicg_tile2_body(
+    Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,
+    const double* __restrict__ part_in, double* __restrict__ part_out,
+    const T* __restrict__ r, const T* __restrict__ rh, const T* __restrict__ a_old,
+    const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L, Blk blk) {
+  constexpr int NZT = TZ / 2;
+  constexpr int NT = NZT * TY;
+  T c1, c2;
+  if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, c1, c2)) return;
+  const FieldOp<T, MODE> F{r, a_old, v_in, c1, c2};
+  using Raw = typename FieldOp<T, MODE>::Raw;
+  struct Raw2 {
+    Raw a, b;
+  };
+  auto ld2 = [&](const T* p, int64_t q) -> Pair<T> {
+    return *reinterpret_cast<const Pair<T>*>(p + q);
+  };
+  auto load2 = [&](int64_t q) -> Raw2 {
+    const Pair<T> v = ld2(F.v, q), r2 = ld2(F.r, q);
+    Pair<T> a = {(T)0, (T)0};
+    if (MODE == kModeA) a = ld2(F.a, q);
+    Raw2 w;
+    w.a.v = v.a; w.a.a = a.a; w.a.r = r2.a;
+    w.b.v = v.b; w.b.a = a.b; w.b.r = r2.b;
+    return w;
+  };
+  auto eval2 = [&](const Raw2& w) -> Pair<T> { return Pair<T>{F.eval(w.a), F.eval(w.b)}; };
+
+  const int nb = blk.n;
+  int b = blk.id;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  const int ty_i = b % tg.nty;
+  const int rest = b / tg.nty;
+  const int tz_i = rest % tg.ntz;
+  const int cx_i = rest / tg.ntz;
+  const int z0 = tz_i * TZ, y0 = ty_i * TY, x0 = cx_i * tg.xc;
+  const int x1 = min(x0 + tg.xc, g.nx);
+
+  const int t = threadIdx.x;
+  const int lz = t % NZT, ly = t / NZT;
+  const int yr = y0 + ly;
+  const bool active = yr < g.ny;
+  const int64_t col = (int64_t)(yr % g.ny) * g.isy + (z0 + 2 * lz);
+
+  // halo: threads [0, 2*NZT) load the y-halo rows in pairs, the next 2*TY
+  // threads one z-halo cell each
+  int hslot_y = 0, hslot_z = 0;
+  bool hpair = false, hone = false;
+  int64_t hcol = 0;
+  if (t < 2 * NZT) {
+    hpair = true;
+    hslot_y = (t < NZT) ? 0 : TY + 1;
+    hslot_z = 2 * (t % NZT) + 1;
+    const int hy = (t < NZT) ? (y0 - 1 + g.ny) % g.ny : (y0 + TY) % g.ny;
+    hcol = (int64_t)hy * g.isy + z0 + 2 * (t % NZT);
+  } else if (t < 2 * NZT + 2 * TY) {
+    hone = true;
+    const int u = t - 2 * NZT;
+    hslot_y = u % TY + 1;
+    hslot_z = (u < TY) ? 0 : TZ + 1;
+    const int hy = (y0 + u % TY) % g.ny;
+    const int hz = (u < TY) ? (z0 - 1 + g.nz) % g.nz : (z0 + TZ) % g.nz;
+    hcol = (int64_t)hy * g.isy + hz;
+  }
+  auto loadh = [&](int64_t q) -> Raw2 {
+    if (hpair) return load2(q);
+    Raw2 w = {};
+    w.a = F.load(q);
+    return w;
+  };
+  auto evalh = [&](const Raw2& w) -> Pair<T> {
+    return Pair<T>{F.eval(w.a), hpair ? F.eval(w.b) : (T)0};
+  };
+  const bool is_halo = hpair || hone;
+
+  __shared__ T plane[2][TY + 2][TZ + 2];
+  auto xoff = [&](int x) -> int64_t {
+    x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
+    return (int64_t)x * g.isx;
+  };
+  Pair<T> fm = eval2(load2(xoff(x0 - 1) + col));
+  Pair<T> fc = eval2(load2(xoff(x0) + col));
+  Pair<T> hv = {(T)0, (T)0};
+  if (is_halo) hv = evalh(loadh(xoff(x0) + hcol));
+  Raw2 rn = load2(xoff(x0 + 1) + col);
+  Raw2 hrn = {};
+  if (is_halo && x0 + 1 < x1) hrn = loadh(xoff(x0 + 1) + hcol);
+  double acc0 = 0.0, acc1 = 0.0;
+  const int zl = 2 * lz + 1;   // LDS column of the first cell
+  for (int x = x0; x < x1; ++x) {
+    const int64_t xo = (int64_t)x * g.isx;
+    Raw2 rnn = {}, hrnn = {};
+    if (x + 2 <= x1) rnn = load2(xoff(x + 2) + col);
+    if (is_halo && x + 2 < x1) hrnn = loadh(xoff(x + 2) + hcol);
+    Pair<T> rhc = {(T)0, (T)0};
+    if (MODE == kModeA && active) rhc = ld2(rh, xo + col);
+    const Pair<T> fp = eval2(rn);
+    Pair<T> hn = {(T)0, (T)0};
+    if (is_halo && x + 1 < x1) hn = evalh(hrn);
+    T(*pl)[TZ + 2] = plane[x & 1];
+    pl[ly + 1][zl] = fc.a;
+    pl[ly + 1][zl + 1] = fc.b;
+    if (is_halo) {
+      pl[hslot_y][hslot_z] = hv.a;
+      if (hpair) pl[hslot_y][hslot_z + 1] = hv.b;
+    }
+    __syncthreads();
+    if (active) {
+      const T af0 = lap7(L, fm.a, fc.a, fp.a, pl[ly][zl], pl[ly + 2][zl], pl[ly + 1][zl - 1],
+                         fc.b);
+      const T af1 = lap7(L, fm.b, fc.b, fp.b, pl[ly][zl + 1], pl[ly + 2][zl + 1], fc.a,
+                         pl[ly + 1][zl + 2]);
+      *reinterpret_cast<Pair<T>*>(f_out + xo + col) = fc;
+      *reinterpret_cast<Pair<T>*>(af_out + xo + col) = Pair<T>{af0, af1};
+      if (MODE == kModeA) {
+        acc0 += (double)af0 * (double)rhc.a;
+        acc0 += (double)af1 * (double)rhc.b;
+      } else {
+        acc0 += (double)fc.a * (double)af0;
+        acc0 += (double)fc.b * (double)af1;
+        acc1 += (double)af0 * (double)af0;
+        acc1 += (double)af1 * (double)af1;
+      }
+    }
+    fm = fc;
+    fc = fp;
+    hv = hn;
+    rn = rnn;
+    hrn = hrnn;
+  }
+  if (MODE == kModeA) {
+    double accv[1] = {acc0};
+    const int ops1[1] = {0};
+    block_reduce<1, NT>(accv, ops1);
+    write_partials<1>(part_out, accv, blk.id);
+  } else {
+    double accv[2] = {acc0, acc1};
+    const int ops2[2] = {0, 0};
+    block_reduce<2, NT>(accv, ops2);
+    write_partials<2>(part_out, accv, blk.id);
+  }
+}
+
+template <typename T, int MODE, int TZ, int TY>
+__global__ void __launch_bounds__(TZ / 2 * TY) bicg_tile2_kernel(
+    Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,
+    const double* __restrict__ part_in, double* __restrict__ part_out,
+    const T* __restrict__ r, const T* __restrict__ rh, const T* __restrict__ a_old,
+    const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L) {
+  bicg_tile2_body<T, MODE, TZ, TY>(g, tg, nblk_in, ctl, part_in, part_out, r, rh, a_old, v_in,
+                                   f_out, af_out, L, this_blk());
 }
 
 template <typename T, int MODE, int TZ, int TY>
@@ -291,17 +507,22 @@ void launch_tile(const TileGrid& tg, hipStream_t s, hipEvent_t e0, hipEvent_t e1
   hipExtLaunchKernelGGL((bicg_tile_kernel<T, MODE, TZ, TY>), dim3(tg.nblk), dim3(TZ * TY), 0, s,  \
                         e0, e1, 0, g, tg, nblk_in, ctl, pin, pout, r, rh, a_old, v_in, f_out,    \
                         af_out, L)
+#define PF_TILE2(TZ, TY)                                                                     \
+  hipExtLaunchKernelGGL((bicg_tile2_kernel<T, MODE, TZ, TY>), dim3(tg.nblk), dim3(TZ / 2 * TY),  \
+                        0, s, e0, e1, 0, g, tg, nblk_in, ctl, pin, pout, r, rh, a_old, v_in,     \
+                        f_out, af_out, L)
+  // tz * ty == 2 * kBlock: two cells per thread
   const int key = tg.tz * 1000 + tg.ty;
   switch (key) {
     case 64004: PF_TILE(64, 4); break;
-    case 64008: PF_TILE(64, 8); break;
     case 32008: PF_TILE(32, 8); break;
-    case 32016: PF_TILE(32, 16); break;
     case 16016: PF_TILE(16, 16); break;
-    case 16032: PF_TILE(16, 32); break;
-    case 8032: PF_TILE(8, 32); break;
-    default: PF_TILE(8, 64); break;
+    case 64008: PF_TILE2(64, 8); break;
+    case 32016: PF_TILE2(32, 16); break;
+    case 16032: PF_TILE2(16, 32); break;
+    default: PF_TILE(8, 32); break;
   }
+#undef PF_TILE2
 #undef PF_TILE
 }
 
@@ -619,19 +840,24 @@ __global__ void __launch_bounds__(kBlock) batch_init_kernel(const BoxDesc<T>* __
 
 // KA of iteration j (parity = j & 1): p[1-parity] from p[parity], v[parity];
 // KB of iteration j: s, t from r and v[1-parity].
-template <typename T, int MODE, int TZ, int TY>
-__global__ void __launch_bounds__(TZ * TY) batch_tile_kernel(const BoxDesc<T>* __restrict__ D,
-                                                             BatchList L, int parity) {
+template <typename T, int MODE, int TZ, int TY, bool PAIR>
+__global__ void __launch_bounds__(PAIR ? TZ / 2 * TY : TZ * TY) batch_tile_kernel(
+    const BoxDesc<T>* __restrict__ D, BatchList L, int parity) {
   PF_BATCH_BOX;
-  if (MODE == kModeA) {
-    const int old = parity, nw = 1 - parity;
-    bicg_tile_kernel_body<T, kModeA, TZ, TY>(d.g, d.tg, d.nblk, d.ctl, d.pC, d.pA, d.r, d.rh,
-                                             d.pk[old], d.vk[old], d.pk[nw], d.vk[nw], d.L, blk);
-  } else {
-    const int cur = 1 - parity;
-    bicg_tile_kernel_body<T, kModeB, TZ, TY>(d.g, d.tg, d.tg.nblk, d.ctl, d.pA, d.pB, d.r, d.rh,
-                                             (const T*)nullptr, d.vk[cur], d.s, d.t, d.L, blk);
-  }
+  const int old = parity, nw = 1 - parity, cur = 1 - parity;
+  const T* a_old = MODE == kModeA ? d.pk[old] : (const T*)nullptr;
+  const T* v_in = MODE == kModeA ? d.vk[old] : d.vk[cur];
+  T* f_out = MODE == kModeA ? d.pk[nw] : d.s;
+  T* af_out = MODE == kModeA ? d.vk[nw] : d.t;
+  const int nin = MODE == kModeA ? d.nblk : d.tg.nblk;
+  const double* pin = MODE == kModeA ? d.pC : d.pA;
+  double* pout = MODE == kModeA ? d.pA : d.pB;
+  if (PAIR)
+    bicg_tile2_body<T, MODE, TZ, TY>(d.g, d.tg, nin, d.ctl, pin, pout, d.r, d.rh, a_old, v_in,
+                                     f_out, af_out, d.L, blk);
+  else
+    bicg_tile_kernel_body<T, MODE, TZ, TY>(d.g, d.tg, nin, d.ctl, pin, pout, d.r, d.rh, a_old,
+                                           v_in, f_out, af_out, d.L, blk);
 }
 
 template <typename T>
@@ -699,6 +925,7 @@ __global__ void __launch_bounds__(kBlock) batch_ctl_kernel(const BoxDesc<T>* __r
     c->res0 = 0.0; c->max_res = 0.0; c->tol = D[b].tol;
     c->iter = 0; c->maxit = D[b].maxit; c->done = 0; c->status = PF_SOLVE_CONVERGED;
     c->halt = 0;
+    c->hist = nullptr; c->hist_cap = 0;
   } else {
     flags[i] = c->done;
   }
@@ -719,18 +946,21 @@ int batch_krylov_launch(int dtype, int what, int parity, const void* desc, const
       case kBatchStats: hipLaunchKernelGGL(batch_stats_kernel<T>, gb, tb, 0, s, D, L); break;
       case kBatchKA:
       case kBatchKB: {
-#define PF_BTILE(TZ, TY)                                                                       \
-  if (what == kBatchKA)                                                                        \
-    hipLaunchKernelGGL((batch_tile_kernel<T, kModeA, TZ, TY>), gb, dim3(TZ * TY), 0, s, D, L,  \
-                       parity);                                                                \
-  else                                                                                         \
-    hipLaunchKernelGGL((batch_tile_kernel<T, kModeB, TZ, TY>), gb, dim3(TZ * TY), 0, s, D, L,  \
-                       parity)
+#define PF_BTILE(TZ, TY, PAIR)                                                                 \
+  if (what == kBatchKA)                                                                          \
+    hipLaunchKernelGGL((batch_tile_kernel<T, kModeA, TZ, TY, PAIR>), gb,                         \
+                       dim3(PAIR ? TZ / 2 * TY : TZ * TY), 0, s, D, L, parity);                  \
+  else                                                                                           \
+    hipLaunchKernelGGL((batch_tile_kernel<T, kModeB, TZ, TY, PAIR>), gb,                         \
+                       dim3(PAIR ? TZ / 2 * TY : TZ * TY), 0, s, D, L, parity)
         switch (tz * 1000 + ty) {
-          case 64004: PF_BTILE(64, 4); break;
-          case 32008: PF_BTILE(32, 8); break;
-          case 16016: PF_BTILE(16, 16); break;
-          case 8032: PF_BTILE(8, 32); break;
+          case 64004: PF_BTILE(64, 4, false); break;
+          case 32008: PF_BTILE(32, 8, false); break;
+          case 16016: PF_BTILE(16, 16, false); break;
+          case 8032: PF_BTILE(8, 32, false); break;
+          case 64008: PF_BTILE(64, 8, true); break;
+          case 32016: PF_BTILE(32, 16, true); break;
+          case 16032: PF_BTILE(16, 32, true); break;
           default: return fail(PF_EINVAL, "batch_krylov_launch: unsupported tile shape");
         }
 #undef PF_BTILE
@@ -794,7 +1024,8 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
   double* pB = part_buf(h, kPartB);
   double* pC = part_buf(h, kPartC);
 
-  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(1), 0, s, h->ctl, tol, maxit);
+  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(1), 0, s, h->ctl, tol, maxit, h->hist,
+                     h->hist_cap);
   PF_LAUNCH_CHECK("ctl_init_kernel");
   const T dxi = (T)(1.0 / h->dx), dyi = (T)(1.0 / h->dy), dzi = (T)(1.0 / h->dz);
   const T sc1 = (T)src_scale, sc2 = (T)(-1.0 * vol);
@@ -828,7 +1059,7 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
     if (prof) recs[nrec++] = Rec{kind, iter, slot[kind]++};
   };
   int it = 0;  // iterations whose KA has been launched
-  const TileGrid tg = make_tile_grid(g, h->tile_rows);
+  const TileGrid tg = make_tile_grid(g, (int)h->esize, h->tile_rows);
   const int nbA = tg.nblk;   // blocks writing partials A and B
   auto ka = [&](int old, int nw, hipEvent_t e0, hipEvent_t e1) {
     launch_tile<T, kModeA>(tg, s, e0, e1, g, nb, h->ctl, pC, pA, r, rh, P[old], Vv[old], P[nw],
@@ -1075,6 +1306,43 @@ int pf_profile_enable(pf_box* h, int on) {
     h->prof_ms[k] = 0.0;
     h->prof_n[k] = 0;
   }
+  return 0;
+}
+
+int pf_residual_history_enable(pf_box* h, int capacity) {
+  if (int rc = check_box(h)) return rc;
+  if (capacity < 0) return fail(PF_EINVAL, "pf_residual_history_enable: capacity must be >= 0");
+  if (h->hist) {
+    PF_HIP(hipFree(h->hist));
+    h->bytes -= (int64_t)sizeof(double) * h->hist_cap;
+    h->hist = nullptr;
+    h->hist_cap = 0;
+  }
+  if (capacity == 0) return 0;
+  hipError_t e = hipMalloc((void**)&h->hist, sizeof(double) * capacity);
+  if (e != hipSuccess) {
+    h->hist = nullptr;
+    return check_hip(e, "hipMalloc");
+  }
+  h->hist_cap = capacity;
+  h->bytes += (int64_t)sizeof(double) * capacity;
+  return 0;
+}
+
+int pf_residual_history(pf_box* h, double* out, int capacity, int* count, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (!out || !count || capacity < 0) return fail(PF_EINVAL, "pf_residual_history: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  PF_HIP(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
+  PF_HIP(hipStreamSynchronize(s));
+  int n = h->ctl_host->iter;
+  if (n > h->hist_cap) n = h->hist_cap;
+  if (n > capacity) n = capacity;
+  if (n > 0) {
+    PF_HIP(hipMemcpyAsync(out, h->hist, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    PF_HIP(hipStreamSynchronize(s));
+  }
+  *count = n;
   return 0;
 }
 

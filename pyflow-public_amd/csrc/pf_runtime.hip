@@ -88,12 +88,12 @@ int pf_box_create(pf_box** out, int nx, int ny, int nz, double dx, double dy, do
   int64_t nb = (h->g.ncell + kBlock - 1) / kBlock;
   h->nblk = (int)(nb < 1024 ? nb : 1024);
   h->chunk = 16;
-  // 256-thread tiles: 512-thread tiles (half the y-halo share) measured
-  // slower on the 128^3 solve (KA 24.0 vs 23.3 us with dispatch events)
+  // 256-thread blocks; taller tiles come from two cells per thread (512-
+  // thread blocks measured slower on the 128^3 solve)
   h->tile_rows = 1;
   {
     // the tiled Krylov kernels may use more blocks than the flat ones
-    const int64_t tiles = make_tile_grid(h->g, h->tile_rows).nblk;
+    const int64_t tiles = make_tile_grid(h->g, (int)h->esize, h->tile_rows).nblk;
     const int64_t mx = tiles > 1024 ? tiles : 1024;   // >= every reducing grid
     h->part_stride = mx * kMaxPartSlots;
   }
@@ -137,7 +137,7 @@ int pf_box_destroy(pf_box* h) {
   if (!h) return 0;
   void* dev[] = {h->x, h->r, h->rh, h->p[0], h->p[1], h->v[0], h->v[1], h->s, h->t,
                  h->acc[0], h->acc[1], h->acc[2], h->ypong[0], h->ypong[1], h->ypong[2],
-                 h->part, h->ctl, h->stats_dev};
+                 h->part, h->ctl, h->stats_dev, h->hist};
   for (void* p : dev)
     if (p) hipFree(p);
   if (h->prof_ev) {

@@ -15,7 +15,8 @@ PF_F64 = 0
 PF_F32 = 1
 PF_POISSON_BICGSTAB = 0
 PF_POISSON_JACOBI = 1
-SOLVE_STATUS = {0: 'converged', 1: 'maxit', 2: 'breakdown <v,r_hat>=0', 3: 'breakdown <t,t>=0'}
+SOLVE_STATUS = {0: 'converged', 1: 'maxit', 2: 'breakdown <v,r_hat>=0', 3: 'breakdown <t,t>=0',
+                4: 'breakdown rho2*omega=0'}
 
 _lib = None
 
@@ -72,6 +73,8 @@ SIGNATURES = [
     ('pf_batch_chunk_iterations', _I, []),
     ('pf_profile_enable', _I, [_P, _I]),
     ('pf_profile_read', _I, [_P, _I, _PD, ctypes.POINTER(ctypes.c_int64)]),
+    ('pf_residual_history_enable', _I, [_P, _I]),
+    ('pf_residual_history', _I, [_P, _PD, _I, ctypes.POINTER(ctypes.c_int), _P]),
 ]
 
 
@@ -295,6 +298,20 @@ class Box:
             check(lib().pf_profile_read(self.h, k, ctypes.byref(ms), ctypes.byref(n)), 'pf_profile_read')
             out[name] = (ms.value, n.value)
         return out
+
+    def residual_history_enable(self, capacity):
+        """Keep max|r_k|/max|r0| of up to `capacity` BiCGStab iterations per solve (0 = off)."""
+        check(lib().pf_residual_history_enable(self.h, int(capacity)), 'pf_residual_history_enable')
+        self._hist_cap = int(capacity)
+
+    def residual_history(self):
+        """The last solve's per-iteration relative residuals (list of floats)."""
+        cap = getattr(self, '_hist_cap', 0)
+        buf = (ctypes.c_double * max(cap, 1))()
+        n = ctypes.c_int(0)
+        check(lib().pf_residual_history(self.h, buf, cap, ctypes.byref(n), self.stream()),
+              'pf_residual_history')
+        return [buf[i] for i in range(n.value)]
 
     def sfs_mlp(self, u, v, w, params, sfs6):
         if params.dtype != torch.float32 or params.device.type != 'cuda' or not params.is_contiguous():
