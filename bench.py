@@ -40,25 +40,25 @@ HBM_PEAK_GBS = 8000.0
 KA_TRAFFIC_BYTES = {(128, 'f64'): int((38081.5 / 0.50003 + 31987.1) * 1024)}
 
 
-def make_config(N, dtype, nu, tol, maxit, seed):
+def make_config(N, dtype, nu, tol, maxit, seed, solver='bicgstab'):
     import numpy as np
     import torch
     return types.SimpleNamespace(
         configName='passotPouquet', Nx=N, Ny=N, Nz=N, Lx=2 * math.pi, seed=seed, k0=4.0, urms=1.0,
         nproc_x=1, nproc_y=1, nproc_z=1, mu=nu, rho=1.0, simDt=None, numIt=0,
-        SFSmodel='none', advancerName='RK4', equationMode='NS', pSolverMode='bicgstab',
+        SFSmodel='none', advancerName='RK4', equationMode='NS', pSolverMode=solver,
         min_pressure_residual=tol, max_pressure_iterations=maxit,
         dtypeTorch=dtype, dtypeNumpy=np.float64 if dtype == torch.float64 else np.float32,
         writeMonitor=False)
 
 
-def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True):
+def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True, solver='bicgstab'):
     import torch
     from pyflow_amd import PyFlow
     from pyflow_amd.initial_conditions import passot_pouquet, cfl_dt
     u0, v0, w0 = passot_pouquet(N, k0=4.0, urms=1.0, seed=seed)
     dt = cfl_dt(u0, v0, w0, 2 * math.pi / N)
-    cfg = make_config(N, dtype, nu, tol, maxit, seed)
+    cfg = make_config(N, dtype, nu, tol, maxit, seed, solver)
     cfg.simDt = dt
     comms, decomp, geometry, metric, D, t0 = PyFlow.setup(cfg)
     for _ in range(warmup):
@@ -86,12 +86,12 @@ def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True):
                 res=D.max_resP, status={k: status.count(k) for k in set(status)})
 
 
-def gpu_fields(N, dtype, nu, tol, maxit, steps, seed, dt):
+def gpu_fields(N, dtype, nu, tol, maxit, steps, seed, dt, solver='bicgstab'):
     """Interior velocity after `steps` steps from the initial field (for the
     field error against the oracle)."""
     import torch
     from pyflow_amd import PyFlow
-    cfg = make_config(N, dtype, nu, tol, maxit, seed)
+    cfg = make_config(N, dtype, nu, tol, maxit, seed, solver)
     cfg.simDt = dt
     comms, decomp, geometry, metric, D, t0 = PyFlow.setup(cfg)
     for _ in range(steps):
@@ -143,6 +143,9 @@ def main():
     ap.add_argument('--tol', type=float, default=None)
     ap.add_argument('--maxit', type=int, default=3000)
     ap.add_argument('--seed', type=int, default=2024)
+    ap.add_argument('--solver', choices=['bicgstab', 'bicgstab_mpir'], default='bicgstab',
+                    help='bicgstab_mpir: FP64 iterative refinement around BiCGStab (long '
+                         'solves, FP32 storage at tight tolerances)')
     ap.add_argument('--cpu-steps', type=int, default=10)
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run 5 device copies of MIB MiB (byte-counter reference) and exit')
@@ -163,7 +166,7 @@ def main():
     w = 8 if a.dtype == 'f64' else 4
     tol = a.tol if a.tol is not None else (1e-10 if a.dtype == 'f64' else 1e-8)
     r = gpu_run(a.N, dtype, a.nu, tol, a.maxit, a.steps, a.warmup, a.seed,
-                0 if a.no_profile else a.profile_period)
+                0 if a.no_profile else a.profile_period, a.solver)
     ncell = a.N ** 3
     K = a.steps
     value = ncell * K / r['wall'] / 1e6
@@ -176,8 +179,9 @@ def main():
         'warmup': a.warmup, 'ms_per_step': round(ms, 4), 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None, 'dtype': a.dtype, 'data': 'synthetic',
         'config': {'workload': 'decaying isotropic turbulence, periodic %d^3 staggered grid, '
-                               'Passot-Pouquet k0=4 u_rms=1, nu=%g, RK4, BiCGStab tol %g'
-                               % (a.N, a.nu, tol),
+                               'Passot-Pouquet k0=4 u_rms=1, nu=%g, RK4, %s tol %g'
+                               % (a.N, a.nu, 'BiCGStab' if a.solver == 'bicgstab' else
+                                  'BiCGStab in FP64 iterative refinement', tol),
                    'grid': a.N, 'nu': a.nu, 'tol': tol, 'dt': r['dt'],
                    'bicgstab_iters_mean': round(kbar, 2),
                    'bicgstab_last_residual': float('%.3e' % r['res']),
@@ -213,7 +217,7 @@ def main():
                                          '%.1f s)' % (a.cpu_steps, a.N, cb['cores'], cb['iters'],
                                                       cb['wall'])}
         # field error of the GPU path against the oracle after the same steps
-        gf = gpu_fields(a.N, dtype, a.nu, tol, a.maxit, a.cpu_steps, a.seed, r['dt'])
+        gf = gpu_fields(a.N, dtype, a.nu, tol, a.maxit, a.cpu_steps, a.seed, r['dt'], a.solver)
         errs = [float(torch.linalg.norm(x - y) / torch.linalg.norm(y))
                 for x, y in zip(gf, cb['fields'])]
         out['oracle_error'] = {'steps': a.cpu_steps, 'rel_l2_uvw': [float('%.3e' % e) for e in errs],

@@ -58,6 +58,7 @@ extern "C" {
  * product and lands on exactly 0 (long solves, e.g. 512^3 beyond ~900
  * iterations).                                                              */
 #define PF_SOLVE_BREAKDOWN_RHO 4
+#define PF_SOLVE_STALLED       5   /* refinement: an inner solve made no progress */
 
 /* field layouts */
 #define PF_LAYOUT_PADDED   0
@@ -75,6 +76,7 @@ extern "C" {
 /* velocity-correction solver selectors */
 #define PF_POISSON_BICGSTAB 0
 #define PF_POISSON_JACOBI   1
+#define PF_POISSON_BICGSTAB_MPIR 2   /* see pf_bicgstab_mpir_solve */
 
 typedef struct pf_box pf_box;
 
@@ -165,6 +167,38 @@ int pf_bicgstab_solve(pf_box* box, void* DP, void* p, void* source,
                       double* out_res, int* out_iters, void* stream);
 
 /*
+ * The same solve by mixed-precision iterative refinement (no counterpart in
+ * the reference; the remedy for long solves where the reference's BiCGStab
+ * breaks down, see PF_SOLVE_BREAKDOWN_RHO, or where FP32 vectors diverge):
+ * the iterate x and the right-hand side b = -vol*(rho/dt)*source are kept in
+ * FP64; each refinement step rounds the FP64 residual r = b - A x to the box
+ * precision, solves A d = r with the box's BiCGStab (fresh shadow residual)
+ * to the relative tolerance of pf_mpir_configure, and adds d to x in FP64.
+ * Stops when max|b - A x| / max|b| <= tol (the FP64 residual of x, returned
+ * in *out_res), after the refinement-step cap or when `maxit` inner
+ * iterations are used up (PF_SOLVE_MAXIT), or when an inner solve makes no
+ * progress (PF_SOLVE_STALLED).  An inner solve that stops unconverged is
+ * re-run up to its best iterate (the solves are deterministic), so a
+ * degrading inner recurrence costs iterations, not accuracy.  DP := x rounded to the box precision, p += DP.  `source` ends
+ * holding the last inner right-hand side.  *out_iters counts inner
+ * iterations.  Allocates 16 bytes per cell on first use.
+ * pf_vel_corr with solver = PF_POISSON_BICGSTAB_MPIR runs the same solve
+ * (with b from div(u) computed in FP64).
+ */
+int pf_bicgstab_mpir_solve(pf_box* box, void* DP, void* p, void* source,
+                           double rho, double dt, double tol, int maxit,
+                           double* out_res, int* out_iters, void* stream);
+
+/*
+ * Refinement settings of a box (defaults: inner_tol 1e-1, inner_maxit 500,
+ * outer_maxit 100).  Short inner solves keep each BiCGStab run well before
+ * the point where its recurrence degrades (measured on a 512^3 FP32 solve to
+ * 1e-8: inner tolerance 1e-1 took 1797 inner iterations, 1e-2 took 2309, and
+ * inner solves run to 3e-3 or beyond diverged).
+ */
+int pf_mpir_configure(pf_box* box, double inner_tol, int inner_maxit, int outer_maxit);
+
+/*
  * B (interior) = (-vol*L7) A for a padded A whose overlap is already filled.
  * Replaces Pressure_Base.applyOperator (solver/pressure.py:54-84) after its
  * communicate_border call; same nine-term summation order.
@@ -185,7 +219,8 @@ int pf_jacobi_solve(pf_box* box, void* pOld, void* p, void* source,
  * Velocity correction of Domain.velCorr (data/domain.py:373-402), fused:
  * source := div(u*) (metric_uniform.div_vel), source scaled by src_scale
  * (the solver's rho/simDt, pressure.py:125/205), solve for DP with the
- * selected solver (PF_POISSON_BICGSTAB: also -vol scaling and p += DP;
+ * selected solver (PF_POISSON_BICGSTAB or PF_POISSON_BICGSTAB_MPIR: also
+ * -vol scaling and p += DP;
  * PF_POISSON_JACOBI: the reference's Jacobi conventions, DP = pOld), then
  * u -= corr_scale * grad(DP) with corr_scale = the domain's simDt/rho,
  * using the staggered gradient of metric_uniform.grad_P
@@ -390,7 +425,8 @@ int pf_profile_read(pf_box* box, int kernel, double* total_ms, int64_t* launches
  * later pf_bicgstab_solve / pf_vel_corr on the box then stores
  * max|r_k|/max|r0| for k = 1 .. min(iterations, capacity), and
  * pf_residual_history copies the last solve's values into out[0..*count).
- * Synchronizes the stream.
+ * Synchronizes the stream.  (Refinement solves, PF_POISSON_BICGSTAB_MPIR,
+ * keep their inner histories to themselves and record nothing here.)
  */
 int pf_residual_history_enable(pf_box* box, int capacity);
 int pf_residual_history(pf_box* box, double* out, int capacity, int* count, void* stream);

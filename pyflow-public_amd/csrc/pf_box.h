@@ -62,6 +62,15 @@ struct pf_box {
   int last_iters;
   double* hist;              // per-iteration relative residuals of the last solve (optional)
   int hist_cap;              // entries of hist
+  // mixed-precision refinement (allocated on first use): FP64 iterate and
+  // right-hand side, interior layout
+  double* mp_x;
+  double* mp_b;
+  double* mp_hist;           // residual history of the current inner solve
+  int mp_hist_cap;
+  double mp_inner_tol;       // relative tolerance of each inner BiCGStab solve
+  int mp_inner_maxit;        // iteration cap of each inner solve
+  int mp_outer_maxit;        // refinement steps
   int chunk;                 // iterations per launch batch between host checks
   int tile_rows;             // block rows of the tiled Krylov kernels: 256*tile_rows threads
   int64_t bytes;             // device bytes owned by the handle

@@ -22,7 +22,9 @@
 
 #include <hip/hip_ext.h>
 
+#include <algorithm>
 #include <cstdlib>
+#include <vector>
 
 #include "pf_kernels.h"
 
@@ -1008,7 +1010,8 @@ int finalize(pf_box* h, hipStream_t s, const double* part, int nv, const int* op
 
 template <typename T>
 int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, const T* W,
-                  double src_scale, double tol, int maxit, double* out_res, int* out_iters) {
+                  double src_scale, double tol, int maxit, double* out_res, int* out_iters,
+                  bool vol_scale = true, double* hist = nullptr, int hist_cap = -1) {
   const Geom& g = h->g;
   const int nb = h->nblk;
   const double vol = h->dx * h->dy * h->dz;
@@ -1024,11 +1027,14 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
   double* pB = part_buf(h, kPartB);
   double* pC = part_buf(h, kPartC);
 
-  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(1), 0, s, h->ctl, tol, maxit, h->hist,
-                     h->hist_cap);
+  if (hist_cap < 0) {
+    hist = h->hist;
+    hist_cap = h->hist_cap;
+  }
+  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(1), 0, s, h->ctl, tol, maxit, hist, hist_cap);
   PF_LAUNCH_CHECK("ctl_init_kernel");
   const T dxi = (T)(1.0 / h->dx), dyi = (T)(1.0 / h->dy), dzi = (T)(1.0 / h->dz);
-  const T sc1 = (T)src_scale, sc2 = (T)(-1.0 * vol);
+  const T sc1 = (T)src_scale, sc2 = vol_scale ? (T)(-1.0 * vol) : (T)1;
   if (U)
     hipLaunchKernelGGL((bicg_init_kernel<T, true>), dim3(nb), dim3(kBlock), 0, s, g, src,
                        U + g.poff, V + g.poff, W + g.poff, dxi, dyi, dzi, sc1, sc2, r, rh, x,
@@ -1201,6 +1207,214 @@ int div_impl(pf_box* h, hipStream_t s, const T* U, const T* V, const T* W, T* di
   return stats_impl<T>(h, s, U, V, W, divg, out5);
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision iterative refinement (PF_POISSON_BICGSTAB_MPIR).  The
+// reference's BiCGStab keeps its shadow residual fixed for the whole solve;
+// on long solves <r, r_hat> decays to the rounding floor of the inner product
+// and the recurrence breaks down (FP64 at 512^3 after ~900 iterations, FP32
+// already near 1e-3).  Refinement restarts it: the iterate x and the
+// right-hand side b are kept in FP64, each outer step solves A d = r for the
+// FP64 residual r = b - A x (rounded to the box precision) with the box's
+// BiCGStab to a loose tolerance, and adds d to x in FP64.  The stop test is
+// max|b - A x| / max|b| <= tol on the FP64 residual.
+template <typename T, bool FROM_DIV>
+__global__ void __launch_bounds__(kBlock) mp_init_kernel(
+    Geom g, const T* __restrict__ src, const T* __restrict__ U, const T* __restrict__ V,
+    const T* __restrict__ W, double dxi, double dyi, double dzi, double sc1, double sc2,
+    double* __restrict__ b, double* __restrict__ x, T* __restrict__ r_out,
+    double* __restrict__ part) {
+  double m = 0.0;
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    double v;
+    if (FROM_DIV) {
+      int i, j, k;
+      decode(g, idx, i, j, k);
+      const int64_t c = i * g.psx + j * g.psy + k;
+      v = ((double)U[c + g.psx] - (double)U[c]) * dxi;
+      v = v + ((double)V[c + g.psy] - (double)V[c]) * dyi;
+      v = v + ((double)W[c + 1] - (double)W[c]) * dzi;
+    } else {
+      v = (double)src[idx];
+    }
+    v = v * sc1;
+    v = v * sc2;
+    b[idx] = v;
+    x[idx] = 0.0;
+    r_out[idx] = (T)v;
+    m = nmax(m, fabs(v));
+  }
+  double a[1] = {m};
+  const int ops[1] = {1};
+  block_reduce<1>(a, ops);
+  write_partials<1>(part, a);
+}
+
+// x += d, then (second kernel) r = b - A x with max|r|: two passes, so that
+// every neighbour of the stencil reads the updated x.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) mp_update_kernel(Geom g, double* __restrict__ x,
+                                                           const T* __restrict__ d) {
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x)
+    x[idx] = x[idx] + (double)d[idx];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) mp_residual_kernel(
+    Geom g, const double* __restrict__ x, const double* __restrict__ b, Lap<double> L,
+    T* __restrict__ r_out, double* __restrict__ part) {
+  double m = 0.0;
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(g, idx, i, j, k);
+    const int64_t ci = (int64_t)i * g.isx, cj = (int64_t)j * g.isy;
+    const int64_t im = (int64_t)(i == 0 ? g.nx - 1 : i - 1) * g.isx;
+    const int64_t ip = (int64_t)(i == g.nx - 1 ? 0 : i + 1) * g.isx;
+    const int64_t jm = (int64_t)(j == 0 ? g.ny - 1 : j - 1) * g.isy;
+    const int64_t jp = (int64_t)(j == g.ny - 1 ? 0 : j + 1) * g.isy;
+    const int km = k == 0 ? g.nz - 1 : k - 1, kp = k == g.nz - 1 ? 0 : k + 1;
+    const double ax = lap7(L, x[im + cj + k], x[ci + cj + k], x[ip + cj + k], x[ci + jm + k],
+                           x[ci + jp + k], x[ci + cj + km], x[ci + cj + kp]);
+    const double r = b[idx] - ax;
+    r_out[idx] = (T)r;
+    m = nmax(m, fabs(r));
+  }
+  double a[1] = {m};
+  const int ops[1] = {1};
+  block_reduce<1>(a, ops);
+  write_partials<1>(part, a);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) mp_finish_kernel(Geom g, const double* __restrict__ x,
+                                                           T* __restrict__ out) {
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
+       idx += gridDim.x * blockDim.x)
+    out[idx] = (T)x[idx];
+}
+
+int mp_buffers(pf_box* h) {
+  if (h->mp_hist_cap < h->mp_inner_maxit) {
+    if (h->mp_hist) {
+      PF_HIP(hipFree(h->mp_hist));
+      h->bytes -= (int64_t)sizeof(double) * h->mp_hist_cap;
+      h->mp_hist = nullptr;
+      h->mp_hist_cap = 0;
+    }
+    hipError_t e = hipMalloc((void**)&h->mp_hist, sizeof(double) * h->mp_inner_maxit);
+    if (e != hipSuccess) {
+      h->mp_hist = nullptr;
+      return check_hip(e, "hipMalloc");
+    }
+    h->mp_hist_cap = h->mp_inner_maxit;
+    h->bytes += (int64_t)sizeof(double) * h->mp_hist_cap;
+  }
+  if (h->mp_x) return 0;
+  const size_t n = (size_t)h->g.ncell * sizeof(double);
+  hipError_t e = hipMalloc((void**)&h->mp_x, n);
+  if (e != hipSuccess) {
+    h->mp_x = nullptr;
+    return check_hip(e, "hipMalloc");
+  }
+  e = hipMalloc((void**)&h->mp_b, n);
+  if (e != hipSuccess) {
+    hipFree(h->mp_x);
+    h->mp_x = nullptr;
+    h->mp_b = nullptr;
+    return check_hip(e, "hipMalloc");
+  }
+  h->bytes += (int64_t)(2 * n);
+  return 0;
+}
+
+// Leaves the solution, rounded to T, in h->x (as bicgstab_impl does) and the
+// last inner right-hand side in src.  *out_iters counts inner iterations.
+template <typename T>
+int mp_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, const T* W,
+            double src_scale, double tol, int maxit, double* out_res, int* out_iters) {
+  if (int rc = mp_buffers(h)) return rc;
+  const Geom& g = h->g;
+  const int nb = h->nblk;
+  const double vol = h->dx * h->dy * h->dz;
+  double* pS = part_buf(h, kPartS);
+  const double dxi = 1.0 / h->dx, dyi = 1.0 / h->dy, dzi = 1.0 / h->dz;
+  if (U)
+    hipLaunchKernelGGL((mp_init_kernel<T, true>), dim3(nb), dim3(kBlock), 0, s, g,
+                       (const T*)nullptr, U + g.poff, V + g.poff, W + g.poff, dxi, dyi, dzi,
+                       src_scale, -1.0 * vol, h->mp_b, h->mp_x, src, pS);
+  else
+    hipLaunchKernelGGL((mp_init_kernel<T, false>), dim3(nb), dim3(kBlock), 0, s, g,
+                       (const T*)src, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
+                       dxi, dyi, dzi, src_scale, -1.0 * vol, h->mp_b, h->mp_x, src, pS);
+  PF_LAUNCH_CHECK("mp_init_kernel");
+  const int ops[1] = {1};
+  double bmax = 0.0;
+  if (int rc = finalize<T>(h, s, pS, 1, ops, &bmax)) return rc;
+  int status = PF_SOLVE_MAXIT;
+  double res = 1.0;
+  int total = 0;
+  if (!(bmax > 0.0)) {   // b == 0: x = 0 is exact
+    status = PF_SOLVE_CONVERGED;
+    res = 0.0;
+  }
+  const Lap<double> L = make_lap<double>(h);
+  for (int outer = 0; status != PF_SOLVE_CONVERGED && outer < h->mp_outer_maxit && total < maxit;
+       ++outer) {
+    const int cap = std::min(h->mp_inner_maxit, maxit - total);
+    double rin = 0.0;
+    int iin = 0;
+    int st = bicgstab_impl<T>(h, s, src, nullptr, nullptr, nullptr, 1.0, h->mp_inner_tol, cap,
+                              &rin, &iin, false, h->mp_hist, h->mp_hist_cap);
+    if (st < 0) return st;
+    total += iin;
+    if (iin == 0) {
+      status = PF_SOLVE_STALLED;
+      break;
+    }
+    if (st != PF_SOLVE_CONVERGED) {
+      // The inner recurrence degraded before reaching its tolerance: re-run
+      // it (deterministically, same bits) up to its best iterate.
+      std::vector<double> hv(iin);
+      PF_HIP(hipMemcpyAsync(hv.data(), h->mp_hist, sizeof(double) * iin, hipMemcpyDeviceToHost, s));
+      PF_HIP(hipStreamSynchronize(s));
+      int kbest = -1;
+      for (int k = 0; k < iin; ++k)
+        if (hv[k] < 1.0 && (kbest < 0 || hv[k] < hv[kbest])) kbest = k;
+      if (kbest < 0) {
+        status = PF_SOLVE_STALLED;
+        break;
+      }
+      if (kbest + 1 < iin && total + kbest + 1 <= maxit) {
+        st = bicgstab_impl<T>(h, s, src, nullptr, nullptr, nullptr, 1.0, h->mp_inner_tol,
+                              kbest + 1, &rin, &iin, false, h->mp_hist, h->mp_hist_cap);
+        if (st < 0) return st;
+        total += iin;
+      } else if (!(rin < 1.0)) {
+        status = PF_SOLVE_STALLED;
+        break;
+      }
+    }
+    hipLaunchKernelGGL(mp_update_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g, h->mp_x,
+                       (const T*)h->x);
+    PF_LAUNCH_CHECK("mp_update_kernel");
+    hipLaunchKernelGGL(mp_residual_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g,
+                       (const double*)h->mp_x, (const double*)h->mp_b, L, src, pS);
+    PF_LAUNCH_CHECK("mp_residual_kernel");
+    double rmax = 0.0;
+    if (int rc = finalize<T>(h, s, pS, 1, ops, &rmax)) return rc;
+    res = rmax / bmax;
+    if (res <= tol) status = PF_SOLVE_CONVERGED;
+  }
+  hipLaunchKernelGGL(mp_finish_kernel<T>, dim3(nb), dim3(kBlock), 0, s, g,
+                     (const double*)h->mp_x, (T*)h->x);
+  PF_LAUNCH_CHECK("mp_finish_kernel");
+  if (out_res) *out_res = res;
+  if (out_iters) *out_iters = total;
+  return status;
+}
+
 template <typename T>
 int vel_corr_impl(pf_box* h, hipStream_t s, T* U, T* V, T* W, T* DP, T* p, T* src, int solver,
                   double src_scale, double corr_scale, double tol, int iters, double* out_res,
@@ -1209,8 +1423,11 @@ int vel_corr_impl(pf_box* h, hipStream_t s, T* U, T* V, T* W, T* DP, T* p, T* sr
   const int nbf = launch_blocks(g);
   const T dxi = (T)(1.0 / h->dx), dyi = (T)(1.0 / h->dy), dzi = (T)(1.0 / h->dz);
   const T neg_c = (T)(-corr_scale);
-  if (solver == PF_POISSON_BICGSTAB) {
-    int st = bicgstab_impl<T>(h, s, src, U, V, W, src_scale, tol, iters, out_res, out_iters);
+  if (solver == PF_POISSON_BICGSTAB || solver == PF_POISSON_BICGSTAB_MPIR) {
+    const int st = solver == PF_POISSON_BICGSTAB
+                       ? bicgstab_impl<T>(h, s, src, U, V, W, src_scale, tol, iters, out_res,
+                                          out_iters)
+                       : mp_impl<T>(h, s, src, U, V, W, src_scale, tol, iters, out_res, out_iters);
     if (st < 0) return st;
     if (int rc = scatter_dp<T>(h, s, DP, p)) return rc;
     hipLaunchKernelGGL(project_kernel<T>, dim3(nbf), dim3(kBlock), 0, s, g, (const T*)h->x, U, V,
@@ -1258,6 +1475,38 @@ int pf_bicgstab_solve(pf_box* h, void* DP, void* p, void* source, double rho, do
   return st;
 }
 
+int pf_bicgstab_mpir_solve(pf_box* h, void* DP, void* p, void* source, double rho, double dt,
+                           double tol, int maxit, double* out_res, int* out_iters, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (!DP || !p || !source) return fail(PF_EINVAL, "pf_bicgstab_mpir_solve: NULL field pointer");
+  if (!(dt != 0.0)) return fail(PF_EINVAL, "pf_bicgstab_mpir_solve: dt must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  int st;
+  if (h->dtype == PF_F64) {
+    st = mp_impl<double>(h, s, (double*)source, nullptr, nullptr, nullptr, rho / dt, tol, maxit,
+                         out_res, out_iters);
+    if (st >= 0)
+      if (int rc = scatter_dp<double>(h, s, (double*)DP, (double*)p)) return rc;
+  } else {
+    st = mp_impl<float>(h, s, (float*)source, nullptr, nullptr, nullptr, rho / dt, tol, maxit,
+                        out_res, out_iters);
+    if (st >= 0)
+      if (int rc = scatter_dp<float>(h, s, (float*)DP, (float*)p)) return rc;
+  }
+  return st;
+}
+
+int pf_mpir_configure(pf_box* h, double inner_tol, int inner_maxit, int outer_maxit) {
+  if (int rc = check_box(h)) return rc;
+  if (!(inner_tol > 0.0 && inner_tol < 1.0) || inner_maxit < 1 || outer_maxit < 1)
+    return fail(PF_EINVAL, "pf_mpir_configure: need 0 < inner_tol < 1, inner_maxit >= 1, "
+                           "outer_maxit >= 1");
+  h->mp_inner_tol = inner_tol;
+  h->mp_inner_maxit = inner_maxit;
+  h->mp_outer_maxit = outer_maxit;
+  return 0;
+}
+
 int pf_jacobi_solve(pf_box* h, void* pOld, void* p, void* source, double rho, double dt, int nit,
                     double* out_res, void* stream) {
   if (int rc = check_box(h)) return rc;
@@ -1278,7 +1527,8 @@ int pf_vel_corr(pf_box* h, void* u, void* v, void* w, void* DP, void* p, void* s
   if (int rc = check_box(h)) return rc;
   if (!u || !v || !w || !DP || !p || !source)
     return fail(PF_EINVAL, "pf_vel_corr: NULL field pointer");
-  if (solver != PF_POISSON_BICGSTAB && solver != PF_POISSON_JACOBI)
+  if (solver != PF_POISSON_BICGSTAB && solver != PF_POISSON_JACOBI &&
+      solver != PF_POISSON_BICGSTAB_MPIR)
     return fail(PF_EINVAL, "pf_vel_corr: unknown solver");
   if (solver == PF_POISSON_JACOBI && iters < 1)
     return fail(PF_EINVAL, "pf_vel_corr: Jacobi needs at least one sweep");

@@ -88,6 +88,9 @@ int pf_box_create(pf_box** out, int nx, int ny, int nz, double dx, double dy, do
   int64_t nb = (h->g.ncell + kBlock - 1) / kBlock;
   h->nblk = (int)(nb < 1024 ? nb : 1024);
   h->chunk = 16;
+  h->mp_inner_tol = 1e-1;
+  h->mp_inner_maxit = 500;
+  h->mp_outer_maxit = 100;
   // 256-thread blocks; taller tiles come from two cells per thread (512-
   // thread blocks measured slower on the 128^3 solve)
   h->tile_rows = 1;
@@ -137,7 +140,7 @@ int pf_box_destroy(pf_box* h) {
   if (!h) return 0;
   void* dev[] = {h->x, h->r, h->rh, h->p[0], h->p[1], h->v[0], h->v[1], h->s, h->t,
                  h->acc[0], h->acc[1], h->acc[2], h->ypong[0], h->ypong[1], h->ypong[2],
-                 h->part, h->ctl, h->stats_dev, h->hist};
+                 h->part, h->ctl, h->stats_dev, h->hist, h->mp_x, h->mp_b, h->mp_hist};
   for (void* p : dev)
     if (p) hipFree(p);
   if (h->prof_ev) {

@@ -15,8 +15,9 @@ PF_F64 = 0
 PF_F32 = 1
 PF_POISSON_BICGSTAB = 0
 PF_POISSON_JACOBI = 1
+PF_POISSON_BICGSTAB_MPIR = 2
 SOLVE_STATUS = {0: 'converged', 1: 'maxit', 2: 'breakdown <v,r_hat>=0', 3: 'breakdown <t,t>=0',
-                4: 'breakdown rho2*omega=0'}
+                4: 'breakdown rho2*omega=0', 5: 'stalled'}
 
 _lib = None
 
@@ -74,6 +75,8 @@ SIGNATURES = [
     ('pf_profile_enable', _I, [_P, _I]),
     ('pf_profile_read', _I, [_P, _I, _PD, ctypes.POINTER(ctypes.c_int64)]),
     ('pf_residual_history_enable', _I, [_P, _I]),
+    ('pf_bicgstab_mpir_solve', _I, [_P, _P, _P, _P, _D, _D, _D, _I, _PD, ctypes.POINTER(_I), _P]),
+    ('pf_mpir_configure', _I, [_P, _D, _I, _I]),
     ('pf_residual_history', _I, [_P, _PD, _I, ctypes.POINTER(ctypes.c_int), _P]),
 ]
 
@@ -232,6 +235,20 @@ class Box:
                                            float(tol), int(maxit), ctypes.byref(res),
                                            ctypes.byref(its), self.stream()), 'pf_bicgstab_solve')
         return res.value, its.value, st
+
+    def bicgstab_mpir(self, DP, p, source, rho, dt, tol, maxit):
+        res = ctypes.c_double(0.0)
+        its = ctypes.c_int(0)
+        st = check(lib().pf_bicgstab_mpir_solve(self.h, self.padded(DP, 'DP'), self.padded(p, 'p'),
+                                                self.interior(source, 'source'), float(rho),
+                                                float(dt), float(tol), int(maxit),
+                                                ctypes.byref(res), ctypes.byref(its),
+                                                self.stream()), 'pf_bicgstab_mpir_solve')
+        return res.value, its.value, st
+
+    def mpir_configure(self, inner_tol=1e-1, inner_maxit=500, outer_maxit=100):
+        check(lib().pf_mpir_configure(self.h, float(inner_tol), int(inner_maxit), int(outer_maxit)),
+              'pf_mpir_configure')
 
     def jacobi(self, pOld, p, source, rho, dt, nit):
         res = ctypes.c_double(0.0)

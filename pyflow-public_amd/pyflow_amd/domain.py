@@ -152,6 +152,13 @@ class Domain:
                                                         inputConfig.rho, simDt,
                                                         inputConfig.min_pressure_residual,
                                                         inputConfig.max_pressure_iterations)
+            elif mode == 'bicgstab_mpir':
+                self.poisson = pressure.solver_bicgstab_mpir(
+                    comms, decomp, metric, geometry, inputConfig.rho, simDt,
+                    inputConfig.min_pressure_residual, inputConfig.max_pressure_iterations,
+                    getattr(inputConfig, 'mpir_inner_tol', 1e-1),
+                    getattr(inputConfig, 'mpir_inner_maxit', 500),
+                    getattr(inputConfig, 'mpir_outer_maxit', 100))
             elif mode == 'RedBlackGS':
                 raise Exception('\nRed-black GS not yet implemented\n')
             elif mode == 'none':
@@ -176,7 +183,7 @@ class Domain:
     def velCorr(self, simDt):
         """domain.py:373-402 as one fused call: divergence, solve, projection."""
         P = self.poisson
-        if P.kind == _native.PF_POISSON_BICGSTAB:
+        if P.kind != _native.PF_POISSON_JACOBI:
             res, its, st = self.box.vel_corr(self.state_u_P.var, self.state_v_P.var,
                                              self.state_w_P.var, self.state_DP_P.var,
                                              self.state_p_P.var, self.source_P, P.kind,
@@ -210,7 +217,7 @@ class Domain:
         if self.doPressure:
             P = self.poisson
             A = (self.state_u_adj_P.var, self.state_v_adj_P.var, self.state_w_adj_P.var)
-            if P.kind == _native.PF_POISSON_BICGSTAB:
+            if P.kind != _native.PF_POISSON_JACOBI:
                 res, its, st = self.box.vel_corr(*A, self.state_DP_P.var, self.state_p_P.var,
                                                  self.source_P, P.kind, P.rho / P.simDt,
                                                  +simDt / self.rho, P.min_residual,

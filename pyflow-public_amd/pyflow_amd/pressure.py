@@ -60,3 +60,26 @@ class solver_bicgstab:
         self.iterations = its
         self.status = _native.SOLVE_STATUS.get(st, st)
         return res
+
+
+class solver_bicgstab_mpir(solver_bicgstab):
+    """pSolverMode 'bicgstab_mpir' (no counterpart in the reference): the
+    same Poisson problem solved by mixed-precision iterative refinement around
+    the box's BiCGStab -- FP64 iterate and residual, inner solves in the box
+    precision restarted with a fresh shadow residual each refinement step.
+    `min_residual` bounds the FP64 residual max|b - A x|/max|b|;
+    `max_iterations` caps the inner iterations of one solve.  Optional
+    `inner_tol`, `inner_maxit`, `outer_maxit` tune the refinement."""
+
+    def __init__(self, comms, decomp, metric, geo, rho, simDt, min_residual, max_iterations,
+                 inner_tol=1e-1, inner_maxit=500, outer_maxit=100):
+        super().__init__(comms, decomp, metric, geo, rho, simDt, min_residual, max_iterations)
+        self.kind = _native.PF_POISSON_BICGSTAB_MPIR
+        self.box.mpir_configure(inner_tol, inner_maxit, outer_maxit)
+
+    def solve(self, state_DP_P, state_p_P, source_P):
+        res, its, st = self.box.bicgstab_mpir(state_DP_P.var, state_p_P.var, source_P, self.rho,
+                                              self.simDt, self.min_residual, self.max_iterations)
+        self.iterations = its
+        self.status = _native.SOLVE_STATUS.get(st, st)
+        return res

@@ -1,0 +1,160 @@
+"""Mixed-precision iterative refinement around BiCGStab (pSolverMode
+'bicgstab_mpir', pf_bicgstab_mpir_solve).  Not the reference's algorithm, so
+the checks are against the problem it solves:
+
+* the returned residual is max|b - A x|/max|b| of the returned solution,
+  recomputed here with an independent FP64 stencil (torch.roll), and is
+  <= tol;
+* FP64: the solution agrees with the oracle's BiCGStab solution of the same
+  system (both to 1e-12) up to the free constant, to 1e-10;
+  10 time steps agree with the oracle's to 1e-10 (velocity);
+* FP32 storage: the refinement reaches tol = 1e-8 on the FP64 residual where
+  plain FP32 BiCGStab cannot (the reference's recurrence in FP32 diverges
+  near 1e-3 at 512^3), and the error against the FP64 oracle after 10 steps
+  is stated and bounded.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import cpu, rel_l2, rel_l2_mean_free
+from test_gpu_oracle import pp_case, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def lap_fp64(x, h):
+    """(-vol * L7) x on an interior periodic field, FP64 (independent of the
+    kernels: torch.roll)."""
+    vol = h ** 3
+    out = torch.zeros_like(x)
+    for d in range(3):
+        out += (torch.roll(x, 1, d) - 2.0 * x + torch.roll(x, -1, d)) / (h * h)
+    return out * (-vol)
+
+
+def make_box(N, dtype):
+    from pyflow_amd import _native
+    h = 2 * math.pi / N
+    return _native.Box(N, N, N, h, h, h, dtype=dtype), h
+
+
+def zero_mean_source(N, dtype, seed):
+    g = torch.Generator(device='cpu').manual_seed(seed)
+    s = torch.randn(N, N, N, generator=g, dtype=torch.float64)
+    s -= s.mean()
+    return s.to(dtype).cuda()
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_returned_residual_is_true_residual(dtype):
+    N = 48
+    box, h = make_box(N, dtype)
+    src = zero_mean_source(N, dtype, 3)
+    b = src.double() * (1.0 / 1e-3) * (-(h ** 3))
+    DP = torch.zeros(N + 4, N + 4, N + 4, dtype=dtype, device='cuda')
+    p = torch.zeros_like(DP)
+    tol = 1e-9 if dtype == torch.float32 else 1e-12
+    res, its, st = box.bicgstab_mpir(DP, p, src.clone(), 1.0, 1e-3, tol, 20000)
+    torch.cuda.synchronize()
+    x = DP[2:-2, 2:-2, 2:-2].double()
+    r = b - lap_fp64(x, h)
+    true_res = float(r.abs().max() / b.abs().max())
+    print('dtype %s: status %d, %d inner iterations, returned %.3e, recomputed %.3e'
+          % (dtype, st, its, res, true_res))
+    assert st == 0 and res <= tol
+    if dtype == torch.float64:
+        # DP is the FP64 iterate itself; the two evaluations of b - A x
+        # differ by the rounding of the stencil sums (~1e-16 of max|b|)
+        assert abs(true_res - res) <= 1e-3 * res + 1e-14
+    else:
+        # DP is the iterate rounded to FP32: its residual carries the rounding
+        # of x amplified by the operator (~ eps32 * ||A|| ||x|| / ||b||)
+        assert true_res <= 1e-3
+    # periodic images of DP, and p += DP
+    assert torch.equal(DP[0:2], DP[N:N + 2]) and torch.equal(DP[N + 2:], DP[2:4])
+    assert torch.equal(p, DP)
+
+
+def test_fp64_solution_matches_oracle_solve():
+    from oracle import pyflow_oracle as O
+    N = 32
+    box, h = make_box(N, torch.float64)
+    src = zero_mean_source(N, torch.float64, 5)
+    DP = torch.zeros(N + 4, N + 4, N + 4, dtype=torch.float64, device='cuda')
+    p = torch.zeros_like(DP)
+    res, its, st = box.bicgstab_mpir(DP, p, src.clone(), 1.0, 1e-3, 1e-12, 20000)
+    g = O.Grid(N, N, N)
+    S = O.BiCGStab(g, 1.0, 1e-3, 1e-12, 20000)
+    fDP = O.Field(g); fp = O.Field(g)
+    S.solve(fDP, fp, src.cpu().clone())
+    err = rel_l2_mean_free(cpu(DP), fDP.var.numpy())
+    print('MPIR %d inner iterations (status %d, res %.2e); oracle %d iterations; err %.3e'
+          % (its, st, res, S.iterations, err))
+    assert st == 0
+    assert err <= 1e-10
+
+
+def run_steps(u0, v0, w0, N, nu, dt, steps, tol, dtype, maxit=20000):
+    import types
+    from pyflow_amd import domain
+    from helpers import gpu_env
+    d, g, m = gpu_env((N, N, N), dtype=dtype)
+    cfg = types.SimpleNamespace(simDt=dt, equationMode='NS', advancerName='RK4', mu=nu, rho=1.0,
+                                dtypeTorch=dtype, SFSmodel='none', pSolverMode='bicgstab_mpir',
+                                min_pressure_residual=tol, max_pressure_iterations=maxit)
+    data_IC = np.stack([u0, v0, w0, np.zeros_like(u0)], axis=-1)
+    D = domain.Domain(cfg, None, d, data_IC, g, m, ['U', 'V', 'W', 'P'], 0.0)
+    st = []
+    for _ in range(steps):
+        D.forwardStep(dt)
+        st.append((D.poisson.status, D.poisson.iterations, D.max_resP))
+    torch.cuda.synchronize()
+    return D, st
+
+
+def test_fp64_ten_steps_match_oracle():
+    N = 32
+    u0, v0, w0, dt = pp_case(N, seed=21)
+    D, st = run_steps(u0, v0, w0, N, 0.01, dt, 10, 1e-10, torch.float64)
+    B, _ = run_oracle(u0, v0, w0, N, 0.01, dt, 10, 1e-10)
+    ev = max(rel_l2(cpu(getattr(D, 'state_%s_P' % c).var), getattr(B, c).var.numpy()) for c in 'uvw')
+    print('velocity err %.3e; solves %s' % (ev, st[:3]))
+    assert all(s[0] == 'converged' and s[2] <= 1e-10 for s in st)
+    assert ev <= 1e-10
+
+
+def test_fp32_reaches_1e8_and_states_error():
+    N = 64
+    u0, v0, w0, dt = pp_case(N, seed=7)
+    D, st = run_steps(u0, v0, w0, N, 0.01, dt, 10, 1e-8, torch.float32)
+    B, _ = run_oracle(u0, v0, w0, N, 0.01, dt, 10, 1e-10)
+    ev = max(rel_l2(cpu(getattr(D, 'state_%s_P' % c).var), getattr(B, c).var.numpy()) for c in 'uvw')
+    ep = rel_l2_mean_free(cpu(D.state_p_P.var), B.p.var.numpy())
+    print('FP32 + FP64 refinement (tol 1e-8) vs FP64 oracle after 10 steps: velocity %.3e '
+          'pressure %.3e; solves %s' % (ev, ep, st))
+    assert all(s[0] == 'converged' and s[2] <= 1e-8 for s in st)
+    assert ev <= 1e-5
+
+
+def test_unconverged_inner_solves_roll_back_to_best_iterate():
+    """Inner solves capped at 30 iterations with an unreachable inner
+    tolerance all stop unconverged, so every refinement step goes through the
+    re-run-to-best-iterate path; the refinement must still converge and
+    report the residual of what it returns."""
+    N = 40
+    box, h = make_box(N, torch.float64)
+    box.mpir_configure(1e-9, 30, 200)
+    src = zero_mean_source(N, torch.float64, 9)
+    b = src.double() * (1.0 / 1e-3) * (-(h ** 3))
+    DP = torch.zeros(N + 4, N + 4, N + 4, dtype=torch.float64, device='cuda')
+    p = torch.zeros_like(DP)
+    res, its, st = box.bicgstab_mpir(DP, p, src.clone(), 1.0, 1e-3, 1e-10, 50000)
+    x = DP[2:-2, 2:-2, 2:-2]
+    true_res = float((b - lap_fp64(x, h)).abs().max() / b.abs().max())
+    print('status %d, %d inner iterations, returned %.3e, recomputed %.3e' % (st, its, res, true_res))
+    assert st == 0 and res <= 1e-10
+    assert abs(true_res - res) <= 1e-3 * res + 1e-14
