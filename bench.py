@@ -121,15 +121,23 @@ def cpu_baseline(u0, v0, w0, N, nu, dt, tol, maxit, steps=1):
 
 def calibrate_copy(mib):
     """A plain device copy of known size (the reference point for the HBM byte
-    counters): 5 copies of `mib` MiB."""
+    counters): 5 copies of `mib` MiB, timed with events (read + write bytes per
+    second of the last 4, i.e. the achievable streaming bandwidth)."""
     import torch
     n = int(mib) << 20
     a = torch.empty(n, dtype=torch.uint8, device='cuda')
     b = torch.ones(n, dtype=torch.uint8, device='cuda')
-    for _ in range(5):
+    a.copy_(b)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(4):
         a.copy_(b)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({'copy_bytes': n, 'copies': 5}))
+    ms = e0.elapsed_time(e1) / 4
+    print(json.dumps({'copy_bytes': n, 'copies': 5, 'ms_per_copy': round(ms, 4),
+                      'GB/s_read_plus_write': round(2 * n / ms / 1e6, 1)}))
 
 
 def main():

@@ -213,6 +213,23 @@ __device__ __forceinline__ bool tile_scalars(KrylovCtl* __restrict__ ctl,
   return true;
 }
 
+// Tile of a block.  The hardware deals consecutive block ids round robin
+// over the 8 XCDs; the ids are renumbered so that each XCD gets a contiguous
+// run of tiles, z fastest, then y.  A run then covers whole z rows of tiles,
+// so the z-halo cells (single-cell loads whose cache lines belong to the
+// z-neighbour tile) and all but the run's edge y halos are lines that a
+// neighbour tile on the same XCD brings into that XCD's L2.
+__device__ __forceinline__ void tile_of_block(const TileGrid& tg, const Blk& blk, int& tz_i,
+                                              int& ty_i, int& cx_i) {
+  const int nb = blk.n;
+  int b = blk.id;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  tz_i = b % tg.ntz;
+  const int rest = b / tg.ntz;
+  ty_i = rest % tg.nty;
+  cx_i = rest / tg.nty;
+}
+
 template <typename T, int MODE, int TZ, int TY>
 __device__ __forceinline__ void bicg_tile_kernel_body(
     Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,
@@ -224,16 +241,8 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
   if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, c1, c2)) return;
   const FieldOp<T, MODE> F{r, a_old, v_in, c1, c2};
 
-  // XCD-aware block order: the hardware deals consecutive block ids round
-  // robin over 8 XCDs; renumber so each XCD gets a contiguous run of tiles
-  // (y fastest, so y-neighbour tiles share an L2).
-  const int nb = blk.n;
-  int b = blk.id;
-  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-  const int ty_i = b % tg.nty;
-  const int rest = b / tg.nty;
-  const int tz_i = rest % tg.ntz;
-  const int cx_i = rest / tg.ntz;
+  int tz_i, ty_i, cx_i;
+  tile_of_block(tg, blk, tz_i, ty_i, cx_i);
   const int z0 = tz_i * TZ, y0 = ty_i * TY, x0 = cx_i * tg.xc;
   const int x1 = min(x0 + tg.xc, g.nx);
 
@@ -366,13 +375,8 @@ __device__ __forceinline__ void bicg_tile2_body(
   };
   auto eval2 = [&](const Raw2& w) -> Pair<T> { return Pair<T>{F.eval(w.a), F.eval(w.b)}; };
 
-  const int nb = blk.n;
-  int b = blk.id;
-  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-  const int ty_i = b % tg.nty;
-  const int rest = b / tg.nty;
-  const int tz_i = rest % tg.ntz;
-  const int cx_i = rest / tg.ntz;
+  int tz_i, ty_i, cx_i;
+  tile_of_block(tg, blk, tz_i, ty_i, cx_i);
   const int z0 = tz_i * TZ, y0 = ty_i * TY, x0 = cx_i * tg.xc;
   const int x1 = min(x0 + tg.xc, g.nx);
 
