@@ -180,7 +180,7 @@ def main():
     value = ncell * K / r['wall'] / 1e6
     ms = r['wall'] / K * 1e3
     kbar = sum(r['its']) / len(r['its'])
-    alg_bytes = sum(ncell * w * (65 + 17 * k) for k in r['its'])
+    alg_bytes = sum(ncell * w * (65 + 15 * k) for k in r['its'])
     out = {
         'metric': 'Mcell-steps/s (RK4 + BiCGStab time step, %d^3 %s)' % (a.N, a.dtype.upper()),
         'value': round(value, 3), 'unit': 'Mcell-steps/s', 'n_gpus': 1, 'steps': K,
@@ -199,7 +199,7 @@ def main():
     }
     if r['prof']:
         per = {}
-        for name, wmul in (('KA', 6), ('KB', 4), ('KC', 7)):
+        for name, wmul in (('KA', 6), ('KB', 2), ('KC', 7)):
             tot, n = r['prof'][name]
             if n:
                 avg = tot / n / 1e3

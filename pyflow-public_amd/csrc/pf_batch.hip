@@ -105,14 +105,13 @@ BoxDesc<T> make_desc(pf_box* h, void* const* f, void* src, const double* prm, do
   d.p = (T*)f[7];
   d.src = (T*)src;
   d.x = (T*)h->x;
-  d.r = (T*)h->r;
+  d.rk[0] = (T*)h->r;
+  d.rk[1] = (T*)h->s;   // second residual buffer
   d.rh = (T*)h->rh;
   d.pk[0] = (T*)h->p[0];
   d.pk[1] = (T*)h->p[1];
   d.vk[0] = (T*)h->v[0];
   d.vk[1] = (T*)h->v[1];
-  d.s = (T*)h->s;
-  d.t = (T*)h->t;
   d.ctl = h->ctl;
   d.pA = part_buf(h, 0);
   d.pB = part_buf(h, 1);
@@ -238,9 +237,7 @@ int record_chunk(pf_batch* b) {
   for (int q = 0; q < kChunk; ++q) {
     const int par = q & 1;
     if (int rc = tiles_launch(b, L, kBatchKB, par)) return rc;
-    if (int rc = batch_krylov_launch(b->dtype, kBatchKC, par, b->desc, L.red.view(), L.red.total,
-                                     0, 0, b->st))
-      return rc;
+    if (int rc = tiles_launch(b, L, kBatchKC, par)) return rc;
     if (int rc = tiles_launch(b, L, kBatchKA, 1 - par)) return rc;
   }
   if (int rc = batch_ctl_launch(b->dtype, kBatchCtlFlags, b->desc, nullptr, b->n, b->flags_dev,

@@ -24,6 +24,7 @@ struct KrylovCtl {
   int done;        // set only by KA/KB, whose early returns are then uniform
   int status;       // PF_SOLVE_* code
   int halt;         // set by KC on a <t,t> breakdown; the next KA turns it into done
+  int npart;       // per-block partials in the <r,r_hat>, max|r| buffer (init or KC grid)
   int hist_cap;     // entries of hist (0: no history kept)
   double* hist;     // hist[k-1] = max|r_k| / max|r0| for k = 1 .. min(iter, hist_cap)
 };

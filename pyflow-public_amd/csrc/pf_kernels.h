@@ -155,7 +155,7 @@ struct BoxDesc {
   T* DP;                    // pressure correction (padded)
   T* p;                     // pressure (padded), += DP
   T* src;                   // Poisson source (interior)
-  T *x, *r, *rh, *pk[2], *vk[2], *s, *t;   // Krylov vectors (interior)
+  T *x, *rk[2], *rh, *pk[2], *vk[2];       // Krylov vectors (interior)
   KrylovCtl* ctl;
   double *pA, *pB, *pC, *pS;                 // partial-sum buffers
   double* out;              // 8 doubles: maxU, maxV, maxW, sum(u^2..), maxDiv, res, iters, status

@@ -5,7 +5,12 @@
 // one per global reduction of the recurrence:
 //   KA: p = (p - omega*v)*beta + r, v = A p,        partial <v, r_hat>
 //   KB: s = r - alpha*v,            t = A s,        partials <s,t>, <t,t>
-//   KC: x += alpha*p; x += omega*s; r = s - omega*t, partials <r,r_hat>, max|r|
+//   KC: s, t again; x += alpha*p; x += omega*s; r' = s - omega*t,
+//                                                   partials <r',r_hat>, max|r'|
+// KB stores nothing: KC recomputes s and t = A s from r and v (the same
+// operations, so the same bits), which costs a stencil but saves writing and
+// re-reading s and t (15 instead of 17 vector passes per iteration).  r is
+// double-buffered because KC reads r around each cell while writing r'.
 // The scalars never leave the device: every block of a kernel reduces the
 // previous kernel's per-block partials itself (same data, same order, same
 // bits in every block), and block 0 records them in the box's KrylovCtl.
@@ -34,13 +39,14 @@ enum { kPartA = 0, kPartB = 1, kPartC = 2, kPartS = 3 };
 
 // ---------------------------------------------------------------------------
 // Control block
-__global__ void ctl_init_kernel(KrylovCtl* ctl, double tol, int maxit, double* hist,
+__global__ void ctl_init_kernel(KrylovCtl* ctl, double tol, int maxit, int npart, double* hist,
                                 int hist_cap) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     ctl->rho1 = 1.0; ctl->rho2 = 1.0; ctl->alpha = 0.0; ctl->omega = 1.0;
     ctl->res0 = 0.0; ctl->max_res = 0.0; ctl->tol = tol;
     ctl->iter = 0; ctl->maxit = maxit; ctl->done = 0; ctl->status = PF_SOLVE_CONVERGED;
     ctl->halt = 0;
+    ctl->npart = npart;
     ctl->hist = hist; ctl->hist_cap = hist ? hist_cap : 0;
   }
 }
@@ -117,16 +123,16 @@ __global__ void __launch_bounds__(kBlock) bicg_init_kernel(
 // loads of plane x+1 (and of the next halo) are issued before the barrier of
 // plane x.  Cells of partial tiles wrap periodically, so the LDS plane always
 // holds valid neighbours; only their outputs are masked.
-enum { kModeA = 0, kModeB = 1 };
+enum { kModeA = 0, kModeB = 1, kModeC = 2 };
 
 // Field recomputed at a cell: KA p = (p_old - omega v_old)*beta + r,
-// KB s = r - alpha v.
+// KB and KC s = r - alpha v.
 template <typename T, int MODE>
 struct FieldOp {
   const T* __restrict__ r;
   const T* __restrict__ a;   // KA: p_old; KB: unused
   const T* __restrict__ v;   // KA: v_old; KB: v
-  T c1, c2;                  // KA: (-omega, beta); KB: (-alpha, -)
+  T c1, c2;                  // KA: (-omega, beta); KB, KC: (-alpha, -)
   struct Raw {
     T v, a, r;
   };
@@ -145,16 +151,52 @@ struct FieldOp {
   __device__ __forceinline__ T at(int64_t q) const { return eval(load(q)); }
 };
 
-// The scalars of KA / KB from the control block and the previous kernel's
-// partials; false when the block must return (solve done, converged,
-// capped or broken down).
+// Scalars of a tiled kernel: the FieldOp coefficients, and for KC alpha,
+// omega, -omega and the <t,t> == 0 breakdown flag.
+template <typename T>
+struct TileScal {
+  T c1 = (T)0, c2 = (T)0;
+  T ta = (T)0, to = (T)0, tmo = (T)0;
+  bool brk = false;
+};
+
+// The scalars of KA / KB / KC from the control block and the previous
+// kernel's partials; false when the block must return (solve done,
+// converged, capped or broken down).
 template <typename T, int MODE, int NT>
 __device__ __forceinline__ bool tile_scalars(KrylovCtl* __restrict__ ctl,
                                              const double* __restrict__ part_in, int nblk_in,
-                                             const Blk& blk, T& c1, T& c2) {
+                                             const Blk& blk, TileScal<T>& sc) {
   const KrylovCtl c0 = snapshot(ctl);
   if (c0.done) return false;
-  c2 = (T)0;
+  T& c1 = sc.c1;
+  T& c2 = sc.c2;
+  if (MODE == kModeC) {
+    double red[2];
+    const int ops[2] = {0, 0};
+    reduce_partials<2, NT>(part_in, nblk_in, red, ops);
+    const double buf1 = red[0], buf2 = red[1];
+    sc.ta = (T)c0.alpha;
+    c1 = (T)(-c0.alpha);
+    if (buf2 == 0) {                                  // pressure.py:251, 266-267
+      sc.brk = true;
+      if (lead(blk)) {
+        ctl->status = PF_SOLVE_BREAKDOWN_B;
+        ctl->halt = 1;
+      }
+      return true;
+    }
+    const double omega = buf1 / buf2;                 // pressure.py:268
+    if (lead(blk)) {
+      ctl->omega = omega;
+      ctl->rho2 = c0.rho1;                            // pressure.py:276
+      ctl->iter = c0.iter + 1;
+      ctl->npart = blk.n;
+    }
+    sc.to = (T)omega;
+    sc.tmo = (T)(-omega);
+    return true;
+  }
   if (MODE == kModeA) {
     if (c0.halt) {
       if (lead(blk)) ctl->done = 1;
@@ -162,7 +204,7 @@ __device__ __forceinline__ bool tile_scalars(KrylovCtl* __restrict__ ctl,
     }
     double red[2];
     const int ops[2] = {0, 1};
-    reduce_partials<2, NT>(part_in, nblk_in, red, ops);
+    reduce_partials<2, NT>(part_in, c0.npart, red, ops);   // init's or KC's partials
     const double rho1 = red[0];
     if (c0.iter == 0) {
       if (lead(blk)) ctl->res0 = red[1];
@@ -237,9 +279,9 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
     const T* __restrict__ r, const T* __restrict__ rh, const T* __restrict__ a_old,
     const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L, Blk blk) {
   constexpr int NT = TZ * TY;
-  T c1, c2;
-  if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, c1, c2)) return;
-  const FieldOp<T, MODE> F{r, a_old, v_in, c1, c2};
+  TileScal<T> sc;
+  if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, sc)) return;
+  const FieldOp<T, MODE> F{r, a_old, v_in, sc.c1, sc.c2};
 
   int tz_i, ty_i, cx_i;
   tile_of_block(tg, blk, tz_i, ty_i, cx_i);
@@ -252,6 +294,14 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
   const bool active = zr < g.nz && yr < g.ny;
   const int z = zr % g.nz, y = yr % g.ny;
   const int64_t col = (int64_t)y * g.isy + z;
+  if (MODE == kModeC && sc.brk) {       // <t,t> == 0: only x += alpha*p
+    if (active)
+      for (int x = x0; x < x1; ++x) {
+        const int64_t q = (int64_t)x * g.isx + col;
+        f_out[q] = fma(a_old[q], sc.ta, f_out[q]);
+      }
+    return;
+  }
 
   // halo assignment: 2*TZ y-halo cells, then 2*TY z-halo cells
   int hz = -1, hy = -1, hslot_y = 0, hslot_z = 0;
@@ -295,7 +345,9 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
     Raw rnn = {}, hrnn = {};
     if (x + 2 <= x1) rnn = F.load(xoff(x + 2) + col);
     if (is_halo && x + 2 < x1) hrnn = F.load(xoff(x + 2) + hcol);
-    const T rhc = (MODE == kModeA && active) ? rh[xo + col] : (T)0;
+    const T rhc = (MODE != kModeB && active) ? rh[xo + col] : (T)0;
+    const T xc = (MODE == kModeC && active) ? f_out[xo + col] : (T)0;
+    const T pc = (MODE == kModeC && active) ? a_old[xo + col] : (T)0;
     const T fp = F.eval(rn);
     const T hn = (is_halo && x + 1 < x1) ? F.eval(hrn) : (T)0;
     T(*pl)[TZ + 2] = plane[x & 1];
@@ -306,13 +358,22 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
       const T fym = pl[ly][lz + 1], fyp = pl[ly + 2][lz + 1];
       const T fzm = pl[ly + 1][lz], fzp = pl[ly + 1][lz + 2];
       const T af = lap7(L, fm, fc, fp, fym, fyp, fzm, fzp);
-      f_out[xo + col] = fc;
-      af_out[xo + col] = af;
       if (MODE == kModeA) {
+        f_out[xo + col] = fc;
+        af_out[xo + col] = af;
         acc0 += (double)af * (double)rhc;
-      } else {
+      } else if (MODE == kModeB) {
         acc0 += (double)fc * (double)af;
         acc1 += (double)af * (double)af;
+      } else {
+        // KC: fc = s, af = t = A s
+        T xv = fma(pc, sc.ta, xc);                  // pressure.py:251
+        xv = fma(fc, sc.to, xv);                    // pressure.py:270
+        f_out[xo + col] = xv;
+        const T rnew = fma(af, sc.tmo, fc);         // pressure.py:274-275
+        af_out[xo + col] = rnew;
+        acc0 += (double)rnew * (double)rhc;
+        acc1 = nmax(acc1, fabs((double)rnew));
       }
     }
     fm = fc;
@@ -327,8 +388,9 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
     block_reduce<1, NT>(accv, ops1);
     write_partials<1>(part_out, accv, blk.id);
   } else {
+    // KB: <s,t>, <t,t>; KC: <r,r_hat>, max|r|
     double accv[2] = {acc0, acc1};
-    const int ops2[2] = {0, 0};
+    const int ops2[2] = {0, MODE == kModeC ? 1 : 0};
     block_reduce<2, NT>(accv, ops2);
     write_partials<2>(part_out, accv, blk.id);
   }
@@ -354,9 +416,9 @@ __device__ __forceinline__ void bicg_tile2_body(
     const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L, Blk blk) {
   constexpr int NZT = TZ / 2;
   constexpr int NT = NZT * TY;
-  T c1, c2;
-  if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, c1, c2)) return;
-  const FieldOp<T, MODE> F{r, a_old, v_in, c1, c2};
+  TileScal<T> sc;
+  if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, sc)) return;
+  const FieldOp<T, MODE> F{r, a_old, v_in, sc.c1, sc.c2};
   using Raw = typename FieldOp<T, MODE>::Raw;
   struct Raw2 {
     Raw a, b;
@@ -385,6 +447,16 @@ __device__ __forceinline__ void bicg_tile2_body(
   const int yr = y0 + ly;
   const bool active = yr < g.ny;
   const int64_t col = (int64_t)(yr % g.ny) * g.isy + (z0 + 2 * lz);
+  if (MODE == kModeC && sc.brk) {       // <t,t> == 0: only x += alpha*p
+    if (active)
+      for (int x = x0; x < x1; ++x) {
+        const int64_t q = (int64_t)x * g.isx + col;
+        const Pair<T> pv = ld2(a_old, q), xv = ld2(f_out, q);
+        *reinterpret_cast<Pair<T>*>(f_out + q) =
+            Pair<T>{fma(pv.a, sc.ta, xv.a), fma(pv.b, sc.ta, xv.b)};
+      }
+    return;
+  }
 
   // halo: threads [0, 2*NZT) load the y-halo rows in pairs, the next 2*TY
   // threads one z-halo cell each
@@ -436,8 +508,12 @@ __device__ __forceinline__ void bicg_tile2_body(
     Raw2 rnn = {}, hrnn = {};
     if (x + 2 <= x1) rnn = load2(xoff(x + 2) + col);
     if (is_halo && x + 2 < x1) hrnn = loadh(xoff(x + 2) + hcol);
-    Pair<T> rhc = {(T)0, (T)0};
-    if (MODE == kModeA && active) rhc = ld2(rh, xo + col);
+    Pair<T> rhc = {(T)0, (T)0}, xc = {(T)0, (T)0}, pc = {(T)0, (T)0};
+    if (MODE != kModeB && active) rhc = ld2(rh, xo + col);
+    if (MODE == kModeC && active) {
+      xc = ld2(f_out, xo + col);
+      pc = ld2(a_old, xo + col);
+    }
     const Pair<T> fp = eval2(rn);
     Pair<T> hn = {(T)0, (T)0};
     if (is_halo && x + 1 < x1) hn = evalh(hrn);
@@ -454,16 +530,29 @@ __device__ __forceinline__ void bicg_tile2_body(
                          fc.b);
       const T af1 = lap7(L, fm.b, fc.b, fp.b, pl[ly][zl + 1], pl[ly + 2][zl + 1], fc.a,
                          pl[ly + 1][zl + 2]);
-      *reinterpret_cast<Pair<T>*>(f_out + xo + col) = fc;
-      *reinterpret_cast<Pair<T>*>(af_out + xo + col) = Pair<T>{af0, af1};
       if (MODE == kModeA) {
+        *reinterpret_cast<Pair<T>*>(f_out + xo + col) = fc;
+        *reinterpret_cast<Pair<T>*>(af_out + xo + col) = Pair<T>{af0, af1};
         acc0 += (double)af0 * (double)rhc.a;
         acc0 += (double)af1 * (double)rhc.b;
-      } else {
+      } else if (MODE == kModeB) {
         acc0 += (double)fc.a * (double)af0;
         acc0 += (double)fc.b * (double)af1;
         acc1 += (double)af0 * (double)af0;
         acc1 += (double)af1 * (double)af1;
+      } else {
+        // KC: fc = s, af = t = A s (pressure.py:251, 270, 274-275)
+        T xa = fma(pc.a, sc.ta, xc.a);
+        T xb = fma(pc.b, sc.ta, xc.b);
+        xa = fma(fc.a, sc.to, xa);
+        xb = fma(fc.b, sc.to, xb);
+        *reinterpret_cast<Pair<T>*>(f_out + xo + col) = Pair<T>{xa, xb};
+        const T ra = fma(af0, sc.tmo, fc.a), rb = fma(af1, sc.tmo, fc.b);
+        *reinterpret_cast<Pair<T>*>(af_out + xo + col) = Pair<T>{ra, rb};
+        acc0 += (double)ra * (double)rhc.a;
+        acc0 += (double)rb * (double)rhc.b;
+        acc1 = nmax(acc1, fabs((double)ra));
+        acc1 = nmax(acc1, fabs((double)rb));
       }
     }
     fm = fc;
@@ -478,8 +567,9 @@ __device__ __forceinline__ void bicg_tile2_body(
     block_reduce<1, NT>(accv, ops1);
     write_partials<1>(part_out, accv, blk.id);
   } else {
+    // KB: <s,t>, <t,t>; KC: <r,r_hat>, max|r|
     double accv[2] = {acc0, acc1};
-    const int ops2[2] = {0, 0};
+    const int ops2[2] = {0, MODE == kModeC ? 1 : 0};
     block_reduce<2, NT>(accv, ops2);
     write_partials<2>(part_out, accv, blk.id);
   }
@@ -530,66 +620,6 @@ void launch_tile(const TileGrid& tg, hipStream_t s, hipEvent_t e0, hipEvent_t e1
   }
 #undef PF_TILE2
 #undef PF_TILE
-}
-
-// KC: omega = <s,t>/<t,t>, x += alpha*p, x += omega*s, r = s - omega*t.
-template <typename T>
-__device__ __forceinline__ void bicg_kc_kernel_body(
-    Geom g, int nblk, KrylovCtl* __restrict__ ctl, const double* __restrict__ partB,
-    double* __restrict__ partC, T* __restrict__ x, const T* __restrict__ p,
-    const T* __restrict__ s, const T* __restrict__ t, T* __restrict__ r,
-    const T* __restrict__ rh, Blk blk) {
-  const KrylovCtl c0 = snapshot(ctl);
-  if (c0.done) return;
-  const double alpha = c0.alpha;
-  const double rho1 = c0.rho1;
-  const int k = c0.iter;
-  double red[2];
-  const int ops[2] = {0, 0};
-  reduce_partials<2>(partB, nblk, red, ops);
-  const double buf1 = red[0], buf2 = red[1];
-  const T ta = (T)alpha;
-  if (buf2 == 0) {                                  // pressure.py:251, 266-267
-    for (uint32_t idx = blk.id * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
-         idx += blk.n * blockDim.x)
-      x[idx] = fma(p[idx], ta, x[idx]);
-    if (lead(blk)) {
-      ctl->status = PF_SOLVE_BREAKDOWN_B;
-      ctl->halt = 1;
-    }
-    return;
-  }
-  const double omega = buf1 / buf2;                 // pressure.py:268
-  if (lead(blk)) {
-    ctl->omega = omega;
-    ctl->rho2 = rho1;                               // pressure.py:276
-    ctl->iter = k + 1;
-  }
-  const T to = (T)omega, tmo = (T)(-omega);
-  double acc[2] = {0.0, 0.0};
-  for (uint32_t idx = blk.id * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
-       idx += blk.n * blockDim.x) {
-    const T sc = s[idx];
-    T xv = fma(p[idx], ta, x[idx]);                 // pressure.py:251
-    xv = fma(sc, to, xv);                           // pressure.py:270
-    x[idx] = xv;
-    const T rn = fma(t[idx], tmo, sc);              // pressure.py:274-275
-    r[idx] = rn;
-    acc[0] += (double)rn * (double)rh[idx];
-    acc[1] = nmax(acc[1], fabs((double)rn));
-  }
-  const int ops2[2] = {0, 1};
-  block_reduce<2>(acc, ops2);
-  write_partials<2>(partC, acc, blk.id);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kBlock) bicg_kc_kernel(
-    Geom g, int nblk, KrylovCtl* __restrict__ ctl, const double* __restrict__ partB,
-    double* __restrict__ partC, T* __restrict__ x, const T* __restrict__ p,
-    const T* __restrict__ s, const T* __restrict__ t, T* __restrict__ r,
-    const T* __restrict__ rh) {
-  bicg_kc_kernel_body<T>(g, nblk, ctl, partB, partC, x, p, s, t, r, rh, this_blk());
 }
 
 // DP (padded) := x with periodic overlap (pressure.py:293), then
@@ -840,39 +870,32 @@ __global__ void __launch_bounds__(kBlock) batch_init_kernel(const BoxDesc<T>* __
   PF_BATCH_BOX;
   const int64_t o = d.g.poff;
   bicg_init_kernel_body<T, true>(d.g, d.src, d.u[0] + o, d.u[1] + o, d.u[2] + o, d.dxi, d.dyi,
-                                 d.dzi, d.sc1, d.sc2, d.r, d.rh, d.x, d.pk[0], d.vk[0], d.pC,
+                                 d.dzi, d.sc1, d.sc2, d.rk[0], d.rh, d.x, d.pk[0], d.vk[0], d.pC,
                                  blk);
 }
 
-// KA of iteration j (parity = j & 1): p[1-parity] from p[parity], v[parity];
-// KB of iteration j: s, t from r and v[1-parity].
+// Iteration j has parity j & 1.  KA of iteration j (launched with parity
+// j & 1): p[1-parity] from p[parity], v[parity] and r[parity]; KB and KC of
+// iteration j (launched with parity j & 1): s, t from r[parity] and
+// v[1-parity]; KC also reads p[1-parity] and writes r[1-parity].
 template <typename T, int MODE, int TZ, int TY, bool PAIR>
 __global__ void __launch_bounds__(PAIR ? TZ / 2 * TY : TZ * TY) batch_tile_kernel(
     const BoxDesc<T>* __restrict__ D, BatchList L, int parity) {
   PF_BATCH_BOX;
   const int old = parity, nw = 1 - parity, cur = 1 - parity;
-  const T* a_old = MODE == kModeA ? d.pk[old] : (const T*)nullptr;
+  const T* a_old = MODE == kModeA ? d.pk[old] : (MODE == kModeC ? d.pk[cur] : (const T*)nullptr);
   const T* v_in = MODE == kModeA ? d.vk[old] : d.vk[cur];
-  T* f_out = MODE == kModeA ? d.pk[nw] : d.s;
-  T* af_out = MODE == kModeA ? d.vk[nw] : d.t;
-  const int nin = MODE == kModeA ? d.nblk : d.tg.nblk;
-  const double* pin = MODE == kModeA ? d.pC : d.pA;
-  double* pout = MODE == kModeA ? d.pA : d.pB;
+  T* f_out = MODE == kModeA ? d.pk[nw] : (MODE == kModeC ? d.x : (T*)nullptr);
+  T* af_out = MODE == kModeA ? d.vk[nw] : (MODE == kModeC ? d.rk[1 - parity] : (T*)nullptr);
+  const double* pin = MODE == kModeA ? d.pC : (MODE == kModeB ? d.pA : d.pB);
+  double* pout = MODE == kModeA ? d.pA : (MODE == kModeB ? d.pB : d.pC);
+  const T* r = d.rk[parity];
   if (PAIR)
-    bicg_tile2_body<T, MODE, TZ, TY>(d.g, d.tg, nin, d.ctl, pin, pout, d.r, d.rh, a_old, v_in,
-                                     f_out, af_out, d.L, blk);
+    bicg_tile2_body<T, MODE, TZ, TY>(d.g, d.tg, d.tg.nblk, d.ctl, pin, pout, r, d.rh, a_old,
+                                     v_in, f_out, af_out, d.L, blk);
   else
-    bicg_tile_kernel_body<T, MODE, TZ, TY>(d.g, d.tg, nin, d.ctl, pin, pout, d.r, d.rh, a_old,
-                                           v_in, f_out, af_out, d.L, blk);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kBlock) batch_kc_kernel(const BoxDesc<T>* __restrict__ D,
-                                                          BatchList L, int parity) {
-  PF_BATCH_BOX;
-  const int cur = 1 - parity;
-  bicg_kc_kernel_body<T>(d.g, d.tg.nblk, d.ctl, d.pB, d.pC, d.x, d.pk[cur], d.s, d.t, d.r, d.rh,
-                         blk);
+    bicg_tile_kernel_body<T, MODE, TZ, TY>(d.g, d.tg, d.tg.nblk, d.ctl, pin, pout, r, d.rh,
+                                           a_old, v_in, f_out, af_out, d.L, blk);
 }
 
 template <typename T>
@@ -931,6 +954,7 @@ __global__ void __launch_bounds__(kBlock) batch_ctl_kernel(const BoxDesc<T>* __r
     c->res0 = 0.0; c->max_res = 0.0; c->tol = D[b].tol;
     c->iter = 0; c->maxit = D[b].maxit; c->done = 0; c->status = PF_SOLVE_CONVERGED;
     c->halt = 0;
+    c->npart = D[b].nblk;
     c->hist = nullptr; c->hist_cap = 0;
   } else {
     flags[i] = c->done;
@@ -946,18 +970,21 @@ int batch_krylov_launch(int dtype, int what, int parity, const void* desc, const
     const dim3 gb(nblocks), tb(kBlock);
     switch (what) {
       case kBatchInit: hipLaunchKernelGGL(batch_init_kernel<T>, gb, tb, 0, s, D, L); break;
-      case kBatchKC: hipLaunchKernelGGL(batch_kc_kernel<T>, gb, tb, 0, s, D, L, parity); break;
       case kBatchScatter: hipLaunchKernelGGL(batch_scatter_kernel<T>, gb, tb, 0, s, D, L); break;
       case kBatchProject: hipLaunchKernelGGL(batch_project_kernel<T>, gb, tb, 0, s, D, L); break;
       case kBatchStats: hipLaunchKernelGGL(batch_stats_kernel<T>, gb, tb, 0, s, D, L); break;
       case kBatchKA:
-      case kBatchKB: {
+      case kBatchKB:
+      case kBatchKC: {
 #define PF_BTILE(TZ, TY, PAIR)                                                                 \
   if (what == kBatchKA)                                                                          \
     hipLaunchKernelGGL((batch_tile_kernel<T, kModeA, TZ, TY, PAIR>), gb,                         \
                        dim3(PAIR ? TZ / 2 * TY : TZ * TY), 0, s, D, L, parity);                  \
-  else                                                                                           \
+  else if (what == kBatchKB)                                                                     \
     hipLaunchKernelGGL((batch_tile_kernel<T, kModeB, TZ, TY, PAIR>), gb,                         \
+                       dim3(PAIR ? TZ / 2 * TY : TZ * TY), 0, s, D, L, parity);                  \
+  else                                                                                           \
+    hipLaunchKernelGGL((batch_tile_kernel<T, kModeC, TZ, TY, PAIR>), gb,                         \
                        dim3(PAIR ? TZ / 2 * TY : TZ * TY), 0, s, D, L, parity)
         switch (tz * 1000 + ty) {
           case 64004: PF_BTILE(64, 4, false); break;
@@ -1021,12 +1048,10 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
   const double vol = h->dx * h->dy * h->dz;
   const Lap<T> L = make_lap<T>(h);
   T* x = (T*)h->x;
-  T* r = (T*)h->r;
+  T* Rr[2] = {(T*)h->r, (T*)h->s};   // r of even / odd iterations
   T* rh = (T*)h->rh;
   T* P[2] = {(T*)h->p[0], (T*)h->p[1]};
   T* Vv[2] = {(T*)h->v[0], (T*)h->v[1]};
-  T* sv = (T*)h->s;
-  T* tv = (T*)h->t;
   double* pA = part_buf(h, kPartA);
   double* pB = part_buf(h, kPartB);
   double* pC = part_buf(h, kPartC);
@@ -1035,18 +1060,19 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
     hist = h->hist;
     hist_cap = h->hist_cap;
   }
-  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(1), 0, s, h->ctl, tol, maxit, hist, hist_cap);
+  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(1), 0, s, h->ctl, tol, maxit, nb, hist,
+                     hist_cap);
   PF_LAUNCH_CHECK("ctl_init_kernel");
   const T dxi = (T)(1.0 / h->dx), dyi = (T)(1.0 / h->dy), dzi = (T)(1.0 / h->dz);
   const T sc1 = (T)src_scale, sc2 = vol_scale ? (T)(-1.0 * vol) : (T)1;
   if (U)
     hipLaunchKernelGGL((bicg_init_kernel<T, true>), dim3(nb), dim3(kBlock), 0, s, g, src,
-                       U + g.poff, V + g.poff, W + g.poff, dxi, dyi, dzi, sc1, sc2, r, rh, x,
+                       U + g.poff, V + g.poff, W + g.poff, dxi, dyi, dzi, sc1, sc2, Rr[0], rh, x,
                        P[0], Vv[0], pC);
   else
     hipLaunchKernelGGL((bicg_init_kernel<T, false>), dim3(nb), dim3(kBlock), 0, s, g, src,
                        (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, dxi, dyi, dzi,
-                       sc1, sc2, r, rh, x, P[0], Vv[0], pC);
+                       sc1, sc2, Rr[0], rh, x, P[0], Vv[0], pC);
   PF_LAUNCH_CHECK("bicg_init_kernel");
 
   // KA of iteration 0, then batches of (KB, KC, KA) until the device says done.
@@ -1070,12 +1096,14 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
   };
   int it = 0;  // iterations whose KA has been launched
   const TileGrid tg = make_tile_grid(g, (int)h->esize, h->tile_rows);
-  const int nbA = tg.nblk;   // blocks writing partials A and B
-  auto ka = [&](int old, int nw, hipEvent_t e0, hipEvent_t e1) {
-    launch_tile<T, kModeA>(tg, s, e0, e1, g, nb, h->ctl, pC, pA, r, rh, P[old], Vv[old], P[nw],
-                           Vv[nw], L);
+  const int nbA = tg.nblk;   // blocks of KA, KB and KC
+  // KA of iteration j reads r[j & 1] (its partial count comes from the
+  // control block: the init kernel's for j = 0, KC's after)
+  auto ka = [&](int j, int old, int nw, hipEvent_t e0, hipEvent_t e1) {
+    launch_tile<T, kModeA>(tg, s, e0, e1, g, nbA, h->ctl, pC, pA, Rr[j & 1], rh, P[old], Vv[old],
+                           P[nw], Vv[nw], L);
   };
-  launch(0, 0, [&](hipEvent_t e0, hipEvent_t e1) { ka(0, 1, e0, e1); });
+  launch(0, 0, [&](hipEvent_t e0, hipEvent_t e1) { ka(0, 0, 1, e0, e1); });
   PF_LAUNCH_CHECK("bicg_tile_kernel");
   const int cap = maxit < 0 ? 0 : maxit;
   bool first = true;
@@ -1086,18 +1114,19 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
     if (m > 1024) m = 1024;
     first = false;
     for (int q = 0; q < m; ++q) {
-      const int cur = (it + 1) & 1;   // buffers written by iteration `it`
+      const int cur = (it + 1) & 1;   // p, v written by iteration `it`
+      const int rc = it & 1;          // r read by iteration `it`
       launch(1, it, [&](hipEvent_t e0, hipEvent_t e1) {
-        launch_tile<T, kModeB>(tg, s, e0, e1, g, nbA, h->ctl, pA, pB, r, rh, (const T*)nullptr,
-                               Vv[cur], sv, tv, L);
+        launch_tile<T, kModeB>(tg, s, e0, e1, g, nbA, h->ctl, pA, pB, Rr[rc], rh,
+                               (const T*)nullptr, Vv[cur], (T*)nullptr, (T*)nullptr, L);
       });
       launch(2, it, [&](hipEvent_t e0, hipEvent_t e1) {
-        hipExtLaunchKernelGGL(bicg_kc_kernel<T>, dim3(nb), dim3(kBlock), 0, s, e0, e1, 0, g, nbA,
-                              h->ctl, pB, pC, x, P[cur], sv, tv, r, rh);
+        launch_tile<T, kModeC>(tg, s, e0, e1, g, nbA, h->ctl, pB, pC, Rr[rc], rh, P[cur],
+                               Vv[cur], x, Rr[1 - rc], L);
       });
       ++it;
       const int nxt = (it + 1) & 1;
-      launch(0, it, [&](hipEvent_t e0, hipEvent_t e1) { ka(cur, nxt, e0, e1); });
+      launch(0, it, [&](hipEvent_t e0, hipEvent_t e1) { ka(it, cur, nxt, e0, e1); });
     }
     PF_LAUNCH_CHECK("bicgstab batch");
     PF_HIP(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
