@@ -9,11 +9,13 @@ steps when fNameOut is set).  simulate(inputConfig) is the same loop
 returning the Domain and the per-step monitor values for programmatic use.
 """
 
+import math
 import os
 import sys
 import time
 
 import numpy as np
+import torch
 
 from . import domain
 from . import geometry as geo
@@ -63,6 +65,44 @@ def _write_restart(inputConfig, D, simTime):
     dr.writeNGArestart_parallel(fname + '_' + timeStr, D.data_all_CPU)
 
 
+def _target_file(inputConfig, it):
+    """inputConfig.getTargetFileName(it) (PyFlow.py:487) when the input
+    configuration defines it, else targetFileBaseStr + '%08d' (domain.py:224)."""
+    f = getattr(inputConfig, 'getTargetFileName', None)
+    if f is not None:
+        return f(it)
+    return inputConfig.targetFileBaseStr + '{:08d}'.format(it)
+
+
+def _load_target(inputConfig, D, itCount, simDt, verbose):
+    """PyFlow.py:486-526: read the window's target file, optionally project
+    it, seed the adjoint with sign(u - u_T) and return (model error, target
+    TKE).  The target's overlap cells are filled after reading; the
+    reference leaves them as they were, so its target divergence (and the
+    sign seed in the overlap) reads stale values at the box faces."""
+    from . import dataReader as dr
+    fname = _target_file(inputConfig, getattr(inputConfig, 'startFileIt', 0) + itCount)
+    dr.readNGArestart_parallel(fname, D.target_data_all_CPU)
+    T = (D.state_u_T, D.state_v_T, D.state_w_T)
+    for t in T:
+        t.update_border()
+    mu_, mv, mw, tke, mdiv = D.box.stats(*(t.var for t in T), D.source_P)
+    if verbose:
+        print('Target data initial TKE={:8.3E}, vel_div={:8.3E}'.format(tke, mdiv))
+    D.TargetDataVelCorr(simDt)
+    mu_, mv, mw, tke, mdiv = D.box.stats(*(t.var for t in T), D.source_P)
+    if verbose:
+        print('Target data final   TKE={:8.3E}, vel_div={:8.3E}, res_P={:8.3E}'
+              .format(tke, mdiv, D.max_resP))
+    U = (D.state_u_P, D.state_v_P, D.state_w_P)
+    A = (D.state_u_adj_P, D.state_v_adj_P, D.state_w_adj_P)
+    err = 0.0
+    for a, u, t in zip(A, U, T):
+        a.var.copy_(torch.sign(u.var - t.var))                     # PyFlow.py:518-520
+        err += float((u.interior() - t.interior()).abs().mean().item())   # PyFlow.py:523-531
+    return err, tke
+
+
 def simulate(inputConfig, verbose=True):
     comms, decomp, geometry, metric, D, startTime = setup(inputConfig)
     simDt = inputConfig.simDt
@@ -83,6 +123,11 @@ def simulate(inputConfig, verbose=True):
     st = D.stats()
     history = [dict(step=0, time=simTime, CFL=cfl(st), **st)]
     adj_history = []
+    train_history = []
+    if adjointTraining and not adjointVerification and not hasattr(getattr(D, 'sfsmodel', None),
+                                                                   'finalize'):
+        raise Exception('adjoint training against target data needs a trainable closure '
+                        "(SFSmodel 'MLP')")
     caseName = getattr(inputConfig, 'caseName', 'PyFlow')
     monitor = None
     if getattr(inputConfig, 'writeMonitor', True):
@@ -141,12 +186,12 @@ def simulate(inputConfig, verbose=True):
         if not adjointTraining:
             continue
         # adjoint inner loop (PyFlow.py:471-607)
-        if not adjointVerification:
-            raise NotImplementedError('adjoint training against target data files is not part '
-                                      'of this build; set adjointVerification')
-        D.state_u_adj_P.var.copy_(2.0 * (D.state_u_P.var - 0.0))      # PyFlow.py:480-482
-        D.state_v_adj_P.var.copy_(0.0 * (D.state_v_P.var - 0.0))
-        D.state_w_adj_P.var.copy_(0.0 * (D.state_w_P.var - 0.0))
+        if adjointVerification:
+            D.state_u_adj_P.var.copy_(2.0 * (D.state_u_P.var - 0.0))      # PyFlow.py:480-482
+            D.state_v_adj_P.var.copy_(0.0 * (D.state_v_P.var - 0.0))
+            D.state_w_adj_P.var.copy_(0.0 * (D.state_w_P.var - 0.0))
+        else:
+            model_error, target_TKE = _load_target(inputConfig, D, itCount, simDt, verbose)
         for a in (D.state_u_adj_P, D.state_v_adj_P, D.state_w_adj_P):  # PyFlow.py:528-530
             a.var.div_(npts)
         if verbose:
@@ -167,7 +212,17 @@ def simulate(inputConfig, verbose=True):
                 print("  Adj {:6d}   {:8.3E}   {:8.3E}   {:8.3E}   {:8.3E}   {:8.3E}   {:9s}   "
                       "{:9s}    {:8.3E}".format(itCount - itCountInnerUp, simTime, maxCFL, maxU,
                                                 maxV, maxW, ' ', ' ', D.max_resP))
-        new_adj = float(D.state_u_adj_P.var[nn, nn, nn].item())        # PyFlow.py:580-582
+        if adjointVerification:
+            new_adj = float(D.state_u_adj_P.var[nn, nn, nn].item())    # PyFlow.py:580-582
+        else:
+            D.sfsmodel.finalize(comms)                                 # PyFlow.py:585-588
+            D.sfsmodel.save()
+            norm_error = model_error / math.sqrt(target_TKE) if target_TKE > 0 else float('nan')
+            train_history.append(dict(epoch=D.sfsmodel.epoch, TKE=history[-1]['TKE'],
+                                      model_error=model_error, norm_error=norm_error))
+            if verbose:
+                print('Done adjoint iteration, training error={:12.5E}, norm error={:12.5E}'
+                      .format(model_error, norm_error))
         D.restore(numStepsInner)                                       # PyFlow.py:600-606
         simTime = simTimeCheckpoint + numStepsInner * simDt
     wall = time.time() - t0
@@ -175,6 +230,7 @@ def simulate(inputConfig, verbose=True):
         monitor.close()
     res = Result(D, geometry, history, wall)
     res.adj_history = adj_history
+    res.train_history = train_history
     res.objective = new_obj
     res.adjoint = new_adj
     return res

@@ -36,6 +36,19 @@ def readNGAconfig(fName, verbose=False):
     return xGrid, yGrid, zGrid, xper, yper, zper
 
 
+def writeNGAconfig(fName, xGrid, yGrid, zGrid, configName='PyFlow', xper=1, yper=1, zper=1,
+                   icyl=0):
+    """The config layout readNGAconfig reads (the reference only reads it):
+    name, flags, sizes, grid points, then an all-zero int32 nx*ny mask."""
+    nx, ny, nz = len(xGrid) - 1, len(yGrid) - 1, len(zGrid) - 1
+    with open(fName, 'wb') as f:
+        f.write(configName.ljust(64)[:64].encode('UTF-8'))
+        f.write(struct.pack('<7i', icyl, xper, yper, zper, nx, ny, nz))
+        for g in (xGrid, yGrid, zGrid):
+            f.write(np.asarray(g, dtype='<f8').tobytes())
+        f.write(np.zeros(nx * ny, dtype='<i4').tobytes())
+
+
 def _read_header(f):
     nx, ny, nz, nvar = struct.unpack('<4i', f.read(16))
     dt, time = struct.unpack('<2d', f.read(16))

@@ -89,6 +89,18 @@ class Domain:
                            for _ in range(3)]
             self.check_u_P, self.check_v_P, self.check_w_P = (c.permute(1, 2, 3, 0)
                                                               for c in self._check)
+            # domain.py:189-226: target states for training against data
+            # files (read per window by the run loop)
+            if not getattr(inputConfig, 'adjointVerification', False):
+                self.useTargetData = True
+                self.numItTargetComp = self.numCheckpointIt
+                self.state_u_T = state.state_P(decomp, zeros, need_gradients=False)
+                self.state_v_T = state.state_P(decomp, zeros, need_gradients=False)
+                self.state_w_T = state.state_P(decomp, zeros, need_gradients=False)
+                self.target_data_all = (self.state_u_T, self.state_v_T, self.state_w_T)
+                # all the file's names (the header size), three states read
+                self.target_data_all_CPU = state.data_all_CPU(decomp, startTime, simDt, names,
+                                                              self.target_data_all)
 
         # domain.py:231-251
         self.use_SFSmodel = True
@@ -231,6 +243,25 @@ class Domain:
                                                  P.Num_pressure_iterations)
             self.max_resP = res
         self.adjointAdvancer.step(simDt)
+
+    def TargetDataVelCorr(self, simDt):
+        """domain.py:445-470: project the target velocity (the correction with
+        +simDt/rho, as the reference writes it), if targetDataVelCorr is set."""
+        if not self.targetDataVelCorr:
+            return
+        P = self.poisson
+        T = (self.state_u_T.var, self.state_v_T.var, self.state_w_T.var)
+        if P.kind != _native.PF_POISSON_JACOBI:
+            res, its, st = self.box.vel_corr(*T, self.state_DP_P.var, self.state_p_P.var,
+                                             self.source_P, P.kind, P.rho / P.simDt,
+                                             +simDt / self.rho, P.min_residual, P.max_iterations)
+            P.iterations = its
+            P.status = _native.SOLVE_STATUS.get(st, st)
+        else:
+            res, its, st = self.box.vel_corr(*T, self.state_DP_P.var, self.state_p_P.var,
+                                             self.source_P, P.kind, P.rho / P.simDt,
+                                             +simDt / self.rho, 0.0, P.Num_pressure_iterations)
+        self.max_resP = res
 
     def stats(self):
         """Monitor values of core/PyFlow.py:378-395 in one kernel; the

@@ -9,6 +9,14 @@ FZZ, FXY, FXZ, FYZ.  The reference has no such model (its ML closure is a
 Parameters are one packed FP32 vector W1[20][9] b1[20] W2[20][20] b2[20]
 W3[6][20] b3[6] with the seeded initialisation of torch.nn.Linear
 (uniform in +-1/sqrt(fan_in)).
+
+Training follows the reference's ML closure (sfsmodel/sfsmodel_ML.py:123-349):
+the adjoint step accumulates d(sum tau . taubar)/d theta into `grad` on every
+adjoint RHS evaluation (the fused adjoint kernels write it); `finalize`
+scales it by the time step, keeps the reference's running average over
+`numSumDistr` windows, takes a torch-default RMSprop step (lr 0.01, alpha
+0.99, eps 1e-8) with the averaged gradient, and halves the learning rate
+every `LR_decay` windows after 4/5 of the first period.
 """
 
 import ctypes
@@ -32,6 +40,33 @@ def init_params(seed=0):
         bound = 1.0 / np.sqrt(FAN_IN[name])
         parts.append(rng.uniform(-bound, bound, size=shape).astype(np.float32).ravel())
     return np.concatenate(parts)
+
+
+def rmsprop_step(params, grad, square_avg, lr, alpha=0.99, eps=1e-8):
+    """One step of torch.optim.RMSprop with its defaults (no momentum, not
+    centred, no weight decay), in place on float32 tensors."""
+    square_avg.mul_(alpha).addcmul_(grad, grad, value=1.0 - alpha)
+    avg = square_avg.sqrt().add_(eps)
+    params.addcdiv_(grad, avg, value=-lr)
+
+
+class GradientAverager:
+    """The reference's window-gradient bookkeeping (sfsmodel_ML.py:284-352):
+    accumulate each window's gradient; every `numSumDistr` windows the mean of
+    the last `numSumDistr` becomes the gradient every following step uses
+    (zero until the first mean exists)."""
+
+    def __init__(self, n, numSumDistr, device):
+        self.numSumDistr = int(numSumDistr)
+        self.out = torch.zeros(n, dtype=torch.float64, device=device)
+        self.sum = torch.zeros(n, dtype=torch.float64, device=device)
+
+    def push(self, g, epoch):
+        self.sum += g
+        if (epoch + 1) % self.numSumDistr == 0:
+            self.out = self.sum / float(self.numSumDistr)
+            self.sum = torch.zeros_like(self.sum)
+        return self.out
 
 
 def unpack(flat):
@@ -61,6 +96,57 @@ class residual_stress:
         if geo.type != 'uniform':
             raise Exception('sfsmodel_mlp: geometry type not implemented')
         self.const = geo.dx * geo.dx / 12.0
+        # training state (sfsmodel_ML.py:160-252)
+        dev = self.params.device
+        self.grad = torch.zeros(N_PARAMS, dtype=torch.float64, device=dev)
+        self.simDt = getattr(inputConfig, 'simDt', None)
+        self.LR = float(getattr(inputConfig, 'LR', 0.01))
+        self.epoch = 0
+        self.LR_decay = int(getattr(inputConfig, 'LR_decay', 2500))
+        self.square_avg = torch.zeros(N_PARAMS, dtype=torch.float32, device=dev)
+        self.averager = GradientAverager(N_PARAMS, getattr(inputConfig, 'numSumDistr', 8), dev)
+        self.modelDictSave = getattr(inputConfig, 'modelDictSave', None)
+        if getattr(inputConfig, 'loadModel', False):
+            self.load(inputConfig.modelDictName)
+
+    def param_grad(self):
+        """FP64 accumulator the adjoint kernels add d(sum tau . taubar)/d theta to."""
+        return self.grad
+
+    def finalize(self, comms=None):
+        """sfsmodel_ML.py:284-352 for this closure.  The reference scales its
+        accumulated gradient by -simDt because its 'source' closure is
+        subtracted from the RHS; this closure's stresses enter the RHS with a
+        plus sign, so the same objective gradient is +simDt times the
+        accumulation."""
+        g = self.grad * self.simDt
+        self.last_window_grad = g.clone()
+        avg = self.averager.push(g, self.epoch)
+        rmsprop_step(self.params, avg.to(torch.float32), self.square_avg, self.LR)
+        self.grad.zero_()
+        self.epoch += 1
+        if self.epoch >= 4 * self.LR_decay // 5 and self.epoch % self.LR_decay == 0:
+            self.LR *= 0.5
+
+    def state_dict(self):
+        return {'epoch': self.epoch, 'LR': self.LR, 'params': self.params.cpu(),
+                'square_avg': self.square_avg.cpu(), 'outParams': self.averager.out.cpu(),
+                'sumParams': self.averager.sum.cpu()}
+
+    def save(self):
+        """sfsmodel_ML.py:270-280 (tensors only)."""
+        if self.modelDictSave:
+            torch.save(self.state_dict(), self.modelDictSave)
+
+    def load(self, path):
+        d = torch.load(path, weights_only=True)
+        dev = self.params.device
+        self.epoch = int(d['epoch'])
+        self.LR = float(d['LR'])
+        self.params.copy_(d['params'].to(dev))
+        self.square_avg.copy_(d['square_avg'].to(dev))
+        self.averager.out = d['outParams'].to(dev)
+        self.averager.sum = d['sumParams'].to(dev)
 
     def stresses(self):
         return (self.FXX, self.FYY, self.FZZ, self.FXY, self.FXZ, self.FYZ)
