@@ -146,6 +146,34 @@ int pf_rhs_ns(pf_box* box, const void* u, const void* v, const void* w,
               void* ru, void* rv, void* rw, void* stream);
 
 /*
+ * The same three entry points with a spatially varying viscosity field
+ * VISC_P (padded, the field Domain.forwardStep builds as mu + eddy viscosity,
+ * domain.py:348-352) in place of the constant mu: the normal viscous fluxes
+ * use 2*VISC/rho at their cell centres and the cross fluxes
+ * interp_sc_xy/xz/yz(VISC)/rho at their edges (velocity.py:248-305,
+ * metric_staggered.py:128-144), as the reference rounds them.
+ */
+int pf_rk4_step_visc(pf_box* box, void* u, void* v, void* w, void* ut, void* vt, void* wt,
+                     double dt, const void* visc, double rho, const void* const* sfs6,
+                     void* stream);
+int pf_euler_step_visc(pf_box* box, void* u, void* v, void* w, double dt, const void* visc,
+                       double rho, const void* const* sfs6, void* stream);
+int pf_rhs_ns_visc(pf_box* box, const void* u, const void* v, const void* w, const void* visc,
+                   double rho, const void* const* sfs6, void* ru, void* rv, void* rw,
+                   void* stream);
+
+/*
+ * Smagorinsky eddy viscosity plus the molecular one, written to the padded
+ * viscosity field: visc = mu + rho*(Cs*Delta)^2*|S| with
+ * |S| = sqrt(S11^2+S22^2+S33^2 + 2(S12^2+S13^2+S23^2)) of the trace-free
+ * strain of the nine grad_vel_center gradients.  Replaces
+ * stress_constCs.eddyVisc (sfsmodel/sfsmodel_smagorinsky.py:65-88) followed by
+ * VISC_P.copy_(muMolec + muEddy) (data/domain.py:348-352).
+ */
+int pf_sfs_smagorinsky(pf_box* box, const void* u, const void* v, const void* w, double rho,
+                       double Cs, double Delta, double mu, void* visc, void* stream);
+
+/*
  * Interior cell-centred divergence of padded u, v, w into interior `divg`.
  * Replaces metric_uniform.div_vel (metric/metric_staggered.py:438-453).
  */

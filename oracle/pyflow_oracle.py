@@ -400,6 +400,35 @@ class GradientModel:
             out.mul_(self.const)
 
 
+class Smagorinsky:
+    """Constant-coefficient Smagorinsky eddy viscosity,
+    sfsmodel/sfsmodel_smagorinsky.py:39-88 (modelType 'eddyVisc'; Box adds it
+    to the molecular viscosity before each step, domain.py:348-352)."""
+
+    modelType = 'eddyVisc'
+
+    def __init__(self, g, Cs=0.18, expFilterFac=1.0):
+        self.g = g
+        self.Cs = Cs
+        self.Delta = g.dx * expFilterFac                    # :55-57
+
+    def eddyVisc(self, fu, fv, fw, rho):
+        g = self.g
+        grad_vel_center(g, fu, 'u')                          # :68-70
+        grad_vel_center(g, fv, 'v')
+        grad_vel_center(g, fw, 'w')
+        divOver3 = (fu.grad_x + fv.grad_y + fw.grad_z) / 3.0    # :73
+        S_11 = fu.grad_x - divOver3                          # :76-81
+        S_22 = fv.grad_y - divOver3
+        S_33 = fw.grad_z - divOver3
+        S_12 = 0.5 * (fu.grad_y + fv.grad_x)
+        S_13 = 0.5 * (fu.grad_z + fw.grad_x)
+        S_23 = 0.5 * (fv.grad_z + fw.grad_y)
+        S_mag = torch.sqrt(S_11 * S_11 + S_22 * S_22 + S_33 * S_33 +
+                           2.0 * (S_12 * S_12 + S_13 * S_13 + S_23 * S_23))    # :82-83
+        return rho * (self.Cs * self.Delta) ** 2 * S_mag    # :86
+
+
 def mlp_closure_forward(g, fu, fv, fw, params, const=None):
     """Closure MLP with the 'tensor' contract: 9 cell-centred velocity
     gradients -> Linear(9,20) -> tanh -> Linear(20,20) -> tanh -> Linear(20,6)
@@ -706,6 +735,9 @@ class Box:
 
     def forward_step(self):
         """domain.py:342-367"""
+        if self.sfs is not None and self.sfs.modelType == 'eddyVisc':
+            muEddy = self.sfs.eddyVisc(self.u, self.v, self.w, self.rho)
+            self.VISC.copy_(self.mu + muEddy)                # domain.py:348-352
         if self.sfs is not None and self.sfs.modelType == 'tensor':
             self.sfs.update(self.u, self.v, self.w)
         if self.advancer == 'RK4':

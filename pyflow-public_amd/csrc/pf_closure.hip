@@ -59,9 +59,72 @@ int sfs_gradient_impl(pf_box* h, hipStream_t s, const T* U, const T* V, const T*
   return 0;
 }
 
+// Smagorinsky eddy viscosity (sfsmodel_smagorinsky.py:65-88) added to the
+// molecular viscosity as Domain.forwardStep does (domain.py:348-352):
+// VISC = mu + rho*(Cs*Delta)^2*|S| at every padded cell, |S| from the nine
+// gradients of grad_vel_center in the reference's op order.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) smagorinsky_kernel(Geom g, const T* __restrict__ U,
+                                                             const T* __restrict__ V,
+                                                             const T* __restrict__ W, T dxi,
+                                                             T dyi, T dzi, T cs2, T mu,
+                                                             T* __restrict__ visc) {
+  const int nxo = g.nx + 2 * kNover, nyo = g.ny + 2 * kNover, nzo = g.nz + 2 * kNover;
+  const int64_t total = (int64_t)nxo * nyo * nzo;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int a, b, k;
+    feature_cell(q, nxo, nyo, nzo, a, b, k);
+    T G[3][3];
+    grad_center(U, V, W, g.psx, g.psy, a, b, k, dxi, dyi, dzi, G);
+    const T d3 = ((G[0][0] + G[1][1]) + G[2][2]) / (T)3.0;
+    const T s11 = G[0][0] - d3, s22 = G[1][1] - d3, s33 = G[2][2] - d3;
+    const T half = (T)0.5;
+    const T s12 = half * (G[0][1] + G[1][0]);
+    const T s13 = half * (G[0][2] + G[2][0]);
+    const T s23 = half * (G[1][2] + G[2][1]);
+    T n = s11 * s11;
+    n = n + (s22 * s22);
+    n = n + (s33 * s33);
+    T o = s12 * s12;
+    o = o + (s13 * s13);
+    o = o + (s23 * s23);
+    const T smag = sqrt_rn(n + ((T)2.0 * o));
+    visc[q] = mu + (cs2 * smag);
+  }
+}
+
+template <typename T>
+int smagorinsky_impl(pf_box* h, hipStream_t s, const T* U, const T* V, const T* W, double rho,
+                     double Cs, double Delta, double mu, T* visc) {
+  const Geom& g = h->g;
+  const int64_t total = (int64_t)(g.nx + 4) * (g.ny + 4) * (g.nz + 4);
+  int64_t nb = (total + kBlock - 1) / kBlock;
+  if (nb > 65536) nb = 65536;
+  const double cd = Cs * Delta;
+  const T cs2 = (T)(rho * (cd * cd));   // rho * (Cs*Delta)**2, one Python float
+  hipLaunchKernelGGL(smagorinsky_kernel<T>, dim3((int)nb), dim3(kBlock), 0, s, g, U, V, W,
+                     (T)(1.0 / h->dx), (T)(1.0 / h->dy), (T)(1.0 / h->dz), cs2, (T)mu, visc);
+  PF_LAUNCH_CHECK("smagorinsky_kernel");
+  return 0;
+}
+
 }  // namespace pf
 
 using namespace pf;
+
+extern "C" int pf_sfs_smagorinsky(pf_box* h, const void* u, const void* v, const void* w,
+                                  double rho, double Cs, double Delta, double mu, void* visc,
+                                  void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (!u || !v || !w || !visc) return fail(PF_EINVAL, "pf_sfs_smagorinsky: NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == PF_F64)
+    return smagorinsky_impl<double>(h, s, (const double*)u, (const double*)v, (const double*)w,
+                                    rho, Cs, Delta, mu, (double*)visc);
+  return smagorinsky_impl<float>(h, s, (const float*)u, (const float*)v, (const float*)w, rho,
+                                 Cs, Delta, mu, (float*)visc);
+}
 
 extern "C" int pf_sfs_gradient(pf_box* h, const void* u, const void* v, const void* w,
                                void* const* sfs6, void* stream) {

@@ -137,6 +137,21 @@ __device__ __forceinline__ double wave_sum(double v) {
 // NaN-propagating max (torch.max propagates NaN; fmax would drop it).
 __device__ __forceinline__ double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
 
+// Square root rounded to nearest, as IEEE 754 (and the CPU torch.sqrt the
+// reference uses) defines it.  The device square root may be off by an ulp;
+// one remainder step corrects it: r = x - y*y exactly by an fma, then
+// y + r/(2y), also by an fma (Markstein's correction, IBM J. Res. Dev. 34(1),
+// 1990).  x = 0 returns 0.  Floats go through double: rounding a correctly
+// rounded double square root to float gives the correctly rounded float one
+// (53 >= 2*24 + 2).
+__device__ __forceinline__ double sqrt_rn(double x) {
+  if (!(x > 0.0)) return sqrt(x);
+  const double y = sqrt(x);
+  const double r = fma(-y, y, x);
+  return fma(r, 0.5 / y, y);
+}
+__device__ __forceinline__ float sqrt_rn(float x) { return (float)sqrt_rn((double)x); }
+
 __device__ __forceinline__ double wave_max(double v) {
   for (int off = kWave / 2; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off, kWave));
   return v;

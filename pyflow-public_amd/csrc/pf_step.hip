@@ -18,6 +18,8 @@ namespace pf {
 template <typename T>
 struct Sfs {
   const T* f[6];      // FXX, FYY, FZZ, FXY, FXZ, FYZ at the interior origin, or all NULL
+  const T* visc;      // viscosity field VISC_P at the interior origin (field-viscosity kernels)
+  T rho;              // density, for the field viscosity's 2*VISC/rho and interp_sc(VISC)/rho
 };
 
 // 0.25*(((F(0)+F(-a))+F(-b))+F(-a-b)): interp_sc_xy/xz/yz at an edge.
@@ -29,13 +31,23 @@ __device__ __forceinline__ T edge4(const T* __restrict__ F, int64_t c, int64_t s
 // RHS at interior cell (i,j,k); U,V,W point at the padded interior origin and
 // c is the padded offset of the cell.  ADV = false leaves out the advective
 // terms (the adjoint RHS applies the viscous operator alone to its field).
-template <typename T, bool SFS, bool ADV = true>
+// VF = true takes the viscosity from the field sfs.visc (an eddy viscosity,
+// domain.py:348-352): 2*VISC/rho at the cell centres of the normal fluxes and
+// interp_sc_xy/xz/yz(VISC)/rho at the edges of the cross fluxes
+// (velocity.py:248-305), each rounded as the reference's whole-array ops
+// round them; VF = false uses the constant coefficients of k.
+template <typename T, bool SFS, bool ADV = true, bool VF = false>
 __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __restrict__ V,
                                          const T* __restrict__ W, int64_t c, int64_t sx,
                                          int64_t sy, const StepCoef<T>& k, const Sfs<T>& sfs,
                                          T& ru, T& rv, T& rw) {
   const T dxi = k.dxi, dyi = k.dyi, dzi = k.dzi;
   const T half = (T)0.5;
+  const T* __restrict__ vis = sfs.visc;
+  auto c2 = [&](int64_t o) -> T { return VF ? ((T)2.0 * vis[o]) / sfs.rho : k.c2; };
+  auto exy = [&](int64_t o) -> T { return VF ? edge4(vis, o, sx, sy) / sfs.rho : k.exy; };
+  auto exz = [&](int64_t o) -> T { return VF ? edge4(vis, o, sx, (int64_t)1) / sfs.rho : k.exz; };
+  auto eyz = [&](int64_t o) -> T { return VF ? edge4(vis, o, sy, (int64_t)1) / sfs.rho : k.eyz; };
 #define LD(A, di, dj, dk) A[c + (di) * sx + (dj) * sy + (dk)]
   const T u000 = LD(U, 0, 0, 0), up00 = LD(U, 1, 0, 0), um00 = LD(U, -1, 0, 0);
   const T u0p0 = LD(U, 0, 1, 0), u0m0 = LD(U, 0, -1, 0), u00p = LD(U, 0, 0, 1), u00m = LD(U, 0, 0, -1);
@@ -57,12 +69,12 @@ __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __res
   const T D_zm = ((((up0m - u00m) * dxi) + ((v0pm - v00m) * dyi)) + ((w000 - w00m) * dzi)) / third;
 
   // ---- u: viscous fluxes (velocity.py:248-269) ----
-  T FX_c = (((up00 - u000) * dxi) - D_c) * k.c2;
-  T FX_m = (((u000 - um00) * dxi) - D_xm) * k.c2;
-  T FY_c = (((u0p0 - u000) * dyi) + ((vp00 - v000) * dxi)) * k.exy;
-  T FY_m = (((u000 - u0m0) * dyi) + ((vpm0 - v0m0) * dxi)) * k.exy;
-  T FZ_c = (((u00p - u000) * dzi) + ((wp00 - w000) * dxi)) * k.exz;
-  T FZ_m = (((u000 - u00m) * dzi) + ((wp0m - w00m) * dxi)) * k.exz;
+  T FX_c = (((up00 - u000) * dxi) - D_c) * c2(c);
+  T FX_m = (((u000 - um00) * dxi) - D_xm) * c2(c - sx);
+  T FY_c = (((u0p0 - u000) * dyi) + ((vp00 - v000) * dxi)) * exy(c);
+  T FY_m = (((u000 - u0m0) * dyi) + ((vpm0 - v0m0) * dxi)) * exy(c - sy);
+  T FZ_c = (((u00p - u000) * dzi) + ((wp00 - w000) * dxi)) * exz(c);
+  T FZ_m = (((u000 - u00m) * dzi) + ((wp0m - w00m) * dxi)) * exz(c - 1);
   if (SFS) {
     FX_c = FX_c + sfs.f[0][c];
     FX_m = FX_m + sfs.f[0][c - sx];
@@ -91,12 +103,12 @@ __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __res
   ru = r;
 
   // ---- v: viscous fluxes (velocity.py:272-293) ----
-  FX_c = (((vp00 - v000) * dxi) + ((u0p0 - u000) * dyi)) * k.exy;
-  FX_m = (((v000 - vm00) * dxi) + ((ump0 - um00) * dyi)) * k.exy;
-  FY_c = (((v0p0 - v000) * dyi) - D_c) * k.c2;
-  FY_m = (((v000 - v0m0) * dyi) - D_ym) * k.c2;
-  FZ_c = (((v00p - v000) * dzi) + ((w0p0 - w000) * dyi)) * k.eyz;
-  FZ_m = (((v000 - v00m) * dzi) + ((w0pm - w00m) * dyi)) * k.eyz;
+  FX_c = (((vp00 - v000) * dxi) + ((u0p0 - u000) * dyi)) * exy(c);
+  FX_m = (((v000 - vm00) * dxi) + ((ump0 - um00) * dyi)) * exy(c - sx);
+  FY_c = (((v0p0 - v000) * dyi) - D_c) * c2(c);
+  FY_m = (((v000 - v0m0) * dyi) - D_ym) * c2(c - sy);
+  FZ_c = (((v00p - v000) * dzi) + ((w0p0 - w000) * dyi)) * eyz(c);
+  FZ_m = (((v000 - v00m) * dzi) + ((w0pm - w00m) * dyi)) * eyz(c - 1);
   if (SFS) {
     FX_c = FX_c + edge4(sfs.f[3], c, sx, sy);
     FX_m = FX_m + edge4(sfs.f[3], c - sx, sx, sy);
@@ -125,12 +137,12 @@ __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __res
   rv = r;
 
   // ---- w: viscous fluxes (velocity.py:296-317) ----
-  FX_c = (((wp00 - w000) * dxi) + ((u00p - u000) * dzi)) * k.exz;
-  FX_m = (((w000 - wm00) * dxi) + ((um0p - um00) * dzi)) * k.exz;
-  FY_c = (((w0p0 - w000) * dyi) + ((v00p - v000) * dzi)) * k.eyz;
-  FY_m = (((w000 - w0m0) * dyi) + ((v0mp - v0m0) * dzi)) * k.eyz;
-  FZ_c = (((w00p - w000) * dzi) - D_c) * k.c2;
-  FZ_m = (((w000 - w00m) * dzi) - D_zm) * k.c2;
+  FX_c = (((wp00 - w000) * dxi) + ((u00p - u000) * dzi)) * exz(c);
+  FX_m = (((w000 - wm00) * dxi) + ((um0p - um00) * dzi)) * exz(c - sx);
+  FY_c = (((w0p0 - w000) * dyi) + ((v00p - v000) * dzi)) * eyz(c);
+  FY_m = (((w000 - w0m0) * dyi) + ((v0mp - v0m0) * dzi)) * eyz(c - sy);
+  FZ_c = (((w00p - w000) * dzi) - D_c) * c2(c);
+  FZ_m = (((w000 - w00m) * dzi) - D_zm) * c2(c - 1);
   if (SFS) {
     FX_c = FX_c + edge4(sfs.f[4], c, sx, (int64_t)1);
     FX_m = FX_m + edge4(sfs.f[4], c - sx, sx, (int64_t)1);
@@ -196,7 +208,7 @@ __device__ __forceinline__ void stage_update(const Geom& g, uint32_t idx, int i,
 
 // One RK stage (or Euler step, or a bare RHS evaluation) of the forward RHS;
 // Y is the stage input state (padded, interior origin).
-template <typename T, int KIND, bool SFS>
+template <typename T, int KIND, bool SFS, bool VF = false>
 __device__ __forceinline__ void rk_stage_body(const Geom& g, const CVec3<T>& Y, const CVec3<T>& Un,
                                               const Vec3<T>& acc, const Vec3<T>& Yout,
                                               const Vec3<T>& Uout, const StepCoef<T>& k,
@@ -207,16 +219,16 @@ __device__ __forceinline__ void rk_stage_body(const Geom& g, const CVec3<T>& Y, 
     decode(g, idx, i, j, kk);
     const int64_t c = i * g.psx + j * g.psy + kk;
     T r[3];
-    rhs_cell<T, SFS>(Y.a[0], Y.a[1], Y.a[2], c, g.psx, g.psy, k, sfs, r[0], r[1], r[2]);
+    rhs_cell<T, SFS, true, VF>(Y.a[0], Y.a[1], Y.a[2], c, g.psx, g.psy, k, sfs, r[0], r[1], r[2]);
     stage_update<T, KIND>(g, idx, i, j, kk, c, r, Un, acc, Yout, Uout, k);
   }
 }
 
-template <typename T, int KIND, bool SFS>
+template <typename T, int KIND, bool SFS, bool VF>
 __global__ void __launch_bounds__(kBlock) rk_stage_kernel(Geom g, CVec3<T> Y, CVec3<T> Un,
                                                           Vec3<T> acc, Vec3<T> Yout, Vec3<T> Uout,
                                                           StepCoef<T> k, Sfs<T> sfs) {
-  rk_stage_body<T, KIND, SFS>(g, Y, Un, acc, Yout, Uout, k, sfs, this_blk());
+  rk_stage_body<T, KIND, SFS, VF>(g, Y, Un, acc, Yout, Uout, k, sfs, this_blk());
 }
 
 // ---------------------------------------------------------------------------
@@ -339,7 +351,7 @@ __global__ void __launch_bounds__(kBlock) adj_stage_kernel(Geom g, CVec3<T> Y, C
 template <typename T, int KIND>
 int launch_stage(pf_box* h, hipStream_t s, const void* const Y[3], const void* const Un[3],
                  void* const Yout[3], void* const Uout[3], const StepCoef<T>& k,
-                 const void* const* sfs6) {
+                 const void* const* sfs6, const void* visc = nullptr, double rho = 1.0) {
   const Geom& g = h->g;
   CVec3<T> y, un;
   Vec3<T> acc, yo, uo;
@@ -355,20 +367,25 @@ int launch_stage(pf_box* h, hipStream_t s, const void* const Y[3], const void* c
   }
   Sfs<T> sf;
   for (int m = 0; m < 6; ++m) sf.f[m] = sfs6 ? (const T*)sfs6[m] + g.poff : nullptr;
+  sf.visc = visc ? (const T*)visc + g.poff : nullptr;
+  sf.rho = (T)rho;
   const int nb = launch_blocks(g);
-  if (sfs6)
-    hipLaunchKernelGGL((rk_stage_kernel<T, KIND, true>), dim3(nb), dim3(kBlock), 0, s, g, y, un,
-                       acc, yo, uo, k, sf);
-  else
-    hipLaunchKernelGGL((rk_stage_kernel<T, KIND, false>), dim3(nb), dim3(kBlock), 0, s, g, y, un,
-                       acc, yo, uo, k, sf);
+#define PF_STAGE(SFS, VF)                                                                   \
+  hipLaunchKernelGGL((rk_stage_kernel<T, KIND, SFS, VF>), dim3(nb), dim3(kBlock), 0, s, g, y, \
+                     un, acc, yo, uo, k, sf)
+  if (visc) {
+    if (sfs6) PF_STAGE(true, true); else PF_STAGE(false, true);
+  } else {
+    if (sfs6) PF_STAGE(true, false); else PF_STAGE(false, false);
+  }
+#undef PF_STAGE
   PF_LAUNCH_CHECK("rk_stage_kernel");
   return 0;
 }
 
 template <typename T>
 int rk4_impl(pf_box* h, void* const U[3], void* const Ut[3], double dt, double mu, double rho,
-             const void* const* sfs6, hipStream_t s) {
+             const void* const* sfs6, hipStream_t s, const void* visc = nullptr) {
   const StepCoef<T> k = make_step_coef<T>(h, mu, rho, dt);
   // Stage states alternate between the caller's ut/vt/wt and the box's
   // second buffer so that the last stage state lands in ut/vt/wt
@@ -379,20 +396,20 @@ int rk4_impl(pf_box* h, void* const U[3], void* const Ut[3], double dt, double m
   const void* const Ac[3] = {A[0], A[1], A[2]};
   const void* const Bc[3] = {B[0], B[1], B[2]};
   int rc;
-  if ((rc = launch_stage<T, kStage1>(h, s, Uc, Uc, A, nullptr, k, sfs6))) return rc;
-  if ((rc = launch_stage<T, kStage2>(h, s, Ac, Uc, B, nullptr, k, sfs6))) return rc;
-  if ((rc = launch_stage<T, kStage3>(h, s, Bc, Uc, A, nullptr, k, sfs6))) return rc;
-  if ((rc = launch_stage<T, kStage4>(h, s, Ac, Uc, nullptr, U, k, sfs6))) return rc;
+  if ((rc = launch_stage<T, kStage1>(h, s, Uc, Uc, A, nullptr, k, sfs6, visc, rho))) return rc;
+  if ((rc = launch_stage<T, kStage2>(h, s, Ac, Uc, B, nullptr, k, sfs6, visc, rho))) return rc;
+  if ((rc = launch_stage<T, kStage3>(h, s, Bc, Uc, A, nullptr, k, sfs6, visc, rho))) return rc;
+  if ((rc = launch_stage<T, kStage4>(h, s, Ac, Uc, nullptr, U, k, sfs6, visc, rho))) return rc;
   return 0;
 }
 
 template <typename T>
 int euler_impl(pf_box* h, void* const U[3], double dt, double mu, double rho,
-               const void* const* sfs6, hipStream_t s) {
+               const void* const* sfs6, hipStream_t s, const void* visc = nullptr) {
   const StepCoef<T> k = make_step_coef<T>(h, mu, rho, dt);
   void* const B[3] = {h->ypong[0], h->ypong[1], h->ypong[2]};
   const void* const Uc[3] = {U[0], U[1], U[2]};
-  int rc = launch_stage<T, kEuler>(h, s, Uc, Uc, B, nullptr, k, sfs6);
+  int rc = launch_stage<T, kEuler>(h, s, Uc, Uc, B, nullptr, k, sfs6, visc, rho);
   if (rc) return rc;
   const size_t np = (size_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4) * sizeof(T);
   for (int q = 0; q < 3; ++q) PF_HIP(hipMemcpyAsync(U[q], B[q], np, hipMemcpyDeviceToDevice, s));
@@ -571,6 +588,58 @@ int pf_euler_step(pf_box* h, void* u, void* v, void* w, double dt, double mu, do
   hipStream_t s = (hipStream_t)stream;
   return h->dtype == PF_F64 ? euler_impl<double>(h, U, dt, mu, rho, sfs6, s)
                             : euler_impl<float>(h, U, dt, mu, rho, sfs6, s);
+}
+
+int pf_rk4_step_visc(pf_box* h, void* u, void* v, void* w, void* ut, void* vt, void* wt,
+                     double dt, const void* visc, double rho, const void* const* sfs6,
+                     void* stream) {
+  if (int rc = check_box(h)) return rc;
+  void* U[3] = {u, v, w};
+  void* Ut[3] = {ut, vt, wt};
+  if (int rc = check_ptrs((const void* const*)U, 3, "pf_rk4_step_visc")) return rc;
+  if (int rc = check_ptrs((const void* const*)Ut, 3, "pf_rk4_step_visc")) return rc;
+  if (!visc) return fail(PF_EINVAL, "pf_rk4_step_visc: NULL viscosity field");
+  if (sfs6)
+    if (int rc = check_ptrs(sfs6, 6, "pf_rk4_step_visc(sfs6)")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_rk4_step_visc: rho must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  return h->dtype == PF_F64 ? rk4_impl<double>(h, U, Ut, dt, 0.0, rho, sfs6, s, visc)
+                            : rk4_impl<float>(h, U, Ut, dt, 0.0, rho, sfs6, s, visc);
+}
+
+int pf_euler_step_visc(pf_box* h, void* u, void* v, void* w, double dt, const void* visc,
+                       double rho, const void* const* sfs6, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  void* U[3] = {u, v, w};
+  if (int rc = check_ptrs((const void* const*)U, 3, "pf_euler_step_visc")) return rc;
+  if (!visc) return fail(PF_EINVAL, "pf_euler_step_visc: NULL viscosity field");
+  if (sfs6)
+    if (int rc = check_ptrs(sfs6, 6, "pf_euler_step_visc(sfs6)")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_euler_step_visc: rho must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  return h->dtype == PF_F64 ? euler_impl<double>(h, U, dt, 0.0, rho, sfs6, s, visc)
+                            : euler_impl<float>(h, U, dt, 0.0, rho, sfs6, s, visc);
+}
+
+int pf_rhs_ns_visc(pf_box* h, const void* u, const void* v, const void* w, const void* visc,
+                   double rho, const void* const* sfs6, void* ru, void* rv, void* rw,
+                   void* stream) {
+  if (int rc = check_box(h)) return rc;
+  const void* U[3] = {u, v, w};
+  void* R[3] = {ru, rv, rw};
+  if (int rc = check_ptrs(U, 3, "pf_rhs_ns_visc")) return rc;
+  if (int rc = check_ptrs((const void* const*)R, 3, "pf_rhs_ns_visc")) return rc;
+  if (!visc) return fail(PF_EINVAL, "pf_rhs_ns_visc: NULL viscosity field");
+  if (sfs6)
+    if (int rc = check_ptrs(sfs6, 6, "pf_rhs_ns_visc(sfs6)")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_rhs_ns_visc: rho must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == PF_F64)
+    return launch_stage<double, kRhsOnly>(h, s, U, nullptr, R, nullptr,
+                                          make_step_coef<double>(h, 0.0, rho, 0.0), sfs6, visc,
+                                          rho);
+  return launch_stage<float, kRhsOnly>(h, s, U, nullptr, R, nullptr,
+                                       make_step_coef<float>(h, 0.0, rho, 0.0), sfs6, visc, rho);
 }
 
 int pf_rhs_ns(pf_box* h, const void* u, const void* v, const void* w, double mu, double rho,

@@ -75,6 +75,10 @@ SIGNATURES = [
     ('pf_profile_enable', _I, [_P, _I]),
     ('pf_profile_read', _I, [_P, _I, _PD, ctypes.POINTER(ctypes.c_int64)]),
     ('pf_residual_history_enable', _I, [_P, _I]),
+    ('pf_rk4_step_visc', _I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _D, _P, _P]),
+    ('pf_euler_step_visc', _I, [_P, _P, _P, _P, _D, _P, _D, _P, _P]),
+    ('pf_rhs_ns_visc', _I, [_P, _P, _P, _P, _P, _D, _P, _P, _P, _P, _P]),
+    ('pf_sfs_smagorinsky', _I, [_P, _P, _P, _P, _D, _D, _D, _D, _P, _P]),
     ('pf_bicgstab_mpir_solve', _I, [_P, _P, _P, _P, _D, _D, _D, _I, _PD, ctypes.POINTER(_I), _P]),
     ('pf_mpir_configure', _I, [_P, _D, _I, _I]),
     ('pf_residual_history', _I, [_P, _PD, _I, ctypes.POINTER(ctypes.c_int), _P]),
@@ -222,6 +226,35 @@ class Box:
                               float(mu), float(rho), sa[0] if sa else None,
                               self.interior(ru, 'rhs_u'), self.interior(rv, 'rhs_v'),
                               self.interior(rw, 'rhs_w'), self.stream()), 'pf_rhs_ns')
+
+    def rk4_step_visc(self, u, v, w, ut, vt, wt, dt, visc, rho, sfs6=None):
+        sa = self.sfs_array(sfs6)
+        check(lib().pf_rk4_step_visc(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                     self.padded(w, 'w'), self.padded(ut, 'ut'),
+                                     self.padded(vt, 'vt'), self.padded(wt, 'wt'), float(dt),
+                                     self.padded(visc, 'VISC'), float(rho),
+                                     sa[0] if sa else None, self.stream()), 'pf_rk4_step_visc')
+
+    def euler_step_visc(self, u, v, w, dt, visc, rho, sfs6=None):
+        sa = self.sfs_array(sfs6)
+        check(lib().pf_euler_step_visc(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                       self.padded(w, 'w'), float(dt), self.padded(visc, 'VISC'),
+                                       float(rho), sa[0] if sa else None, self.stream()),
+              'pf_euler_step_visc')
+
+    def rhs_ns_visc(self, u, v, w, visc, rho, ru, rv, rw, sfs6=None):
+        sa = self.sfs_array(sfs6)
+        check(lib().pf_rhs_ns_visc(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                   self.padded(w, 'w'), self.padded(visc, 'VISC'), float(rho),
+                                   sa[0] if sa else None, self.interior(ru, 'rhs_u'),
+                                   self.interior(rv, 'rhs_v'), self.interior(rw, 'rhs_w'),
+                                   self.stream()), 'pf_rhs_ns_visc')
+
+    def sfs_smagorinsky(self, u, v, w, rho, Cs, Delta, mu, visc):
+        check(lib().pf_sfs_smagorinsky(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                       self.padded(w, 'w'), float(rho), float(Cs), float(Delta),
+                                       float(mu), self.padded(visc, 'VISC'), self.stream()),
+              'pf_sfs_smagorinsky')
 
     def div_vel(self, u, v, w, divg):
         check(lib().pf_div_vel(self.h, self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),

@@ -39,8 +39,13 @@ class Euler:
                                  params, work, delta, dparams)
             return
         if _fusable(r):
-            r.box.euler_step(self.state_u.var, self.state_v.var, self.state_w.var,
-                             simDt, r.mu(), r.rho, r.sfs_fields())
+            vf = r.visc_field()
+            if vf is not None:
+                r.box.euler_step_visc(self.state_u.var, self.state_v.var, self.state_w.var,
+                                      simDt, vf, r.rho, r.sfs_fields())
+            else:
+                r.box.euler_step(self.state_u.var, self.state_v.var, self.state_w.var,
+                                 simDt, r.mu(), r.rho, r.sfs_fields())
             return
         r.evaluate(self.state_u, self.state_v, self.state_w)
         for st, k in ((self.state_u, r.rhs_u), (self.state_v, r.rhs_v), (self.state_w, r.rhs_w)):
@@ -71,9 +76,15 @@ class RK4:
             return
         if all(_fusable(getattr(rc, n)) for n in ('rhs1', 'rhs2', 'rhs3', 'rhs4')):
             r = rc.rhs1
-            r.box.rk4_step(self.state_u.var, self.state_v.var, self.state_w.var,
-                           self.state_uTmp.var, self.state_vTmp.var, self.state_wTmp.var,
-                           simDt, r.mu(), r.rho, r.sfs_fields())
+            vf = r.visc_field()
+            if vf is not None:
+                r.box.rk4_step_visc(self.state_u.var, self.state_v.var, self.state_w.var,
+                                    self.state_uTmp.var, self.state_vTmp.var,
+                                    self.state_wTmp.var, simDt, vf, r.rho, r.sfs_fields())
+            else:
+                r.box.rk4_step(self.state_u.var, self.state_v.var, self.state_w.var,
+                               self.state_uTmp.var, self.state_vTmp.var, self.state_wTmp.var,
+                               simDt, r.mu(), r.rho, r.sfs_fields())
             return
         U = (self.state_u, self.state_v, self.state_w)
         Tm = (self.state_uTmp, self.state_vTmp, self.state_wTmp)

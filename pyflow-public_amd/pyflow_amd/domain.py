@@ -182,9 +182,13 @@ class Domain:
         """domain.py:342-367"""
         if self.use_SFSmodel:
             if self.sfsmodel.modelType == 'eddyVisc':
-                muEddy = self.sfsmodel.eddyVisc(self.state_u_P, self.state_v_P, self.state_w_P,
-                                                self.rho, self.metric)
-                self.VISC_P.copy_(self.muMolec + muEddy)
+                if hasattr(self.sfsmodel, 'viscosity'):
+                    self.sfsmodel.viscosity(self.state_u_P, self.state_v_P, self.state_w_P,
+                                            self.rho, self.muMolec, self.VISC_P)
+                else:
+                    muEddy = self.sfsmodel.eddyVisc(self.state_u_P, self.state_v_P,
+                                                    self.state_w_P, self.rho, self.metric)
+                    self.VISC_P.copy_(self.muMolec + muEddy)
                 self.VISC_P.pf_const_mu = None
             elif self.sfsmodel.modelType == 'tensor':
                 self.sfsmodel.update(self.state_u_P, self.state_v_P, self.state_w_P, self.metric)

@@ -1,8 +1,9 @@
 """solver/velocity.py:45-76 and :189-383 on the GPU.
 
-rhs_NavierStokes.evaluate is one fused HIP kernel (pf_rhs_ns); the RK4 and
-Euler advancers of this package do not call it at all but fuse the same RHS
-into their stage kernels (pf_rk4_step / pf_euler_step).
+rhs_NavierStokes.evaluate is one fused HIP kernel (pf_rhs_ns, or
+pf_rhs_ns_visc for a spatially varying VISC field); the RK4 and Euler
+advancers of this package do not call it at all but fuse the same RHS into
+their stage kernels (pf_rk4_step / pf_euler_step and their _visc variants).
 """
 
 import torch
@@ -41,9 +42,13 @@ class rhs_NavierStokes:
     def mu(self):
         mu = constant_viscosity(self.VISC)
         if mu is None:
-            raise NotImplementedError('a spatially varying VISC field (eddy-viscosity closures) '
-                                      'is not supported by the fused RHS kernels')
+            raise NotImplementedError('this operation needs a constant viscosity; VISC is a '
+                                      'spatially varying field')
         return mu
+
+    def visc_field(self):
+        """The padded VISC field when it varies in space (eddy viscosity), else None."""
+        return self.VISC if constant_viscosity(self.VISC) is None else None
 
     def sfs_fields(self):
         """The six stresses of a 'tensor' closure, or None (velocity.py:262-315)."""
@@ -58,8 +63,13 @@ class rhs_NavierStokes:
     def evaluate(self, state_u, state_v, state_w):
         """velocity.py:221-383"""
         ru, rv, rw = self._arrays()
-        self.box.rhs_ns(state_u.var, state_v.var, state_w.var, self.mu(), self.rho,
-                        ru, rv, rw, self.sfs_fields())
+        vf = self.visc_field()
+        if vf is not None:
+            self.box.rhs_ns_visc(state_u.var, state_v.var, state_w.var, vf, self.rho,
+                                 ru, rv, rw, self.sfs_fields())
+        else:
+            self.box.rhs_ns(state_u.var, state_v.var, state_w.var, self.mu(), self.rho,
+                            ru, rv, rw, self.sfs_fields())
 
 
 class ForwardRHS:

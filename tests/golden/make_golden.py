@@ -16,7 +16,10 @@ functions that no closed form or textbook routine pins on its own:
 * rhs_adjPredictor.evaluate (solver/adjoint.py:94-298) and Domain.adjointStep
   (data/domain.py:409-438) with RK4 + BiCGStab and Euler + Jacobi, the
   closure term fed by a stand-in 'source' closure whose stresses are zero
-  (the reference's ML closure architecture is not part of the build).
+  (the reference's ML closure architecture is not part of the build);
+* stress_constCs.eddyVisc (sfsmodel/sfsmodel_smagorinsky.py:65-88), the RHS
+  with the resulting viscosity field, and Domain.forwardStep with
+  SFSmodel 'Smagorinsky' (data/domain.py:348-352).
 
 The reference's MPI layer is replaced by a single-rank periodic stub with the
 semantics of library/parallel.py:301-414 on a periodic Cartesian communicator
@@ -302,6 +305,34 @@ def main():
         out[tag + '_DP'] = ns.state_DP_P.var.numpy().copy()
         out[tag + '_p'] = ns.state_p_P.var.numpy().copy()
         out[tag + '_res'] = float(ns.max_resP)
+
+    # 9. Smagorinsky eddy viscosity (sfsmodel_smagorinsky.py:65-88), the RHS
+    #    with the resulting VISC field (velocity.py:248-305 with a field), and
+    #    two Domain.forwardSteps with SFSmodel 'Smagorinsky' (domain.py:348-352)
+    su, sv, sw = states(u0, v0, w0)
+    smag = sfsmodel_smagorinsky.stress_constCs(types.SimpleNamespace(Cs=0.2), geo)
+    mut = smag.eddyVisc(su, sv, sw, rho, metric)
+    out['smag_mut'] = mut.numpy().copy()
+    VISCf = mu + mut
+    out['smag_visc'] = VISCf.numpy().copy()
+    su, sv, sw = states(u0, v0, w0)
+    r = velocity.rhs_NavierStokes(d, metric, rho, VISCf, smag)
+    r.evaluate(su, sv, sw)
+    out['rhs_smag_u'], out['rhs_smag_v'], out['rhs_smag_w'] = (
+        r.rhs_u.numpy().copy(), r.rhs_v.numpy().copy(), r.rhs_w.numpy().copy())
+    cfg = types.SimpleNamespace(simDt=dt, equationMode='NS', advancerName='RK4', mu=mu, rho=rho,
+                                dtypeTorch=prec, SFSmodel='Smagorinsky', Cs=0.2,
+                                pSolverMode='bicgstab', min_pressure_residual=1e-12,
+                                max_pressure_iterations=500)
+    data_IC = np.stack([u0, v0, w0, zeros], axis=-1)
+    with contextlib.redirect_stdout(io.StringIO()):
+        D = domain.Domain(cfg, comms, d, data_IC, geo, metric, ['U', 'V', 'W', 'P'], 0.0)
+        for _ in range(2):
+            D.forwardStep(dt)
+    for c in ('u', 'v', 'w', 'p'):
+        out['dom_smag_' + c] = getattr(D, 'state_%s_P' % c).var.numpy().copy()
+    out['dom_smag_visc'] = D.VISC_P.numpy().copy()
+    out['dom_smag_res'] = float(D.max_resP)
 
     np.savez_compressed(OUT, **out)
     print('wrote', OUT, 'keys:', len(out))

@@ -185,3 +185,83 @@ def test_domain_forward_step(G, tag, adv, solver, tol, nit):
         assert rel_l2_mean_free(got['p'], G[tag + '_p']) < 1e-10
         assert rel_l2_mean_free(got['DP'], G[tag + '_DP']) < 1e-9
         assert D.max_resP <= 1e-12
+
+
+def _smag_cells(a):
+    """The padded cells whose viscosity the RHS reads (indices 0..n+1 per
+    axis); the reference's values on the last two overlap planes depend on
+    its gradient arrays' history."""
+    return a[:-2, :-2, :-2]
+
+
+def test_smagorinsky_viscosity_to_one_ulp(G):
+    """pf_sfs_smagorinsky: VISC = mu + rho*(Cs*Delta)^2*|S|
+    (sfsmodel_smagorinsky.py:65-88, domain.py:348-352) against the
+    reference's field, and eddyVisc alone (mu = 0).  Every operation is
+    reproduced except one: torch's CPU square root of a tensor with 16 or
+    more elements (the vectorized path) is not correctly rounded -- ~0.9 % of
+    its results are one ulp off the IEEE value, measured against numpy --
+    while the kernel's square root is correctly rounded.  So the fields agree
+    to one ulp (2.3e-16 relative), bit for bit on ~99 % of the cells.  The
+    oracle, which calls torch's square root like the reference, matches the
+    reference bit for bit (test_oracle_golden.py)."""
+    import types
+    from pyflow_amd import sfsmodel_smagorinsky
+    shape, d, g, m = setup(G)
+    su, sv, sw = states(d, G)
+    sm = sfsmodel_smagorinsky.stress_constCs(types.SimpleNamespace(Cs=0.2), g)
+    V = torch.zeros(d.nxo_, d.nyo_, d.nzo_, dtype=torch.float64, device='cuda')
+    sm.viscosity(su, sv, sw, float(G['rho']), float(G['mu']), V)
+    mut = sm.eddyVisc(su, sv, sw, float(G['rho']), m)
+    torch.cuda.synchronize()
+    for got, ref in ((cpu(V), G['smag_visc']), (cpu(mut), G['smag_mut'])):
+        got, ref = _smag_cells(got), _smag_cells(ref)
+        rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+        same = float(np.mean(got == ref))
+        print('Smagorinsky viscosity: %.4f of cells bit-identical, max rel %.2e' % (same, rel.max()))
+        assert rel.max() <= 4.5e-16
+        assert same >= 0.95
+
+
+def test_rhs_with_viscosity_field_bit_exact(G):
+    """rhs_NavierStokes.evaluate with the Smagorinsky VISC field
+    (velocity.py:248-305 with a field: 2*VISC/rho at cell centres,
+    interp_sc_xy/xz/yz(VISC)/rho at edges)."""
+    import types
+    from pyflow_amd import velocity, sfsmodel_smagorinsky
+    shape, d, g, m = setup(G)
+    su, sv, sw = states(d, G)
+    V = torch.tensor(G['smag_visc'], device='cuda')
+    sm = sfsmodel_smagorinsky.stress_constCs(types.SimpleNamespace(Cs=0.2), g)
+    r = velocity.rhs_NavierStokes(d, m, float(G['rho']), V, sm)
+    r.evaluate(su, sv, sw)
+    torch.cuda.synchronize()
+    for c in 'uvw':
+        got = cpu(getattr(r, 'rhs_' + c))
+        assert np.array_equal(got, G['rhs_smag_' + c]), (c, rel_max(got, G['rhs_smag_' + c]))
+
+
+def test_smagorinsky_domain_forward_step(G):
+    """Two Domain.forwardStep calls with SFSmodel 'Smagorinsky' (RK4 +
+    BiCGStab 1e-12): the eddy viscosity of each step, the fused RK4 stages
+    with the field viscosity and the projection; bars as in
+    test_domain_forward_step."""
+    import types
+    from pyflow_amd import domain
+    shape, d, g, m = setup(G)
+    cfg = types.SimpleNamespace(simDt=float(G['dt']), equationMode='NS', advancerName='RK4',
+                                mu=float(G['mu']), rho=float(G['rho']), dtypeTorch=torch.float64,
+                                SFSmodel='Smagorinsky', Cs=0.2, pSolverMode='bicgstab',
+                                min_pressure_residual=1e-12, max_pressure_iterations=500)
+    zeros = np.zeros(shape)
+    data_IC = np.stack([G['u0'], G['v0'], G['w0'], zeros], axis=-1)
+    D = domain.Domain(cfg, None, d, data_IC, g, m, ['U', 'V', 'W', 'P'], 0.0)
+    for _ in range(2):
+        D.forwardStep(float(G['dt']))
+    torch.cuda.synchronize()
+    for c in ('u', 'v', 'w'):
+        assert rel_l2(cpu(getattr(D, 'state_%s_P' % c).var), G['dom_smag_' + c]) < 1e-12, c
+    assert rel_l2_mean_free(cpu(D.state_p_P.var), G['dom_smag_p']) < 1e-10
+    # the second step's viscosity comes from the first step's velocity
+    assert rel_l2(_smag_cells(cpu(D.VISC_P)), _smag_cells(G['dom_smag_visc'])) < 1e-12
+    assert D.max_resP <= 1e-12
