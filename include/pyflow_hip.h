@@ -360,6 +360,20 @@ int pf_adj_euler_step(pf_box* box, const void* u, const void* v, const void* w, 
                       void* va, void* wa, double dt, double mu, double rho, int closure,
                       const float* params, void* work, void* const* delta3, double* dparams,
                       void* stream);
+/* The three with the viscosity field VISC_P (padded) in place of mu: the
+ * reference's adjoint applies its forward viscous operator, the field
+ * included, to the adjoint velocity (solver/adjoint.py:100-170). */
+int pf_rhs_adj_visc(pf_box* box, const void* u, const void* v, const void* w, const void* ua,
+                    const void* va, const void* wa, const void* visc, double rho,
+                    const void* const* delta3, void* ru, void* rv, void* rw, void* stream);
+int pf_adj_rk4_step_visc(pf_box* box, const void* u, const void* v, const void* w, void* ua,
+                         void* va, void* wa, void* uta, void* vta, void* wta, double dt,
+                         const void* visc, double rho, int closure, const float* params,
+                         void* work, void* const* delta3, double* dparams, void* stream);
+int pf_adj_euler_step_visc(pf_box* box, const void* u, const void* v, const void* w, void* ua,
+                           void* va, void* wa, double dt, const void* visc, double rho,
+                           int closure, const float* params, void* work, void* const* delta3,
+                           double* dparams, void* stream);
 size_t pf_adj_work_bytes(const pf_box* box);
 int pf_adj_closure_term(pf_box* box, int kind, const void* u, const void* v, const void* w,
                         const float* params, const void* ua, const void* va, const void* wa,

@@ -292,15 +292,17 @@ AdjCoef<T> make_adj_coef(const pf_box* h) {
 
 // Adjoint RHS at interior cell c.  A* = adjoint fields, U* = forward fields
 // (both padded, interior origin); D = optional closure term (padded).
-template <typename T, bool DELTA>
+// VF: the viscous part takes the viscosity field of vs (the reference's
+// adjoint applies its forward viscous operator, VISC included, to the
+// adjoint field).
+template <typename T, bool DELTA, bool VF = false>
 __device__ __forceinline__ void adj_rhs_cell(const T* __restrict__ Au, const T* __restrict__ Av,
                                              const T* __restrict__ Aw, const T* __restrict__ Uu,
                                              const T* __restrict__ Uv, const T* __restrict__ Uw,
                                              int64_t c, int64_t sx, int64_t sy,
                                              const StepCoef<T>& k, const AdjCoef<T>& a,
-                                             const CVec3<T>& D, T (&r)[3]) {
-  const Sfs<T> none = {{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
-  rhs_cell<T, false, false>(Au, Av, Aw, c, sx, sy, k, none, r[0], r[1], r[2]);
+                                             const CVec3<T>& D, const Sfs<T>& vs, T (&r)[3]) {
+  rhs_cell<T, false, false, VF>(Au, Av, Aw, c, sx, sy, k, vs, r[0], r[1], r[2]);
   const int64_t z = 1;
   T ru = r[0], rv = r[1], rw = r[2];
   // u (adjoint.py:172-192)
@@ -331,19 +333,20 @@ __device__ __forceinline__ void adj_rhs_cell(const T* __restrict__ Au, const T* 
   r[2] = rw;
 }
 
-template <typename T, int KIND, bool DELTA>
+template <typename T, int KIND, bool DELTA, bool VF>
 __global__ void __launch_bounds__(kBlock) adj_stage_kernel(Geom g, CVec3<T> Y, CVec3<T> Uf,
                                                            CVec3<T> D, CVec3<T> Un, Vec3<T> acc,
                                                            Vec3<T> Yout, Vec3<T> Uout,
-                                                           StepCoef<T> k, AdjCoef<T> a) {
+                                                           StepCoef<T> k, AdjCoef<T> a,
+                                                           Sfs<T> vs) {
   for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < (uint32_t)g.ncell;
        idx += gridDim.x * blockDim.x) {
     int i, j, kk;
     decode(g, idx, i, j, kk);
     const int64_t c = i * g.psx + j * g.psy + kk;
     T r[3];
-    adj_rhs_cell<T, DELTA>(Y.a[0], Y.a[1], Y.a[2], Uf.a[0], Uf.a[1], Uf.a[2], c, g.psx, g.psy, k,
-                           a, D, r);
+    adj_rhs_cell<T, DELTA, VF>(Y.a[0], Y.a[1], Y.a[2], Uf.a[0], Uf.a[1], Uf.a[2], c, g.psx,
+                               g.psy, k, a, D, vs, r);
     stage_update<T, KIND>(g, idx, i, j, kk, c, r, Un, acc, Yout, Uout, k);
   }
 }
@@ -419,7 +422,8 @@ int euler_impl(pf_box* h, void* const U[3], double dt, double mu, double rho,
 template <typename T, int KIND>
 int launch_adj_stage(pf_box* h, hipStream_t s, const void* const Y[3], const void* const Uf[3],
                      const void* const* D, const void* const Un[3], void* const Yout[3],
-                     void* const Uout[3], const StepCoef<T>& k) {
+                     void* const Uout[3], const StepCoef<T>& k, const void* visc = nullptr,
+                     double rho = 1.0) {
   const Geom& g = h->g;
   CVec3<T> y, uf, d, un;
   Vec3<T> acc, yo, uo;
@@ -436,13 +440,19 @@ int launch_adj_stage(pf_box* h, hipStream_t s, const void* const Y[3], const voi
     uo.a[q] = Uout && Uout[q] ? (T*)Uout[q] + g.poff : nullptr;
   }
   const AdjCoef<T> a = make_adj_coef<T>(h);
+  Sfs<T> vs = {{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
+  vs.visc = visc ? (const T*)visc + g.poff : nullptr;
+  vs.rho = (T)rho;
   const int nb = launch_blocks(g);
-  if (D)
-    hipLaunchKernelGGL((adj_stage_kernel<T, KIND, true>), dim3(nb), dim3(kBlock), 0, s, g, y, uf,
-                       d, un, acc, yo, uo, k, a);
-  else
-    hipLaunchKernelGGL((adj_stage_kernel<T, KIND, false>), dim3(nb), dim3(kBlock), 0, s, g, y, uf,
-                       d, un, acc, yo, uo, k, a);
+#define PF_ADJ(DELTA, VF)                                                                       \
+  hipLaunchKernelGGL((adj_stage_kernel<T, KIND, DELTA, VF>), dim3(nb), dim3(kBlock), 0, s, g, y, \
+                     uf, d, un, acc, yo, uo, k, a, vs)
+  if (visc) {
+    if (D) PF_ADJ(true, true); else PF_ADJ(false, true);
+  } else {
+    if (D) PF_ADJ(true, false); else PF_ADJ(false, false);
+  }
+#undef PF_ADJ
   PF_LAUNCH_CHECK("adj_stage_kernel");
   return 0;
 }
@@ -472,7 +482,8 @@ static int adj_term(pf_box* h, const AdjClosure& cl, const void* const U[3],
 // structure as the forward step, the forward state U held fixed.
 template <typename T>
 int adj_rk4_impl(pf_box* h, const void* const U[3], void* const Ua[3], void* const Ut[3],
-                 double dt, double mu, double rho, const AdjClosure& cl, hipStream_t s) {
+                 double dt, double mu, double rho, const AdjClosure& cl, hipStream_t s,
+                 const void* visc = nullptr) {
   const StepCoef<T> k = make_step_coef<T>(h, mu, rho, dt);
   void* const A[3] = {Ut[0], Ut[1], Ut[2]};
   void* const B[3] = {h->ypong[0], h->ypong[1], h->ypong[2]};
@@ -482,26 +493,31 @@ int adj_rk4_impl(pf_box* h, const void* const U[3], void* const Ua[3], void* con
   const void* const* D;
   int rc;
   if ((rc = adj_term(h, cl, U, Uc, s, &D))) return rc;
-  if ((rc = launch_adj_stage<T, kStage1>(h, s, Uc, U, D, Uc, A, nullptr, k))) return rc;
+  if ((rc = launch_adj_stage<T, kStage1>(h, s, Uc, U, D, Uc, A, nullptr, k, visc, rho)))
+    return rc;
   if ((rc = adj_term(h, cl, U, Ac, s, &D))) return rc;
-  if ((rc = launch_adj_stage<T, kStage2>(h, s, Ac, U, D, Uc, B, nullptr, k))) return rc;
+  if ((rc = launch_adj_stage<T, kStage2>(h, s, Ac, U, D, Uc, B, nullptr, k, visc, rho)))
+    return rc;
   if ((rc = adj_term(h, cl, U, Bc, s, &D))) return rc;
-  if ((rc = launch_adj_stage<T, kStage3>(h, s, Bc, U, D, Uc, A, nullptr, k))) return rc;
+  if ((rc = launch_adj_stage<T, kStage3>(h, s, Bc, U, D, Uc, A, nullptr, k, visc, rho)))
+    return rc;
   if ((rc = adj_term(h, cl, U, Ac, s, &D))) return rc;
-  if ((rc = launch_adj_stage<T, kStage4>(h, s, Ac, U, D, Uc, nullptr, Ua, k))) return rc;
+  if ((rc = launch_adj_stage<T, kStage4>(h, s, Ac, U, D, Uc, nullptr, Ua, k, visc, rho)))
+    return rc;
   return 0;
 }
 
 template <typename T>
 int adj_euler_impl(pf_box* h, const void* const U[3], void* const Ua[3], double dt, double mu,
-                   double rho, const AdjClosure& cl, hipStream_t s) {
+                   double rho, const AdjClosure& cl, hipStream_t s, const void* visc = nullptr) {
   const StepCoef<T> k = make_step_coef<T>(h, mu, rho, dt);
   void* const B[3] = {h->ypong[0], h->ypong[1], h->ypong[2]};
   const void* const Uc[3] = {Ua[0], Ua[1], Ua[2]};
   const void* const* D;
   int rc;
   if ((rc = adj_term(h, cl, U, Uc, s, &D))) return rc;
-  if ((rc = launch_adj_stage<T, kEuler>(h, s, Uc, U, D, Uc, B, nullptr, k))) return rc;
+  if ((rc = launch_adj_stage<T, kEuler>(h, s, Uc, U, D, Uc, B, nullptr, k, visc, rho)))
+    return rc;
   const size_t np = (size_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4) * sizeof(T);
   for (int q = 0; q < 3; ++q) PF_HIP(hipMemcpyAsync(Ua[q], B[q], np, hipMemcpyDeviceToDevice, s));
   return 0;
@@ -677,9 +693,10 @@ static int adj_closure_args(pf_box* h, int kind, const float* params, void* work
   return 0;
 }
 
-int pf_rhs_adj(pf_box* h, const void* u, const void* v, const void* w, const void* ua,
-               const void* va, const void* wa, double mu, double rho, const void* const* delta3,
-               void* ru, void* rv, void* rw, void* stream) {
+static int rhs_adj_entry(pf_box* h, const void* u, const void* v, const void* w, const void* ua,
+                         const void* va, const void* wa, double mu, const void* visc,
+                         double rho, const void* const* delta3, void* ru, void* rv, void* rw,
+                         void* stream) {
   if (int rc = check_box(h)) return rc;
   const void* U[3] = {u, v, w};
   const void* A[3] = {ua, va, wa};
@@ -693,15 +710,30 @@ int pf_rhs_adj(pf_box* h, const void* u, const void* v, const void* w, const voi
   hipStream_t s = (hipStream_t)stream;
   if (h->dtype == PF_F64)
     return launch_adj_stage<double, kRhsOnly>(h, s, A, U, delta3, nullptr, R, nullptr,
-                                              make_step_coef<double>(h, mu, rho, 0.0));
+                                              make_step_coef<double>(h, mu, rho, 0.0), visc,
+                                              rho);
   return launch_adj_stage<float, kRhsOnly>(h, s, A, U, delta3, nullptr, R, nullptr,
-                                           make_step_coef<float>(h, mu, rho, 0.0));
+                                           make_step_coef<float>(h, mu, rho, 0.0), visc, rho);
 }
 
-int pf_adj_rk4_step(pf_box* h, const void* u, const void* v, const void* w, void* ua, void* va,
-                    void* wa, void* uta, void* vta, void* wta, double dt, double mu, double rho,
-                    int closure, const float* params, void* work, void* const* delta3,
-                    double* dparams, void* stream) {
+int pf_rhs_adj(pf_box* h, const void* u, const void* v, const void* w, const void* ua,
+               const void* va, const void* wa, double mu, double rho, const void* const* delta3,
+               void* ru, void* rv, void* rw, void* stream) {
+  return rhs_adj_entry(h, u, v, w, ua, va, wa, mu, nullptr, rho, delta3, ru, rv, rw, stream);
+}
+
+int pf_rhs_adj_visc(pf_box* h, const void* u, const void* v, const void* w, const void* ua,
+                    const void* va, const void* wa, const void* visc, double rho,
+                    const void* const* delta3, void* ru, void* rv, void* rw, void* stream) {
+  if (!visc) return fail(PF_EINVAL, "pf_rhs_adj_visc: NULL viscosity field");
+  return rhs_adj_entry(h, u, v, w, ua, va, wa, 0.0, visc, rho, delta3, ru, rv, rw, stream);
+}
+
+static int adj_rk4_entry(pf_box* h, const void* u, const void* v, const void* w, void* ua,
+                         void* va, void* wa, void* uta, void* vta, void* wta, double dt,
+                         double mu, const void* visc, double rho, int closure,
+                         const float* params, void* work, void* const* delta3, double* dparams,
+                         void* stream) {
   if (int rc = check_box(h)) return rc;
   const void* U[3] = {u, v, w};
   void* A[3] = {ua, va, wa};
@@ -714,14 +746,31 @@ int pf_adj_rk4_step(pf_box* h, const void* u, const void* v, const void* w, void
   if (int rc = adj_closure_args(h, closure, params, work, delta3, dparams, &cl, "pf_adj_rk4_step"))
     return rc;
   hipStream_t s = (hipStream_t)stream;
-  return h->dtype == PF_F64 ? adj_rk4_impl<double>(h, U, A, At, dt, mu, rho, cl, s)
-                            : adj_rk4_impl<float>(h, U, A, At, dt, mu, rho, cl, s);
+  return h->dtype == PF_F64 ? adj_rk4_impl<double>(h, U, A, At, dt, mu, rho, cl, s, visc)
+                            : adj_rk4_impl<float>(h, U, A, At, dt, mu, rho, cl, s, visc);
 }
 
-int pf_adj_euler_step(pf_box* h, const void* u, const void* v, const void* w, void* ua, void* va,
-                      void* wa, double dt, double mu, double rho, int closure,
-                      const float* params, void* work, void* const* delta3, double* dparams,
-                      void* stream) {
+int pf_adj_rk4_step(pf_box* h, const void* u, const void* v, const void* w, void* ua, void* va,
+                    void* wa, void* uta, void* vta, void* wta, double dt, double mu, double rho,
+                    int closure, const float* params, void* work, void* const* delta3,
+                    double* dparams, void* stream) {
+  return adj_rk4_entry(h, u, v, w, ua, va, wa, uta, vta, wta, dt, mu, nullptr, rho, closure,
+                       params, work, delta3, dparams, stream);
+}
+
+int pf_adj_rk4_step_visc(pf_box* h, const void* u, const void* v, const void* w, void* ua,
+                         void* va, void* wa, void* uta, void* vta, void* wta, double dt,
+                         const void* visc, double rho, int closure, const float* params,
+                         void* work, void* const* delta3, double* dparams, void* stream) {
+  if (!visc) return fail(PF_EINVAL, "pf_adj_rk4_step_visc: NULL viscosity field");
+  return adj_rk4_entry(h, u, v, w, ua, va, wa, uta, vta, wta, dt, 0.0, visc, rho, closure,
+                       params, work, delta3, dparams, stream);
+}
+
+static int adj_euler_entry(pf_box* h, const void* u, const void* v, const void* w, void* ua,
+                           void* va, void* wa, double dt, double mu, const void* visc,
+                           double rho, int closure, const float* params, void* work,
+                           void* const* delta3, double* dparams, void* stream) {
   if (int rc = check_box(h)) return rc;
   const void* U[3] = {u, v, w};
   void* A[3] = {ua, va, wa};
@@ -733,8 +782,25 @@ int pf_adj_euler_step(pf_box* h, const void* u, const void* v, const void* w, vo
                                 "pf_adj_euler_step"))
     return rc;
   hipStream_t s = (hipStream_t)stream;
-  return h->dtype == PF_F64 ? adj_euler_impl<double>(h, U, A, dt, mu, rho, cl, s)
-                            : adj_euler_impl<float>(h, U, A, dt, mu, rho, cl, s);
+  return h->dtype == PF_F64 ? adj_euler_impl<double>(h, U, A, dt, mu, rho, cl, s, visc)
+                            : adj_euler_impl<float>(h, U, A, dt, mu, rho, cl, s, visc);
+}
+
+int pf_adj_euler_step(pf_box* h, const void* u, const void* v, const void* w, void* ua, void* va,
+                      void* wa, double dt, double mu, double rho, int closure,
+                      const float* params, void* work, void* const* delta3, double* dparams,
+                      void* stream) {
+  return adj_euler_entry(h, u, v, w, ua, va, wa, dt, mu, nullptr, rho, closure, params, work,
+                         delta3, dparams, stream);
+}
+
+int pf_adj_euler_step_visc(pf_box* h, const void* u, const void* v, const void* w, void* ua,
+                           void* va, void* wa, double dt, const void* visc, double rho,
+                           int closure, const float* params, void* work, void* const* delta3,
+                           double* dparams, void* stream) {
+  if (!visc) return fail(PF_EINVAL, "pf_adj_euler_step_visc: NULL viscosity field");
+  return adj_euler_entry(h, u, v, w, ua, va, wa, dt, 0.0, visc, rho, closure, params, work,
+                         delta3, dparams, stream);
 }
 
 }  // extern "C"

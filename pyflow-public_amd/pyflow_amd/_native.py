@@ -59,6 +59,11 @@ SIGNATURES = [
     ('pf_adj_rk4_step', _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _I, _P, _P, _PP,
                              _PD, _P]),
     ('pf_adj_euler_step', _I, [_P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _I, _P, _P, _PP, _PD, _P]),
+    ('pf_rhs_adj_visc', _I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _PP, _P, _P, _P, _P]),
+    ('pf_adj_rk4_step_visc', _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _D, _I, _P, _P,
+                                  _PP, _PD, _P]),
+    ('pf_adj_euler_step_visc', _I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _D, _I, _P, _P, _PP, _PD,
+                                    _P]),
     ('pf_adj_work_bytes', ctypes.c_size_t, [_P]),
     ('pf_adj_closure_term', _I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _PP, _PD, _P]),
     ('pf_adj_conv', _I, [_P, _I, _I, _I, _P, _P, _P, _D, _P]),
@@ -413,12 +418,17 @@ class Box:
         return int(lib().pf_adj_work_bytes(self.h))
 
     def rhs_adj(self, u, v, w, ua, va, wa, mu, rho, ru, rv, rw, delta3=None):
+        """mu: the constant viscosity, or the padded VISC field (tensor)."""
         da = self._ptrs(delta3, 'delta') if delta3 is not None else None
-        check(lib().pf_rhs_adj(self.h, self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
-                               self.padded(ua, 'ua'), self.padded(va, 'va'), self.padded(wa, 'wa'),
-                               float(mu), float(rho), da[0] if da else None,
-                               self.interior(ru, 'rhs_u'), self.interior(rv, 'rhs_v'),
-                               self.interior(rw, 'rhs_w'), self.stream()), 'pf_rhs_adj')
+        args = (self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
+                self.padded(ua, 'ua'), self.padded(va, 'va'), self.padded(wa, 'wa'))
+        tail = (float(rho), da[0] if da else None, self.interior(ru, 'rhs_u'),
+                self.interior(rv, 'rhs_v'), self.interior(rw, 'rhs_w'), self.stream())
+        if isinstance(mu, torch.Tensor):
+            check(lib().pf_rhs_adj_visc(self.h, *args, self.padded(mu, 'VISC'), *tail),
+                  'pf_rhs_adj_visc')
+        else:
+            check(lib().pf_rhs_adj(self.h, *args, float(mu), *tail), 'pf_rhs_adj')
 
     def adj_closure_term(self, kind, u, v, w, params, ua, va, wa, work, delta3, dparams=None):
         da = self._ptrs(delta3, 'delta')
@@ -439,24 +449,33 @@ class Box:
 
     def adj_rk4_step(self, u, v, w, ua, va, wa, uta, vta, wta, dt, mu, rho, closure=-1,
                      params=None, work=None, delta3=None, dparams=None):
+        """mu: the constant viscosity, or the padded VISC field (tensor)."""
         da = self._ptrs(delta3, 'delta') if delta3 is not None else None
-        check(lib().pf_adj_rk4_step(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
-                                    self.padded(w, 'w'), self.padded(ua, 'ua'), self.padded(va, 'va'),
-                                    self.padded(wa, 'wa'), self.padded(uta, 'uta'),
-                                    self.padded(vta, 'vta'), self.padded(wta, 'wta'), float(dt),
-                                    float(mu), float(rho), int(closure), self._params(params),
-                                    self._work(work), da[0] if da else None,
-                                    self._dparams(dparams), self.stream()), 'pf_adj_rk4_step')
+        head = (self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
+                self.padded(ua, 'ua'), self.padded(va, 'va'), self.padded(wa, 'wa'),
+                self.padded(uta, 'uta'), self.padded(vta, 'vta'), self.padded(wta, 'wta'),
+                float(dt))
+        tail = (float(rho), int(closure), self._params(params), self._work(work),
+                da[0] if da else None, self._dparams(dparams), self.stream())
+        if isinstance(mu, torch.Tensor):
+            check(lib().pf_adj_rk4_step_visc(self.h, *head, self.padded(mu, 'VISC'), *tail),
+                  'pf_adj_rk4_step_visc')
+        else:
+            check(lib().pf_adj_rk4_step(self.h, *head, float(mu), *tail), 'pf_adj_rk4_step')
 
     def adj_euler_step(self, u, v, w, ua, va, wa, dt, mu, rho, closure=-1, params=None, work=None,
                        delta3=None, dparams=None):
+        """mu: the constant viscosity, or the padded VISC field (tensor)."""
         da = self._ptrs(delta3, 'delta') if delta3 is not None else None
-        check(lib().pf_adj_euler_step(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
-                                      self.padded(w, 'w'), self.padded(ua, 'ua'),
-                                      self.padded(va, 'va'), self.padded(wa, 'wa'), float(dt),
-                                      float(mu), float(rho), int(closure), self._params(params),
-                                      self._work(work), da[0] if da else None,
-                                      self._dparams(dparams), self.stream()), 'pf_adj_euler_step')
+        head = (self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
+                self.padded(ua, 'ua'), self.padded(va, 'va'), self.padded(wa, 'wa'), float(dt))
+        tail = (float(rho), int(closure), self._params(params), self._work(work),
+                da[0] if da else None, self._dparams(dparams), self.stream())
+        if isinstance(mu, torch.Tensor):
+            check(lib().pf_adj_euler_step_visc(self.h, *head, self.padded(mu, 'VISC'), *tail),
+                  'pf_adj_euler_step_visc')
+        else:
+            check(lib().pf_adj_euler_step(self.h, *head, float(mu), *tail), 'pf_adj_euler_step')
 
     def adj_conv(self, d_axis, s2_axis, centre, a, vi, rhs, coef):
         check(lib().pf_adj_conv(self.h, int(d_axis), int(s2_axis), 1 if centre else 0,

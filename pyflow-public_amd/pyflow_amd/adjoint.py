@@ -65,11 +65,10 @@ class rhs_adjPredictor:
         self.rhs_w = torch.zeros(shape, dtype=decomp.prec, device=decomp.device)
 
     def mu(self):
+        """The constant viscosity, or the padded VISC field when it varies
+        (the kernels then read it per cell, as the forward ones do)."""
         mu = constant_viscosity(self.VISC)
-        if mu is None:
-            raise NotImplementedError('a spatially varying VISC field is not supported by the '
-                                      'fused adjoint kernels')
-        return mu
+        return self.VISC if mu is None else mu
 
     def closure_args(self):
         """(kind, params, work, delta3, dparams) for the fused kernels."""

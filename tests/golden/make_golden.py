@@ -19,7 +19,8 @@ functions that no closed form or textbook routine pins on its own:
   (the reference's ML closure architecture is not part of the build);
 * stress_constCs.eddyVisc (sfsmodel/sfsmodel_smagorinsky.py:65-88), the RHS
   with the resulting viscosity field, and Domain.forwardStep with
-  SFSmodel 'Smagorinsky' (data/domain.py:348-352).
+  SFSmodel 'Smagorinsky' (data/domain.py:348-352);
+* the adjoint RHS and an adjoint RK4 step with that viscosity field.
 
 The reference's MPI layer is replaced by a single-rank periodic stub with the
 semantics of library/parallel.py:301-414 on a periodic Cartesian communicator
@@ -333,6 +334,31 @@ def main():
         out['dom_smag_' + c] = getattr(D, 'state_%s_P' % c).var.numpy().copy()
     out['dom_smag_visc'] = D.VISC_P.numpy().copy()
     out['dom_smag_res'] = float(D.max_resP)
+
+    # 10. Adjoint RHS and an adjoint RK4 step (BiCGStab) with the Smagorinsky
+    #     VISC field of section 9 in the viscous part
+    su, sv, sw = states(u0, v0, w0)
+    au, av, aw = states(a0, b0, c0)
+    ar = adjoint.rhs_adjPredictor(d, metric, rho, VISCf, zsrc, su, sv, sw)
+    ar.evaluate(au, av, aw)
+    out['adjrhs_smag_u'], out['adjrhs_smag_v'], out['adjrhs_smag_w'] = (
+        ar.rhs_u.numpy().copy(), ar.rhs_v.numpy().copy(), ar.rhs_w.numpy().copy())
+    su, sv, sw = states(u0, v0, w0)
+    au, av, aw = states(a0, b0, c0)
+    tu, tv, tw = states(a0, b0, c0)
+    arhs = adjoint.AdjointRHS(types.SimpleNamespace(advancerName='RK4'), d, metric, rho, VISCf,
+                              zsrc, su, sv, sw)
+    adv = advancer.RK4(au, tu, av, tv, aw, tw, arhs)
+    poisson = pressure.solver_bicgstab(comms, d, metric, geo, rho, dt, 1e-12, 500)
+    ns = types.SimpleNamespace(
+        doPressure=True, metric=metric, state_u_adj_P=au, state_v_adj_P=av, state_w_adj_P=aw,
+        source_P=torch.zeros(shape, dtype=prec), poisson=poisson,
+        state_DP_P=state.state_P(d, zeros.copy()), state_p_P=state.state_P(d, zeros.copy()),
+        rho=rho, adjointAdvancer=adv, max_resP=0.0)
+    domain.Domain.adjointStep(ns, dt)
+    for c, a in (('u', au), ('v', av), ('w', aw)):
+        out['adj_rk4_smag_' + c] = a.var.numpy().copy()
+    out['adj_rk4_smag_res'] = float(ns.max_resP)
 
     np.savez_compressed(OUT, **out)
     print('wrote', OUT, 'keys:', len(out))

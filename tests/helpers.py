@@ -1,4 +1,4 @@
-"""Shared helpers for the tests: golden fixtures, GPU box set-up, error norms."""
+"""Shared helpers for the tests: golden fixtures, set-up of the HIP-backed domain objects, error norms."""
 
 import math
 import os

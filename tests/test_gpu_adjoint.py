@@ -242,3 +242,28 @@ def test_adjoint_verification_protocol():
                                             tol=1e-13, maxit=1000)
     assert abs(obj0 - oobj) <= 1e-11 * abs(oobj)
     assert abs(adj - oadj) <= 1e-8 * abs(oadj)
+
+
+def test_adjoint_rhs_and_step_with_viscosity_field(G):
+    """The reference's adjoint RHS and an adjoint RK4 step (BiCGStab 1e-12)
+    with the Smagorinsky VISC field in the viscous part: RHS bit-exact, step
+    to the solver's tolerance."""
+    from pyflow_amd import adjoint, state
+    shape, L, d, g, m = env(G)
+    V = torch.tensor(G['smag_visc'], device='cuda')
+    su, sv, sw = (state.state_P(d, G[k].copy()) for k in ('u0', 'v0', 'w0'))
+    au, av, aw = (state.state_P(d, G[k].copy()) for k in ('adj_u0', 'adj_v0', 'adj_w0'))
+    r = adjoint.rhs_adjPredictor(d, m, float(G['rho']), V, None, su, sv, sw)
+    r.evaluate(au, av, aw)
+    torch.cuda.synchronize()
+    for c in 'uvw':
+        assert np.array_equal(cpu(getattr(r, 'rhs_' + c)), G['adjrhs_smag_' + c]), c
+    D, shape, L = adjoint_domain(G, 'RK4', 'bicgstab', 1e-12, 500, sfs='Smagorinsky')
+    D.VISC_P.copy_(V)
+    D.VISC_P.pf_const_mu = None
+    D.adjointStep(float(G['dt']))
+    torch.cuda.synchronize()
+    for c, a in (('u', D.state_u_adj_P), ('v', D.state_v_adj_P), ('w', D.state_w_adj_P)):
+        e = rel_l2(cpu(a.var), G['adj_rk4_smag_' + c])
+        print('adjoint RK4 step with VISC field: %s rel L2 %.3e' % (c, e))
+        assert e < 1e-10

@@ -172,3 +172,15 @@ def test_adjoint_verification_protocol():
     om, _ = A.adjoint_verification_run(g, u, v, w, 3, -h, False, **kw)
     fd = (op - om) / (2 * h)
     assert abs(adj - fd) <= 1e-3 * abs(fd)
+
+
+def test_adjoint_rhs_with_viscosity_field_golden(G):
+    """The adjoint RHS with the Smagorinsky VISC field in its viscous part
+    (adjoint.py:100-170 with a field)."""
+    g = grid(G)
+    fu, fv, fw = (O.Field(g, G[k]) for k in ('u0', 'v0', 'w0'))
+    V = torch.as_tensor(G['smag_visc'])
+    r = A.AdjointRHS(g, float(G['rho']), V, fu, fv, fw)
+    r.evaluate(*(O.Field(g, G[k]) for k in ('adj_u0', 'adj_v0', 'adj_w0')))
+    for c in 'uvw':
+        assert np.array_equal(getattr(r, 'rhs_' + c).numpy(), G['adjrhs_smag_' + c])
