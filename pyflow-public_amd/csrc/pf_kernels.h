@@ -100,16 +100,17 @@ inline int tile_extent_z(int nz) {
 }
 
 // Tile grid for `pair` cells per thread and x chunks of at least `minp`
-// planes (a chunk also loads the planes on either side of it), up to 1024
-// blocks in all.
-inline TileGrid tile_grid_with(const Geom& g, int tz, int pair, int rows, int minp) {
+// planes (a chunk also loads the planes on either side of it), up to
+// `target` blocks in all.
+inline TileGrid tile_grid_with(const Geom& g, int tz, int pair, int rows, int minp,
+                               int target = 1024) {
   TileGrid t;
   t.tz = tz;
   t.ty = pair * rows * kBlock / t.tz;
   t.ntz = (g.nz + t.tz - 1) / t.tz;
   t.nty = (g.ny + t.ty - 1) / t.ty;
   const int tiles = t.ntz * t.nty;
-  int cx = 1024 / tiles;
+  int cx = target / tiles;
   const int maxc = g.nx >= minp ? g.nx / minp : 1;
   if (cx > maxc) cx = maxc;
   if (cx < 1) cx = 1;
@@ -137,6 +138,23 @@ inline TileGrid make_tile_grid(const Geom& g, int esize, int rows = 1) {
     }
   }
   return tile_grid_with(g, tz, 1, rows, 4);
+}
+
+// Tile of a block.  The hardware deals consecutive block ids round robin
+// over the 8 XCDs; the ids are renumbered so that each XCD gets a contiguous
+// run of tiles, z fastest, then y.  A run then covers whole z rows of tiles,
+// so the z-halo cells (single-cell loads whose cache lines belong to the
+// z-neighbour tile) and all but the run's edge y halos are lines that a
+// neighbour tile on the same XCD brings into that XCD's L2.
+__device__ __forceinline__ void tile_of_block(const TileGrid& tg, const Blk& blk, int& tz_i,
+                                              int& ty_i, int& cx_i) {
+  const int nb = blk.n;
+  int b = blk.id;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  tz_i = b % tg.ntz;
+  const int rest = b / tg.ntz;
+  ty_i = rest % tg.nty;
+  cx_i = rest / tg.nty;
 }
 
 // ---------------------------------------------------------------------------

@@ -255,23 +255,6 @@ __device__ __forceinline__ bool tile_scalars(KrylovCtl* __restrict__ ctl,
   return true;
 }
 
-// Tile of a block.  The hardware deals consecutive block ids round robin
-// over the 8 XCDs; the ids are renumbered so that each XCD gets a contiguous
-// run of tiles, z fastest, then y.  A run then covers whole z rows of tiles,
-// so the z-halo cells (single-cell loads whose cache lines belong to the
-// z-neighbour tile) and all but the run's edge y halos are lines that a
-// neighbour tile on the same XCD brings into that XCD's L2.
-__device__ __forceinline__ void tile_of_block(const TileGrid& tg, const Blk& blk, int& tz_i,
-                                              int& ty_i, int& cx_i) {
-  const int nb = blk.n;
-  int b = blk.id;
-  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-  tz_i = b % tg.ntz;
-  const int rest = b / tg.ntz;
-  ty_i = rest % tg.nty;
-  cx_i = rest / tg.nty;
-}
-
 template <typename T, int MODE, int TZ, int TY>
 __device__ __forceinline__ void bicg_tile_kernel_body(
     Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,

@@ -28,19 +28,23 @@ __device__ __forceinline__ T edge4(const T* __restrict__ F, int64_t c, int64_t s
   return (T)0.25 * (((F[c] + F[c - sa]) + F[c - sb]) + F[c - sa - sb]);
 }
 
-// RHS at interior cell (i,j,k); U,V,W point at the padded interior origin and
-// c is the padded offset of the cell.  ADV = false leaves out the advective
+// RHS at interior cell (i,j,k); c is the padded offset of the cell (from the
+// interior origin) in the SFS and viscosity fields.  ADV = false leaves out the advective
 // terms (the adjoint RHS applies the viscous operator alone to its field).
 // VF = true takes the viscosity from the field sfs.visc (an eddy viscosity,
 // domain.py:348-352): 2*VISC/rho at the cell centres of the normal fluxes and
 // interp_sc_xy/xz/yz(VISC)/rho at the edges of the cross fluxes
 // (velocity.py:248-305), each rounded as the reference's whole-array ops
 // round them; VF = false uses the constant coefficients of k.
-template <typename T, bool SFS, bool ADV = true, bool VF = false>
-__device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __restrict__ V,
-                                         const T* __restrict__ W, int64_t c, int64_t sx,
-                                         int64_t sy, const StepCoef<T>& k, const Sfs<T>& sfs,
-                                         T& ru, T& rv, T& rw) {
+//
+// U, V and W are taps: U(di, dj, dk) is u at (i+di, j+dj, k+dk), read from
+// global memory (GlobalTap) or from the LDS planes of the tiled kernel
+// (PlaneTap).  The arithmetic is the same for both; the rhs_cell overload
+// below takes plain pointers to the padded interior origin.
+template <typename T, bool SFS, bool ADV, bool VF, class Tap>
+__device__ __forceinline__ void rhs_cell_tap(const Tap& U, const Tap& V, const Tap& W, int64_t c,
+                                             int64_t sx, int64_t sy, const StepCoef<T>& k,
+                                             const Sfs<T>& sfs, T& ru, T& rv, T& rw) {
   const T dxi = k.dxi, dyi = k.dyi, dzi = k.dzi;
   const T half = (T)0.5;
   const T* __restrict__ vis = sfs.visc;
@@ -48,7 +52,7 @@ __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __res
   auto exy = [&](int64_t o) -> T { return VF ? edge4(vis, o, sx, sy) / sfs.rho : k.exy; };
   auto exz = [&](int64_t o) -> T { return VF ? edge4(vis, o, sx, (int64_t)1) / sfs.rho : k.exz; };
   auto eyz = [&](int64_t o) -> T { return VF ? edge4(vis, o, sy, (int64_t)1) / sfs.rho : k.eyz; };
-#define LD(A, di, dj, dk) A[c + (di) * sx + (dj) * sy + (dk)]
+#define LD(A, di, dj, dk) A(di, dj, dk)
   const T u000 = LD(U, 0, 0, 0), up00 = LD(U, 1, 0, 0), um00 = LD(U, -1, 0, 0);
   const T u0p0 = LD(U, 0, 1, 0), u0m0 = LD(U, 0, -1, 0), u00p = LD(U, 0, 0, 1), u00m = LD(U, 0, 0, -1);
   const T ump0 = LD(U, -1, 1, 0), upm0 = LD(U, 1, -1, 0), um0p = LD(U, -1, 0, 1), up0m = LD(U, 1, 0, -1);
@@ -171,6 +175,25 @@ __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __res
   rw = r;
 }
 
+// Stencil input read from global memory: p is the field at the cell.
+template <typename T>
+struct GlobalTap {
+  const T* __restrict__ p;
+  int64_t sx, sy;
+  __device__ __forceinline__ T operator()(int di, int dj, int dk) const {
+    return p[di * sx + dj * sy + dk];
+  }
+};
+
+template <typename T, bool SFS, bool ADV = true, bool VF = false>
+__device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __restrict__ V,
+                                         const T* __restrict__ W, int64_t c, int64_t sx,
+                                         int64_t sy, const StepCoef<T>& k, const Sfs<T>& sfs,
+                                         T& ru, T& rv, T& rw) {
+  rhs_cell_tap<T, SFS, ADV, VF>(GlobalTap<T>{U + c, sx, sy}, GlobalTap<T>{V + c, sx, sy},
+                                GlobalTap<T>{W + c, sx, sy}, c, sx, sy, k, sfs, ru, rv, rw);
+}
+
 // Stage kinds.
 enum { kStage1 = 1, kStage2 = 2, kStage3 = 3, kStage4 = 4, kEuler = 0, kRhsOnly = 5 };
 
@@ -179,31 +202,53 @@ enum { kStage1 = 1, kStage2 = 2, kStage3 = 3, kStage4 = 4, kEuler = 0, kRhsOnly 
 //   acc   : running k1 + 2k2 + 2k3 (interior)
 //   Yout  : next stage state (padded), or the Euler result, or the RHS (interior)
 //   Uout  : u* for stage 4 (padded; same arrays as Un, read and written at the own cell only)
+// The values a stage reads at its own cell besides the stencil: u^n (all
+// kinds but the bare RHS) and the running sum (stages 2-4).
 template <typename T, int KIND>
-__device__ __forceinline__ void stage_update(const Geom& g, uint32_t idx, int i, int j, int kk,
-                                             int64_t c, const T (&r)[3], const CVec3<T>& Un,
-                                             const Vec3<T>& acc, const Vec3<T>& Yout,
-                                             const Vec3<T>& Uout, const StepCoef<T>& k) {
+struct CellIn {
+  T un[3], ac[3];
+  __device__ __forceinline__ void load(const CVec3<T>& Un, const Vec3<T>& acc, uint32_t idx,
+                                       int64_t c) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      un[q] = KIND != kRhsOnly ? Un.a[q][c] : (T)0;
+      ac[q] = (KIND == kStage2 || KIND == kStage3 || KIND == kStage4) ? acc.a[q][idx] : (T)0;
+    }
+  }
+};
+
+template <typename T, int KIND>
+__device__ __forceinline__ void stage_store(const Geom& g, uint32_t idx, int i, int j, int kk,
+                                            const T (&r)[3], const CellIn<T, KIND>& in,
+                                            const Vec3<T>& acc, const Vec3<T>& Yout,
+                                            const Vec3<T>& Uout, const StepCoef<T>& k) {
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     if (KIND == kRhsOnly) {
       Yout.a[q][idx] = r[q];
     } else if (KIND == kEuler) {
-      const T un = Un.a[q][c];
-      store_padded(g, Yout.a[q] - g.poff, i, j, kk, un + (r[q] * k.dt));
+      store_padded(g, Yout.a[q] - g.poff, i, j, kk, in.un[q] + (r[q] * k.dt));
     } else if (KIND == kStage4) {
-      const T un = Un.a[q][c];
-      const T s = acc.a[q][idx] + r[q];
-      store_padded(g, Uout.a[q] - g.poff, i, j, kk, un + (k.dt6 * s));
+      const T s = in.ac[q] + r[q];
+      store_padded(g, Uout.a[q] - g.poff, i, j, kk, in.un[q] + (k.dt6 * s));
     } else {
-      const T un = Un.a[q][c];
       if (KIND == kStage1)
         acc.a[q][idx] = r[q];
       else
-        acc.a[q][idx] = acc.a[q][idx] + ((T)2.0 * r[q]);
-      store_padded(g, Yout.a[q] - g.poff, i, j, kk, (k.half_dt * r[q]) + un);
+        acc.a[q][idx] = in.ac[q] + ((T)2.0 * r[q]);
+      store_padded(g, Yout.a[q] - g.poff, i, j, kk, (k.half_dt * r[q]) + in.un[q]);
     }
   }
+}
+
+template <typename T, int KIND>
+__device__ __forceinline__ void stage_update(const Geom& g, uint32_t idx, int i, int j, int kk,
+                                             int64_t c, const T (&r)[3], const CVec3<T>& Un,
+                                             const Vec3<T>& acc, const Vec3<T>& Yout,
+                                             const Vec3<T>& Uout, const StepCoef<T>& k) {
+  CellIn<T, KIND> in;
+  in.load(Un, acc, idx, c);
+  stage_store<T, KIND>(g, idx, i, j, kk, r, in, acc, Yout, Uout, k);
 }
 
 // One RK stage (or Euler step, or a bare RHS evaluation) of the forward RHS;
@@ -224,12 +269,132 @@ __device__ __forceinline__ void rk_stage_body(const Geom& g, const CVec3<T>& Y, 
   }
 }
 
-template <typename T, int KIND, bool SFS, bool VF>
-__global__ void __launch_bounds__(kBlock) rk_stage_kernel(Geom g, CVec3<T> Y, CVec3<T> Un,
-                                                          Vec3<T> acc, Vec3<T> Yout, Vec3<T> Uout,
-                                                          StepCoef<T> k, Sfs<T> sfs) {
-  rk_stage_body<T, KIND, SFS, VF>(g, Y, Un, acc, Yout, Uout, k, sfs, this_blk());
+// ---------------------------------------------------------------------------
+// Tiled stage.  A block owns a TZ x TY tile of the (y,z) plane and marches
+// along x over a chunk of planes.  Each plane of the stage input u, v, w
+// (tile plus a one-cell y/z halo with its corners) is read from HBM once and
+// kept in an LDS ring of four planes; the cell at x reads its 33 stencil
+// values from the ring slots of planes x-1, x and x+1.  A flat one-cell-per-
+// thread loop (rk_stage_body, which the batched launches use) re-reads the y
+// and x neighbours from L2 or HBM instead (its consecutive blocks land on
+// different XCDs): about three times the unique bytes at 512^3.  The loads of plane x+2 are in flight while plane x is computed.
+//
+// Ring hazard: iteration x writes plane x+1 into slot (x+1) & 3 before its
+// barrier.  That slot last held plane x-3, read up to iteration x-2, and
+// every thread has finished iteration x-2 before any passes the barrier of
+// x-1, so one barrier per plane suffices.
+//
+// Inputs are padded (two periodic image layers), so every neighbour of an
+// interior cell is an in-bounds padded cell; cells of partial tiles past the
+// padding read the last padded row or column instead (their values reach only
+// cells that store nothing).
+
+// Stencil input read from the LDS ring: m, o, p point at the cell's entry in
+// the slots of planes x-1, x and x+1.
+template <typename T, int RW>
+struct PlaneTap {
+  const T* m;
+  const T* o;
+  const T* p;
+  __device__ __forceinline__ T operator()(int di, int dj, int dk) const {
+    const T* b = di < 0 ? m : (di > 0 ? p : o);
+    return b[dj * RW + dk];
+  }
+};
+
+template <typename T, int KIND, bool SFS, bool VF, int TZ, int TY>
+__global__ void __launch_bounds__(TZ * TY) rk_tile_kernel(Geom g, TileGrid tg, CVec3<T> Y,
+                                                          CVec3<T> Un, Vec3<T> acc, Vec3<T> Yout,
+                                                          Vec3<T> Uout, StepCoef<T> k,
+                                                          Sfs<T> sfs) {
+  constexpr int RW = TZ + 2;           // LDS row: tile row plus the two z-halo cells
+  constexpr int PL = (TY + 2) * RW;    // one field's plane
+  static_assert(TZ * TY >= 2 * TZ + 2 * TY + 4, "halo needs a thread per cell");
+  __shared__ T ring[4][3][PL];
+
+  int tz_i, ty_i, cx_i;
+  tile_of_block(tg, this_blk(), tz_i, ty_i, cx_i);
+  const int z0 = tz_i * TZ, y0 = ty_i * TY, x0 = cx_i * tg.xc;
+  const int x1 = min(x0 + tg.xc, g.nx);
+
+  const int t = threadIdx.x;
+  const int lz = t % TZ, ly = t / TZ;
+  const int z = z0 + lz, y = y0 + ly;
+  const bool active = z < g.nz && y < g.ny;
+  const int64_t col = (int64_t)min(y, g.ny + 1) * g.psy + min(z, g.nz + 1);
+  const int own = (ly + 1) * RW + lz + 1;
+
+  // halo cells: rows y0-1 and y0+TY (2*TZ threads), columns z0-1 and z0+TZ
+  // (2*TY threads), then the four corners
+  int hslot = -1, hy = 0, hz = 0;
+  if (t < 2 * TZ) {
+    const bool hi = t >= TZ;
+    hy = hi ? y0 + TY : y0 - 1;
+    hz = z0 + t % TZ;
+    hslot = (hi ? TY + 1 : 0) * RW + t % TZ + 1;
+  } else if (t < 2 * TZ + 2 * TY) {
+    const int u = t - 2 * TZ;
+    const bool hi = u >= TY;
+    hy = y0 + u % TY;
+    hz = hi ? z0 + TZ : z0 - 1;
+    hslot = (u % TY + 1) * RW + (hi ? TZ + 1 : 0);
+  } else if (t < 2 * TZ + 2 * TY + 4) {
+    const int u = t - 2 * TZ - 2 * TY;
+    hy = (u & 1) ? y0 + TY : y0 - 1;
+    hz = (u & 2) ? z0 + TZ : z0 - 1;
+    hslot = ((u & 1) ? TY + 1 : 0) * RW + ((u & 2) ? TZ + 1 : 0);
+  }
+  const bool is_halo = hslot >= 0;
+  const int64_t hcol = (int64_t)min(hy, g.ny + 1) * g.psy + min(hz, g.nz + 1);
+
+  struct Plane {
+    T c[3], h[3];
+  };
+  auto load_plane = [&](int x) -> Plane {
+    Plane P;
+    const int64_t xo = (int64_t)x * g.psx;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      P.c[q] = Y.a[q][xo + col];
+      P.h[q] = is_halo ? Y.a[q][xo + hcol] : (T)0;
+    }
+    return P;
+  };
+  auto put_plane = [&](int x, const Plane& P) {
+    T(*s)[PL] = ring[(x + 4) & 3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      s[q][own] = P.c[q];
+      if (is_halo) s[q][hslot] = P.h[q];
+    }
+  };
+
+  put_plane(x0 - 1, load_plane(x0 - 1));
+  put_plane(x0, load_plane(x0));
+  Plane nxt = load_plane(x0 + 1);
+  const int64_t pcol = (int64_t)y * g.psy + z, icol = (int64_t)y * g.isy + z;
+  for (int x = x0; x < x1; ++x) {
+    put_plane(x + 1, nxt);
+    if (x + 1 < x1) nxt = load_plane(x + 2);
+    __syncthreads();
+    if (active) {
+      const int64_t c = (int64_t)x * g.psx + pcol;
+      const T* sm = &ring[(x + 3) & 3][0][own];
+      const T* so = &ring[x & 3][0][own];
+      const T* sp = &ring[(x + 1) & 3][0][own];
+      const PlaneTap<T, RW> tu{sm, so, sp}, tv{sm + PL, so + PL, sp + PL},
+          tw{sm + 2 * PL, so + 2 * PL, sp + 2 * PL};
+      T r[3];
+      rhs_cell_tap<T, SFS, true, VF>(tu, tv, tw, c, g.psx, g.psy, k, sfs, r[0], r[1], r[2]);
+      const uint32_t idx = (uint32_t)((int64_t)x * g.isx + icol);
+      stage_update<T, KIND>(g, idx, x, y, z, c, r, Un, acc, Yout, Uout, k);
+    }
+  }
 }
+
+// Tiles of the stage kernel: 32 x 8 cells, chunks of at least 8 planes,
+// up to 1024 blocks.
+inline TileGrid stage_tile_grid(const Geom& g) { return tile_grid_with(g, 32, 1, 1, 8); }
 
 // ---------------------------------------------------------------------------
 // Adjoint RHS of the reference's adjoint step (solver/adjoint.py:94-298):
@@ -372,17 +537,17 @@ int launch_stage(pf_box* h, hipStream_t s, const void* const Y[3], const void* c
   for (int m = 0; m < 6; ++m) sf.f[m] = sfs6 ? (const T*)sfs6[m] + g.poff : nullptr;
   sf.visc = visc ? (const T*)visc + g.poff : nullptr;
   sf.rho = (T)rho;
-  const int nb = launch_blocks(g);
-#define PF_STAGE(SFS, VF)                                                                   \
-  hipLaunchKernelGGL((rk_stage_kernel<T, KIND, SFS, VF>), dim3(nb), dim3(kBlock), 0, s, g, y, \
-                     un, acc, yo, uo, k, sf)
+  const TileGrid tg = stage_tile_grid(g);
+#define PF_STAGE(SFS, VF)                                                                     \
+  hipLaunchKernelGGL((rk_tile_kernel<T, KIND, SFS, VF, 32, 8>), dim3(tg.nblk), dim3(256), 0, s, \
+                     g, tg, y, un, acc, yo, uo, k, sf)
   if (visc) {
     if (sfs6) PF_STAGE(true, true); else PF_STAGE(false, true);
   } else {
     if (sfs6) PF_STAGE(true, false); else PF_STAGE(false, false);
   }
 #undef PF_STAGE
-  PF_LAUNCH_CHECK("rk_stage_kernel");
+  PF_LAUNCH_CHECK("rk_tile_kernel");
   return 0;
 }
 
