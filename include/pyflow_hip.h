@@ -163,6 +163,39 @@ int pf_rhs_ns_visc(pf_box* box, const void* u, const void* v, const void* w, con
                    void* stream);
 
 /*
+ * 'source' closures (sfsmodel modelType 'source', velocity.py:360-383).  The
+ * reference evaluates such a closure inside every RHS evaluation: it
+ * interpolates u, v, w to cell centres into state.var_i (interp_u_xm,
+ * interp_v_ym, interp_w_zm), calls sfsmodel.update(var_i...), and subtracts
+ * GX, GY, GZ interpolated to the faces (interp_sc_x/y/z) from the RHS.
+ *
+ * pf_source_inputs writes into ui, vi, wi (padded) exactly what the
+ * reference's var_i arrays hold at that call: the centre interpolants, and on
+ * the last plane along each component's axis the edge interpolants the
+ * advective terms of the same evaluation left there (one corner cell per
+ * array, never written by the reference's RHS, is left unchanged).
+ * pf_rhs_ns_src and pf_euler_step_src are pf_rhs_ns / pf_euler_step with the
+ * closure term (constant viscosity, no stresses: a source closure is the
+ * run's only closure).  An RK4 step with such a closure is four calls of
+ * pf_rk4_stage_src (stage 1..4), each preceded by pf_rk4_source_inputs of
+ * the same stage (the inputs of that stage's state) and the closure's own
+ * evaluation; the four stages together equal pf_rk4_step's sequence
+ * (advancer.py:75-113).  gx, gy, gz: padded, box dtype.
+ */
+int pf_source_inputs(pf_box* box, const void* u, const void* v, const void* w, void* ui, void* vi,
+                     void* wi, void* stream);
+int pf_rhs_ns_src(pf_box* box, const void* u, const void* v, const void* w, double mu, double rho,
+                  const void* gx, const void* gy, const void* gz, void* ru, void* rv, void* rw,
+                  void* stream);
+int pf_euler_step_src(pf_box* box, void* u, void* v, void* w, double dt, double mu, double rho,
+                      const void* gx, const void* gy, const void* gz, void* stream);
+int pf_rk4_source_inputs(pf_box* box, int stage, void* u, void* v, void* w, void* ut, void* vt,
+                         void* wt, void* ui, void* vi, void* wi, void* stream);
+int pf_rk4_stage_src(pf_box* box, int stage, void* u, void* v, void* w, void* ut, void* vt,
+                     void* wt, double dt, double mu, double rho, const void* gx, const void* gy,
+                     const void* gz, void* stream);
+
+/*
  * Smagorinsky eddy viscosity plus the molecular one, written to the padded
  * viscosity field: visc = mu + rho*(Cs*Delta)^2*|S| with
  * |S| = sqrt(S11^2+S22^2+S33^2 + 2(S12^2+S13^2+S23^2)) of the trace-free

@@ -494,6 +494,18 @@ class NavierStokesRHS:
         interp_uw_y(fw); interp_uv_z(fv); vel_conv_y(g, fw, fv, self.rhs_w)
         interp_w_zm(fw); vel_conv_zz(g, fw, fw, self.rhs_w)
 
+        # 'source' closures: :360-383 (inputs are the var_i arrays as the
+        # interpolations above and these three leave them)
+        if sfs is not None and sfs.modelType == 'source':
+            interp_u_xm(fu); interp_v_ym(fv); interp_w_zm(fw)
+            sfs.update(fu.var_i, fv.var_i, fw.var_i)
+            SC = self.interp_SC
+            for G, interp, rhs in ((sfs.GX, interp_sc_x, self.rhs_u),
+                                   (sfs.GY, interp_sc_y, self.rhs_v),
+                                   (sfs.GZ, interp_sc_z, self.rhs_w)):
+                interp(g, G[..., 0] if G.dim() == 4 else G, SC)
+                rhs.sub_(g.interior(SC))
+
     def viscous(self, fu, fv, fw, tensor):
         """Viscous (and tensor-closure) flux divergence accumulated into
         rhs_u/v/w, velocity.py:235-317."""

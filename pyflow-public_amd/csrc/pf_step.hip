@@ -20,6 +20,7 @@ struct Sfs {
   const T* f[6];      // FXX, FYY, FZZ, FXY, FXZ, FYZ at the interior origin, or all NULL
   const T* visc;      // viscosity field VISC_P at the interior origin (field-viscosity kernels)
   T rho;              // density, for the field viscosity's 2*VISC/rho and interp_sc(VISC)/rho
+  const T* src[3];    // GX, GY, GZ of a 'source' closure at the interior origin (SRC kernels)
 };
 
 // 0.25*(((F(0)+F(-a))+F(-b))+F(-a-b)): interp_sc_xy/xz/yz at an edge.
@@ -41,7 +42,11 @@ __device__ __forceinline__ T edge4(const T* __restrict__ F, int64_t c, int64_t s
 // global memory (GlobalTap) or from the LDS planes of the tiled kernel
 // (PlaneTap).  The arithmetic is the same for both; the rhs_cell overload
 // below takes plain pointers to the padded interior origin.
-template <typename T, bool SFS, bool ADV, bool VF, class Tap>
+//
+// SRC = true subtracts a 'source' closure's GX, GY, GZ interpolated to the
+// faces, 0.5*(G[i] + G[i-1]) along the component's own axis, after the
+// advective terms (velocity.py:360-383).
+template <typename T, bool SFS, bool ADV, bool VF, bool SRC, class Tap>
 __device__ __forceinline__ void rhs_cell_tap(const Tap& U, const Tap& V, const Tap& W, int64_t c,
                                              int64_t sx, int64_t sy, const StepCoef<T>& k,
                                              const Sfs<T>& sfs, T& ru, T& rv, T& rw) {
@@ -104,6 +109,7 @@ __device__ __forceinline__ void rhs_cell_tap(const Tap& U, const Tap& V, const T
     const T uz_c = half * (u000 + u00m), wx_c = half * (w000 + wm00);
     r = r - ((uz_p * wx_p - uz_c * wx_c) * dzi);
   }
+  if (SRC) r = r - (T)0.5 * (sfs.src[0][c] + sfs.src[0][c - sx]);
   ru = r;
 
   // ---- v: viscous fluxes (velocity.py:272-293) ----
@@ -138,6 +144,7 @@ __device__ __forceinline__ void rhs_cell_tap(const Tap& U, const Tap& V, const T
     const T vz_c = half * (v000 + v00m), wy_c = half * (w000 + w0m0);
     r = r - ((vz_p * wy_p - vz_c * wy_c) * dzi);
   }
+  if (SRC) r = r - (T)0.5 * (sfs.src[1][c] + sfs.src[1][c - sy]);
   rv = r;
 
   // ---- w: viscous fluxes (velocity.py:296-317) ----
@@ -172,6 +179,7 @@ __device__ __forceinline__ void rhs_cell_tap(const Tap& U, const Tap& V, const T
     const T a = half * (w00p + w000), b = half * (w000 + w00m);
     r = r - ((a * a - b * b) * dzi);
   }
+  if (SRC) r = r - (T)0.5 * (sfs.src[2][c] + sfs.src[2][c - (int64_t)1]);
   rw = r;
 }
 
@@ -190,7 +198,7 @@ __device__ __forceinline__ void rhs_cell(const T* __restrict__ U, const T* __res
                                          const T* __restrict__ W, int64_t c, int64_t sx,
                                          int64_t sy, const StepCoef<T>& k, const Sfs<T>& sfs,
                                          T& ru, T& rv, T& rw) {
-  rhs_cell_tap<T, SFS, ADV, VF>(GlobalTap<T>{U + c, sx, sy}, GlobalTap<T>{V + c, sx, sy},
+  rhs_cell_tap<T, SFS, ADV, VF, false>(GlobalTap<T>{U + c, sx, sy}, GlobalTap<T>{V + c, sx, sy},
                                 GlobalTap<T>{W + c, sx, sy}, c, sx, sy, k, sfs, ru, rv, rw);
 }
 
@@ -302,7 +310,7 @@ struct PlaneTap {
   }
 };
 
-template <typename T, int KIND, bool SFS, bool VF, int TZ, int TY>
+template <typename T, int KIND, bool SFS, bool VF, bool SRC, int TZ, int TY>
 __global__ void __launch_bounds__(TZ * TY) rk_tile_kernel(Geom g, TileGrid tg, CVec3<T> Y,
                                                           CVec3<T> Un, Vec3<T> acc, Vec3<T> Yout,
                                                           Vec3<T> Uout, StepCoef<T> k,
@@ -385,7 +393,7 @@ __global__ void __launch_bounds__(TZ * TY) rk_tile_kernel(Geom g, TileGrid tg, C
       const PlaneTap<T, RW> tu{sm, so, sp}, tv{sm + PL, so + PL, sp + PL},
           tw{sm + 2 * PL, so + 2 * PL, sp + 2 * PL};
       T r[3];
-      rhs_cell_tap<T, SFS, true, VF>(tu, tv, tw, c, g.psx, g.psy, k, sfs, r[0], r[1], r[2]);
+      rhs_cell_tap<T, SFS, true, VF, SRC>(tu, tv, tw, c, g.psx, g.psy, k, sfs, r[0], r[1], r[2]);
       const uint32_t idx = (uint32_t)((int64_t)x * g.isx + icol);
       stage_update<T, KIND>(g, idx, x, y, z, c, r, Un, acc, Yout, Uout, k);
     }
@@ -519,7 +527,8 @@ __global__ void __launch_bounds__(kBlock) adj_stage_kernel(Geom g, CVec3<T> Y, C
 template <typename T, int KIND>
 int launch_stage(pf_box* h, hipStream_t s, const void* const Y[3], const void* const Un[3],
                  void* const Yout[3], void* const Uout[3], const StepCoef<T>& k,
-                 const void* const* sfs6, const void* visc = nullptr, double rho = 1.0) {
+                 const void* const* sfs6, const void* visc = nullptr, double rho = 1.0,
+                 const void* const* src3 = nullptr) {
   const Geom& g = h->g;
   CVec3<T> y, un;
   Vec3<T> acc, yo, uo;
@@ -537,14 +546,18 @@ int launch_stage(pf_box* h, hipStream_t s, const void* const Y[3], const void* c
   for (int m = 0; m < 6; ++m) sf.f[m] = sfs6 ? (const T*)sfs6[m] + g.poff : nullptr;
   sf.visc = visc ? (const T*)visc + g.poff : nullptr;
   sf.rho = (T)rho;
+  for (int q = 0; q < 3; ++q) sf.src[q] = src3 ? (const T*)src3[q] + g.poff : nullptr;
   const TileGrid tg = stage_tile_grid(g);
-#define PF_STAGE(SFS, VF)                                                                     \
-  hipLaunchKernelGGL((rk_tile_kernel<T, KIND, SFS, VF, 32, 8>), dim3(tg.nblk), dim3(256), 0, s, \
-                     g, tg, y, un, acc, yo, uo, k, sf)
-  if (visc) {
-    if (sfs6) PF_STAGE(true, true); else PF_STAGE(false, true);
+#define PF_STAGE(SFS, VF, SRC)                                                          \
+  hipLaunchKernelGGL((rk_tile_kernel<T, KIND, SFS, VF, SRC, 32, 8>), dim3(tg.nblk), dim3(256), \
+                     0, s, g, tg, y, un, acc, yo, uo, k, sf)
+  // a 'source' closure is the only closure of its run (constant viscosity, no stresses)
+  if (src3) {
+    PF_STAGE(false, false, true);
+  } else if (visc) {
+    if (sfs6) PF_STAGE(true, true, false); else PF_STAGE(false, true, false);
   } else {
-    if (sfs6) PF_STAGE(true, false); else PF_STAGE(false, false);
+    if (sfs6) PF_STAGE(true, false, false); else PF_STAGE(false, false, false);
   }
 #undef PF_STAGE
   PF_LAUNCH_CHECK("rk_tile_kernel");
@@ -582,6 +595,87 @@ int euler_impl(pf_box* h, void* const U[3], double dt, double mu, double rho,
   const size_t np = (size_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4) * sizeof(T);
   for (int q = 0; q < 3; ++q) PF_HIP(hipMemcpyAsync(U[q], B[q], np, hipMemcpyDeviceToDevice, s));
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Inputs of a 'source' closure (velocity.py:365-371): the closure receives
+// state_u.var_i, state_v.var_i, state_w.var_i right after interp_u_xm,
+// interp_v_ym and interp_w_zm.  Those write every plane but the last along
+// their own axis; the last plane still holds what the advective terms of the
+// same RHS evaluation wrote there last (velocity.py:322-355):
+//   u: k >= 1 interp_uv_z (x-plane nxo-1), k = 0 interp_uw_y;
+//   v: k >= 1 interp_uv_z (y-plane nyo-1), k = 0 interp_vw_x;
+//   w: j >= 1 interp_uw_y (z-plane nzo-1), j = 0 interp_vw_x;
+// and one corner of that plane ((nxo-1,0,0), (0,nyo-1,0), (0,0,nzo-1)) is
+// written by none of them, so it keeps the value it had (zero from the
+// allocation in state.py:125 unless a caller wrote it).  All values are
+// 0.5*(f[a] + f[a - e]) as metric_staggered.py:149-188 forms them.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) source_inputs_kernel(Geom g, CVec3<T> U, Vec3<T> I) {
+  const int nxo = g.nx + 2 * kNover, nyo = g.ny + 2 * kNover, nzo = g.nz + 2 * kNover;
+  const int64_t n = (int64_t)nxo * nyo * nzo;
+  const T half = (T)0.5;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(p % nzo);
+    const int j = (int)((p / nzo) % nyo);
+    const int i = (int)(p / ((int64_t)nzo * nyo));
+    const T* u = U.a[0];
+    const T* v = U.a[1];
+    const T* w = U.a[2];
+    // u: x-centre interpolant, or the last x-plane's edge values
+    if (i < nxo - 1) I.a[0][p] = half * (u[p + g.psx] + u[p]);
+    else if (k >= 1) I.a[0][p] = half * (u[p] + u[p - 1]);
+    else if (j >= 1) I.a[0][p] = half * (u[p] + u[p - g.psy]);
+    // v
+    if (j < nyo - 1) I.a[1][p] = half * (v[p + g.psy] + v[p]);
+    else if (k >= 1) I.a[1][p] = half * (v[p] + v[p - 1]);
+    else if (i >= 1) I.a[1][p] = half * (v[p] + v[p - g.psx]);
+    // w
+    if (k < nzo - 1) I.a[2][p] = half * (w[p + 1] + w[p]);
+    else if (j >= 1) I.a[2][p] = half * (w[p] + w[p - g.psy]);
+    else if (i >= 1) I.a[2][p] = half * (w[p] + w[p - g.psx]);
+  }
+}
+
+template <typename T>
+int source_inputs_impl(pf_box* h, const void* const U[3], void* const I[3], hipStream_t s) {
+  CVec3<T> u;
+  Vec3<T> in;
+  for (int q = 0; q < 3; ++q) {
+    u.a[q] = (const T*)U[q];
+    in.a[q] = (T*)I[q];
+  }
+  const int64_t n = (int64_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4);
+  int64_t nb = (n + kBlock - 1) / kBlock;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(source_inputs_kernel<T>, dim3((unsigned)nb), dim3(kBlock), 0, s, h->g, u, in);
+  PF_LAUNCH_CHECK("source_inputs_kernel");
+  return 0;
+}
+
+// The stage input and output buffers of rk4_impl's sequence (advancer.py:75-113):
+// stage 1 U -> A, stage 2 A -> B, stage 3 B -> A, stage 4 A -> U, with A the
+// caller's ut/vt/wt and B the box's second buffer.
+inline const void* rk4_stage_input(pf_box* h, int stage, void* const U[3], void* const Ut[3],
+                                   int q) {
+  return stage == 1 ? U[q] : (stage == 3 ? h->ypong[q] : Ut[q]);
+}
+
+template <typename T>
+int rk4_stage_src_impl(pf_box* h, int stage, void* const U[3], void* const Ut[3], double dt,
+                       double mu, double rho, const void* const G[3], hipStream_t s) {
+  const StepCoef<T> k = make_step_coef<T>(h, mu, rho, dt);
+  void* const B[3] = {h->ypong[0], h->ypong[1], h->ypong[2]};
+  const void* Y[3];
+  for (int q = 0; q < 3; ++q) Y[q] = rk4_stage_input(h, stage, U, Ut, q);
+  const void* const Uc[3] = {U[0], U[1], U[2]};
+  switch (stage) {
+    case 1: return launch_stage<T, kStage1>(h, s, Y, Uc, Ut, nullptr, k, nullptr, nullptr, rho, G);
+    case 2: return launch_stage<T, kStage2>(h, s, Y, Uc, B, nullptr, k, nullptr, nullptr, rho, G);
+    case 3: return launch_stage<T, kStage3>(h, s, Y, Uc, Ut, nullptr, k, nullptr, nullptr, rho, G);
+    default: return launch_stage<T, kStage4>(h, s, Y, Uc, nullptr, U, k, nullptr, nullptr, rho, G);
+  }
 }
 
 template <typename T, int KIND>
@@ -841,6 +935,97 @@ int pf_rhs_ns(pf_box* h, const void* u, const void* v, const void* w, double mu,
                                        make_step_coef<float>(h, mu, rho, 0.0), sfs6);
 }
 
+int pf_source_inputs(pf_box* h, const void* u, const void* v, const void* w, void* ui, void* vi,
+                     void* wi, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  const void* U[3] = {u, v, w};
+  void* I[3] = {ui, vi, wi};
+  if (int rc = check_ptrs(U, 3, "pf_source_inputs")) return rc;
+  if (int rc = check_ptrs((const void* const*)I, 3, "pf_source_inputs")) return rc;
+  for (int q = 0; q < 3; ++q)
+    if (I[q] == U[0] || I[q] == U[1] || I[q] == U[2])
+      return fail(PF_EINVAL, "pf_source_inputs: outputs must not alias the velocity");
+  hipStream_t s = (hipStream_t)stream;
+  return h->dtype == PF_F64 ? source_inputs_impl<double>(h, U, I, s)
+                            : source_inputs_impl<float>(h, U, I, s);
+}
+
+int pf_rk4_source_inputs(pf_box* h, int stage, void* u, void* v, void* w, void* ut, void* vt,
+                         void* wt, void* ui, void* vi, void* wi, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (stage < 1 || stage > 4) return fail(PF_EINVAL, "pf_rk4_source_inputs: stage must be 1..4");
+  void* U[3] = {u, v, w};
+  void* Ut[3] = {ut, vt, wt};
+  if (int rc = check_ptrs((const void* const*)U, 3, "pf_rk4_source_inputs")) return rc;
+  if (int rc = check_ptrs((const void* const*)Ut, 3, "pf_rk4_source_inputs")) return rc;
+  const void* Y[3];
+  for (int q = 0; q < 3; ++q) Y[q] = rk4_stage_input(h, stage, U, Ut, q);
+  return pf_source_inputs(h, Y[0], Y[1], Y[2], ui, vi, wi, stream);
+}
+
+int pf_rk4_stage_src(pf_box* h, int stage, void* u, void* v, void* w, void* ut, void* vt,
+                     void* wt, double dt, double mu, double rho, const void* gx, const void* gy,
+                     const void* gz, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (stage < 1 || stage > 4) return fail(PF_EINVAL, "pf_rk4_stage_src: stage must be 1..4");
+  void* U[3] = {u, v, w};
+  void* Ut[3] = {ut, vt, wt};
+  const void* G[3] = {gx, gy, gz};
+  if (int rc = check_ptrs((const void* const*)U, 3, "pf_rk4_stage_src")) return rc;
+  if (int rc = check_ptrs((const void* const*)Ut, 3, "pf_rk4_stage_src")) return rc;
+  if (int rc = check_ptrs(G, 3, "pf_rk4_stage_src(G)")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_rk4_stage_src: rho must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  return h->dtype == PF_F64 ? rk4_stage_src_impl<double>(h, stage, U, Ut, dt, mu, rho, G, s)
+                            : rk4_stage_src_impl<float>(h, stage, U, Ut, dt, mu, rho, G, s);
+}
+
+int pf_euler_step_src(pf_box* h, void* u, void* v, void* w, double dt, double mu, double rho,
+                      const void* gx, const void* gy, const void* gz, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  void* U[3] = {u, v, w};
+  const void* G[3] = {gx, gy, gz};
+  if (int rc = check_ptrs((const void* const*)U, 3, "pf_euler_step_src")) return rc;
+  if (int rc = check_ptrs(G, 3, "pf_euler_step_src(G)")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_euler_step_src: rho must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  const void* const Uc[3] = {u, v, w};
+  void* const B[3] = {h->ypong[0], h->ypong[1], h->ypong[2]};
+  int rc;
+  if (h->dtype == PF_F64)
+    rc = launch_stage<double, kEuler>(h, s, Uc, Uc, B, nullptr,
+                                      make_step_coef<double>(h, mu, rho, dt), nullptr, nullptr,
+                                      rho, G);
+  else
+    rc = launch_stage<float, kEuler>(h, s, Uc, Uc, B, nullptr,
+                                     make_step_coef<float>(h, mu, rho, dt), nullptr, nullptr, rho,
+                                     G);
+  if (rc) return rc;
+  const size_t np = (size_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4) * h->esize;
+  for (int q = 0; q < 3; ++q) PF_HIP(hipMemcpyAsync(U[q], B[q], np, hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+int pf_rhs_ns_src(pf_box* h, const void* u, const void* v, const void* w, double mu, double rho,
+                  const void* gx, const void* gy, const void* gz, void* ru, void* rv, void* rw,
+                  void* stream) {
+  if (int rc = check_box(h)) return rc;
+  const void* U[3] = {u, v, w};
+  void* R[3] = {ru, rv, rw};
+  const void* G[3] = {gx, gy, gz};
+  if (int rc = check_ptrs(U, 3, "pf_rhs_ns_src")) return rc;
+  if (int rc = check_ptrs((const void* const*)R, 3, "pf_rhs_ns_src")) return rc;
+  if (int rc = check_ptrs(G, 3, "pf_rhs_ns_src(G)")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_rhs_ns_src: rho must be nonzero");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == PF_F64)
+    return launch_stage<double, kRhsOnly>(h, s, U, nullptr, R, nullptr,
+                                          make_step_coef<double>(h, mu, rho, 0.0), nullptr,
+                                          nullptr, rho, G);
+  return launch_stage<float, kRhsOnly>(h, s, U, nullptr, R, nullptr,
+                                       make_step_coef<float>(h, mu, rho, 0.0), nullptr, nullptr,
+                                       rho, G);
+}
 
 static int adj_closure_args(pf_box* h, int kind, const float* params, void* work,
                             void* const* delta3, double* dparams, AdjClosure* cl,

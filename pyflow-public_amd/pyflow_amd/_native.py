@@ -87,6 +87,11 @@ SIGNATURES = [
     ('pf_bicgstab_mpir_solve', _I, [_P, _P, _P, _P, _D, _D, _D, _I, _PD, ctypes.POINTER(_I), _P]),
     ('pf_mpir_configure', _I, [_P, _D, _I, _I]),
     ('pf_residual_history', _I, [_P, _PD, _I, ctypes.POINTER(ctypes.c_int), _P]),
+    ('pf_source_inputs', _I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    ('pf_rhs_ns_src', _I, [_P, _P, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
+    ('pf_euler_step_src', _I, [_P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P]),
+    ('pf_rk4_source_inputs', _I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    ('pf_rk4_stage_src', _I, [_P, _I, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P]),
 ]
 
 
@@ -254,6 +259,52 @@ class Box:
                                    sa[0] if sa else None, self.interior(ru, 'rhs_u'),
                                    self.interior(rv, 'rhs_v'), self.interior(rw, 'rhs_w'),
                                    self.stream()), 'pf_rhs_ns_visc')
+
+    # -- 'source' closures (velocity.py:360-383) ------------------------------
+    def source_fields(self, G):
+        """GX, GY, GZ as padded pointers; (nxo, nyo, nzo, 1) arrays, the
+        reference's shape (sfsmodel_ML.py:146-151), are taken as padded."""
+        out = []
+        for name, g in zip(('GX', 'GY', 'GZ'), G):
+            if isinstance(g, torch.Tensor) and g.dim() == 4 and g.shape[3] == 1:
+                g = g[..., 0]
+            out.append(self.padded(g, name))
+        return out
+
+    def source_inputs(self, u, v, w, ui, vi, wi):
+        check(lib().pf_source_inputs(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                     self.padded(w, 'w'), self.padded(ui, 'u.var_i'),
+                                     self.padded(vi, 'v.var_i'), self.padded(wi, 'w.var_i'),
+                                     self.stream()), 'pf_source_inputs')
+
+    def rhs_ns_src(self, u, v, w, mu, rho, G, ru, rv, rw):
+        g = self.source_fields(G)
+        check(lib().pf_rhs_ns_src(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                  self.padded(w, 'w'), float(mu), float(rho), *g,
+                                  self.interior(ru, 'rhs_u'), self.interior(rv, 'rhs_v'),
+                                  self.interior(rw, 'rhs_w'), self.stream()), 'pf_rhs_ns_src')
+
+    def euler_step_src(self, u, v, w, dt, mu, rho, G):
+        g = self.source_fields(G)
+        check(lib().pf_euler_step_src(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
+                                      self.padded(w, 'w'), float(dt), float(mu), float(rho), *g,
+                                      self.stream()), 'pf_euler_step_src')
+
+    def rk4_source_inputs(self, stage, u, v, w, ut, vt, wt, ui, vi, wi):
+        check(lib().pf_rk4_source_inputs(self.h, int(stage), self.padded(u, 'u'),
+                                         self.padded(v, 'v'), self.padded(w, 'w'),
+                                         self.padded(ut, 'ut'), self.padded(vt, 'vt'),
+                                         self.padded(wt, 'wt'), self.padded(ui, 'u.var_i'),
+                                         self.padded(vi, 'v.var_i'), self.padded(wi, 'w.var_i'),
+                                         self.stream()), 'pf_rk4_source_inputs')
+
+    def rk4_stage_src(self, stage, u, v, w, ut, vt, wt, dt, mu, rho, G):
+        g = self.source_fields(G)
+        check(lib().pf_rk4_stage_src(self.h, int(stage), self.padded(u, 'u'), self.padded(v, 'v'),
+                                     self.padded(w, 'w'), self.padded(ut, 'ut'),
+                                     self.padded(vt, 'vt'), self.padded(wt, 'wt'), float(dt),
+                                     float(mu), float(rho), *g, self.stream()),
+              'pf_rk4_stage_src')
 
     def sfs_smagorinsky(self, u, v, w, rho, Cs, Delta, mu, visc):
         check(lib().pf_sfs_smagorinsky(self.h, self.padded(u, 'u'), self.padded(v, 'v'),

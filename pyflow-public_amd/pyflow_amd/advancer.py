@@ -38,6 +38,14 @@ class Euler:
                                  self.state_v.var, self.state_w.var, simDt, r.mu(), r.rho, kind,
                                  params, work, delta, dparams)
             return
+        if _fusable(r) and r.source_model() is not None:
+            src = r.source_model()
+            S = (self.state_u, self.state_v, self.state_w)
+            r.box.source_inputs(*(st.var for st in S), *(st.var_i for st in S))
+            src.update(*(st.var_i for st in S))
+            r.box.euler_step_src(*(st.var for st in S), simDt, r.mu(), r.rho,
+                                 (src.GX, src.GY, src.GZ))
+            return
         if _fusable(r):
             vf = r.visc_field()
             if vf is not None:
@@ -53,6 +61,11 @@ class Euler:
 
 
 class RK4:
+    """advancer.py:75-113.  With a 'source' closure each stage's closure is
+    evaluated on that stage's state (stage 1 on state_u, stages 2-4 on
+    state_uTmp, whose var_i arrays receive the inputs), each stage's RHS
+    object calling its own closure as rhs_NavierStokes.evaluate does."""
+
     def __init__(self, state_u, state_uTmp, state_v, state_vTmp, state_w, state_wTmp, rhsClass):
         self.state_u = state_u
         self.state_v = state_v
@@ -76,6 +89,9 @@ class RK4:
             return
         if all(_fusable(getattr(rc, n)) for n in ('rhs1', 'rhs2', 'rhs3', 'rhs4')):
             r = rc.rhs1
+            if r.source_model() is not None:
+                self._step_source(simDt)
+                return
             vf = r.visc_field()
             if vf is not None:
                 r.box.rk4_step_visc(self.state_u.var, self.state_v.var, self.state_w.var,
@@ -88,6 +104,9 @@ class RK4:
             return
         U = (self.state_u, self.state_v, self.state_w)
         Tm = (self.state_uTmp, self.state_vTmp, self.state_wTmp)
+        if any(_fusable(getattr(rc, n)) and getattr(rc, n).source_model() is not None
+               for n in ('rhs1', 'rhs2', 'rhs3', 'rhs4')):
+            raise NotImplementedError("a 'source' closure needs this package's RHS for all stages")
         rc.rhs1.evaluate(*U)
         prev = rc.rhs1
         for nxt in (rc.rhs2, rc.rhs3, rc.rhs4):
@@ -99,3 +118,17 @@ class RK4:
             ks = [getattr(getattr(rc, n), 'rhs_' + 'uvw'[i]) for n in ('rhs1', 'rhs2', 'rhs3', 'rhs4')]
             acc = ks[0] + 2.0 * ks[1] + 2.0 * ks[2] + ks[3]
             u.ZAXPY(simDt / 6.0, acc, u.interior())
+
+    def _step_source(self, simDt):
+        rc = self.rhsClass
+        U = (self.state_u, self.state_v, self.state_w)
+        Tm = (self.state_uTmp, self.state_vTmp, self.state_wTmp)
+        for stage, name in enumerate(('rhs1', 'rhs2', 'rhs3', 'rhs4'), start=1):
+            r = getattr(rc, name)
+            src = r.source_model()
+            holder = U if stage == 1 else Tm
+            I = [st.var_i for st in holder]
+            r.box.rk4_source_inputs(stage, *(st.var for st in U), *(st.var for st in Tm), *I)
+            src.update(*I)
+            r.box.rk4_stage_src(stage, *(st.var for st in U), *(st.var for st in Tm), simDt,
+                                r.mu(), r.rho, (src.GX, src.GY, src.GZ))

@@ -4,6 +4,13 @@ rhs_NavierStokes.evaluate is one fused HIP kernel (pf_rhs_ns, or
 pf_rhs_ns_visc for a spatially varying VISC field); the RK4 and Euler
 advancers of this package do not call it at all but fuse the same RHS into
 their stage kernels (pf_rk4_step / pf_euler_step and their _visc variants).
+
+A 'source' closure (velocity.py:360-383) is evaluated inside every RHS
+evaluation, as the reference does: pf_source_inputs writes the closure's
+inputs into state.var_i exactly as the reference's interpolations leave them,
+the closure's update() fills GX, GY, GZ, and the fused kernel subtracts their
+face interpolants (pf_rhs_ns_src; the advancers run pf_rk4_stage_src /
+pf_euler_step_src stage by stage with the closure evaluated in between).
 """
 
 import torch
@@ -56,13 +63,25 @@ class rhs_NavierStokes:
         if m == 'tensor':
             s = self.sfsmodel
             return (s.FXX, s.FYY, s.FZZ, s.FXY, s.FXZ, s.FYZ)
-        if m == 'source':
-            raise NotImplementedError("'source' closures (sfsmodel_ML) are not part of this build")
+        return None
+
+    def source_model(self):
+        """The closure when it is a 'source' model (velocity.py:360-383), else None."""
+        if getattr(self.sfsmodel, 'modelType', None) == 'source':
+            return self.sfsmodel
         return None
 
     def evaluate(self, state_u, state_v, state_w):
         """velocity.py:221-383"""
         ru, rv, rw = self._arrays()
+        src = self.source_model()
+        if src is not None:
+            self.box.source_inputs(state_u.var, state_v.var, state_w.var,
+                                   state_u.var_i, state_v.var_i, state_w.var_i)
+            src.update(state_u.var_i, state_v.var_i, state_w.var_i)
+            self.box.rhs_ns_src(state_u.var, state_v.var, state_w.var, self.mu(), self.rho,
+                                (src.GX, src.GY, src.GZ), ru, rv, rw)
+            return
         vf = self.visc_field()
         if vf is not None:
             self.box.rhs_ns_visc(state_u.var, state_v.var, state_w.var, vf, self.rho,

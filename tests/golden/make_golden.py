@@ -20,7 +20,10 @@ functions that no closed form or textbook routine pins on its own:
 * stress_constCs.eddyVisc (sfsmodel/sfsmodel_smagorinsky.py:65-88), the RHS
   with the resulting viscosity field, and Domain.forwardStep with
   SFSmodel 'Smagorinsky' (data/domain.py:348-352);
-* the adjoint RHS and an adjoint RK4 step with that viscosity field.
+* the adjoint RHS and an adjoint RK4 step with that viscosity field;
+* the 'source' closure contract (solver/velocity.py:360-383): the RHS, an RK4
+  step and an Euler step with a recording stand-in closure
+  (source_stub.RecordingSource), and the arrays the reference hands it.
 
 The reference's MPI layer is replaced by a single-rank periodic stub with the
 semantics of library/parallel.py:301-414 on a periodic Cartesian communicator
@@ -359,6 +362,37 @@ def main():
     for c, a in (('u', au), ('v', av), ('w', aw)):
         out['adj_rk4_smag_' + c] = a.var.numpy().copy()
     out['adj_rk4_smag_res'] = float(ns.max_resP)
+
+    # 11. 'source' closure contract with the recording stand-in: the RHS, the
+    #     inputs it receives (var_i after the RHS's interpolations), an RK4 step
+    #     (closure evaluated in each stage) and an Euler step
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from source_stub import RecordingSource
+    su, sv, sw = states(u0, v0, w0)
+    stub = RecordingSource()
+    r = velocity.rhs_NavierStokes(d, metric, rho, VISC, stub)
+    r.evaluate(su, sv, sw)
+    out['rhs_src_u'], out['rhs_src_v'], out['rhs_src_w'] = (
+        r.rhs_u.numpy().copy(), r.rhs_v.numpy().copy(), r.rhs_w.numpy().copy())
+    for c, t in zip('uvw', stub.calls[0]):
+        out['src_in_' + c] = t.numpy().copy()
+    cfg = types.SimpleNamespace(advancerName='RK4', equationMode='NS', pSolverMode='bicgstab')
+    su, sv, sw = states(u0, v0, w0)
+    tu, tv, tw = states(u0, v0, w0)
+    stub = RecordingSource()
+    frhs = velocity.ForwardRHS(cfg, d, metric, rho, VISC, stub)
+    advancer.RK4(su, tu, sv, tv, sw, tw, frhs).step(dt)
+    for c, s_ in zip('uvw', (su, sv, sw)):
+        out['rk4_src_' + c] = s_.var.numpy().copy()
+    for c, t in zip('uvw', stub.calls[3]):
+        out['rk4_src_in4_' + c] = t.numpy().copy()
+    cfg = types.SimpleNamespace(advancerName='Euler', equationMode='NS', pSolverMode='bicgstab')
+    su, sv, sw = states(u0, v0, w0)
+    stub = RecordingSource()
+    frhs = velocity.ForwardRHS(cfg, d, metric, rho, VISC, stub)
+    advancer.Euler(su, sv, sw, frhs).step(dt)
+    for c, s_ in zip('uvw', (su, sv, sw)):
+        out['euler_src_' + c] = s_.var.numpy().copy()
 
     np.savez_compressed(OUT, **out)
     print('wrote', OUT, 'keys:', len(out))

@@ -2,6 +2,7 @@
 
 import math
 import os
+import sys
 import types
 
 import numpy as np
@@ -13,6 +14,16 @@ GOLDEN = os.path.join(HERE, 'golden', 'reference_vectors.npz')
 
 def golden():
     return np.load(GOLDEN)
+
+
+def recording_source():
+    """The stand-in 'source' closure the golden vectors were written with
+    (tests/golden/source_stub.py)."""
+    d = os.path.join(HERE, 'golden')
+    if d not in sys.path:
+        sys.path.insert(0, d)
+    from source_stub import RecordingSource
+    return RecordingSource()
 
 
 def rel_l2(a, b):

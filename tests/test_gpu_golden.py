@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import golden, gpu_env, cpu, rel_l2, rel_max, rel_l2_mean_free
+from helpers import golden, gpu_env, cpu, rel_l2, rel_max, rel_l2_mean_free, recording_source
 
 pytestmark = pytest.mark.gpu
 
@@ -265,3 +265,56 @@ def test_smagorinsky_domain_forward_step(G):
     # the second step's viscosity comes from the first step's velocity
     assert rel_l2(_smag_cells(cpu(D.VISC_P)), _smag_cells(G['dom_smag_visc'])) < 1e-12
     assert D.max_resP <= 1e-12
+
+
+def _const_visc(d, G):
+    VISC = torch.ones(d.nxo_, d.nyo_, d.nzo_, dtype=torch.float64, device='cuda') * float(G['mu'])
+    VISC.pf_const_mu = float(G['mu'])
+    return VISC
+
+
+def test_source_closure_rhs_bit_exact(G):
+    """'source' closure contract (velocity.py:360-383) with the recording
+    stand-in: the arrays the closure receives (pf_source_inputs) and the RHS
+    with its face-interpolated GX, GY, GZ subtracted (pf_rhs_ns_src)."""
+    from pyflow_amd import velocity
+    shape, d, g, m = setup(G)
+    su, sv, sw = states(d, G)
+    stub = recording_source()
+    r = velocity.rhs_NavierStokes(d, m, float(G['rho']), _const_visc(d, G), stub)
+    r.evaluate(su, sv, sw)
+    torch.cuda.synchronize()
+    assert len(stub.calls) == 1
+    for c, t in zip('uvw', stub.calls[0]):
+        assert np.array_equal(cpu(t), G['src_in_' + c]), c
+    for c in 'uvw':
+        got = cpu(getattr(r, 'rhs_' + c))
+        assert np.array_equal(got, G['rhs_src_' + c]), (c, rel_max(got, G['rhs_src_' + c]))
+
+
+@pytest.mark.parametrize('adv', ['RK4', 'Euler'])
+def test_source_closure_steps_bit_exact(G, adv):
+    """RK4 (closure evaluated on each stage's state, pf_rk4_source_inputs +
+    pf_rk4_stage_src) and Euler (pf_euler_step_src) with the stand-in."""
+    from pyflow_amd import velocity, advancer
+    import types
+    shape, d, g, m = setup(G)
+    su, sv, sw = states(d, G)
+    stub = recording_source()
+    cfg = types.SimpleNamespace(advancerName=adv, equationMode='NS')
+    frhs = velocity.ForwardRHS(cfg, d, m, float(G['rho']), _const_visc(d, G), stub)
+    dt = float(G['dt'])
+    if adv == 'RK4':
+        tu, tv, tw = states(d, G)
+        advancer.RK4(su, tu, sv, tv, sw, tw, frhs).step(dt)
+        key = 'rk4_src_'
+        assert len(stub.calls) == 4
+        for c, t in zip('uvw', stub.calls[3]):
+            assert np.array_equal(cpu(t), G['rk4_src_in4_' + c]), c
+    else:
+        advancer.Euler(su, sv, sw, frhs).step(dt)
+        key = 'euler_src_'
+    torch.cuda.synchronize()
+    for c, s_ in zip('uvw', (su, sv, sw)):
+        got = cpu(s_.var)
+        assert np.array_equal(got, G[key + c]), (c, rel_max(got, G[key + c]))

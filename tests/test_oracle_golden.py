@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import golden
+from helpers import golden, recording_source
 from oracle import pyflow_oracle as O
 
 
@@ -69,6 +69,56 @@ def test_rk4_and_euler(G):
     O.euler_step(fu, fv, fw, O.NavierStokesRHS(g, float(G['rho']), visc(g, G)), dt)
     for c, f in zip('uvw', (fu, fv, fw)):
         assert np.array_equal(f.var.numpy(), G['euler_' + c])
+
+
+def test_source_closure_contract(G):
+    """velocity.py:360-383 with a recording stand-in closure: the arrays it
+    receives (var_i, including the last planes the advective interpolations
+    leave), the RHS, and RK4 / Euler steps that evaluate it in every stage."""
+    g = grid(G)
+    rho, dt = float(G['rho']), float(G['dt'])
+    stub = recording_source()
+    r = O.NavierStokesRHS(g, rho, visc(g, G), stub)
+    r.evaluate(*fields(g, G))
+    for c, t in zip('uvw', stub.calls[0]):
+        assert np.array_equal(t.numpy(), G['src_in_' + c]), c
+        assert np.array_equal(getattr(r, 'rhs_' + c).numpy(), G['rhs_src_' + c]), c
+    fu, fv, fw = fields(g, G)
+    tu, tv, tw = fields(g, G)
+    stub = recording_source()
+    O.rk4_step(fu, fv, fw, tu, tv, tw, [O.NavierStokesRHS(g, rho, visc(g, G), stub)
+                                        for _ in range(4)], dt)
+    assert len(stub.calls) == 4
+    for c, f, t in zip('uvw', (fu, fv, fw), stub.calls[3]):
+        assert np.array_equal(f.var.numpy(), G['rk4_src_' + c]), c
+        assert np.array_equal(t.numpy(), G['rk4_src_in4_' + c]), c
+    fu, fv, fw = fields(g, G)
+    O.euler_step(fu, fv, fw, O.NavierStokesRHS(g, rho, visc(g, G), recording_source()), dt)
+    for c, f in zip('uvw', (fu, fv, fw)):
+        assert np.array_equal(f.var.numpy(), G['euler_src_' + c]), c
+
+
+def test_source_inputs_stale_planes(G):
+    """The closure's inputs where the centre interpolant does not reach: the
+    last plane along each component's own axis holds the edge interpolant
+    the advective terms wrote there last, and one corner of it is never
+    written (zero from the allocation)."""
+    u = np.pad(G['u0'], 2, mode='wrap')
+    v = np.pad(G['v0'], 2, mode='wrap')
+    w = np.pad(G['w0'], 2, mode='wrap')
+    ui, vi, wi = G['src_in_u'], G['src_in_v'], G['src_in_w']
+    assert np.array_equal(ui[:-1], 0.5 * (u[1:] + u[:-1]))
+    assert np.array_equal(ui[-1, :, 1:], 0.5 * (u[-1, :, 1:] + u[-1, :, :-1]))
+    assert np.array_equal(ui[-1, 1:, 0], 0.5 * (u[-1, 1:, 0] + u[-1, :-1, 0]))
+    assert ui[-1, 0, 0] == 0.0
+    assert np.array_equal(vi[:, :-1], 0.5 * (v[:, 1:] + v[:, :-1]))
+    assert np.array_equal(vi[:, -1, 1:], 0.5 * (v[:, -1, 1:] + v[:, -1, :-1]))
+    assert np.array_equal(vi[1:, -1, 0], 0.5 * (v[1:, -1, 0] + v[:-1, -1, 0]))
+    assert vi[0, -1, 0] == 0.0
+    assert np.array_equal(wi[:, :, :-1], 0.5 * (w[:, :, 1:] + w[:, :, :-1]))
+    assert np.array_equal(wi[:, 1:, -1], 0.5 * (w[:, 1:, -1] + w[:, :-1, -1]))
+    assert np.array_equal(wi[1:, 0, -1], 0.5 * (w[1:, 0, -1] + w[:-1, 0, -1]))
+    assert wi[0, 0, -1] == 0.0
 
 
 def test_bicgstab_history(G):
