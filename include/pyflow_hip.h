@@ -196,6 +196,23 @@ int pf_rk4_stage_src(pf_box* box, int stage, void* u, void* v, void* w, void* ut
                      const void* gz, void* stream);
 
 /*
+ * The reference's 'source' ML closure, residual_stress.update(u_P, v_P, w_P)
+ * of sfsmodel/sfsmodel_ML.py:354-404 with ML_Closure (:468-593),
+ * Finite_Difference_ConCat (:408-465) and NeuralNetworkModel2 (:76-116):
+ * ui, vi, wi (padded, box dtype; the var_i arrays pf_source_inputs writes) ->
+ * gx, gy, gz (padded, box dtype).  Features and network in FP32 as the
+ * reference evaluates them; the network on MFMA.
+ * params: device FP32, pf_sfs_ml_source_params() values: torch.nn.Linear
+ * layout (weight[out][in] then bias[out]) of fc1, fcG, fcG2 (273 -> 200),
+ * fc2, fc3 (200 -> 200), fc4 (200 -> 18), in that order.  norm21: host, the
+ * 21 FP32 normalisation constants (sfsmodel_ML.py:252-258).  A workspace of
+ * 30 floats per padded cell is allocated in the box on first use.
+ */
+int pf_sfs_ml_source(pf_box* box, const float* params, const float* norm21, const void* ui,
+                     const void* vi, const void* wi, void* gx, void* gy, void* gz, void* stream);
+int64_t pf_sfs_ml_source_params(void);
+
+/*
  * Smagorinsky eddy viscosity plus the molecular one, written to the padded
  * viscosity field: visc = mu + rho*(Cs*Delta)^2*|S| with
  * |S| = sqrt(S11^2+S22^2+S33^2 + 2(S12^2+S13^2+S23^2)) of the trace-free

@@ -23,6 +23,7 @@ say "parity unpinned" in their docstring.
 
 import math
 
+import numpy as np
 import torch
 
 NOVER = 2  # overlap depth, parallel.py:201
@@ -459,6 +460,69 @@ def mlp_closure_forward(g, fu, fv, fw, params, const=None):
 # ---------------------------------------------------------------------------
 # Momentum right-hand side: rhs_NavierStokes (solver/velocity.py:189-383)
 # ---------------------------------------------------------------------------
+class MLSource:
+    """The reference's 'source' ML closure, sfsmodel/sfsmodel_ML.py:354-593
+    (update, Finite_Difference_ConCat, ML_Closure) with NeuralNetworkModel2
+    (:76-116), restated with torch CPU ops in FP32 as the reference runs it.
+    ``sd`` is the network's state_dict.  Shifts wrap over the padded extent
+    (the reference's torch.cat of first and last planes; torch.roll moves the
+    same values), and every direction uses dx (:408-465, :468-593)."""
+
+    modelType = 'source'
+
+    # sfsmodel_ML.py:252-258
+    NORM = 10.0 * np.array([1.0233662e+00, 1.0241578e+00, 1.0196122e+00,
+                            5.0992206e+02, 5.2713013e+02, 5.2689368e+02, 5.2550342e+02,
+                            5.1325964e+02, 5.2564642e+02, 5.3132422e+02, 5.2574353e+02,
+                            5.1027972e+02, 9.5621462e+05, 1.2852805e+06, 1.2698791e+06,
+                            1.2744681e+06, 9.6111162e+05, 1.2722050e+06, 1.2891108e+06,
+                            1.2891108e+06, 9.6587112e+05])
+
+    def __init__(self, g, sd):
+        self.g = g
+        self.sd = {k: v.to(torch.float32) for k, v in sd.items()}
+        self.norm = torch.tensor(self.NORM, dtype=torch.float32)
+        self.last_output = None
+
+    def _d1(self, f, dim):
+        return (torch.roll(f, -1, dim) - torch.roll(f, 1, dim)) / (2 * self.g.dx)
+
+    def _d2(self, f, dim):
+        return (torch.roll(f, -1, dim) - 2 * f + torch.roll(f, 1, dim)) / (self.g.dx * self.g.dx)
+
+    def features(self, fu, fv, fw):
+        """The 21 inputs per padded cell (:372-378), float32."""
+        f3 = [a.to(torch.float32) for a in (fu, fv, fw)]
+        first = [self._d1(f, d) for f in f3 for d in range(3)]
+        second = [self._d2(f, d) for f in f3 for d in range(3)]
+        return torch.stack(f3 + first + second, 3)
+
+    def network(self, x):
+        """NeuralNetworkModel2.forward (:88-116)."""
+        sd = self.sd
+        lin = lambda n, a: torch.nn.functional.linear(a, sd[n + '.weight'], sd[n + '.bias'])
+        H1 = torch.tanh(lin('fc1', x))
+        H2 = torch.tanh(lin('fc2', H1))
+        G = torch.tanh(lin('fcG', x))
+        Gate2 = torch.tanh(lin('fcG2', x))
+        H3 = torch.tanh(lin('fc3', G * H2))
+        return lin('fc4', H3 * Gate2)
+
+    def update(self, fu, fv, fw, requires_grad=False):
+        x = self.features(fu, fv, fw)
+        xn = x / self.norm
+        parts = [xn]
+        for d in (1, 2):                                     # :413-447 order
+            for dim in range(3):
+                parts.append(torch.roll(xn, -d, dim))
+                parts.append(torch.roll(xn, d, dim))
+        p = self.network(torch.cat(parts, 3))
+        self.last_output = p
+        c = [self._d1(p[..., q:q + 1], q % 3) for q in range(9)]
+        G = [(c[3 * k] + c[3 * k + 1]) + c[3 * k + 2] for k in range(3)]   # :586-588
+        self.GX, self.GY, self.GZ = (a.to(self.g.dtype) for a in G)
+
+
 class NavierStokesRHS:
     """rhs_NavierStokes, solver/velocity.py:189-383 (viscous + tensor-SFS
     fluxes, then the nine advective terms)."""

@@ -72,6 +72,10 @@ struct pf_box {
   double mp_inner_tol;       // relative tolerance of each inner BiCGStab solve
   int mp_inner_maxit;        // iteration cap of each inner solve
   int mp_outer_maxit;        // refinement steps
+  // 'source' ML closure workspace (allocated on first use): 21 normalised
+  // features and 9 network outputs per padded cell, FP32
+  float* src_ws;
+  size_t src_ws_bytes;
   int chunk;                 // iterations per launch batch between host checks
   int tile_rows;             // block rows of the tiled Krylov kernels: 256*tile_rows threads
   int64_t bytes;             // device bytes owned by the handle

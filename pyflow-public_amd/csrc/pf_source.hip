@@ -1,0 +1,365 @@
+// The reference's 'source' ML closure (sfsmodel/sfsmodel_ML.py:76-116 and
+// :354-593) on the GPU: residual_stress.update(u_P, v_P, w_P) -> GX, GY, GZ.
+//
+// Three kernels:
+//  1. features (Finite_Difference_ConCat, :408-465): per padded cell the 21
+//     FP32 values u, v, w, their first differences (f[+1] - f[-1])/(2 dx) and
+//     second differences ((f[+1] - 2 f) + f[-1])/(dx dx) along x, y, z, all
+//     divided by the normalisation table (:252-258, ML_Closure :449-463).  As
+//     in the reference, the wrap-around is that of the padded array itself
+//     (torch.cat of its first and last planes) and every direction uses dx;
+//  2. the network (NeuralNetworkModel2.forward, :88-116) on MFMA in FP32
+//     (v_mfma_f32_16x16x4_f32): input = the cell's 21 values and those of its
+//     12 neighbours at +-1 and +-2 along each axis (padded wrap again, :413-447),
+//     L1 = tanh(fc1 x), G = tanh(fcG x), Gate2 = tanh(fcG2 x),
+//     H3 = tanh(fc3 (G * tanh(fc2 L1))), out = fc4 (H3 * Gate2); only output
+//     channels 0-8 are used (:466-589);
+//  3. divergence: GX = (d/dx p0 + d/dy p1) + d/dz p2 with central differences
+//     (p[+1] - p[-1])/(2 dx) over the padded wrap, likewise GY from p3-p5 and GZ
+//     from p6-p8 (:466-589), cast to the box dtype.
+//
+// The network keeps cells on the MFMA N dimension (16 per wave) and neurons
+// on M, so a layer's accumulator tile (lane l holds neurons 4*(l/16)+i of cell
+// l%16) is, element by element, the next layer's B operand: the K index of
+// k-step (t, i) in lane group g is neuron 16 t + 4 g + i, and the weights (A
+// operand) are read with the same permutation.  Activations never leave the
+// registers.  The first layer's K runs over (shift, feature) with the 21
+// features of a shift padded to 24, so each k-step reads one neighbour.
+// Summation order differs from the reference's CPU GEMM, so results agree to
+// FP32 rounding, not bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "pf_kernels.h"
+
+namespace pf {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kSrcFeat = 21;                    // features per cell (:372-378)
+constexpr int kSrcShifts = 13;                  // own cell + 12 shifted copies (:413-447)
+constexpr int kSrcIn = kSrcFeat * kSrcShifts;   // 273 = num_inputs (:74)
+constexpr int kSrcHid = 200;                    // H (:73)
+constexpr int kSrcOut = 18;                     // fc4 outputs (:87)
+constexpr int kSrcUsed = 9;                     // output channels the closure uses
+constexpr int kSrcTiles = 13;                   // 16-neuron tiles covering 200
+// packed FP32 parameters, torch.nn.Linear layout (W[out][in], then b[out]),
+// in the order fc1, fcG, fcG2, fc2, fc3, fc4
+constexpr int64_t kOffFc1W = 0;
+constexpr int64_t kOffFc1B = kOffFc1W + (int64_t)kSrcHid * kSrcIn;
+constexpr int64_t kOffGW = kOffFc1B + kSrcHid;
+constexpr int64_t kOffGB = kOffGW + (int64_t)kSrcHid * kSrcIn;
+constexpr int64_t kOffG2W = kOffGB + kSrcHid;
+constexpr int64_t kOffG2B = kOffG2W + (int64_t)kSrcHid * kSrcIn;
+constexpr int64_t kOffFc2W = kOffG2B + kSrcHid;
+constexpr int64_t kOffFc2B = kOffFc2W + (int64_t)kSrcHid * kSrcHid;
+constexpr int64_t kOffFc3W = kOffFc2B + kSrcHid;
+constexpr int64_t kOffFc3B = kOffFc3W + (int64_t)kSrcHid * kSrcHid;
+constexpr int64_t kOffFc4W = kOffFc3B + kSrcHid;
+constexpr int64_t kOffFc4B = kOffFc4W + (int64_t)kSrcOut * kSrcHid;
+constexpr int64_t kSrcParams = kOffFc4B + kSrcOut;
+
+struct SrcGeom {
+  int nxo, nyo, nzo;
+  int64_t m;      // padded cells
+  float d2;       // (float)(2*dx)
+  float dd;       // (float)(dx*dx)
+};
+
+struct Norm21 {
+  float v[kSrcFeat];
+};
+
+__device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// Flat index of the cell at (i,j,k) moved by shift sh of the reference's
+// concatenation order: own, x+1, x-1, y+1, y-1, z+1, z-1, x+2, x-2, y+2, y-2,
+// z+2, z-2 (:449-462), wrapping over the padded extents.
+__device__ __forceinline__ int64_t shifted(const SrcGeom& s, int i, int j, int k, int sh) {
+  if (sh > 0) {
+    const int d = sh <= 6 ? 1 : 2;
+    const int q = (sh - 1) % 6;          // 0 x+, 1 x-, 2 y+, 3 y-, 4 z+, 5 z-
+    const int o = (q & 1) ? -d : d;
+    if (q < 2) i = wrap(i + o, s.nxo);
+    else if (q < 4) j = wrap(j + o, s.nyo);
+    else k = wrap(k + o, s.nzo);
+  }
+  return ((int64_t)i * s.nyo + j) * s.nzo + k;
+}
+
+__device__ __forceinline__ void decode_padded(const SrcGeom& s, int64_t p, int& i, int& j, int& k) {
+  k = (int)(p % s.nzo);
+  const int64_t r = p / s.nzo;
+  j = (int)(r % s.nyo);
+  i = (int)(r / s.nyo);
+}
+
+// 1. features, feature-major: Fn[f * m + cell]
+template <typename T>
+__global__ void __launch_bounds__(kBlock) src_features_kernel(SrcGeom s, const T* __restrict__ u,
+                                                              const T* __restrict__ v,
+                                                              const T* __restrict__ w,
+                                                              float* __restrict__ Fn, Norm21 nrm) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < s.m;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode_padded(s, p, i, j, k);
+    const int64_t sy = s.nzo, sx = (int64_t)s.nyo * s.nzo;
+    const int64_t xp = (int64_t)wrap(i + 1, s.nxo) * sx + j * sy + k;
+    const int64_t xm = (int64_t)wrap(i - 1, s.nxo) * sx + j * sy + k;
+    const int64_t yp = (int64_t)i * sx + wrap(j + 1, s.nyo) * sy + k;
+    const int64_t ym = (int64_t)i * sx + wrap(j - 1, s.nyo) * sy + k;
+    const int64_t zp = (int64_t)i * sx + j * sy + wrap(k + 1, s.nzo);
+    const int64_t zm = (int64_t)i * sx + j * sy + wrap(k - 1, s.nzo);
+    const T* f3[3] = {u, v, w};
+    float F[kSrcFeat];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const T* f = f3[q];
+      const float c = (float)f[p];
+      const float nb[6] = {(float)f[xp], (float)f[xm], (float)f[yp],
+                           (float)f[ym], (float)f[zp], (float)f[zm]};
+      F[q] = c;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float fp = nb[2 * a], fm = nb[2 * a + 1];
+        F[3 + 3 * q + a] = (fp - fm) / s.d2;                      // :439-463
+        F[12 + 3 * q + a] = ((fp - 2.0f * c) + fm) / s.dd;        // :412-433
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < kSrcFeat; ++f) Fn[f * s.m + p] = F[f] / nrm.v[f];
+  }
+}
+
+// A operand of a weight matrix W[rows][ld] for neuron row r and input column
+// kc (zero outside the matrix).
+__device__ __forceinline__ float wload(const float* __restrict__ W, int ld, int rows, int cols,
+                                       int r, int kc) {
+  return (r < rows && kc < cols) ? W[(int64_t)r * ld + kc] : 0.f;
+}
+
+__device__ __forceinline__ f32x4 mfma(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// tanh(acc + b) in accumulator layout: element i of tile t is neuron
+// 16 t + 4 g + i.
+__device__ __forceinline__ void bias_tanh(f32x4 (&a)[kSrcTiles], const float* __restrict__ b,
+                                          int g) {
+#pragma unroll
+  for (int t = 0; t < kSrcTiles; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int n = 16 * t + 4 * g + i;
+      a[t][i] = n < kSrcHid ? tanhf(a[t][i] + b[n]) : 0.f;
+    }
+}
+
+// One hidden layer on register-resident activations: out = W x over the
+// permuted K order (see the file comment).
+__device__ __forceinline__ void hidden_layer(const f32x4 (&x)[kSrcTiles],
+                                             const float* __restrict__ W, int col, int g,
+                                             f32x4 (&out)[kSrcTiles]) {
+#pragma unroll
+  for (int o = 0; o < kSrcTiles; ++o) out[o] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < kSrcTiles; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int kc = 16 * t + 4 * g + i;
+      const float b = x[t][i];
+#pragma unroll
+      for (int o = 0; o < kSrcTiles; ++o)
+        out[o] = mfma(wload(W, kSrcHid, kSrcHid, kSrcHid, 16 * o + col, kc), b, out[o]);
+    }
+}
+
+// First-layer product of one or two 273-input matrices over the gathered
+// input (k-step (sh, sub): features 4 sub + g of shift sh).
+template <int NMAT>
+__device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __restrict__ Fn,
+                                            bool valid, int ci, int cj, int ck,
+                                            const float* __restrict__ W0,
+                                            const float* __restrict__ W1, int col, int g,
+                                            f32x4 (&a0)[kSrcTiles], f32x4 (&a1)[kSrcTiles]) {
+#pragma unroll
+  for (int o = 0; o < kSrcTiles; ++o) {
+    a0[o] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (NMAT > 1) a1[o] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  for (int sh = 0; sh < kSrcShifts; ++sh) {
+    const int64_t nb = valid ? shifted(s, ci, cj, ck, sh) : 0;
+#pragma unroll
+    for (int sub = 0; sub < 6; ++sub) {
+      const int f = 4 * sub + g;
+      const float b = (valid && f < kSrcFeat) ? Fn[(int64_t)f * s.m + nb] : 0.f;
+      const int kc = f < kSrcFeat ? sh * kSrcFeat + f : kSrcIn;   // kSrcIn: no column
+#pragma unroll
+      for (int o = 0; o < kSrcTiles; ++o) {
+        a0[o] = mfma(wload(W0, kSrcIn, kSrcHid, kSrcIn, 16 * o + col, kc), b, a0[o]);
+        if (NMAT > 1)
+          a1[o] = mfma(wload(W1, kSrcIn, kSrcHid, kSrcIn, 16 * o + col, kc), b, a1[o]);
+      }
+    }
+  }
+}
+
+// 2. the network: 64 cells per 256-thread block, 16 per wave; P[c * m + cell]
+// for channels c = 0..8.
+__global__ void __launch_bounds__(256) src_network_kernel(SrcGeom s, const float* __restrict__ Fn,
+                                                          const float* __restrict__ prm,
+                                                          float* __restrict__ P) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int col = lane & 15, g = lane >> 4;
+  const int64_t cell = ((int64_t)blockIdx.x * 4 + wave) * 16 + col;
+  const bool valid = cell < s.m;
+  int ci = 0, cj = 0, ck = 0;
+  if (valid) decode_padded(s, cell, ci, cj, ck);
+
+  f32x4 h1[kSrcTiles], gt[kSrcTiles];
+  // L1 = tanh(fc1 x), G = tanh(fcG x)  (:91-100)
+  input_layer<2>(s, Fn, valid, ci, cj, ck, prm + kOffFc1W, prm + kOffGW, col, g, h1, gt);
+  bias_tanh(h1, prm + kOffFc1B, g);
+  bias_tanh(gt, prm + kOffGB, g);
+  // H2 = tanh(fc2 L1); H2_G = G * H2  (:94-104)
+  f32x4 hg[kSrcTiles];
+  hidden_layer(h1, prm + kOffFc2W, col, g, hg);
+  bias_tanh(hg, prm + kOffFc2B, g);
+#pragma unroll
+  for (int t = 0; t < kSrcTiles; ++t) hg[t] = gt[t] * hg[t];
+  // Gate2 = tanh(fcG2 x)  (:102)
+  f32x4 g2[kSrcTiles];
+  input_layer<1>(s, Fn, valid, ci, cj, ck, prm + kOffG2W, nullptr, col, g, g2, g2);
+  bias_tanh(g2, prm + kOffG2B, g);
+  // H3 = tanh(fc3 H2_G); H4 = H3 * Gate2  (:108-112)
+  f32x4 h3[kSrcTiles];
+  hidden_layer(hg, prm + kOffFc3W, col, g, h3);
+  bias_tanh(h3, prm + kOffFc3B, g);
+#pragma unroll
+  for (int t = 0; t < kSrcTiles; ++t) h3[t] = h3[t] * g2[t];
+  // out = fc4 H4  (:114); one 16-neuron tile holds channels 0..15
+  f32x4 o4 = f32x4{0.f, 0.f, 0.f, 0.f};
+  const float* W4 = prm + kOffFc4W;
+#pragma unroll
+  for (int t = 0; t < kSrcTiles; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      o4 = mfma(wload(W4, kSrcHid, kSrcOut, kSrcHid, col, 16 * t + 4 * g + i), h3[t][i], o4);
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c = 4 * g + i;
+      if (c < kSrcUsed) P[(int64_t)c * s.m + cell] = o4[i] + prm[kOffFc4B + c];
+    }
+  }
+}
+
+// 3. GX, GY, GZ from the network's channels 0-8
+template <typename T>
+__global__ void __launch_bounds__(kBlock) src_divergence_kernel(SrcGeom s,
+                                                                const float* __restrict__ P,
+                                                                T* __restrict__ gx,
+                                                                T* __restrict__ gy,
+                                                                T* __restrict__ gz) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < s.m;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode_padded(s, p, i, j, k);
+    const int64_t sy = s.nzo, sx = (int64_t)s.nyo * s.nzo;
+    const int64_t xp = (int64_t)wrap(i + 1, s.nxo) * sx + j * sy + k;
+    const int64_t xm = (int64_t)wrap(i - 1, s.nxo) * sx + j * sy + k;
+    const int64_t yp = (int64_t)i * sx + wrap(j + 1, s.nyo) * sy + k;
+    const int64_t ym = (int64_t)i * sx + wrap(j - 1, s.nyo) * sy + k;
+    const int64_t zp = (int64_t)i * sx + j * sy + wrap(k + 1, s.nzo);
+    const int64_t zm = (int64_t)i * sx + j * sy + wrap(k - 1, s.nzo);
+    T* out[3] = {gx, gy, gz};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float* p0 = P + (int64_t)(3 * q) * s.m;
+      const float* p1 = p0 + s.m;
+      const float* p2 = p1 + s.m;
+      const float dxp = (p0[xp] - p0[xm]) / s.d2;
+      const float dyp = (p1[yp] - p1[ym]) / s.d2;
+      const float dzp = (p2[zp] - p2[zm]) / s.d2;
+      out[q][p] = (T)((dxp + dyp) + dzp);                  // :586-588
+    }
+  }
+}
+
+template <typename T>
+int ml_source_impl(pf_box* h, const float* prm, const Norm21& nrm, const void* const U[3],
+                   void* const G[3], hipStream_t st) {
+  SrcGeom s;
+  s.nxo = h->nx + 2 * kNover;
+  s.nyo = h->ny + 2 * kNover;
+  s.nzo = h->nz + 2 * kNover;
+  s.m = (int64_t)s.nxo * s.nyo * s.nzo;
+  s.d2 = (float)(2.0 * h->dx);
+  s.dd = (float)(h->dx * h->dx);
+  const size_t need = (size_t)(kSrcFeat + kSrcUsed) * (size_t)s.m * sizeof(float);
+  if (h->src_ws_bytes < need) {
+    if (h->src_ws) {
+      PF_HIP(hipFree(h->src_ws));
+      h->bytes -= (int64_t)h->src_ws_bytes;
+      h->src_ws = nullptr;
+      h->src_ws_bytes = 0;
+    }
+    hipError_t e = hipMalloc((void**)&h->src_ws, need);
+    if (e != hipSuccess) {
+      h->src_ws = nullptr;
+      return check_hip(e, "hipMalloc");
+    }
+    h->src_ws_bytes = need;
+    h->bytes += (int64_t)need;
+  }
+  float* Fn = h->src_ws;
+  float* P = Fn + (size_t)kSrcFeat * s.m;
+  int64_t nb = (s.m + kBlock - 1) / kBlock;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(src_features_kernel<T>, dim3((unsigned)nb), dim3(kBlock), 0, st, s,
+                     (const T*)U[0], (const T*)U[1], (const T*)U[2], Fn, nrm);
+  PF_LAUNCH_CHECK("src_features_kernel");
+  const int64_t nnet = (s.m + 63) / 64;
+  hipLaunchKernelGGL(src_network_kernel, dim3((unsigned)nnet), dim3(256), 0, st, s,
+                     (const float*)Fn, prm, P);
+  PF_LAUNCH_CHECK("src_network_kernel");
+  hipLaunchKernelGGL(src_divergence_kernel<T>, dim3((unsigned)nb), dim3(kBlock), 0, st, s,
+                     (const float*)P, (T*)G[0], (T*)G[1], (T*)G[2]);
+  PF_LAUNCH_CHECK("src_divergence_kernel");
+  return 0;
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+extern "C" {
+
+int pf_sfs_ml_source(pf_box* h, const float* params, const float* norm21, const void* ui,
+                     const void* vi, const void* wi, void* gx, void* gy, void* gz,
+                     void* stream) {
+  if (int rc = check_box(h)) return rc;
+  const void* U[3] = {ui, vi, wi};
+  void* G[3] = {gx, gy, gz};
+  if (!params) return fail(PF_EINVAL, "pf_sfs_ml_source: NULL params");
+  if (!norm21) return fail(PF_EINVAL, "pf_sfs_ml_source: NULL normalisation table");
+  for (int q = 0; q < 3; ++q) {
+    if (!U[q] || !G[q]) return fail(PF_EINVAL, "pf_sfs_ml_source: NULL field");
+    for (int r = 0; r < 3; ++r)
+      if (G[q] == U[r]) return fail(PF_EINVAL, "pf_sfs_ml_source: outputs alias the inputs");
+  }
+  Norm21 nrm;
+  for (int f = 0; f < kSrcFeat; ++f) {
+    nrm.v[f] = norm21[f];
+    if (!(nrm.v[f] != 0.f)) return fail(PF_EINVAL, "pf_sfs_ml_source: zero normalisation entry");
+  }
+  if (!(h->dx > 0.0)) return fail(PF_EINVAL, "pf_sfs_ml_source: dx must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  return h->dtype == PF_F64 ? ml_source_impl<double>(h, params, nrm, U, G, s)
+                            : ml_source_impl<float>(h, params, nrm, U, G, s);
+}
+
+int64_t pf_sfs_ml_source_params(void) { return kSrcParams; }
+
+}  // extern "C"

@@ -92,6 +92,8 @@ SIGNATURES = [
     ('pf_euler_step_src', _I, [_P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P]),
     ('pf_rk4_source_inputs', _I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     ('pf_rk4_stage_src', _I, [_P, _I, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P]),
+    ('pf_sfs_ml_source', _I, [_P, _P, ctypes.POINTER(ctypes.c_float), _P, _P, _P, _P, _P, _P, _P]),
+    ('pf_sfs_ml_source_params', ctypes.c_int64, []),
 ]
 
 
@@ -305,6 +307,23 @@ class Box:
                                      self.padded(vt, 'vt'), self.padded(wt, 'wt'), float(dt),
                                      float(mu), float(rho), *g, self.stream()),
               'pf_rk4_stage_src')
+
+    def sfs_ml_source(self, params, norm21, ui, vi, wi, G):
+        """The reference's 'source' ML closure: var_i arrays -> GX, GY, GZ."""
+        n = int(lib().pf_sfs_ml_source_params())
+        if not isinstance(params, torch.Tensor) or params.dtype != torch.float32 or \
+                params.device.type != 'cuda' or params.numel() != n or not params.is_contiguous():
+            raise ValueError('params must be a contiguous float32 GPU tensor of %d values' % n)
+        nrm = (ctypes.c_float * 21)(*[float(x) for x in norm21])
+        g = []
+        for name, a in zip(('GX', 'GY', 'GZ'), G):
+            if a.dim() == 4 and a.shape[3] == 1:
+                a = a[..., 0]
+            g.append(self.padded(a, name))
+        check(lib().pf_sfs_ml_source(self.h, ctypes.c_void_p(params.data_ptr()), nrm,
+                                     self.padded(ui, 'u'), self.padded(vi, 'v'),
+                                     self.padded(wi, 'w'), *g, self.stream()),
+              'pf_sfs_ml_source')
 
     def sfs_smagorinsky(self, u, v, w, rho, Cs, Delta, mu, visc):
         check(lib().pf_sfs_smagorinsky(self.h, self.padded(u, 'u'), self.padded(v, 'v'),

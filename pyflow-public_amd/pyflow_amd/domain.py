@@ -113,7 +113,11 @@ class Domain:
             from . import sfsmodel_mlp
             self.sfsmodel = sfsmodel_mlp.residual_stress(inputConfig, decomp, geometry, metric)
         elif SFSmodelName == 'ML':
-            raise NotImplementedError("the reference's 'source' ML closure is not part of this build")
+            from . import sfsmodel_ML
+            self.sfsmodel = sfsmodel_ML.residual_stress(inputConfig, decomp, geometry)
+            if adjointTraining:
+                raise NotImplementedError("adjoint training through the 'source' ML closure "
+                                          "(adjoint.py:241-286) is not part of this build")
         else:
             self.use_SFSmodel = False
             self.sfsmodel = sfsmodel_smagorinsky.stress_constCs(inputConfig, geometry)

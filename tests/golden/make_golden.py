@@ -23,7 +23,10 @@ functions that no closed form or textbook routine pins on its own:
 * the adjoint RHS and an adjoint RK4 step with that viscosity field;
 * the 'source' closure contract (solver/velocity.py:360-383): the RHS, an RK4
   step and an Euler step with a recording stand-in closure
-  (source_stub.RecordingSource), and the arrays the reference hands it.
+  (source_stub.RecordingSource), and the arrays the reference hands it;
+* the reference's 'source' ML closure itself (sfsmodel/sfsmodel_ML.py:354-593,
+  NeuralNetworkModel2 with the seeded weights of source_stub.ml_source_params)
+  on those arrays: GX, GY, GZ and the network's output channels 0-8.
 
 The reference's MPI layer is replaced by a single-rank periodic stub with the
 semantics of library/parallel.py:301-414 on a periodic Cartesian communicator
@@ -393,6 +396,32 @@ def main():
     advancer.Euler(su, sv, sw, frhs).step(dt)
     for c, s_ in zip('uvw', (su, sv, sw)):
         out['euler_src_' + c] = s_.var.numpy().copy()
+
+    # 12. the reference's 'source' ML closure on the inputs of section 11
+    import sfsmodel_ML
+    from source_stub import ml_source_params, ml_state_dict
+    d.comm = None
+    cfg = types.SimpleNamespace(simDt=dt)
+    with contextlib.redirect_stdout(io.StringIO()):
+        ml = sfsmodel_ML.residual_stress(cfg, d, geo)
+    ml.model.load_state_dict(ml_state_dict(ml_source_params()))
+
+    class Recorder(torch.nn.Module):
+        def __init__(self, m):
+            super().__init__()
+            self.m = m
+            self.out = None
+
+        def forward(self, x):
+            y = self.m(x)
+            self.out = y.detach().clone()
+            return y
+
+    ml.model = Recorder(ml.model)
+    ml.update(*(torch.tensor(out['src_in_' + c]) for c in 'uvw'))
+    out['mlsrc_GX'], out['mlsrc_GY'], out['mlsrc_GZ'] = (
+        ml.GX.numpy().copy(), ml.GY.numpy().copy(), ml.GZ.numpy().copy())
+    out['mlsrc_p'] = ml.model.out[..., :9].numpy().copy()
 
     np.savez_compressed(OUT, **out)
     print('wrote', OUT, 'keys:', len(out))

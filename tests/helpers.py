@@ -16,6 +16,17 @@ def golden():
     return np.load(GOLDEN)
 
 
+def ml_source_params(seed=7):
+    """Packed weights of the reference's ML closure network used by the golden
+    vectors (tests/golden/source_stub.py), and the same as a state_dict."""
+    d = os.path.join(HERE, 'golden')
+    if d not in sys.path:
+        sys.path.insert(0, d)
+    from source_stub import ml_source_params as gen, ml_state_dict
+    flat = gen(seed)
+    return flat, ml_state_dict(flat)
+
+
 def recording_source():
     """The stand-in 'source' closure the golden vectors were written with
     (tests/golden/source_stub.py)."""

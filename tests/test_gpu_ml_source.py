@@ -1,0 +1,124 @@
+"""The reference's 'source' ML closure (sfsmodel/sfsmodel_ML.py:354-593,
+NeuralNetworkModel2 of :76-116) on the GPU (pf_sfs_ml_source).
+
+Features and the divergence are elementwise FP32 arithmetic as the reference
+does it; the network runs on MFMA in FP32 with another summation order than
+the reference's CPU GEMM, and GX, GY, GZ are differences of neighbouring
+network outputs (|G| ~ 1e-3 against outputs ~5e-2 here), so they carry FP32
+rounding amplified by that cancellation.  Bars: relative L2 5e-5 on GX, GY,
+GZ against the reference's own values (golden vectors) and against the
+oracle on a second, ragged box (measured 1.3e-6 to 5e-6); velocities of two
+Domain steps with the closure 1e-10 against the oracle's steps (measured
+1.2e-11; the closure term is dt*G, small against u).
+"""
+
+import math
+import types
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import golden, gpu_env, cpu, rel_l2, ml_source_params
+from oracle import pyflow_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_G = 5e-5
+
+
+@pytest.fixture(scope='module')
+def G():
+    return golden()
+
+
+def closure(d, g, flat, dt=1e-3):
+    from pyflow_amd import sfsmodel_ML
+    return sfsmodel_ML.residual_stress(types.SimpleNamespace(simDt=dt, mlSourceParams=flat), d, g)
+
+
+def test_closure_against_reference(G):
+    shape = tuple(int(x) for x in G['shape'])
+    L = tuple(float(x) for x in G['L'])
+    d, g, m = gpu_env(shape, L, torch.float64)
+    flat, _ = ml_source_params()
+    ml = closure(d, g, flat)
+    ml.update(*(torch.tensor(G['src_in_' + c], device='cuda') for c in 'uvw'))
+    torch.cuda.synchronize()
+    for n in ('GX', 'GY', 'GZ'):
+        got, ref = cpu(getattr(ml, n)), G['mlsrc_' + n]
+        err = rel_l2(got, ref)
+        print('%s rel L2 vs reference %.2e (max|G| %.2e)' % (n, err, np.abs(ref).max()))
+        assert got.shape == ref.shape
+        assert err < TOL_G, (n, err)
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_closure_against_oracle_ragged(dtype):
+    shape = (13, 9, 11)
+    L = (2 * math.pi, 1.4 * math.pi, 1.9 * math.pi)
+    d, g, m = gpu_env(shape, L, dtype)
+    og = O.Grid(*shape, Lx=L[0], Ly=L[1], Lz=L[2], dtype=dtype)
+    flat, sd = ml_source_params(11)
+    rng = np.random.Generator(np.random.PCG64(5))
+    ins = [torch.tensor(rng.standard_normal(tuple(n + 4 for n in shape)), dtype=dtype)
+           for _ in range(3)]
+    ml = closure(d, g, flat)
+    ml.update(*(a.to('cuda') for a in ins))
+    torch.set_num_threads(1)
+    ref = O.MLSource(og, sd)
+    ref.update(*ins)
+    torch.cuda.synchronize()
+    for n in ('GX', 'GY', 'GZ'):
+        got, want = cpu(getattr(ml, n)), getattr(ref, n).to(torch.float64).numpy()
+        err = rel_l2(got, want)
+        print('%s %s rel L2 vs oracle %.2e (max|G| %.2e)' % (dtype, n, err, np.abs(want).max()))
+        assert err < TOL_G, (n, err)
+
+
+def test_domain_steps_with_ml_closure(G):
+    """Two Domain.forwardStep calls (RK4 + BiCGStab 1e-12) with
+    SFSmodel='ML': the closure is evaluated on each RK stage's state."""
+    from pyflow_amd import domain
+    shape = tuple(int(x) for x in G['shape'])
+    L = tuple(float(x) for x in G['L'])
+    d, g, m = gpu_env(shape, L, torch.float64)
+    flat, sd = ml_source_params()
+    dt, mu, rho = float(G['dt']), float(G['mu']), float(G['rho'])
+    cfg = types.SimpleNamespace(simDt=dt, equationMode='NS', advancerName='RK4', mu=mu, rho=rho,
+                                dtypeTorch=torch.float64, SFSmodel='ML', mlSourceParams=flat,
+                                pSolverMode='bicgstab', min_pressure_residual=1e-12,
+                                max_pressure_iterations=500)
+    zeros = np.zeros(shape)
+    data_IC = np.stack([G['u0'], G['v0'], G['w0'], zeros], axis=-1)
+    D = domain.Domain(cfg, None, d, data_IC, g, m, ['U', 'V', 'W', 'P'], 0.0)
+    torch.set_num_threads(1)
+    og = O.Grid(*shape, Lx=L[0], Ly=L[1], Lz=L[2])
+    B = O.Box(og, G['u0'], G['v0'], G['w0'], mu=mu, rho=rho, dt=dt, tol=1e-12, maxit=500,
+              sfs=O.MLSource(og, sd))
+    for _ in range(2):
+        D.forwardStep(dt)
+        B.forward_step()
+    torch.cuda.synchronize()
+    for c, f in zip('uvw', (B.u, B.v, B.w)):
+        err = rel_l2(cpu(getattr(D, 'state_%s_P' % c).var), f.var.numpy())
+        print('%s rel L2 vs oracle after 2 steps %.2e' % (c, err))
+        assert err < 1e-10, (c, err)
+    assert float(torch.abs(D.sfsmodel.GX).max()) > 0.0
+
+
+def test_checkpoint_in_reference_format(tmp_path):
+    """A dict with 'model_state_dict' (the reference's save format,
+    sfsmodel_ML.py:270-280) loads with weights_only=True and packs to the
+    same vector."""
+    from pyflow_amd import sfsmodel_ML
+    flat, sd = ml_source_params(3)
+    path = str(tmp_path / 'model.pt')
+    torch.save({'epoch': 4, 'LR': 0.005, 'model_state_dict': sd}, path)
+    d, g, m = gpu_env((8, 8, 8), (2 * math.pi,) * 3, torch.float64)
+    ml = sfsmodel_ML.residual_stress(types.SimpleNamespace(simDt=1e-3, loadModel=True,
+                                                           modelDictName=path), d, g)
+    assert np.array_equal(ml.params.cpu().numpy(), flat)
+    assert ml.epoch == 4 and ml.LR == 0.005
+    with pytest.raises(NotImplementedError):
+        ml.update(ml.GX[..., 0], ml.GY[..., 0], ml.GZ[..., 0], requires_grad=True)

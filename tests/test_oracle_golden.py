@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import golden, recording_source
+from helpers import golden, recording_source, ml_source_params
 from oracle import pyflow_oracle as O
 
 
@@ -119,6 +119,21 @@ def test_source_inputs_stale_planes(G):
     assert np.array_equal(wi[:, 1:, -1], 0.5 * (w[:, 1:, -1] + w[:, :-1, -1]))
     assert np.array_equal(wi[1:, 0, -1], 0.5 * (w[1:, 0, -1] + w[:-1, 0, -1]))
     assert wi[0, 0, -1] == 0.0
+
+
+def test_ml_source_closure(G):
+    """The reference's 'source' ML closure (sfsmodel_ML.py:354-593) on the
+    inputs its RHS hands it: network output channels 0-8 and GX, GY, GZ.  The
+    oracle issues the same FP32 torch ops (one thread, as the vectors were
+    written), so they agree bit for bit."""
+    torch.set_num_threads(1)
+    g = grid(G)
+    _, sd = ml_source_params()
+    ml = O.MLSource(g, sd)
+    ml.update(*(torch.tensor(G['src_in_' + c]) for c in 'uvw'))
+    assert np.array_equal(ml.last_output[..., :9].numpy(), G['mlsrc_p'])
+    for n in ('GX', 'GY', 'GZ'):
+        assert np.array_equal(getattr(ml, n).numpy(), G['mlsrc_' + n]), n
 
 
 def test_bicgstab_history(G):
