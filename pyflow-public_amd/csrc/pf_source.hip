@@ -25,6 +25,9 @@
 // operand) are read with the same permutation.  Activations never leave the
 // registers.  The first layer's K runs over (shift, feature) with the 21
 // features of a shift padded to 24, so each k-step reads one neighbour.
+// Weights reach the MFMAs through LDS: each block stages one chunk of input
+// columns at a time for its four waves, the next chunk's loads in flight
+// while the current one is consumed.
 // Summation order differs from the reference's CPU GEMM, so results agree to
 // FP32 rounding, not bit for bit.
 #include <hip/hip_runtime.h>
@@ -155,52 +158,134 @@ __device__ __forceinline__ void bias_tanh(f32x4 (&a)[kSrcTiles], const float* __
     }
 }
 
+// Weights go through LDS: a block stages a chunk of input columns of a
+// weight matrix once for its four waves, transposed (chunk[column][neuron]).
+// A half-wave reads 16 neurons of two columns; the pitch puts those 32 reads
+// on 32 banks: the hidden layers' column index steps by 4 between lane groups
+// (pitch 212: 4 * 212 = 16 mod 32), the input layers' by 1 (pitch 208).
+constexpr int kPitchHid = 212;
+constexpr int kPitchIn = 208;
+constexpr int kStageCols = 24;
+constexpr int kStageFloats = 2 * kStageCols * kPitchIn;   // >= 16 * kPitchHid
+
+// One thread's share of a staged chunk: chunk[j * PITCH + r] = W[r][c0 + j]
+// for r < nrows and j < nvalid, else 0 (208 neuron rows, NC columns, NMAT
+// matrices one after another).  load() fills registers, store() writes them
+// to LDS, so the next chunk's loads are in flight while the current one is
+// consumed.
+template <int PITCH, int NC, int NMAT>
+struct Stager {
+  static constexpr int kPer = NMAT * NC * 16 * kSrcTiles;
+  static constexpr int kN = (kPer + 255) / 256;
+  float v[kN];
+  __device__ __forceinline__ void load(const float* __restrict__ W0, const float* __restrict__ W1,
+                                       int ld, int nrows, int c0, int nvalid) {
+#pragma unroll
+    for (int e = 0; e < kN; ++e) {
+      const int q = threadIdx.x + 256 * e;
+      const int mat = q / (NC * 16 * kSrcTiles);
+      const int qq = q - mat * (NC * 16 * kSrcTiles);
+      const int r = qq / NC, j = qq - r * NC;
+      const float* W = mat ? W1 : W0;
+      v[e] = (q < kPer && r < nrows && j < nvalid) ? W[(int64_t)r * ld + c0 + j] : 0.f;
+    }
+  }
+  __device__ __forceinline__ void store(float* __restrict__ chunk) const {
+#pragma unroll
+    for (int e = 0; e < kN; ++e) {
+      const int q = threadIdx.x + 256 * e;
+      if (q < kPer) {
+        const int mat = q / (NC * 16 * kSrcTiles);
+        const int qq = q - mat * (NC * 16 * kSrcTiles);
+        const int r = qq / NC, j = qq - r * NC;
+        chunk[mat * kStageCols * kPitchIn + j * PITCH + r] = v[e];
+      }
+    }
+  }
+};
+
 // One hidden layer on register-resident activations: out = W x over the
-// permuted K order (see the file comment).
+// permuted K order (see the file comment), one 16-column chunk of W per t,
+// double-buffered in LDS (buf[0], buf[1]); one barrier per chunk.
 __device__ __forceinline__ void hidden_layer(const f32x4 (&x)[kSrcTiles],
-                                             const float* __restrict__ W, int col, int g,
+                                             const float* __restrict__ W, float* buf0,
+                                             float* buf1, int col, int g,
                                              f32x4 (&out)[kSrcTiles]) {
+  auto nvalid = [](int t) { return t < kSrcTiles - 1 ? 16 : kSrcHid - 16 * t; };
+  Stager<kPitchHid, 16, 1> st;
+  st.load(W, W, kSrcHid, kSrcHid, 0, nvalid(0));
+  st.store(buf0);
+  __syncthreads();
 #pragma unroll
   for (int o = 0; o < kSrcTiles; ++o) out[o] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int t = 0; t < kSrcTiles; ++t)
+  for (int t = 0; t < kSrcTiles; ++t) {
+    if (t + 1 < kSrcTiles) st.load(W, W, kSrcHid, kSrcHid, 16 * (t + 1), nvalid(t + 1));
+    const float* chunk = (t & 1) ? buf1 : buf0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int kc = 16 * t + 4 * g + i;
       const float b = x[t][i];
+      const float* a = chunk + (4 * g + i) * kPitchHid + col;
 #pragma unroll
-      for (int o = 0; o < kSrcTiles; ++o)
-        out[o] = mfma(wload(W, kSrcHid, kSrcHid, kSrcHid, 16 * o + col, kc), b, out[o]);
+      for (int o = 0; o < kSrcTiles; ++o) out[o] = mfma(a[16 * o], b, out[o]);
     }
+    if (t + 1 < kSrcTiles) st.store((t & 1) ? buf0 : buf1);
+    __syncthreads();
+  }
 }
 
 // First-layer product of one or two 273-input matrices over the gathered
-// input (k-step (sh, sub): features 4 sub + g of shift sh).
+// input (k-step (sh, sub): features 4 sub + g of shift sh; one chunk of 21
+// columns per shift and matrix), double-buffered like hidden_layer.
 template <int NMAT>
 __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __restrict__ Fn,
                                             bool valid, int ci, int cj, int ck,
                                             const float* __restrict__ W0,
-                                            const float* __restrict__ W1, int col, int g,
+                                            const float* __restrict__ W1, float* buf0,
+                                            float* buf1, int col, int g,
                                             f32x4 (&a0)[kSrcTiles], f32x4 (&a1)[kSrcTiles]) {
+  auto gather = [&](int sh, float (&b)[6]) {
+    const int64_t nb = valid ? shifted(s, ci, cj, ck, sh) : 0;
+#pragma unroll
+    for (int sub = 0; sub < 6; ++sub) {
+      const int f = 4 * sub + g;
+      b[sub] = (valid && f < kSrcFeat) ? Fn[(int64_t)f * s.m + nb] : 0.f;
+    }
+  };
+  Stager<kPitchIn, kStageCols, NMAT> st;
+  st.load(W0, W1, kSrcIn, kSrcHid, 0, kSrcFeat);
+  st.store(buf0);
+  float b[6], bn[6];
+  gather(0, b);
+  __syncthreads();
 #pragma unroll
   for (int o = 0; o < kSrcTiles; ++o) {
     a0[o] = f32x4{0.f, 0.f, 0.f, 0.f};
     if (NMAT > 1) a1[o] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
   for (int sh = 0; sh < kSrcShifts; ++sh) {
-    const int64_t nb = valid ? shifted(s, ci, cj, ck, sh) : 0;
+    const bool more = sh + 1 < kSrcShifts;
+    if (more) {
+      st.load(W0, W1, kSrcIn, kSrcHid, (sh + 1) * kSrcFeat, kSrcFeat);
+      gather(sh + 1, bn);
+    }
+    const float* c0 = (sh & 1) ? buf1 : buf0;
+    const float* c1 = c0 + (NMAT > 1 ? kStageCols * kPitchIn : 0);
 #pragma unroll
     for (int sub = 0; sub < 6; ++sub) {
-      const int f = 4 * sub + g;
-      const float b = (valid && f < kSrcFeat) ? Fn[(int64_t)f * s.m + nb] : 0.f;
-      const int kc = f < kSrcFeat ? sh * kSrcFeat + f : kSrcIn;   // kSrcIn: no column
+      const int row = (4 * sub + g) * kPitchIn + col;
 #pragma unroll
       for (int o = 0; o < kSrcTiles; ++o) {
-        a0[o] = mfma(wload(W0, kSrcIn, kSrcHid, kSrcIn, 16 * o + col, kc), b, a0[o]);
-        if (NMAT > 1)
-          a1[o] = mfma(wload(W1, kSrcIn, kSrcHid, kSrcIn, 16 * o + col, kc), b, a1[o]);
+        a0[o] = mfma(c0[row + 16 * o], b[sub], a0[o]);
+        if (NMAT > 1) a1[o] = mfma(c1[row + 16 * o], b[sub], a1[o]);
       }
     }
+    if (more) {
+      st.store((sh & 1) ? buf0 : buf1);
+#pragma unroll
+      for (int sub = 0; sub < 6; ++sub) b[sub] = bn[sub];
+    }
+    __syncthreads();
   }
 }
 
@@ -209,6 +294,9 @@ __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __res
 __global__ void __launch_bounds__(256) src_network_kernel(SrcGeom s, const float* __restrict__ Fn,
                                                           const float* __restrict__ prm,
                                                           float* __restrict__ P) {
+  __shared__ float chunks[2 * kStageFloats];
+  float* buf0 = chunks;
+  float* buf1 = chunks + kStageFloats;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
@@ -219,22 +307,24 @@ __global__ void __launch_bounds__(256) src_network_kernel(SrcGeom s, const float
 
   f32x4 h1[kSrcTiles], gt[kSrcTiles];
   // L1 = tanh(fc1 x), G = tanh(fcG x)  (:91-100)
-  input_layer<2>(s, Fn, valid, ci, cj, ck, prm + kOffFc1W, prm + kOffGW, col, g, h1, gt);
+  input_layer<2>(s, Fn, valid, ci, cj, ck, prm + kOffFc1W, prm + kOffGW, buf0, buf1, col, g,
+                 h1, gt);
   bias_tanh(h1, prm + kOffFc1B, g);
   bias_tanh(gt, prm + kOffGB, g);
   // H2 = tanh(fc2 L1); H2_G = G * H2  (:94-104)
   f32x4 hg[kSrcTiles];
-  hidden_layer(h1, prm + kOffFc2W, col, g, hg);
+  hidden_layer(h1, prm + kOffFc2W, buf0, buf1, col, g, hg);
   bias_tanh(hg, prm + kOffFc2B, g);
 #pragma unroll
   for (int t = 0; t < kSrcTiles; ++t) hg[t] = gt[t] * hg[t];
   // Gate2 = tanh(fcG2 x)  (:102)
   f32x4 g2[kSrcTiles];
-  input_layer<1>(s, Fn, valid, ci, cj, ck, prm + kOffG2W, nullptr, col, g, g2, g2);
+  input_layer<1>(s, Fn, valid, ci, cj, ck, prm + kOffG2W, prm + kOffG2W, buf0, buf1, col, g,
+                 g2, g2);
   bias_tanh(g2, prm + kOffG2B, g);
   // H3 = tanh(fc3 H2_G); H4 = H3 * Gate2  (:108-112)
   f32x4 h3[kSrcTiles];
-  hidden_layer(hg, prm + kOffFc3W, col, g, h3);
+  hidden_layer(hg, prm + kOffFc3W, buf0, buf1, col, g, h3);
   bias_tanh(h3, prm + kOffFc3B, g);
 #pragma unroll
   for (int t = 0; t < kSrcTiles; ++t) h3[t] = h3[t] * g2[t];
