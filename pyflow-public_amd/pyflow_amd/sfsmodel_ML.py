@@ -120,3 +120,45 @@ class residual_stress:
 
     def finalize(self, comms=None):
         raise NotImplementedError("training the 'source' ML closure is not part of this build")
+
+
+def main(argv=None):
+    """Time one closure evaluation on random padded fields:
+    python -m pyflow_amd.sfsmodel_ML --N 64 --dtype f32 --reps 5"""
+    import argparse
+    import math
+    import types
+    from . import _native
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--N', type=int, default=64)
+    ap.add_argument('--dtype', choices=('f32', 'f64'), default='f32')
+    ap.add_argument('--reps', type=int, default=5)
+    a = ap.parse_args(argv)
+    dt = torch.float32 if a.dtype == 'f32' else torch.float64
+    n = a.N
+    h = 2 * math.pi / n
+    box = _native.Box(n, n, n, h, h, h, dtype=dt, device=torch.device('cuda'))
+    d = types.SimpleNamespace(prec=dt, device=torch.device('cuda'), box=box, nxo_=n + 4,
+                              nyo_=n + 4, nzo_=n + 4)
+    geo = types.SimpleNamespace(type='uniform', dx=h)
+    ml = residual_stress(types.SimpleNamespace(simDt=1e-3), d, geo)
+    gen = torch.Generator(device='cuda').manual_seed(0)
+    U = [torch.randn((n + 4,) * 3, dtype=dt, device='cuda', generator=gen) for _ in range(3)]
+    ml.update(*U)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.reps):
+        ml.update(*U)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.reps
+    flop = 2.0 * (n + 4) ** 3 * sum(o * i for _, o, i in LAYERS)
+    print('ML source closure %d^3 %s: %.3f ms per evaluation, %.1f TFLOP/s of the network'
+          % (n, a.dtype, ms, flop / ms / 1e9), flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    import sys
+    sys.exit(main())
