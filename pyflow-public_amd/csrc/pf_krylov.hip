@@ -262,9 +262,8 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
     const T* __restrict__ r, const T* __restrict__ rh, const T* __restrict__ a_old,
     const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L, Blk blk) {
   constexpr int NT = TZ * TY;
-  TileScal<T> sc;
-  if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, sc)) return;
-  const FieldOp<T, MODE> F{r, a_old, v_in, sc.c1, sc.c2};
+  // coefficients are set once the scalars are known (below)
+  FieldOp<T, MODE> F{r, a_old, v_in, (T)0, (T)0};
 
   int tz_i, ty_i, cx_i;
   tile_of_block(tg, blk, tz_i, ty_i, cx_i);
@@ -277,14 +276,6 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
   const bool active = zr < g.nz && yr < g.ny;
   const int z = zr % g.nz, y = yr % g.ny;
   const int64_t col = (int64_t)y * g.isy + z;
-  if (MODE == kModeC && sc.brk) {       // <t,t> == 0: only x += alpha*p
-    if (active)
-      for (int x = x0; x < x1; ++x) {
-        const int64_t q = (int64_t)x * g.isx + col;
-        f_out[q] = fma(a_old[q], sc.ta, f_out[q]);
-      }
-    return;
-  }
 
   // halo assignment: 2*TZ y-halo cells, then 2*TY z-halo cells
   int hz = -1, hy = -1, hslot_y = 0, hslot_z = 0;
@@ -314,14 +305,34 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
     x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
     return (int64_t)x * g.isx;
   };
-  T fm = F.at(xoff(x0 - 1) + col);
-  T fc = F.at(xoff(x0) + col);
-  T hv = is_halo ? F.at(xoff(x0) + hcol) : (T)0;
+  // The stored vectors of the first planes do not depend on the solver
+  // scalars: their loads are issued before the control block and the
+  // previous kernel's partials are read and reduced, and overlap them.
+  const Raw rm0 = F.load(xoff(x0 - 1) + col);
+  const Raw rc0 = F.load(xoff(x0) + col);
+  Raw rh0 = {};
+  if (is_halo) rh0 = F.load(xoff(x0) + hcol);
   // the raw loads of plane x+1 are in flight while plane x is computed, and
   // those of plane x+2 are issued before plane x's barrier
   Raw rn = F.load(xoff(x0 + 1) + col);
   Raw hrn = {};
   if (is_halo && x0 + 1 < x1) hrn = F.load(xoff(x0 + 1) + hcol);
+
+  TileScal<T> sc;
+  if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, sc)) return;
+  F.c1 = sc.c1;
+  F.c2 = sc.c2;
+  if (MODE == kModeC && sc.brk) {       // <t,t> == 0: only x += alpha*p
+    if (active)
+      for (int x = x0; x < x1; ++x) {
+        const int64_t q = (int64_t)x * g.isx + col;
+        f_out[q] = fma(a_old[q], sc.ta, f_out[q]);
+      }
+    return;
+  }
+  T fm = F.eval(rm0);
+  T fc = F.eval(rc0);
+  T hv = is_halo ? F.eval(rh0) : (T)0;
   double acc0 = 0.0, acc1 = 0.0;
   for (int x = x0; x < x1; ++x) {
     const int64_t xo = (int64_t)x * g.isx;
@@ -399,9 +410,8 @@ __device__ __forceinline__ void bicg_tile2_body(
     const T* __restrict__ v_in, T* __restrict__ f_out, T* __restrict__ af_out, Lap<T> L, Blk blk) {
   constexpr int NZT = TZ / 2;
   constexpr int NT = NZT * TY;
-  TileScal<T> sc;
-  if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, sc)) return;
-  const FieldOp<T, MODE> F{r, a_old, v_in, sc.c1, sc.c2};
+  // coefficients are set once the scalars are known (below)
+  FieldOp<T, MODE> F{r, a_old, v_in, (T)0, (T)0};
   using Raw = typename FieldOp<T, MODE>::Raw;
   struct Raw2 {
     Raw a, b;
@@ -430,16 +440,6 @@ __device__ __forceinline__ void bicg_tile2_body(
   const int yr = y0 + ly;
   const bool active = yr < g.ny;
   const int64_t col = (int64_t)(yr % g.ny) * g.isy + (z0 + 2 * lz);
-  if (MODE == kModeC && sc.brk) {       // <t,t> == 0: only x += alpha*p
-    if (active)
-      for (int x = x0; x < x1; ++x) {
-        const int64_t q = (int64_t)x * g.isx + col;
-        const Pair<T> pv = ld2(a_old, q), xv = ld2(f_out, q);
-        *reinterpret_cast<Pair<T>*>(f_out + q) =
-            Pair<T>{fma(pv.a, sc.ta, xv.a), fma(pv.b, sc.ta, xv.b)};
-      }
-    return;
-  }
 
   // halo: threads [0, 2*NZT) load the y-halo rows in pairs, the next 2*TY
   // threads one z-halo cell each
@@ -477,13 +477,36 @@ __device__ __forceinline__ void bicg_tile2_body(
     x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
     return (int64_t)x * g.isx;
   };
-  Pair<T> fm = eval2(load2(xoff(x0 - 1) + col));
-  Pair<T> fc = eval2(load2(xoff(x0) + col));
-  Pair<T> hv = {(T)0, (T)0};
-  if (is_halo) hv = evalh(loadh(xoff(x0) + hcol));
+  // The stored vectors of the first planes do not depend on the solver
+  // scalars: their loads are issued before the control block and the
+  // previous kernel's partials are read and reduced, and overlap them.
+  const Raw2 rm0 = load2(xoff(x0 - 1) + col);
+  const Raw2 rc0 = load2(xoff(x0) + col);
+  Raw2 rh0 = {};
+  if (is_halo) rh0 = loadh(xoff(x0) + hcol);
   Raw2 rn = load2(xoff(x0 + 1) + col);
   Raw2 hrn = {};
   if (is_halo && x0 + 1 < x1) hrn = loadh(xoff(x0 + 1) + hcol);
+
+  TileScal<T> sc;
+  if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, sc)) return;
+  F.c1 = sc.c1;
+  F.c2 = sc.c2;
+  if (MODE == kModeC && sc.brk) {       // <t,t> == 0: only x += alpha*p
+    if (active)
+      for (int x = x0; x < x1; ++x) {
+        const int64_t q = (int64_t)x * g.isx + col;
+        const Pair<T> pv = ld2(a_old, q), xv = ld2(f_out, q);
+        *reinterpret_cast<Pair<T>*>(f_out + q) =
+            Pair<T>{fma(pv.a, sc.ta, xv.a), fma(pv.b, sc.ta, xv.b)};
+      }
+    return;
+  }
+
+  Pair<T> fm = eval2(rm0);
+  Pair<T> fc = eval2(rc0);
+  Pair<T> hv = {(T)0, (T)0};
+  if (is_halo) hv = evalh(rh0);
   double acc0 = 0.0, acc1 = 0.0;
   const int zl = 2 * lz + 1;   // LDS column of the first cell
   for (int x = x0; x < x1; ++x) {
