@@ -487,6 +487,20 @@ __device__ __forceinline__ void bicg_tile2_body(
   Raw2 rn = load2(xoff(x0 + 1) + col);
   Raw2 hrn = {};
   if (is_halo && x0 + 1 < x1) hrn = loadh(xoff(x0 + 1) + hcol);
+  // the vectors KA and KC read at the cell itself: r_hat (KA, KC), x and p (KC)
+  struct Own {
+    Pair<T> rh, x, p;
+  };
+  auto load_own = [&](int64_t q) -> Own {
+    Own o = {{(T)0, (T)0}, {(T)0, (T)0}, {(T)0, (T)0}};
+    if (MODE != kModeB && active) o.rh = ld2(rh, q);
+    if (MODE == kModeC && active) {
+      o.x = ld2(f_out, q);
+      o.p = ld2(a_old, q);
+    }
+    return o;
+  };
+  Own own = load_own((int64_t)x0 * g.isx + col);
 
   TileScal<T> sc;
   if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, sc)) return;
@@ -514,12 +528,9 @@ __device__ __forceinline__ void bicg_tile2_body(
     Raw2 rnn = {}, hrnn = {};
     if (x + 2 <= x1) rnn = load2(xoff(x + 2) + col);
     if (is_halo && x + 2 < x1) hrnn = loadh(xoff(x + 2) + hcol);
-    Pair<T> rhc = {(T)0, (T)0}, xc = {(T)0, (T)0}, pc = {(T)0, (T)0};
-    if (MODE != kModeB && active) rhc = ld2(rh, xo + col);
-    if (MODE == kModeC && active) {
-      xc = ld2(f_out, xo + col);
-      pc = ld2(a_old, xo + col);
-    }
+    // this plane's own-cell vectors were loaded one plane ahead
+    const Pair<T> rhc = own.rh, xc = own.x, pc = own.p;
+    if (x + 1 < x1) own = load_own(xo + g.isx + col);
     const Pair<T> fp = eval2(rn);
     Pair<T> hn = {(T)0, (T)0};
     if (is_halo && x + 1 < x1) hn = evalh(hrn);
