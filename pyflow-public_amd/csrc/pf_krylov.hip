@@ -484,9 +484,17 @@ __device__ __forceinline__ void bicg_tile2_body(
   const Raw2 rc0 = load2(xoff(x0) + col);
   Raw2 rh0 = {};
   if (is_halo) rh0 = loadh(xoff(x0) + hcol);
+  // Planes ahead in flight while plane x is computed: KB keeps x+1 and x+2
+  // (its short per-plane work left each iteration waiting on its loads:
+  // 12.3 -> 11.6 us at 128^3 FP64, 268 -> 232 us at 512^3 FP32), KA and KC
+  // keep x+1 (with x+2 as well, KC went 690 -> 849 us at 512^3 FP32)
+  constexpr bool kDeep = MODE == kModeB;
   Raw2 rn = load2(xoff(x0 + 1) + col);
   Raw2 hrn = {};
   if (is_halo && x0 + 1 < x1) hrn = loadh(xoff(x0 + 1) + hcol);
+  Raw2 rn2 = {}, hrn2 = {};
+  if (kDeep && x0 + 2 <= x1) rn2 = load2(xoff(x0 + 2) + col);
+  if (kDeep && is_halo && x0 + 2 < x1) hrn2 = loadh(xoff(x0 + 2) + hcol);
   // the vectors KA and KC read at the cell itself: r_hat (KA, KC), x and p (KC)
   struct Own {
     Pair<T> rh, x, p;
@@ -530,9 +538,11 @@ __device__ __forceinline__ void bicg_tile2_body(
   const int zl = 2 * lz + 1;   // LDS column of the first cell
   for (int x = x0; x < x1; ++x) {
     const int64_t xo = (int64_t)x * g.isx;
-    Raw2 rnn = {}, hrnn = {};
-    if (x + 2 <= x1) rnn = load2(xoff(x + 2) + col);
-    if (is_halo && x + 2 < x1) hrnn = loadh(xoff(x + 2) + hcol);
+    // the loads of plane x + 1 + (kDeep ? 2 : 1)
+    const int xl = kDeep ? x + 3 : x + 2;
+    Raw2 rn3 = {}, hrn3 = {};
+    if (xl <= x1) rn3 = load2(xoff(xl) + col);
+    if (is_halo && xl < x1) hrn3 = loadh(xoff(xl) + hcol);
     const Own cur = kAhead ? own : load_own(xo + col);
     if (kAhead && x + 1 < x1) own = load_own(xo + g.isx + col);
     const Pair<T> rhc = cur.rh, xc = cur.x, pc = cur.p;
@@ -580,8 +590,15 @@ __device__ __forceinline__ void bicg_tile2_body(
     fm = fc;
     fc = fp;
     hv = hn;
-    rn = rnn;
-    hrn = hrnn;
+    if (kDeep) {
+      rn = rn2;
+      hrn = hrn2;
+      rn2 = rn3;
+      hrn2 = hrn3;
+    } else {
+      rn = rn3;
+      hrn = hrn3;
+    }
   }
   if (MODE == kModeA) {
     double accv[1] = {acc0};
