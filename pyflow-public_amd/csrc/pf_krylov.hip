@@ -317,6 +317,11 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
   Raw rn = F.load(xoff(x0 + 1) + col);
   Raw hrn = {};
   if (is_halo && x0 + 1 < x1) hrn = F.load(xoff(x0 + 1) + hcol);
+  // KB keeps plane x+2 in flight as well (as in the paired kernel)
+  constexpr bool kDeep = MODE == kModeB;
+  Raw rn2 = {}, hrn2 = {};
+  if (kDeep && x0 + 2 <= x1) rn2 = F.load(xoff(x0 + 2) + col);
+  if (kDeep && is_halo && x0 + 2 < x1) hrn2 = F.load(xoff(x0 + 2) + hcol);
 
   TileScal<T> sc;
   if (!tile_scalars<T, MODE, NT>(ctl, part_in, nblk_in, blk, sc)) return;
@@ -336,9 +341,10 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
   double acc0 = 0.0, acc1 = 0.0;
   for (int x = x0; x < x1; ++x) {
     const int64_t xo = (int64_t)x * g.isx;
+    const int xl = kDeep ? x + 3 : x + 2;
     Raw rnn = {}, hrnn = {};
-    if (x + 2 <= x1) rnn = F.load(xoff(x + 2) + col);
-    if (is_halo && x + 2 < x1) hrnn = F.load(xoff(x + 2) + hcol);
+    if (xl <= x1) rnn = F.load(xoff(xl) + col);
+    if (is_halo && xl < x1) hrnn = F.load(xoff(xl) + hcol);
     const T rhc = (MODE != kModeB && active) ? rh[xo + col] : (T)0;
     const T xc = (MODE == kModeC && active) ? f_out[xo + col] : (T)0;
     const T pc = (MODE == kModeC && active) ? a_old[xo + col] : (T)0;
@@ -373,8 +379,15 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
     fm = fc;
     fc = fp;
     hv = hn;
-    rn = rnn;
-    hrn = hrnn;
+    if (kDeep) {
+      rn = rn2;
+      hrn = hrn2;
+      rn2 = rnn;
+      hrn2 = hrnn;
+    } else {
+      rn = rnn;
+      hrn = hrnn;
+    }
   }
   if (MODE == kModeA) {
     double accv[1] = {acc0};
