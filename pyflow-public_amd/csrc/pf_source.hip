@@ -27,7 +27,7 @@
 // features of a shift padded to 24, so each k-step reads one neighbour.
 // Weights reach the MFMAs through LDS: each block stages one chunk of input
 // columns at a time for its four waves, the next chunk's loads in flight
-// while the current one is consumed.
+// while the current one is consumed; a 128-bit LDS read feeds four MFMAs.
 // Summation order differs from the reference's CPU GEMM, so results agree to
 // FP32 rounding, not bit for bit.
 #include <hip/hip_runtime.h>
@@ -159,23 +159,28 @@ __device__ __forceinline__ void bias_tanh(f32x4 (&a)[kSrcTiles], const float* __
 }
 
 // Weights go through LDS: a block stages a chunk of input columns of a
-// weight matrix once for its four waves, transposed (chunk[column][neuron]).
-// A half-wave reads 16 neurons of two columns; the pitch puts those 32 reads
-// on 32 banks: the hidden layers' column index steps by 4 between lane groups
-// (pitch 212: 4 * 212 = 16 mod 32), the input layers' by 1 (pitch 208).
-constexpr int kPitchHid = 212;
-constexpr int kPitchIn = 208;
-constexpr int kStageCols = 24;
-constexpr int kStageFloats = 2 * kStageCols * kPitchIn;   // >= 16 * kPitchHid
+// weight matrix once for its four waves, row-major (chunk[neuron][column]),
+// so one 128-bit LDS read gives a lane the A operands of four consecutive
+// k-steps.  Pitches (in floats) put the 8 lanes of each pass of a 128-bit
+// read on distinct banks: 20 for the 16-column chunks of the hidden layers,
+// 28 for the 24-column chunks (21 features of a shift) of the input layers.
+constexpr int kPitchHid = 20;
+constexpr int kPitchIn = 28;
+constexpr int kStageCols = 24;                             // 21 features of a shift, padded
+constexpr int kRows = 16 * kSrcTiles;                      // 208 neurons
+constexpr int kMatIn = kRows * kPitchIn;                   // one input-layer chunk
+constexpr int kStageFloats = 2 * kMatIn;                   // >= kRows * kPitchHid
 
-// One thread's share of a staged chunk: chunk[j * PITCH + r] = W[r][c0 + j]
-// for r < nrows and j < nvalid, else 0 (208 neuron rows, NC columns, NMAT
-// matrices one after another).  load() fills registers, store() writes them
-// to LDS, so the next chunk's loads are in flight while the current one is
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+
+// One thread's share of a staged chunk: chunk[mat * kMatIn + r * PITCH + j] =
+// W[r][c0 + j] for r < nrows and j < nvalid, else 0 (208 neuron rows, NC
+// columns, NMAT matrices).  load() fills registers, store() writes them to
+// LDS, so the next chunk's loads are in flight while the current one is
 // consumed.
 template <int PITCH, int NC, int NMAT>
 struct Stager {
-  static constexpr int kPer = NMAT * NC * 16 * kSrcTiles;
+  static constexpr int kPer = NMAT * NC * kRows;
   static constexpr int kN = (kPer + 255) / 256;
   float v[kN];
   __device__ __forceinline__ void load(const float* __restrict__ W0, const float* __restrict__ W1,
@@ -183,8 +188,8 @@ struct Stager {
 #pragma unroll
     for (int e = 0; e < kN; ++e) {
       const int q = threadIdx.x + 256 * e;
-      const int mat = q / (NC * 16 * kSrcTiles);
-      const int qq = q - mat * (NC * 16 * kSrcTiles);
+      const int mat = q / (NC * kRows);
+      const int qq = q - mat * (NC * kRows);
       const int r = qq / NC, j = qq - r * NC;
       const float* W = mat ? W1 : W0;
       v[e] = (q < kPer && r < nrows && j < nvalid) ? W[(int64_t)r * ld + c0 + j] : 0.f;
@@ -195,10 +200,10 @@ struct Stager {
     for (int e = 0; e < kN; ++e) {
       const int q = threadIdx.x + 256 * e;
       if (q < kPer) {
-        const int mat = q / (NC * 16 * kSrcTiles);
-        const int qq = q - mat * (NC * 16 * kSrcTiles);
+        const int mat = q / (NC * kRows);
+        const int qq = q - mat * (NC * kRows);
         const int r = qq / NC, j = qq - r * NC;
-        chunk[mat * kStageCols * kPitchIn + j * PITCH + r] = v[e];
+        chunk[mat * kMatIn + r * PITCH + j] = v[e];
       }
     }
   }
@@ -206,7 +211,8 @@ struct Stager {
 
 // One hidden layer on register-resident activations: out = W x over the
 // permuted K order (see the file comment), one 16-column chunk of W per t,
-// double-buffered in LDS (buf[0], buf[1]); one barrier per chunk.
+// double-buffered in LDS (buf0, buf1); one barrier per chunk.  A lane reads
+// W[16 o + col][16 t + 4 g .. + 3] with one 128-bit read, one tile ahead.
 __device__ __forceinline__ void hidden_layer(const f32x4 (&x)[kSrcTiles],
                                              const float* __restrict__ W, float* buf0,
                                              float* buf1, int col, int g,
@@ -221,13 +227,15 @@ __device__ __forceinline__ void hidden_layer(const f32x4 (&x)[kSrcTiles],
 #pragma unroll
   for (int t = 0; t < kSrcTiles; ++t) {
     if (t + 1 < kSrcTiles) st.load(W, W, kSrcHid, kSrcHid, 16 * (t + 1), nvalid(t + 1));
-    const float* chunk = (t & 1) ? buf1 : buf0;
+    const float* base = ((t & 1) ? buf1 : buf0) + col * kPitchHid + 4 * g;
+    f32x4 a = *reinterpret_cast<const f32x4*>(base);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float b = x[t][i];
-      const float* a = chunk + (4 * g + i) * kPitchHid + col;
+    for (int o = 0; o < kSrcTiles; ++o) {
+      f32x4 an = a;
+      if (o + 1 < kSrcTiles) an = *reinterpret_cast<const f32x4*>(base + 16 * (o + 1) * kPitchHid);
 #pragma unroll
-      for (int o = 0; o < kSrcTiles; ++o) out[o] = mfma(a[16 * o], b, out[o]);
+      for (int i = 0; i < 4; ++i) out[o] = mfma(a[i], x[t][i], out[o]);
+      a = an;
     }
     if (t + 1 < kSrcTiles) st.store((t & 1) ? buf0 : buf1);
     __syncthreads();
@@ -235,8 +243,10 @@ __device__ __forceinline__ void hidden_layer(const f32x4 (&x)[kSrcTiles],
 }
 
 // First-layer product of one or two 273-input matrices over the gathered
-// input (k-step (sh, sub): features 4 sub + g of shift sh; one chunk of 21
-// columns per shift and matrix), double-buffered like hidden_layer.
+// input, one chunk of 21 columns (one shift) per matrix and step, double-
+// buffered like hidden_layer.  K order within a shift: four k-steps i with
+// lane group g on feature 4 g + i (one 128-bit read), then two with
+// 16 + 2 g + i (one 64-bit read; features 21-23 are zero).
 template <int NMAT>
 __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __restrict__ Fn,
                                             bool valid, int ci, int cj, int ck,
@@ -247,9 +257,9 @@ __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __res
   auto gather = [&](int sh, float (&b)[6]) {
     const int64_t nb = valid ? shifted(s, ci, cj, ck, sh) : 0;
 #pragma unroll
-    for (int sub = 0; sub < 6; ++sub) {
-      const int f = 4 * sub + g;
-      b[sub] = (valid && f < kSrcFeat) ? Fn[(int64_t)f * s.m + nb] : 0.f;
+    for (int i = 0; i < 6; ++i) {
+      const int f = i < 4 ? 4 * g + i : 16 + 2 * g + (i - 4);
+      b[i] = (valid && f < kSrcFeat) ? Fn[(int64_t)f * s.m + nb] : 0.f;
     }
   };
   Stager<kPitchIn, kStageCols, NMAT> st;
@@ -269,21 +279,34 @@ __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __res
       st.load(W0, W1, kSrcIn, kSrcHid, (sh + 1) * kSrcFeat, kSrcFeat);
       gather(sh + 1, bn);
     }
-    const float* c0 = (sh & 1) ? buf1 : buf0;
-    const float* c1 = c0 + (NMAT > 1 ? kStageCols * kPitchIn : 0);
+    const float* c0 = ((sh & 1) ? buf1 : buf0) + col * kPitchIn;
+    const float* c1 = c0 + (NMAT > 1 ? kMatIn : 0);
 #pragma unroll
-    for (int sub = 0; sub < 6; ++sub) {
-      const int row = (4 * sub + g) * kPitchIn + col;
+    for (int o = 0; o < kSrcTiles; ++o) {
+      const int row = 16 * o * kPitchIn;
+      const f32x4 wa = *reinterpret_cast<const f32x4*>(c0 + row + 4 * g);
+      const f32x2v wb = *reinterpret_cast<const f32x2v*>(c0 + row + 16 + 2 * g);
+      f32x4 va = wa;
+      f32x2v vb = wb;
+      if (NMAT > 1) {
+        va = *reinterpret_cast<const f32x4*>(c1 + row + 4 * g);
+        vb = *reinterpret_cast<const f32x2v*>(c1 + row + 16 + 2 * g);
+      }
 #pragma unroll
-      for (int o = 0; o < kSrcTiles; ++o) {
-        a0[o] = mfma(c0[row + 16 * o], b[sub], a0[o]);
-        if (NMAT > 1) a1[o] = mfma(c1[row + 16 * o], b[sub], a1[o]);
+      for (int i = 0; i < 4; ++i) {
+        a0[o] = mfma(wa[i], b[i], a0[o]);
+        if (NMAT > 1) a1[o] = mfma(va[i], b[i], a1[o]);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a0[o] = mfma(wb[i], b[4 + i], a0[o]);
+        if (NMAT > 1) a1[o] = mfma(vb[i], b[4 + i], a1[o]);
       }
     }
     if (more) {
       st.store((sh & 1) ? buf0 : buf1);
 #pragma unroll
-      for (int sub = 0; sub < 6; ++sub) b[sub] = bn[sub];
+      for (int i = 0; i < 6; ++i) b[i] = bn[i];
     }
     __syncthreads();
   }
@@ -294,7 +317,7 @@ __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __res
 __global__ void __launch_bounds__(256) src_network_kernel(SrcGeom s, const float* __restrict__ Fn,
                                                           const float* __restrict__ prm,
                                                           float* __restrict__ P) {
-  __shared__ float chunks[2 * kStageFloats];
+  __shared__ __align__(16) float chunks[2 * kStageFloats];
   float* buf0 = chunks;
   float* buf1 = chunks + kStageFloats;
   const int lane = threadIdx.x & 63;
