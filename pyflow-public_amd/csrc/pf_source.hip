@@ -281,17 +281,31 @@ __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __res
     }
     const float* c0 = ((sh & 1) ? buf1 : buf0) + col * kPitchIn;
     const float* c1 = c0 + (NMAT > 1 ? kMatIn : 0);
+    // the operands of tile o + 1 are read while tile o's MFMAs run
+    struct Ops {
+      f32x4 wa, va;
+      f32x2v wb, vb;
+    };
+    auto read_ops = [&](int o) -> Ops {
+      const int row = 16 * o * kPitchIn;
+      Ops q;
+      q.wa = *reinterpret_cast<const f32x4*>(c0 + row + 4 * g);
+      q.wb = *reinterpret_cast<const f32x2v*>(c0 + row + 16 + 2 * g);
+      q.va = q.wa;
+      q.vb = q.wb;
+      if (NMAT > 1) {
+        q.va = *reinterpret_cast<const f32x4*>(c1 + row + 4 * g);
+        q.vb = *reinterpret_cast<const f32x2v*>(c1 + row + 16 + 2 * g);
+      }
+      return q;
+    };
+    Ops nx = read_ops(0);
 #pragma unroll
     for (int o = 0; o < kSrcTiles; ++o) {
-      const int row = 16 * o * kPitchIn;
-      const f32x4 wa = *reinterpret_cast<const f32x4*>(c0 + row + 4 * g);
-      const f32x2v wb = *reinterpret_cast<const f32x2v*>(c0 + row + 16 + 2 * g);
-      f32x4 va = wa;
-      f32x2v vb = wb;
-      if (NMAT > 1) {
-        va = *reinterpret_cast<const f32x4*>(c1 + row + 4 * g);
-        vb = *reinterpret_cast<const f32x2v*>(c1 + row + 16 + 2 * g);
-      }
+      const Ops cu = nx;
+      if (o + 1 < kSrcTiles) nx = read_ops(o + 1);
+      const f32x4 wa = cu.wa, va = cu.va;
+      const f32x2v wb = cu.wb, vb = cu.vb;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         a0[o] = mfma(wa[i], b[i], a0[o]);
