@@ -13,7 +13,10 @@ time step as soon as its own solve is done (continuous batching at the
 granularity of Krylov iterations: no request waits for a slower one).
 Requests that arrive while others run wait in a FIFO queue and are admitted
 as soon as their HBM footprint fits under the admission limit and a batch
-slot is free; finished requests leave when their last step ends.
+slot is free; an admitted request's initial field is built by a loader
+thread on its own stream while the batch keeps running (pf_batch_advance
+releases the GIL), and it joins the batch once built; finished requests
+leave when their last step ends.
 Boxes and field tensors are pooled by size, so steady-state admission does
 not allocate.
 
@@ -24,6 +27,8 @@ same bits alone, in any mix, and through the plain single-box path
 """
 
 import math
+import queue as _queue
+import threading
 import time
 
 import numpy as np
@@ -203,6 +208,33 @@ def _record(req, row, npts):
     req.iters.append(int(row[6]))
 
 
+class _Loader(threading.Thread):
+    """Builds admitted requests' initial states (Slot.load) on a stream of
+    its own, so that the batch's kernels keep the GPU busy meanwhile."""
+
+    def __init__(self, device):
+        super().__init__(daemon=True)
+        self.device = device
+        self.stream = torch.cuda.Stream(device=device)
+        self.todo = _queue.Queue()
+        self.done = _queue.Queue()
+
+    def run(self):
+        torch.cuda.set_device(self.device)
+        while True:
+            item = self.todo.get()
+            if item is None:
+                return
+            req, slot = item
+            try:
+                with torch.cuda.stream(self.stream):
+                    slot.load(req)
+                self.stream.synchronize()
+                self.done.put((req, slot, None))
+            except Exception as e:   # handed to the serving loop, which raises it
+                self.done.put((req, slot, e))
+
+
 class BatchServer:
     """Continuous-batching server on one GPU.  hbm_cap: admission limit in
     bytes for resident slots (idle pooled slots are evicted to make room)."""
@@ -223,19 +255,29 @@ class BatchServer:
     def serve(self, requests, keep_results=False, realtime=True):
         """Serve `requests` (arrival times in seconds from now when realtime,
         else all queued at once).  Returns a summary dict."""
+        loader = _Loader(self.device)
+        loader.start()
+        try:
+            return self._serve(requests, keep_results, realtime, loader)
+        finally:
+            loader.todo.put(None)
+            loader.join()
+
+    def _serve(self, requests, keep_results, realtime, loader):
         pending = sorted(requests, key=lambda r: r.arrival)
         queue = []
         active = []          # [req, slot, inside a step, step start time]
+        loading = 0          # admitted requests whose initial state is being built
         t0 = time.perf_counter()
         i = 0
         dirty = True
-        while i < len(pending) or queue or active:
+        while i < len(pending) or queue or active or loading:
             now = time.perf_counter() - t0
             while i < len(pending) and (not realtime or pending[i].arrival <= now):
                 queue.append(pending[i])
                 i += 1
             # admission: FIFO while a slot is free and the footprint fits
-            while queue and len(active) < self.capacity:
+            while queue and len(active) + loading < self.capacity:
                 req = queue[0]
                 need = self.pool.footprint(req.N)
                 if self.pool.bytes_resident + need > self.hbm_cap:
@@ -243,14 +285,24 @@ class BatchServer:
                 if not self.pool.free.get(req.N):
                     self.pool.evict(need, self.hbm_cap)
                 slot = self.pool.take(req.N)
-                slot.load(req)
+                loader.todo.put((req, slot))
+                loading += 1
+                queue.pop(0)
+            self.max_resident = max(self.max_resident, self.pool.bytes_resident)
+            # built requests join the batch; wait for one only when nothing runs
+            while loading:
+                try:
+                    req, slot, err = loader.done.get(block=not active, timeout=0.01)
+                except _queue.Empty:
+                    break
+                if err is not None:
+                    raise err
+                loading -= 1
                 req.admitted = time.perf_counter()
                 active.append([req, slot, False, None])
-                queue.pop(0)
                 dirty = True
-            self.max_resident = max(self.max_resident, self.pool.bytes_resident)
             if not active:
-                if i < len(pending):
+                if i < len(pending) and not loading:
                     time.sleep(max(0.0, min(0.01, pending[i].arrival - now)))
                 continue
             if dirty:
