@@ -7,7 +7,8 @@ padding read clamped cells and store nothing), the x axis splits into
 chunks of 10, 10, 10 and 7 planes, and the three spacings differ.  RK4, Euler
 and the bare RHS, with a constant viscosity, a viscosity field and tensor
 stresses, are compared with the oracle's restatement of velocity.py:221-383
-and advancer.py:49-113.  Elementwise arithmetic only: bit-exact.
+and advancer.py:49-113, in FP64 and in FP32 (the oracle then issues the
+same ops on float32 tensors).  Elementwise arithmetic only: bit-exact.
 """
 
 import math
@@ -34,13 +35,13 @@ class Tensor6:
     modelType = 'tensor'
 
 
-def make_case(visc_field, stresses):
-    og = O.Grid(*SHAPE, Lx=L[0], Ly=L[1], Lz=L[2])
+def make_case(visc_field, stresses, dtype=torch.float64):
+    og = O.Grid(*SHAPE, Lx=L[0], Ly=L[1], Lz=L[2], dtype=dtype)
     u = [O.Field(og, rnd(s)) for s in (1, 2, 3)]
     if visc_field:
         VISC = og.from_interior(MU * (1.0 + 0.5 * np.abs(rnd(4))))
     else:
-        VISC = torch.ones(og.nxo, og.nyo, og.nzo, dtype=torch.float64) * MU
+        VISC = torch.ones(og.nxo, og.nyo, og.nzo, dtype=dtype) * MU
     sfs = None
     if stresses:
         sfs = Tensor6()
@@ -51,7 +52,7 @@ def make_case(visc_field, stresses):
 
 def gpu_box(og):
     from pyflow_amd import _native
-    return _native.Box(*SHAPE, og.dx, og.dy, og.dz, dtype=torch.float64,
+    return _native.Box(*SHAPE, og.dx, og.dy, og.dz, dtype=og.dtype,
                        device=torch.device('cuda'))
 
 
@@ -67,11 +68,13 @@ def sfs6(sfs):
 
 CASES = [(False, False), (True, False), (False, True), (True, True)]
 IDS = ['const-mu', 'visc-field', 'stresses', 'visc-field+stresses']
+DTYPES = pytest.mark.parametrize('dtype', [torch.float64, torch.float32], ids=['f64', 'f32'])
 
 
+@DTYPES
 @pytest.mark.parametrize('visc_field,stresses', CASES, ids=IDS)
-def test_rk4_step_bit_exact(visc_field, stresses):
-    og, u, VISC, sfs = make_case(visc_field, stresses)
+def test_rk4_step_bit_exact(visc_field, stresses, dtype):
+    og, u, VISC, sfs = make_case(visc_field, stresses, dtype)
     box = gpu_box(og)
     U = [dev(f.var) for f in u]
     Ut = [torch.zeros_like(a) for a in U]
@@ -89,13 +92,14 @@ def test_rk4_step_bit_exact(visc_field, stresses):
         assert np.array_equal(cpu(Ut[q]), t[q].var.numpy()), ('Y4', q)
 
 
+@DTYPES
 @pytest.mark.parametrize('visc_field,stresses', CASES, ids=IDS)
-def test_euler_step_and_rhs_bit_exact(visc_field, stresses):
-    og, u, VISC, sfs = make_case(visc_field, stresses)
+def test_euler_step_and_rhs_bit_exact(visc_field, stresses, dtype):
+    og, u, VISC, sfs = make_case(visc_field, stresses, dtype)
     box = gpu_box(og)
     U = [dev(f.var) for f in u]
     S = sfs6(sfs)
-    R = [torch.empty(SHAPE, dtype=torch.float64, device='cuda') for _ in range(3)]
+    R = [torch.empty(SHAPE, dtype=dtype, device='cuda') for _ in range(3)]
     if visc_field:
         box.rhs_ns_visc(*U, dev(VISC), RHO, *R, S)
         box.euler_step_visc(*U, DT, dev(VISC), RHO, S)
