@@ -191,3 +191,72 @@ def rollout(stepper, u, v, w, nsteps, params=None):
     for _ in range(nsteps):
         u, v, w = step(stepper, u, v, w, params)
     return u, v, w
+
+
+def main(argv=None):
+    """Time one closure-training window: K checkpointed steps forward and the
+    reverse sweep back to u^0 and the MLP parameters, from a Passot-Pouquet
+    field (k0 = 4, u_rms = 1, dt at CFL 0.3):
+    python -m pyflow_amd.autodiff --N 64 --steps 20 --dtype f32 --closure mlp"""
+    import argparse
+    import math
+    import numpy as np
+    from .initial_conditions import passot_pouquet, cfl_dt
+    from .sfsmodel_mlp import init_params
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--N', type=int, default=64)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--dtype', choices=('f32', 'f64'), default='f32')
+    ap.add_argument('--closure', choices=('none', 'gradient', 'mlp'), default='mlp')
+    ap.add_argument('--nu', type=float, default=0.005)
+    ap.add_argument('--tol', type=float, default=None,
+                    help='Poisson tolerance (default 1e-6 in FP32, 1e-12 in FP64)')
+    ap.add_argument('--reps', type=int, default=2)
+    a = ap.parse_args(argv)
+    dt_ = torch.float32 if a.dtype == 'f32' else torch.float64
+    tol = a.tol if a.tol is not None else (1e-6 if a.dtype == 'f32' else 1e-12)
+    closure = None if a.closure == 'none' else a.closure
+    n, K = a.N, a.steps
+    h = 2 * math.pi / n
+    dev = torch.device('cuda')
+    u0 = passot_pouquet(n, k0=4.0, urms=1.0, seed=3)
+    dt = cfl_dt(*u0, h)
+    box = _native.Box(n, n, n, h, h, h, dtype=dt_, device=dev)
+    st = StepVJP(box, a.nu, 1.0, dt, closure=closure, tol=tol)
+    rng = np.random.Generator(np.random.PCG64(7))
+    c = [torch.tensor(rng.standard_normal((n, n, n)), dtype=dt_, device=dev) for _ in range(3)]
+    params = None
+    if closure == 'mlp':
+        params = torch.tensor(init_params(13), device=dev, requires_grad=True)
+    fwd, rev = [], []
+    for rep in range(a.reps + 1):
+        x = [torch.tensor(t, dtype=dt_, device=dev, requires_grad=True) for t in u0]
+        if params is not None:
+            params.grad = None
+        st.iterations = []
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        torch.cuda.synchronize()
+        e[0].record()
+        out = rollout(st, *x, K, params)
+        loss = sum(torch.sum(o * cc) for o, cc in zip(out, c))
+        e[1].record()
+        loss.backward()
+        e[2].record()
+        torch.cuda.synchronize()
+        if rep > 0:            # the first window warms up
+            fwd.append(e[0].elapsed_time(e[1]))
+            rev.append(e[1].elapsed_time(e[2]))
+    its = st.iterations
+    nf = len(its) // 2
+    f, r = min(fwd), min(rev)
+    print('closure-training window %d^3 %s closure=%s: %d steps, forward %.1f ms '
+          '(%.2f ms/step), reverse %.1f ms (%.2f ms/step), window %.1f ms; '
+          'Poisson iterations per solve: forward %.1f, reverse %.1f'
+          % (n, a.dtype, a.closure, K, f, f / K, r, r / K, f + r,
+             sum(its[:nf]) / max(nf, 1), sum(its[nf:]) / max(len(its) - nf, 1)), flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    import sys
+    sys.exit(main())
