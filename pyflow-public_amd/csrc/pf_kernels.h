@@ -78,6 +78,11 @@ __device__ __forceinline__ T lap7(const Lap<T>& L, T xm, T c, T xp, T ym, T yp, 
   return b;
 }
 
+// Krylov tiles whose x chunks have at most this many planes take the
+// one-plane-ahead register schedule (PIPE in pf_krylov.hip), longer chunks
+// the rotating one.
+constexpr int kPipeChunk = 16;
+
 // Tiled KA / KB decomposition (see pf_krylov.hip): a block owns a TZ x TY
 // tile of the (y,z) plane and marches along x over a chunk of planes.
 struct TileGrid {
