@@ -35,9 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM-side bytes per KA launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this bench's workload (one pass per counter), each scaled by
 # the counter's reading for a plain 512 MiB device copy (FETCH 0.500,
-# WRITE 1.000): (38184.8 KiB / 0.500 + 32098.8 KiB) per dispatch.  Keyed by
+# WRITE 1.000): (38403.6 KiB / 0.500 + 32099.1 KiB) per dispatch.  Keyed by
 # (grid, dtype); other workloads report null.
-KA_TRAFFIC_BYTES = {(128, 'f64'): int((38184.8 / 0.50003 + 32098.8) * 1024)}
+KA_TRAFFIC_BYTES = {(128, 'f64'): int((38403.6 / 0.50003 + 32099.1) * 1024)}
 
 
 def make_config(N, dtype, nu, tol, maxit, seed, solver='bicgstab'):
@@ -208,7 +208,7 @@ def main():
                              'bytes_per_launch': wmul * w * ncell, 'GB/s': round(gbs, 1)}
         out['kernels'] = per
         if 'KA' in per:
-            out['roofline'] = {'kernel': 'bicg_tile_kernel<KA>', 'bound': 'hbm',
+            out['roofline'] = {'kernel': 'KA: p update, v = A p, <v, r_hat> (bicg_tile2_kernel / bicg_tile_kernel)', 'bound': 'hbm',
                                'achieved': per['KA']['GB/s'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                                'frac': round(per['KA']['GB/s'] / HBM_PEAK_GBS, 4),
                                'traffic': KA_TRAFFIC_BYTES.get((a.N, a.dtype)),
