@@ -315,9 +315,8 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
   Raw rh0 = {};
   if (PIPE || is_halo) rh0 = F.load(xoff(x0) + hcol);
   // PIPE: register set c/h = column and halo of plane x+1, rh/x/p = the own
-  // cell at plane x.  !PIPE: the loads of plane x+1 are in flight while
-  // plane x is computed (x+2 as well for KB), the own cell is read in its
-  // iteration.
+  // cell at plane x.  !PIPE: plane x+1 is loaded ahead (x+2 as well for KB)
+  // and the loads rotate into place, the own cell is read in its iteration.
   struct Ahead {
     Raw c, h;
     T rh, x, p;
@@ -466,10 +465,11 @@ struct alignas(2 * sizeof(T)) Pair {
 // the loads repeat the last plane they need, threads without a halo cell
 // read a discarded pair), because the compiler waits for every load in
 // flight where a load or its use sits under a branch.
-// !PIPE (long chunks, 512^3): the column loads of plane x+1 (x+2 for KB) are
-// in flight while plane x is computed, the own-cell vectors are read in
-// their iteration (one plane ahead for FP32 KC).  On long chunks this
-// schedule, fewer loads in flight, reaches the bandwidth of a plain stream;
+// !PIPE (long chunks, 512^3): iteration x issues the column loads of plane
+// x+2 (x+3 for KB) at its top and rotates them into place at its end, and
+// reads the own-cell vectors in their iteration (one plane ahead for FP32
+// KC), so it waits on its own loads.  On long chunks, with four waves per
+// SIMD, this schedule still reaches the bandwidth of a plain stream, and
 // PIPE was slower there (512^3 FP32: KA 683 -> 735 us, KC 690 -> 828 us).
 template <typename T, int MODE, int TZ, int TY, bool PIPE>
 __device__ __forceinline__ void bicg_tile2_body(
@@ -574,12 +574,10 @@ __device__ __forceinline__ void bicg_tile2_body(
   Raw2 rh0 = {};
   if (PIPE || is_halo) rh0 = loadh(xoff(x0) + hcol);
   // PIPE: register set c/h = column and halo of plane x+1, o = own cells of
-  // plane x.  !PIPE: KB keeps planes x+1 and x+2 in flight (its short
-  // per-plane work left each iteration waiting on its loads: 12.3 -> 11.6
-  // us at 128^3 FP64, 268 -> 232 us at 512^3 FP32), KA and KC x+1 (with x+2
-  // as well, KC went 690 -> 849 us at 512^3 FP32); FP32 KC loads its own-cell
-  // vectors one plane ahead (730 -> 690 us at 512^3; ahead of time KA went
-  // 675 -> 743 us there and FP64 KC at 128^3 19.57 -> 19.89 us).
+  // plane x.  !PIPE: KB loads planes x+1 and x+2 ahead (268 -> 232 us at
+  // 512^3 FP32), KA and KC x+1 (with x+2 as well, KC went 690 -> 849 us at
+  // 512^3 FP32); FP32 KC loads its own-cell vectors one plane ahead (730 ->
+  // 690 us at 512^3; ahead of time KA went 675 -> 743 us there).
   struct Ahead {
     Raw2 c, h;
     Own o;
