@@ -255,6 +255,36 @@ __device__ __forceinline__ bool tile_scalars(KrylovCtl* __restrict__ ctl,
   return true;
 }
 
+// Two cells of a z row, loaded and stored with one vector access.
+template <typename T>
+struct alignas(2 * sizeof(T)) Pair {
+  T a, b;
+};
+
+// Output store of the tile kernels.  NT: non-temporal, used by the rotating
+// schedule of large grids, whose Krylov set does not fit the Infinity Cache
+// (512^3 FP32: KA 674 -> 659, KB 230 -> 217, KC 691 -> 674 us); on the
+// short-chunk grids (128^3 FP64, whose set fits) it was slower (KA 20.6 ->
+// 22.5 us), the next kernel's reads no longer finding the data there.
+template <bool NT, typename T>
+__device__ __forceinline__ void store2(T* p, const Pair<T>& v) {
+  if constexpr (NT) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 w = {v.a, v.b};
+    __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+  } else {
+    *reinterpret_cast<Pair<T>*>(p) = v;
+  }
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ void store1(T* p, T v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 // One cell per thread; PIPE selects the load schedule as in bicg_tile2_body.
 template <typename T, int MODE, int TZ, int TY, bool PIPE>
 __device__ __forceinline__ void bicg_tile_kernel_body(
@@ -372,8 +402,8 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
       const T fzm = pl[ly + 1][lz], fzp = pl[ly + 1][lz + 2];
       const T af = lap7(L, fm, fc, fp, fym, fyp, fzm, fzp);
       if (MODE == kModeA) {
-        f_out[xo + col] = fc;
-        af_out[xo + col] = af;
+        store1<!PIPE>(f_out + xo + col, fc);
+        store1<!PIPE>(af_out + xo + col, af);
         acc0 += (double)af * (double)rhc;
       } else if (MODE == kModeB) {
         acc0 += (double)fc * (double)af;
@@ -382,9 +412,9 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
         // KC: fc = s, af = t = A s
         T xv = fma(pc, sc.ta, xc);                  // pressure.py:251
         xv = fma(fc, sc.to, xv);                    // pressure.py:270
-        f_out[xo + col] = xv;
+        store1<!PIPE>(f_out + xo + col, xv);
         const T rnew = fma(af, sc.tmo, fc);         // pressure.py:274-275
-        af_out[xo + col] = rnew;
+        store1<!PIPE>(af_out + xo + col, rnew);
         acc0 += (double)rnew * (double)rhc;
         acc1 = nmax(acc1, fabs((double)rnew));
       }
@@ -442,12 +472,6 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
     write_partials<2>(part_out, accv, blk.id);
   }
 }
-
-// Two cells of a z row, loaded and stored with one vector access.
-template <typename T>
-struct alignas(2 * sizeof(T)) Pair {
-  T a, b;
-};
 
 // KA / KB with two z-neighbour cells per thread: a 256-thread block owns a
 // TZ x TY tile (TZ/2 threads per row, TY = 512/TZ rows), so the y halo is a
@@ -639,8 +663,8 @@ __device__ __forceinline__ void bicg_tile2_body(
       const T af1 = lap7(L, fm.b, fc.b, fp.b, pl[ly][zl + 1], pl[ly + 2][zl + 1], fc.a,
                          pl[ly + 1][zl + 2]);
       if (MODE == kModeA) {
-        *reinterpret_cast<Pair<T>*>(f_out + xo + col) = fc;
-        *reinterpret_cast<Pair<T>*>(af_out + xo + col) = Pair<T>{af0, af1};
+        store2<!PIPE>(f_out + xo + col, fc);
+        store2<!PIPE>(af_out + xo + col, Pair<T>{af0, af1});
         acc0 += (double)af0 * (double)rhc.a;
         acc0 += (double)af1 * (double)rhc.b;
       } else if (MODE == kModeB) {
@@ -654,9 +678,9 @@ __device__ __forceinline__ void bicg_tile2_body(
         T xb = fma(pc.b, sc.ta, xc.b);
         xa = fma(fc.a, sc.to, xa);
         xb = fma(fc.b, sc.to, xb);
-        *reinterpret_cast<Pair<T>*>(f_out + xo + col) = Pair<T>{xa, xb};
+        store2<!PIPE>(f_out + xo + col, Pair<T>{xa, xb});
         const T ra = fma(af0, sc.tmo, fc.a), rb = fma(af1, sc.tmo, fc.b);
-        *reinterpret_cast<Pair<T>*>(af_out + xo + col) = Pair<T>{ra, rb};
+        store2<!PIPE>(af_out + xo + col, Pair<T>{ra, rb});
         acc0 += (double)ra * (double)rhc.a;
         acc0 += (double)rb * (double)rhc.b;
         acc1 = nmax(acc1, fabs((double)ra));

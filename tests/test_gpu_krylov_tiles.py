@@ -1,7 +1,13 @@
-"""The tiled Krylov kernels' paired-cell path (two z-neighbour cells per
-thread, 64 x 8 tiles), which the solver takes when a box gives it at least
-512 blocks that way: 128 x 128 x 64 and larger (smaller boxes take the
-one-cell path that the other solver tests cover).
+"""The tiled Krylov kernels on the decompositions the other solver tests
+do not reach:
+
+* 128 x 128 x 64: two z-neighbour cells per thread (64 x 8 tiles), taken
+  when a box gives at least 512 blocks that way, here with x chunks of 4
+  planes - the one-plane-ahead load schedule of short chunks;
+* 136 x 1024 x 64: the same tiles on chunks of 17 planes - the rotating
+  load schedule and non-temporal stores of long chunks (512^3 takes it);
+* 85 x 1024 x 45: one cell per thread (45 is no multiple of the 16-wide
+  tile), chunks of 17 planes - the rotating schedule of that kernel.
 
 Ten BiCGStab iterations (pressure.py:229-290) on a random mean-free source
 against the oracle's ten: the per-iteration relative residual and the
@@ -22,26 +28,30 @@ from oracle import pyflow_oracle as O
 
 pytestmark = pytest.mark.gpu
 
-SHAPE = (128, 128, 64)
-L = (2 * math.pi, 2 * math.pi, math.pi)
+SHAPES = {'paired-short-chunks': ((128, 128, 64), (2 * math.pi, 2 * math.pi, math.pi)),
+          'paired-long-chunks': ((136, 1024, 64), (2 * math.pi, 8 * math.pi, math.pi)),
+          'one-cell-long-chunks': ((85, 1024, 45), (2 * math.pi, 8 * math.pi, 1.1 * math.pi))}
 NIT = 10
 RHO, DT = 1.0, 1e-3
 
 
-def source(seed=3):
-    a = np.random.Generator(np.random.PCG64(seed)).standard_normal(SHAPE)
+def source(shape, seed=3):
+    a = np.random.Generator(np.random.PCG64(seed)).standard_normal(shape)
     return a - a.mean()
 
 
-@pytest.mark.parametrize('dtype,bar', [(torch.float64, 1e-10), (torch.float32, 1e-4)])
-def test_paired_path_against_oracle(dtype, bar):
+@pytest.mark.parametrize('case', list(SHAPES))
+@pytest.mark.parametrize('dtype,bar', [(torch.float64, 1e-10), (torch.float32, 1e-4)],
+                         ids=['f64', 'f32'])
+def test_tile_paths_against_oracle(case, dtype, bar):
     from pyflow_amd import _native
+    SHAPE, L = SHAPES[case]
     og = O.Grid(*SHAPE, Lx=L[0], Ly=L[1], Lz=L[2])
     box = _native.Box(*SHAPE, og.dx, og.dy, og.dz, dtype=dtype, device=torch.device('cuda'))
     P = tuple(n + 4 for n in SHAPE)
     DP = torch.zeros(P, dtype=dtype, device='cuda')
     p = torch.zeros(P, dtype=dtype, device='cuda')
-    src = torch.tensor(source(), dtype=dtype, device='cuda')
+    src = torch.tensor(source(SHAPE), dtype=dtype, device='cuda')
     box.residual_history_enable(NIT + 4)
     res, its, status = box.bicgstab(DP, p, src, RHO, DT, 1e-30, NIT)
     hg = np.array(box.residual_history())
@@ -49,14 +59,14 @@ def test_paired_path_against_oracle(dtype, bar):
     torch.set_num_threads(max(1, min(16, torch.get_num_threads())))
     fDP, fp = O.Field(og), O.Field(og)
     B = O.BiCGStab(og, RHO, DT, 1e-30, NIT)
-    B.solve(fDP, fp, torch.tensor(source()))
+    B.solve(fDP, fp, torch.tensor(source(SHAPE)))
     ho = np.array(B.history)
 
     assert its == NIT == B.iterations
     rel = np.abs(hg - ho) / ho
     err = rel_l2(cpu(DP), fDP.var.numpy())
-    print('%s: residual history max rel diff %.2e, solution rel L2 %.2e'
-          % (dtype, rel.max(), err))
+    print('%s %s: residual history max rel diff %.2e, solution rel L2 %.2e'
+          % (case, dtype, rel.max(), err))
     assert rel.max() <= bar
     assert err <= bar
     assert res == pytest.approx(hg[-1], rel=0, abs=0)
