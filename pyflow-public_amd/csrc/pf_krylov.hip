@@ -261,8 +261,9 @@ struct alignas(2 * sizeof(T)) Pair {
   T a, b;
 };
 
-// Output store of the tile kernels.  NT: non-temporal, used by the rotating
-// schedule of large grids, whose Krylov set does not fit the Infinity Cache
+// Output store of the tile kernels.  NT: non-temporal (NTS of the bodies),
+// used by the rotating schedule of large grids and by batched launches,
+// whose Krylov sets do not fit the Infinity Cache
 // (512^3 FP32: KA 674 -> 659, KB 230 -> 217, KC 691 -> 674 us); on the
 // short-chunk grids (128^3 FP64, whose set fits) it was slower (KA 20.6 ->
 // 22.5 us), the next kernel's reads no longer finding the data there.
@@ -286,7 +287,7 @@ __device__ __forceinline__ void store1(T* p, T v) {
 }
 
 // One cell per thread; PIPE selects the load schedule as in bicg_tile2_body.
-template <typename T, int MODE, int TZ, int TY, bool PIPE>
+template <typename T, int MODE, int TZ, int TY, bool PIPE, bool NTS = !PIPE>
 __device__ __forceinline__ void bicg_tile_kernel_body(
     Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,
     const double* __restrict__ part_in, double* __restrict__ part_out,
@@ -402,8 +403,8 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
       const T fzm = pl[ly + 1][lz], fzp = pl[ly + 1][lz + 2];
       const T af = lap7(L, fm, fc, fp, fym, fyp, fzm, fzp);
       if (MODE == kModeA) {
-        store1<!PIPE>(f_out + xo + col, fc);
-        store1<!PIPE>(af_out + xo + col, af);
+        store1<NTS>(f_out + xo + col, fc);
+        store1<NTS>(af_out + xo + col, af);
         acc0 += (double)af * (double)rhc;
       } else if (MODE == kModeB) {
         acc0 += (double)fc * (double)af;
@@ -412,9 +413,9 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
         // KC: fc = s, af = t = A s
         T xv = fma(pc, sc.ta, xc);                  // pressure.py:251
         xv = fma(fc, sc.to, xv);                    // pressure.py:270
-        store1<!PIPE>(f_out + xo + col, xv);
+        store1<NTS>(f_out + xo + col, xv);
         const T rnew = fma(af, sc.tmo, fc);         // pressure.py:274-275
-        store1<!PIPE>(af_out + xo + col, rnew);
+        store1<NTS>(af_out + xo + col, rnew);
         acc0 += (double)rnew * (double)rhc;
         acc1 = nmax(acc1, fabs((double)rnew));
       }
@@ -495,7 +496,7 @@ __device__ __forceinline__ void bicg_tile_kernel_body(
 // KC), so it waits on its own loads.  On long chunks, with four waves per
 // SIMD, this schedule still reaches the bandwidth of a plain stream, and
 // PIPE was slower there (512^3 FP32: KA 683 -> 735 us, KC 690 -> 828 us).
-template <typename T, int MODE, int TZ, int TY, bool PIPE>
+template <typename T, int MODE, int TZ, int TY, bool PIPE, bool NTS = !PIPE>
 __device__ __forceinline__ void bicg_tile2_body(
     Geom g, TileGrid tg, int nblk_in, KrylovCtl* __restrict__ ctl,
     const double* __restrict__ part_in, double* __restrict__ part_out,
@@ -663,8 +664,8 @@ __device__ __forceinline__ void bicg_tile2_body(
       const T af1 = lap7(L, fm.b, fc.b, fp.b, pl[ly][zl + 1], pl[ly + 2][zl + 1], fc.a,
                          pl[ly + 1][zl + 2]);
       if (MODE == kModeA) {
-        store2<!PIPE>(f_out + xo + col, fc);
-        store2<!PIPE>(af_out + xo + col, Pair<T>{af0, af1});
+        store2<NTS>(f_out + xo + col, fc);
+        store2<NTS>(af_out + xo + col, Pair<T>{af0, af1});
         acc0 += (double)af0 * (double)rhc.a;
         acc0 += (double)af1 * (double)rhc.b;
       } else if (MODE == kModeB) {
@@ -678,9 +679,9 @@ __device__ __forceinline__ void bicg_tile2_body(
         T xb = fma(pc.b, sc.ta, xc.b);
         xa = fma(fc.a, sc.to, xa);
         xb = fma(fc.b, sc.to, xb);
-        store2<!PIPE>(f_out + xo + col, Pair<T>{xa, xb});
+        store2<NTS>(f_out + xo + col, Pair<T>{xa, xb});
         const T ra = fma(af0, sc.tmo, fc.a), rb = fma(af1, sc.tmo, fc.b);
-        store2<!PIPE>(af_out + xo + col, Pair<T>{ra, rb});
+        store2<NTS>(af_out + xo + col, Pair<T>{ra, rb});
         acc0 += (double)ra * (double)rhc.a;
         acc0 += (double)rb * (double)rhc.b;
         acc1 = nmax(acc1, fabs((double)ra));
@@ -1073,12 +1074,14 @@ __global__ void __launch_bounds__(PAIR ? TZ / 2 * TY : TZ * TY) batch_tile_kerne
   const double* pin = MODE == kModeA ? d.pC : (MODE == kModeB ? d.pA : d.pB);
   double* pout = MODE == kModeA ? d.pA : (MODE == kModeB ? d.pB : d.pC);
   const T* r = d.rk[parity];
-  // served boxes are small (short x chunks): the one-plane-ahead schedule
+  // served boxes are small (short x chunks): the one-plane-ahead schedule;
+  // non-temporal stores, since the boxes of a batch together outgrow the
+  // Infinity Cache (48-request serving benchmark: 243.2 -> 246.2 Mcell-steps/s)
   if (PAIR)
-    bicg_tile2_body<T, MODE, TZ, TY, true>(d.g, d.tg, d.tg.nblk, d.ctl, pin, pout, r, d.rh, a_old,
+    bicg_tile2_body<T, MODE, TZ, TY, true, true>(d.g, d.tg, d.tg.nblk, d.ctl, pin, pout, r, d.rh, a_old,
                                      v_in, f_out, af_out, d.L, blk);
   else
-    bicg_tile_kernel_body<T, MODE, TZ, TY, true>(d.g, d.tg, d.tg.nblk, d.ctl, pin, pout, r, d.rh,
+    bicg_tile_kernel_body<T, MODE, TZ, TY, true, true>(d.g, d.tg, d.tg.nblk, d.ctl, pin, pout, r, d.rh,
                                            a_old, v_in, f_out, af_out, d.L, blk);
 }
 
