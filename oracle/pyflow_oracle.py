@@ -438,9 +438,10 @@ def mlp_closure_forward(g, fu, fv, fw, params, const=None):
 
     The MLP has no reference implementation (it replaces the arithmetic of
     sfsmodel_gradient.py:77-148 behind the same interface); this FP64 torch
-    version is its oracle.  parity unpinned against the reference; pinned by
-    the closed-form check that zero weights give zero stresses and by
-    autograd-consistency tests.
+    version is its oracle.  Pinned by closed forms (tests/test_oracle_golden:
+    zero weights with output biases; selection weights that fix the feature
+    order, the orientation of each weight matrix and the dx^2/12 scale) and
+    by autograd-consistency tests.
     """
     grad_vel_center(g, fu, 'u')
     grad_vel_center(g, fv, 'v')

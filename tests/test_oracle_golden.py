@@ -233,3 +233,51 @@ def test_smagorinsky_closed_forms():
     expect = 1.3 * (0.17 * g.dx) ** 2 * a / np.sqrt(2.0)
     inner = (slice(3, -3),) * 3
     assert np.allclose(mut[inner].numpy(), expect, rtol=1e-12, atol=0)
+
+
+def test_mlp_closure_closed_forms(G):
+    """The closure MLP has no reference implementation; its oracle is pinned
+    by weights for which the network has a closed form.  (1) Zero weights
+    with output biases: every stress is its bias times dx^2/12.  (2) Weights
+    that select: hidden unit j of the first layer reads feature perm[j] (a
+    9-cycle, so a transposed matrix reads perm^-1) plus a bias, unit j of
+    the second reads unit q[j] of the first times a (another 9-cycle), the
+    third reads unit m times c and, for m < 3,
+    unit m + 6 times c*d, so that every feature reaches a stress with a
+    weight of its own: stress m is (c*T(m) + c*d*T(m+6)) * dx^2/12 with
+    T(j) = tanh(a*tanh(f[perm[q[j]]] + b1[q[j]])).  The features are the nine
+    cell-centred gradients in the order u_x, u_y, u_z, v_x, ..., w_z, from
+    the oracle's grad_vel_center (pinned by the gradient model's vectors)."""
+    g = grid(G)
+    fu, fv, fw = fields(g, G)
+    scale = g.dx * g.dx / 12.0
+    z = lambda *s: np.zeros(s)   # noqa: E731
+    b3 = np.array([0.3, -0.2, 0.1, 0.05, -0.4, 0.25])
+    out = O.mlp_closure_forward(g, fu, fv, fw, [z(20, 9), z(20), z(20, 20), z(20), z(6, 20), b3])
+    for m in range(6):
+        assert torch.allclose(out[m], torch.full_like(out[m], b3[m] * scale), rtol=0, atol=1e-15)
+
+    perm = [(j + 4) % 9 for j in range(9)]
+    W1 = z(20, 9)
+    for j in range(9):
+        W1[j, perm[j]] = 1.0
+    b1 = np.concatenate([np.linspace(-0.3, 0.3, 9), z(11)])
+    a, c, d = 0.7, 1.9, -0.45
+    q = [(j + 2) % 9 for j in range(9)]
+    W2 = z(20, 20)
+    for j in range(9):
+        W2[j, q[j]] = a
+    W3 = z(6, 20)
+    for m in range(6):
+        W3[m, m] = c
+        if m < 3:
+            W3[m, m + 6] = c * d
+    out = O.mlp_closure_forward(g, fu, fv, fw, [W1, b1, W2, z(20), W3, z(6)])
+    feats = []
+    for f, comp in ((fu, 'u'), (fv, 'v'), (fw, 'w')):
+        O.grad_vel_center(g, f, comp)
+        feats += [f.grad_x.numpy().copy(), f.grad_y.numpy().copy(), f.grad_z.numpy().copy()]
+    T = [np.tanh(a * np.tanh(feats[perm[q[j]]] + b1[q[j]])) for j in range(9)]
+    for m in range(6):
+        want = c * T[m] + (c * d * T[m + 6] if m < 3 else 0.0)
+        np.testing.assert_allclose(out[m].numpy(), want * scale, rtol=1e-13, atol=1e-15)
