@@ -7,7 +7,11 @@ do not reach:
 * 136 x 1024 x 64: the same tiles on chunks of 17 planes - the rotating
   load schedule and non-temporal stores of long chunks (512^3 takes it);
 * 85 x 1024 x 45: one cell per thread (45 is no multiple of the 16-wide
-  tile), chunks of 17 planes - the rotating schedule of that kernel.
+  tile), chunks of 17 planes - the rotating schedule of that kernel;
+* 91 x 256 x 64 (paired) and 38 x 19 x 45 (one cell per thread): chunks of
+  5 planes, an odd count that leaves the two-fold unrolled loop of the
+  one-plane-ahead schedule one plane to do on its own, and a last chunk of
+  1 and 3 planes.
 
 Ten BiCGStab iterations (pressure.py:229-290) on a random mean-free source
 against the oracle's ten: the per-iteration relative residual and the
@@ -30,7 +34,9 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = {'paired-short-chunks': ((128, 128, 64), (2 * math.pi, 2 * math.pi, math.pi)),
           'paired-long-chunks': ((136, 1024, 64), (2 * math.pi, 8 * math.pi, math.pi)),
-          'one-cell-long-chunks': ((85, 1024, 45), (2 * math.pi, 8 * math.pi, 1.1 * math.pi))}
+          'one-cell-long-chunks': ((85, 1024, 45), (2 * math.pi, 8 * math.pi, 1.1 * math.pi)),
+          'paired-odd-ragged-chunks': ((91, 256, 64), (2 * math.pi, 4 * math.pi, math.pi)),
+          'one-cell-odd-ragged-chunks': ((38, 19, 45), (2 * math.pi, 1.1 * math.pi, 2.4 * math.pi))}
 NIT = 10
 RHO, DT = 1.0, 1e-3
 
