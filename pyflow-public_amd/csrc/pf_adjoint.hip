@@ -441,8 +441,24 @@ __global__ void __launch_bounds__(64) closure_vjp_local_kernel(
   const int nxo = g.nx + 2 * kNover, nyo = g.ny + 2 * kNover, nzo = g.nz + 2 * kNover;
   const int64_t total = (int64_t)nxo * nyo * nzo;
   const int lane = threadIdx.x;
-  double acc[(kNp + 63) / 64];
-  for (int s = 0; s < (kNp + 63) / 64; ++s) acc[s] = 0.0;
+  constexpr int kNs = (kNp + 63) / 64;
+  double acc[kNs];
+  // LDS row of a cell: x (9), h1, h2, z1bar, z2bar (20 each), obar (6), 1
+  constexpr int kSx = 0, kSh1 = 9, kSh2 = 29, kSz1 = 49, kSz2 = 69, kSob = 89, kSone = 95;
+  __shared__ float sv[PGRAD ? 64 : 1][96];
+  // the two row entries whose product is parameter t * 64 + lane's term
+  int ia[kNs], ib[kNs];
+#pragma unroll
+  for (int t = 0; t < kNs; ++t) {
+    acc[t] = 0.0;
+    const int p = t * 64 + lane;
+    if (p < kB1) { ia[t] = kSz1 + p / 9; ib[t] = kSx + p % 9; }
+    else if (p < kW2) { ia[t] = kSz1 + (p - kB1); ib[t] = kSone; }
+    else if (p < kB2) { ia[t] = kSz2 + (p - kW2) / 20; ib[t] = kSh1 + (p - kW2) % 20; }
+    else if (p < kW3) { ia[t] = kSz2 + (p - kB2); ib[t] = kSone; }
+    else if (p < kB3) { ia[t] = kSob + (p - kW3) / 20; ib[t] = kSh2 + (p - kW3) % 20; }
+    else { ia[t] = kSob + (p < kNp ? p - kB3 : 0); ib[t] = kSone; }
+  }
   for (int64_t base = (int64_t)blockIdx.x * 64; base < total; base += (int64_t)gridDim.x * 64) {
     const int64_t q = base + lane;
     const bool valid = q < total;
@@ -498,36 +514,52 @@ __global__ void __launch_bounds__(64) closure_vjp_local_kernel(
         for (int d = 0; d < 3; ++d) gbar[(int64_t)(3 * c + d) * total + q] = Gb[c][d];
     }
     if (KIND == 1 && PGRAD) {
-      // parameter gradients: wave sums of per-cell outer products
-      for (int p = 0; p < kNp; ++p) {
-        float v;
-        if (p < kB1) v = z1b[p / 9] * mc.x[p % 9];
-        else if (p < kW2) v = z1b[p - kB1];
-        else if (p < kB2) v = z2b[(p - kW2) / 20] * mc.h1[(p - kW2) % 20];
-        else if (p < kW3) v = z2b[p - kB2];
-        else if (p < kB3) v = ob[(p - kW3) / 20] * mc.h2[(p - kW3) % 20];
-        else v = ob[p - kB3];
-        const double sv = wave_sum((double)v);
-        if (lane == (p & 63)) acc[p >> 6] += sv;
+      // parameter gradients: sums over the wave's 64 cells of per-cell outer
+      // products (float products, double sums).  The per-cell vectors go to
+      // LDS, and lane l sums parameters l, l + 64, ... over the cells in a
+      // fixed order.
+      float* row = sv[lane];
+      for (int f = 0; f < 9; ++f) row[kSx + f] = mc.x[f];
+      for (int r = 0; r < 20; ++r) {
+        row[kSh1 + r] = mc.h1[r];
+        row[kSh2 + r] = mc.h2[r];
+        row[kSz1 + r] = z1b[r];
+        row[kSz2 + r] = z2b[r];
       }
+      for (int m = 0; m < 6; ++m) row[kSob + m] = ob[m];
+      row[kSone] = 1.f;
+      __syncthreads();   // the block is one wave
+#pragma unroll
+      for (int t = 0; t < kNs; ++t) {
+        if (t * 64 + lane < kNp) {
+          double sum = 0.0;
+#pragma unroll 8
+          for (int c = 0; c < 64; ++c) sum += (double)(sv[c][ia[t]] * sv[c][ib[t]]);
+          acc[t] += sum;
+        }
+      }
+      __syncthreads();   // before the next cells overwrite the rows
     }
   }
   if (KIND == 1 && PGRAD) {
-    for (int s = 0; s < (kNp + 63) / 64; ++s) {
-      const int p = s * 64 + lane;
-      if (p < kNp) wpart[(int64_t)blockIdx.x * kNp + p] = acc[s];
+#pragma unroll
+    for (int t = 0; t < kNs; ++t) {
+      const int p = t * 64 + lane;
+      if (p < kNp) wpart[(int64_t)blockIdx.x * kNp + p] = acc[t];
     }
   }
 }
 
-// dtheta[p] += sum over waves of wpart[w][p] (fixed order)
+// dtheta[p] += sum over waves of wpart[w][p] (fixed order): one block per
+// parameter, its threads striding over the waves, then a block reduction.
 __global__ void __launch_bounds__(kBlock) mlp_param_reduce_kernel(const double* __restrict__ wpart,
                                                                   int nw, double* __restrict__ dth) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kNp) return;
-  double s = 0.0;
-  for (int w = 0; w < nw; ++w) s += wpart[(int64_t)w * kNp + p];
-  dth[p] += s;
+  const int p = blockIdx.x;
+  double v[1] = {0.0};
+  for (int w = threadIdx.x; w < nw; w += kBlock) v[0] += wpart[(int64_t)w * kNp + p];
+  const int ops[1] = {0};
+  block_reduce<1, kBlock>(v, ops);
+  if (threadIdx.x == 0) dth[p] += v[0];
 }
 
 // Reverse of the feature stencil (grad_center): fbar_c(p) for the padded
@@ -631,8 +663,8 @@ static int closure_vjp_impl(pf_box* h, int kind, const void* const U[3], const f
                          v, w, dxi, dyi, dzi, cst, tb, params, (T*)gbar9, wpart);
     PF_LAUNCH_CHECK("closure_vjp_local_kernel");
     if (kind == 1 && dparams) {
-      hipLaunchKernelGGL(mlp_param_reduce_kernel, dim3((kNp + kBlock - 1) / kBlock), dim3(kBlock),
-                         0, s, wpart, nw, dparams);
+      hipLaunchKernelGGL(mlp_param_reduce_kernel, dim3(kNp), dim3(kBlock), 0, s, wpart, nw,
+                         dparams);
       PF_LAUNCH_CHECK("mlp_param_reduce_kernel");
     }
     Vec3<T> fb = {{(T*)F[0], (T*)F[1], (T*)F[2]}};
