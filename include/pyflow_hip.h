@@ -23,9 +23,13 @@
  *     reference's .cpu() reductions.
  *   - Every kernel that writes a padded field also fills its overlap cells
  *     with periodic images, as state_P.update_border does.
- *   - No entry point allocates device memory except pf_box_create; all
- *     workspace belongs to the box.  A box may be used by one host thread
- *     and one stream at a time.
+ *   - Device memory is allocated only by pf_box_create and by the set-up
+ *     calls that reserve optional workspace in a box (pf_mpir_configure,
+ *     pf_sfs_ml_source_reserve, pf_residual_history_enable; pf_profile_enable
+ *     creates events).  Every other entry point allocates nothing and fails
+ *     with PF_EINVAL when the workspace it needs was not reserved, so a box's
+ *     pf_box_device_bytes is final once its set-up calls are done.  A box may
+ *     be used by one host thread and one stream at a time.
  */
 #ifndef PYFLOW_HIP_H
 #define PYFLOW_HIP_H
@@ -205,12 +209,17 @@ int pf_rk4_stage_src(pf_box* box, int stage, void* u, void* v, void* w, void* ut
  * params: device FP32, pf_sfs_ml_source_params() values: torch.nn.Linear
  * layout (weight[out][in] then bias[out]) of fc1, fcG, fcG2 (273 -> 200),
  * fc2, fc3 (200 -> 200), fc4 (200 -> 18), in that order.  norm21: host, the
- * 21 FP32 normalisation constants (sfsmodel_ML.py:252-258).  A workspace of
- * 30 floats per padded cell is allocated in the box on first use.
+ * 21 FP32 normalisation constants (sfsmodel_ML.py:252-258).  Needs the
+ * box's workspace of 30 floats per padded cell, reserved once by
+ * pf_sfs_ml_source_reserve (PF_EINVAL otherwise).
  */
 int pf_sfs_ml_source(pf_box* box, const float* params, const float* norm21, const void* ui,
                      const void* vi, const void* wi, void* gx, void* gy, void* gz, void* stream);
 int64_t pf_sfs_ml_source_params(void);
+/* Reserve the closure's workspace in the box (set-up call, allocates; made
+ * where residual_stress.__init__ allocates its workspace arrays,
+ * sfsmodel_ML.py:145-151).  Idempotent. */
+int pf_sfs_ml_source_reserve(pf_box* box);
 
 /*
  * Smagorinsky eddy viscosity plus the molecular one, written to the padded
@@ -259,7 +268,8 @@ int pf_bicgstab_solve(pf_box* box, void* DP, void* p, void* source,
  * re-run up to its best iterate (the solves are deterministic), so a
  * degrading inner recurrence costs iterations, not accuracy.  DP := x rounded to the box precision, p += DP.  `source` ends
  * holding the last inner right-hand side.  *out_iters counts inner
- * iterations.  Allocates 16 bytes per cell on first use.
+ * iterations.  Uses the FP64 workspace (16 bytes per cell) that
+ * pf_mpir_configure reserves; PF_EINVAL if it was never called on the box.
  * pf_vel_corr with solver = PF_POISSON_BICGSTAB_MPIR runs the same solve
  * (with b from div(u) computed in FP64).
  */
@@ -272,7 +282,10 @@ int pf_bicgstab_mpir_solve(pf_box* box, void* DP, void* p, void* source,
  * outer_maxit 100).  Short inner solves keep each BiCGStab run well before
  * the point where its recurrence degrades (measured on a 512^3 FP32 solve to
  * 1e-8: inner tolerance 1e-1 took 1797 inner iterations, 1e-2 took 2309, and
- * inner solves run to 3e-3 or beyond diverged).
+ * inner solves run to 3e-3 or beyond diverged).  Set-up call: reserves the
+ * refinement workspace (FP64 iterate and right-hand side, 16 bytes per cell,
+ * and an inner residual history of inner_maxit doubles) so that the solves
+ * allocate nothing.
  */
 int pf_mpir_configure(pf_box* box, double inner_tol, int inner_maxit, int outer_maxit);
 

@@ -1515,7 +1515,10 @@ __global__ void __launch_bounds__(kBlock) mp_finish_kernel(Geom g, const double*
     out[idx] = (T)x[idx];
 }
 
-int mp_buffers(pf_box* h) {
+// FP64 iterate, right-hand side and inner residual history of the
+// refinement solver, reserved by pf_mpir_configure (a set-up call) so that
+// the solve itself allocates nothing.
+int mp_reserve(pf_box* h) {
   if (h->mp_hist_cap < h->mp_inner_maxit) {
     if (h->mp_hist) {
       PF_HIP(hipFree(h->mp_hist));
@@ -1554,7 +1557,9 @@ int mp_buffers(pf_box* h) {
 template <typename T>
 int mp_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, const T* W,
             double src_scale, double tol, int maxit, double* out_res, int* out_iters) {
-  if (int rc = mp_buffers(h)) return rc;
+  if (!h->mp_x || !h->mp_b || h->mp_hist_cap < h->mp_inner_maxit)
+    return fail(PF_EINVAL, "refinement solve: workspace not reserved (call pf_mpir_configure "
+                           "on the box first)");
   const Geom& g = h->g;
   const int nb = h->nblk;
   const double vol = h->dx * h->dy * h->dz;
@@ -1724,7 +1729,7 @@ int pf_mpir_configure(pf_box* h, double inner_tol, int inner_maxit, int outer_ma
   h->mp_inner_tol = inner_tol;
   h->mp_inner_maxit = inner_maxit;
   h->mp_outer_maxit = outer_maxit;
-  return 0;
+  return mp_reserve(h);
 }
 
 int pf_jacobi_solve(pf_box* h, void* pOld, void* p, void* source, double rho, double dt, int nit,

@@ -414,6 +414,13 @@ __global__ void __launch_bounds__(kBlock) src_divergence_kernel(SrcGeom s,
   }
 }
 
+// Workspace of one evaluation: 21 normalised features and 9 network outputs
+// per padded cell, FP32.
+inline size_t src_ws_need(const pf_box* h) {
+  const size_t m = (size_t)(h->nx + 2 * kNover) * (h->ny + 2 * kNover) * (h->nz + 2 * kNover);
+  return (size_t)(kSrcFeat + kSrcUsed) * m * sizeof(float);
+}
+
 template <typename T>
 int ml_source_impl(pf_box* h, const float* prm, const Norm21& nrm, const void* const U[3],
                    void* const G[3], hipStream_t st) {
@@ -424,22 +431,9 @@ int ml_source_impl(pf_box* h, const float* prm, const Norm21& nrm, const void* c
   s.m = (int64_t)s.nxo * s.nyo * s.nzo;
   s.d2 = (float)(2.0 * h->dx);
   s.dd = (float)(h->dx * h->dx);
-  const size_t need = (size_t)(kSrcFeat + kSrcUsed) * (size_t)s.m * sizeof(float);
-  if (h->src_ws_bytes < need) {
-    if (h->src_ws) {
-      PF_HIP(hipFree(h->src_ws));
-      h->bytes -= (int64_t)h->src_ws_bytes;
-      h->src_ws = nullptr;
-      h->src_ws_bytes = 0;
-    }
-    hipError_t e = hipMalloc((void**)&h->src_ws, need);
-    if (e != hipSuccess) {
-      h->src_ws = nullptr;
-      return check_hip(e, "hipMalloc");
-    }
-    h->src_ws_bytes = need;
-    h->bytes += (int64_t)need;
-  }
+  if (h->src_ws_bytes < src_ws_need(h))
+    return fail(PF_EINVAL, "pf_sfs_ml_source: workspace not reserved (call "
+                           "pf_sfs_ml_source_reserve on the box first)");
   float* Fn = h->src_ws;
   float* P = Fn + (size_t)kSrcFeat * s.m;
   int64_t nb = (s.m + kBlock - 1) / kBlock;
@@ -488,5 +482,25 @@ int pf_sfs_ml_source(pf_box* h, const float* params, const float* norm21, const 
 }
 
 int64_t pf_sfs_ml_source_params(void) { return kSrcParams; }
+
+int pf_sfs_ml_source_reserve(pf_box* h) {
+  if (int rc = check_box(h)) return rc;
+  const size_t need = src_ws_need(h);
+  if (h->src_ws_bytes >= need) return 0;
+  if (h->src_ws) {
+    PF_HIP(hipFree(h->src_ws));
+    h->bytes -= (int64_t)h->src_ws_bytes;
+    h->src_ws = nullptr;
+    h->src_ws_bytes = 0;
+  }
+  hipError_t e = hipMalloc((void**)&h->src_ws, need);
+  if (e != hipSuccess) {
+    h->src_ws = nullptr;
+    return check_hip(e, "hipMalloc");
+  }
+  h->src_ws_bytes = need;
+  h->bytes += (int64_t)need;
+  return 0;
+}
 
 }  // extern "C"

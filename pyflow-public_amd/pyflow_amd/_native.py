@@ -94,6 +94,7 @@ SIGNATURES = [
     ('pf_rk4_stage_src', _I, [_P, _I, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P]),
     ('pf_sfs_ml_source', _I, [_P, _P, ctypes.POINTER(ctypes.c_float), _P, _P, _P, _P, _P, _P, _P]),
     ('pf_sfs_ml_source_params', ctypes.c_int64, []),
+    ('pf_sfs_ml_source_reserve', _I, [_P]),
 ]
 
 
@@ -325,6 +326,10 @@ class Box:
                                      self.padded(wi, 'w'), *g, self.stream()),
               'pf_sfs_ml_source')
 
+    def sfs_ml_source_reserve(self):
+        """Reserve the ML closure's workspace in the box (allocates once)."""
+        check(lib().pf_sfs_ml_source_reserve(self.h), 'pf_sfs_ml_source_reserve')
+
     def sfs_smagorinsky(self, u, v, w, rho, Cs, Delta, mu, visc):
         check(lib().pf_sfs_smagorinsky(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
                                        self.padded(w, 'w'), float(rho), float(Cs), float(Delta),
@@ -355,6 +360,8 @@ class Box:
         return res.value, its.value, st
 
     def mpir_configure(self, inner_tol=1e-1, inner_maxit=500, outer_maxit=100):
+        """Refinement settings; reserves the refinement workspace (allocates),
+        which bicgstab_mpir and vel_corr with the refinement solver need."""
         check(lib().pf_mpir_configure(self.h, float(inner_tol), int(inner_maxit), int(outer_maxit)),
               'pf_mpir_configure')
 

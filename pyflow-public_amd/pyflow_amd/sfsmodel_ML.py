@@ -83,6 +83,8 @@ class residual_stress:
         self.GX = torch.zeros(shape, dtype=self.prec, device=self.device)
         self.GY = torch.zeros(shape, dtype=self.prec, device=self.device)
         self.GZ = torch.zeros(shape, dtype=self.prec, device=self.device)
+        # the kernels' feature and network-output workspace, reserved once
+        self.box.sfs_ml_source_reserve()
         if geo.type != 'uniform':
             raise Exception('sfsmodel_ML: geometry type not implemented')
         self.dx = geo.dx

@@ -150,3 +150,29 @@ def test_run_loop_with_ml_closure():
           % (diff, g, 3 * 2e-3 * g, r_ml.history[-1]['maxDiv']))
     assert 0.0 < diff < 30 * 2e-3 * g
     assert r_ml.history[-1]['max_resP'] <= 1e-10
+
+
+def test_closure_workspace_is_reserved_at_set_up():
+    """pf_sfs_ml_source allocates nothing: the closure reserves its workspace
+    (30 floats per padded cell) when it is built, a box without it is refused,
+    and the box's device bytes stay the same across evaluations."""
+    from pyflow_amd import _native
+    n = 12
+    h = 2 * math.pi / n
+    flat, _ = ml_source_params()
+    params = torch.tensor(flat, dtype=torch.float32, device='cuda')
+    U = [torch.randn((n + 4,) * 3, dtype=torch.float32, device='cuda') for _ in range(3)]
+    Gs = [torch.zeros((n + 4,) * 3, dtype=torch.float32, device='cuda') for _ in range(3)]
+    from pyflow_amd.sfsmodel_ML import NORM
+    bare = _native.Box(n, n, n, h, h, h, dtype=torch.float32)
+    with pytest.raises(RuntimeError, match='pf_sfs_ml_source_reserve'):
+        bare.sfs_ml_source(params, NORM, *U, Gs)
+    d, g, m = gpu_env((n, n, n), (2 * math.pi,) * 3, torch.float32)
+    before = d.box.device_bytes
+    ml = closure(d, g, flat)
+    reserved = d.box.device_bytes
+    assert reserved == before + 30 * 4 * (n + 4) ** 3
+    for _ in range(2):
+        ml.update(*U)
+        torch.cuda.synchronize()
+        assert d.box.device_bytes == reserved

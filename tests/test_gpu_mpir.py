@@ -39,7 +39,9 @@ def lap_fp64(x, h):
 def make_box(N, dtype):
     from pyflow_amd import _native
     h = 2 * math.pi / N
-    return _native.Box(N, N, N, h, h, h, dtype=dtype), h
+    box = _native.Box(N, N, N, h, h, h, dtype=dtype)
+    box.mpir_configure()
+    return box, h
 
 
 def zero_mean_source(N, dtype, seed):
@@ -158,3 +160,26 @@ def test_unconverged_inner_solves_roll_back_to_best_iterate():
     print('status %d, %d inner iterations, returned %.3e, recomputed %.3e' % (st, its, res, true_res))
     assert st == 0 and res <= 1e-10
     assert abs(true_res - res) <= 1e-3 * res + 1e-14
+
+
+def test_refinement_workspace_is_reserved_at_set_up():
+    """The refinement solve allocates nothing: it needs the workspace that
+    pf_mpir_configure reserves, and the box's device bytes do not change
+    across solves (serving admission counts them once)."""
+    from pyflow_amd import _native
+    N = 24
+    h = 2 * math.pi / N
+    box = _native.Box(N, N, N, h, h, h, dtype=torch.float64)
+    src = zero_mean_source(N, torch.float64, 9)
+    DP = torch.zeros(N + 4, N + 4, N + 4, dtype=torch.float64, device='cuda')
+    p = torch.zeros_like(DP)
+    with pytest.raises(RuntimeError, match='pf_mpir_configure'):
+        box.bicgstab_mpir(DP, p, src.clone(), 1.0, 1e-3, 1e-10, 5000)
+    before = box.device_bytes
+    box.mpir_configure()
+    reserved = box.device_bytes
+    assert reserved == before + 16 * N ** 3 + 8 * 500
+    for _ in range(2):
+        res, its, st = box.bicgstab_mpir(DP, p, src.clone(), 1.0, 1e-3, 1e-10, 5000)
+        assert st == 0 and res <= 1e-10
+        assert box.device_bytes == reserved
