@@ -142,6 +142,26 @@ int pf_euler_step(pf_box* box, void* u, void* v, void* w,
                   double dt, double mu, double rho, const void* const* sfs6, void* stream);
 
 /*
+ * pf_rk4_step without a box and without any allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops or harnesses that forbid hipMalloc).  The
+ * box is described by its sizes, spacings and dtype; u,v,w,ut,vt,wt (padded)
+ * and sfs6 mean what they mean for pf_rk4_step and receive the same bits.
+ * The step's workspace (3 interior + 3 padded fields of the dtype, each
+ * rounded up to 256 bytes) lives in a fixed per-GPU device array of
+ * pf_rk4_step_noalloc_max_bytes() bytes (64 MiB: up to 96^3 in FP64 or
+ * 128^3 in FP32) that each GPU the library runs on holds until the process
+ * ends; larger boxes are rejected with PF_EINVAL before any GPU work.  All
+ * calls on one GPU share that array: calls that can overlap on one GPU
+ * (other streams, other threads) race and must be ordered by the caller;
+ * calls on one stream are ordered by it.
+ */
+int pf_rk4_step_noalloc(int nx, int ny, int nz, double dx, double dy, double dz, int dtype,
+                        void* u, void* v, void* w, void* ut, void* vt, void* wt, double dt,
+                        double mu, double rho, const void* const* sfs6, void* stream);
+int64_t pf_rk4_step_noalloc_max_bytes(void);
+
+/*
  * One evaluation of the Navier-Stokes RHS into interior arrays ru, rv, rw.
  * Replaces rhs_NavierStokes.evaluate (solver/velocity.py:221-383).
  */

@@ -11,6 +11,9 @@
 // contractions.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <mutex>
+
 #include "pf_kernels.h"
 
 namespace pf {
@@ -1151,6 +1154,115 @@ int pf_adj_euler_step_visc(pf_box* h, const void* u, const void* v, const void* 
   if (!visc) return fail(PF_EINVAL, "pf_adj_euler_step_visc: NULL viscosity field");
   return adj_euler_entry(h, u, v, w, ua, va, wa, dt, 0.0, visc, rho, closure, params, work,
                          delta3, dparams, stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free RK4 step (pf_rk4_step_noalloc).  Its workspace -- the three
+// interior accumulators k1 + 2k2 + 2k3 and the second buffer of the stage
+// states (three padded fields) that a box otherwise owns -- lives in one
+// file-scope device array, so the call needs neither a box nor hipMalloc.
+// Each GPU holds its own copy of the array for the life of the process; its
+// device address is looked up once per device (hipGetSymbolAddress) and kept.
+// Nothing in it needs clearing: stage 1 writes every accumulator cell before
+// stages 2-4 read them, and stage 2 writes every cell of the second buffer
+// (interior and periodic images) before stage 3 reads it.
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)64 << 20;   // 64 MiB per GPU
+constexpr size_t kScratchAlign = 256;
+constexpr int kScratchMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchMaxDevices];   // device address of `scratch`, per device
+
+size_t scratch_round(size_t n) { return (n + kScratchAlign - 1) / kScratchAlign * kScratchAlign; }
+
+// Bytes of scratch one step needs: 3 interior + 3 padded fields.
+size_t rk4_scratch_need(int nx, int ny, int nz, size_t esize) {
+  const size_t ni = (size_t)nx * ny * nz * esize;
+  const size_t np = (size_t)(nx + 4) * (ny + 4) * (nz + 4) * esize;
+  return 3 * scratch_round(ni) + 3 * scratch_round(np);
+}
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device, with that device current).
+int scratch_on_current_device(unsigned char** out) {
+  int dev = -1;
+  PF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchMaxDevices)
+    return fail(PF_EINVAL, "pf_rk4_step_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return check_hip(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(PF_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = (unsigned char*)g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t pf_rk4_step_noalloc_max_bytes(void) { return (int64_t)kScratchBytes; }
+
+int pf_rk4_step_noalloc(int nx, int ny, int nz, double dx, double dy, double dz, int dtype,
+                        void* u, void* v, void* w, void* ut, void* vt, void* wt, double dt,
+                        double mu, double rho, const void* const* sfs6, void* stream) {
+  // every argument is checked before the first HIP call
+  if (nx < 4 || ny < 4 || nz < 4)
+    return fail(PF_EINVAL, "pf_rk4_step_noalloc: every dimension must be >= 4 (two overlap cells)");
+  if ((int64_t)nx * ny * nz >= (1ll << 31))
+    return fail(PF_EINVAL, "pf_rk4_step_noalloc: nx*ny*nz must be < 2^31");
+  if (dtype != PF_F64 && dtype != PF_F32) return fail(PF_EINVAL, "pf_rk4_step_noalloc: bad dtype");
+  if (!(dx > 0) || !(dy > 0) || !(dz > 0))
+    return fail(PF_EINVAL, "pf_rk4_step_noalloc: spacings must be > 0");
+  void* U[3] = {u, v, w};
+  void* Ut[3] = {ut, vt, wt};
+  if (int rc = check_ptrs((const void* const*)U, 3, "pf_rk4_step_noalloc")) return rc;
+  if (int rc = check_ptrs((const void* const*)Ut, 3, "pf_rk4_step_noalloc")) return rc;
+  if (sfs6)
+    if (int rc = check_ptrs(sfs6, 6, "pf_rk4_step_noalloc(sfs6)")) return rc;
+  if (!(rho != 0.0)) return fail(PF_EINVAL, "pf_rk4_step_noalloc: rho must be nonzero");
+  const size_t esize = dtype == PF_F64 ? 8 : 4;
+  const size_t need = rk4_scratch_need(nx, ny, nz, esize);
+  if (need > kScratchBytes)
+    return fail(PF_EINVAL, "pf_rk4_step_noalloc: a " + std::to_string(nx) + "x" +
+                               std::to_string(ny) + "x" + std::to_string(nz) + " box needs " +
+                               std::to_string(need) + " bytes of scratch, more than the " +
+                               std::to_string(kScratchBytes) + " it has (use pf_rk4_step)");
+  unsigned char* base = nullptr;
+  if (int rc = scratch_on_current_device(&base)) return rc;
+
+  // A box view on the stack: geometry, spacing and the scratch as workspace.
+  pf_box b;
+  memset(&b, 0, sizeof(b));
+  b.nx = nx; b.ny = ny; b.nz = nz;
+  b.dx = dx; b.dy = dy; b.dz = dz;
+  b.dtype = dtype;
+  b.esize = esize;
+  b.g = make_geom(nx, ny, nz);
+  size_t off = 0;
+  const size_t ni = (size_t)nx * ny * nz * esize;
+  const size_t np = (size_t)(nx + 4) * (ny + 4) * (nz + 4) * esize;
+  for (int q = 0; q < 3; ++q) {
+    b.acc[q] = base + off;
+    off += scratch_round(ni);
+  }
+  for (int q = 0; q < 3; ++q) {
+    b.ypong[q] = base + off;
+    off += scratch_round(np);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  return dtype == PF_F64 ? rk4_impl<double>(&b, U, Ut, dt, mu, rho, sfs6, s)
+                         : rk4_impl<float>(&b, U, Ut, dt, mu, rho, sfs6, s);
 }
 
 }  // extern "C"

@@ -95,6 +95,9 @@ SIGNATURES = [
     ('pf_sfs_ml_source', _I, [_P, _P, ctypes.POINTER(ctypes.c_float), _P, _P, _P, _P, _P, _P, _P]),
     ('pf_sfs_ml_source_params', ctypes.c_int64, []),
     ('pf_sfs_ml_source_reserve', _I, [_P]),
+    ('pf_rk4_step_noalloc', _I, [_I, _I, _I, _D, _D, _D, _I, _P, _P, _P, _P, _P, _P, _D, _D, _D,
+                                 _PP, _P]),
+    ('pf_rk4_step_noalloc_max_bytes', ctypes.c_int64, []),
 ]
 
 

@@ -46,3 +46,24 @@ def test_abi_version_and_error_path_without_gpu():
     rc = L.pf_box_create(ctypes.byref(h), 2, 8, 8, 1.0, 1.0, 1.0, 0)
     assert rc < 0 and 'dimension' in _native.last_error()
     assert L.pf_fill_halo(None, None, None) < 0
+
+
+def test_noalloc_rk4_rejects_oversized_box_before_any_gpu_work():
+    """pf_rk4_step_noalloc checks its scratch limit before its first HIP call
+    (the pointers below are never dereferenced)."""
+    import torch  # noqa: F401
+    from pyflow_amd import _native
+    L = _native.lib()
+    cap = int(L.pf_rk4_step_noalloc_max_bytes())
+    assert cap == 64 << 20
+    fake = [ctypes.c_void_p(4096 * (q + 1)) for q in range(6)]
+    # 128^3 FP64 needs 3*(128^3 + 132^3)*8 bytes, more than the scratch holds
+    rc = L.pf_rk4_step_noalloc(128, 128, 128, 0.1, 0.1, 0.1, _native.PF_F64, *fake,
+                               1e-3, 0.01, 1.0, None, None)
+    assert rc == -1 and 'bytes of scratch' in _native.last_error()
+    rc = L.pf_rk4_step_noalloc(3, 8, 8, 0.1, 0.1, 0.1, _native.PF_F64, *fake,
+                               1e-3, 0.01, 1.0, None, None)
+    assert rc == -1 and 'dimension' in _native.last_error()
+    rc = L.pf_rk4_step_noalloc(8, 8, 8, 0.1, 0.1, 0.1, _native.PF_F64, *fake[:5], None,
+                               1e-3, 0.01, 1.0, None, None)
+    assert rc == -1 and 'NULL' in _native.last_error()
