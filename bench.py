@@ -7,14 +7,20 @@ a relative L-inf residual of 1e-10 -- the reference's forward loop
 (Domain.forwardStep + the per-step monitor values) through this package's
 PyFlow.setup.  A step is one full time step of that loop.
 
-Prints one JSON line.  `value` is Mcell-steps/s of the timed steps; the
-roofline object is for the dominant kernel (KA of BiCGStab) with its average
-launch duration measured during the timed steps with start/stop hipEvents
-recorded by the dispatch of every 32nd launch of each Krylov kernel (sampling
-keeps the events' own cost out of `value`); `cpu_baseline` times the CPU oracle (a
-torch-CPU FP64 restatement of the reference's op sequence) on a bounded
-sample of the same workload on this host, and `oracle_error` is the relative
-L2 error of the GPU velocity against the oracle's after those steps.
+Prints one JSON line.  `value` is Mcell-steps/s of the timed FP64 steps.
+The roofline object is for the Krylov kernel with the largest measured time
+(KA, KB and KC run once per iteration each; their average launch durations
+come from start/stop hipEvents recorded by the dispatch of every 32nd launch
+of each kernel during the timed steps, a sample that keeps the events' own
+cost out of `value`).  `cpu_baseline` times the CPU oracle (a torch-CPU FP64
+restatement of the reference's op sequence) on a bounded sample of the same
+workload on this host, and `oracle_error` is the relative L2 error of the GPU
+velocity against the oracle's after those steps.  The other legs of the
+metric ride along: `f32` (the same workload with FP32 storage and FP64
+Krylov reductions at tolerance 1e-8, its error against the FP64 oracle) and
+`serving` (a bounded Poisson stream of mixed-size requests served with
+continuous batching and one after another: throughput, p50/p99 step latency,
+requests with bit-identical results).
 """
 
 import argparse
@@ -31,13 +37,23 @@ for p in (ROOT, os.path.join(ROOT, 'pyflow-public_amd')):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+INFINITY_CACHE_BYTES = 256 << 20   # MI355X last-level (MALL) cache
 
-# HBM-side bytes per KA launch measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on this bench's workload (one pass per counter), each scaled by
-# the counter's reading for a plain 512 MiB device copy (FETCH 0.500,
-# WRITE 1.000): (38403.6 KiB / 0.500 + 32099.1 KiB) per dispatch.  Keyed by
-# (grid, dtype); other workloads report null.
-KA_TRAFFIC_BYTES = {(128, 'f64'): int((38403.6 / 0.50003 + 32099.1) * 1024)}
+# HBM-side bytes per launch of each Krylov kernel measured with rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE on this bench's default workload (one pass
+# per counter), each scaled by the counter's reading for a plain 512 MiB
+# device copy (FETCH 0.500, WRITE 1.000): FETCH/0.500 + WRITE KiB per
+# dispatch.  Keyed by (grid, dtype, kernel); other workloads report null.
+KRYLOV_TRAFFIC_BYTES = {
+    (128, 'f64', 'KA'): int((38403.6 / 0.50003 + 32099.1) * 1024),
+    (128, 'f64', 'KB'): int((18283.6 / 0.50003 + 15.7) * 1024),
+    (128, 'f64', 'KC'): int((44520.7 / 0.50003 + 32205.1) * 1024),
+}
+KERNEL_NAMES = {'KA': 'KA: p update, v = A p, <v, r_hat> (bicg_tile2_kernel mode 0)',
+                'KB': 'KB: s = r - alpha v, t = A s, <s,t>, <t,t> (bicg_tile2_kernel mode 1)',
+                'KC': 'KC: s, t = A s again, x and r updates, <r,r_hat>, max|r| '
+                      '(bicg_tile2_kernel mode 2)'}
+KERNEL_WORDS = {'KA': 6, 'KB': 2, 'KC': 7}   # algorithmic vector passes per launch
 
 
 def make_config(N, dtype, nu, tol, maxit, seed, solver='bicgstab'):
@@ -86,6 +102,57 @@ def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True, solver=
                 res=D.max_resP, status={k: status.count(k) for k in set(status)})
 
 
+def measure(N, dt_name, nu, tol, maxit, steps, warmup, seed, period, solver):
+    """Timed steps of one workload: throughput, per-kernel bandwidth and the
+    roofline of the Krylov kernel with the largest measured time."""
+    import torch
+    dtype = torch.float64 if dt_name == 'f64' else torch.float32
+    w = 8 if dt_name == 'f64' else 4
+    r = gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, period, solver)
+    ncell = N ** 3
+    value = ncell * steps / r['wall'] / 1e6
+    kbar = sum(r['its']) / len(r['its'])
+    kt = sum(r['its'])
+    # bytes per step: 65 vector passes outside BiCGStab plus 15 per iteration
+    # (what the kernels must move; KC recomputes s and t) -- and the 17 per
+    # iteration of a minimally fused step that stores s and t
+    alg = sum(ncell * w * (65 + 15 * k) for k in r['its'])
+    alg17 = sum(ncell * w * (65 + 17 * k) for k in r['its'])
+    out = {'value': round(value, 3), 'ms_per_step': round(r['wall'] / steps * 1e3, 4),
+           'bicgstab_iters_mean': round(kbar, 2),
+           'hbm_gbs_algorithmic': round(alg / r['wall'] / 1e9, 2),
+           'step_roofline_frac': round(alg / r['wall'] / 1e9 / HBM_PEAK_GBS, 4),
+           'step_roofline_frac_17w': round(alg17 / r['wall'] / 1e9 / HBM_PEAK_GBS, 4),
+           '_run': r}
+    krylov_set = 9 * ncell * w
+    out['infinity_cache'] = {'krylov_set_bytes': krylov_set,
+                             'cache_bytes': INFINITY_CACHE_BYTES,
+                             'set_fits': krylov_set <= INFINITY_CACHE_BYTES}
+    if r['prof']:
+        per = {}
+        for name in ('KA', 'KB', 'KC'):
+            tot, n = r['prof'][name]
+            if n:
+                avg = tot / n / 1e3
+                nbytes = KERNEL_WORDS[name] * w * ncell
+                per[name] = {'avg_us': round(avg * 1e6, 3), 'timed_launches': n,
+                             'launches': kt, 'est_total_ms': round(avg * kt * 1e3, 3),
+                             'bytes_per_launch': nbytes, 'GB/s': round(nbytes / avg / 1e9, 1)}
+        out['kernels'] = per
+        if per:
+            # every kernel runs once per iteration: the largest total is the
+            # largest average launch
+            top = max(per, key=lambda k: per[k]['est_total_ms'])
+            p = per[top]
+            out['roofline'] = {'kernel': KERNEL_NAMES[top], 'bound': 'hbm',
+                               'achieved': p['GB/s'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                               'frac': round(p['GB/s'] / HBM_PEAK_GBS, 4),
+                               'traffic': KRYLOV_TRAFFIC_BYTES.get((N, dt_name, top)),
+                               'bytes_per_launch': p['bytes_per_launch'],
+                               'chosen_by': 'largest measured total time of KA, KB, KC'}
+    return out
+
+
 def gpu_fields(N, dtype, nu, tol, maxit, steps, seed, dt, solver='bicgstab'):
     """Interior velocity after `steps` steps from the initial field (for the
     field error against the oracle)."""
@@ -100,12 +167,32 @@ def gpu_fields(N, dtype, nu, tol, maxit, steps, seed, dt, solver='bicgstab'):
     return [s.interior().double().cpu() for s in (D.state_u_P, D.state_v_P, D.state_w_P)]
 
 
+def host_cpu():
+    """(model name, visible cores, threads to use): the threads are the
+    process's CPU share -- OMP_NUM_THREADS when the host sets it, else every
+    core the process may run on."""
+    model = 'unknown'
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    visible = len(os.sched_getaffinity(0))
+    share = os.environ.get('OMP_NUM_THREADS')
+    threads = min(visible, int(share)) if share and share.isdigit() and int(share) > 0 \
+        else visible
+    return model, visible, max(1, threads)
+
+
 def cpu_baseline(u0, v0, w0, N, nu, dt, tol, maxit, steps=1):
     """The CPU oracle (torch-CPU FP64) on `steps` steps of the same workload;
     returns its throughput and final interior velocity."""
     import torch
     from oracle import pyflow_oracle as O
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    model, visible, cores = host_cpu()
     torch.set_num_threads(cores)
     g = O.Grid(N, N, N)
     B = O.Box(g, u0, v0, w0, mu=nu, rho=1.0, dt=dt, tol=tol, maxit=maxit)
@@ -114,8 +201,8 @@ def cpu_baseline(u0, v0, w0, N, nu, dt, tol, maxit, steps=1):
         B.forward_step()
         B.stats()
     wall = time.perf_counter() - t
-    return dict(value=N ** 3 * steps / wall / 1e6, wall=wall, cores=cores,
-                iters=B.poisson.iterations,
+    return dict(value=N ** 3 * steps / wall / 1e6, wall=wall, cores=cores, model=model,
+                visible=visible, iters=B.poisson.iterations,
                 fields=[B.u.interior().clone(), B.v.interior().clone(), B.w.interior().clone()])
 
 
@@ -158,6 +245,10 @@ def main():
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run 5 device copies of MIB MiB (byte-counter reference) and exit')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-f32', action='store_true',
+                    help='skip the FP32-storage leg of the default FP64 run')
+    ap.add_argument('--serving-requests', type=int, default=24,
+                    help='requests of the serving leg (0 = skip it)')
     ap.add_argument('--no-profile', action='store_true')
     ap.add_argument('--profile-period', type=int, default=32,
                     help='time every n-th launch of each Krylov kernel with dispatch events')
@@ -171,65 +262,77 @@ def main():
         return
     import torch
     dtype = torch.float64 if a.dtype == 'f64' else torch.float32
-    w = 8 if a.dtype == 'f64' else 4
     tol = a.tol if a.tol is not None else (1e-10 if a.dtype == 'f64' else 1e-8)
-    r = gpu_run(a.N, dtype, a.nu, tol, a.maxit, a.steps, a.warmup, a.seed,
-                0 if a.no_profile else a.profile_period, a.solver)
-    ncell = a.N ** 3
-    K = a.steps
-    value = ncell * K / r['wall'] / 1e6
-    ms = r['wall'] / K * 1e3
-    kbar = sum(r['its']) / len(r['its'])
-    alg_bytes = sum(ncell * w * (65 + 15 * k) for k in r['its'])
+    metric = 'Mcell-steps/s + HBM GB/s at 128^3 FP64/FP32; served p50/p99 step latency'
+    leg = measure(a.N, a.dtype, a.nu, tol, a.maxit, a.steps, a.warmup, a.seed,
+                  0 if a.no_profile else a.profile_period, a.solver)
+    r = leg.pop('_run')
     out = {
-        'metric': 'Mcell-steps/s (RK4 + BiCGStab time step, %d^3 %s)' % (a.N, a.dtype.upper()),
-        'value': round(value, 3), 'unit': 'Mcell-steps/s', 'n_gpus': 1, 'steps': K,
-        'warmup': a.warmup, 'ms_per_step': round(ms, 4), 'higher_is_better': True,
-        'scaling': 'weak', 'vs_baseline': None, 'dtype': a.dtype, 'data': 'synthetic',
+        'metric': metric, 'value': leg['value'], 'unit': 'Mcell-steps/s', 'n_gpus': 1,
+        'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': leg['ms_per_step'],
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': a.dtype,
+        'data': 'synthetic (seeded Passot-Pouquet field)',
         'config': {'workload': 'decaying isotropic turbulence, periodic %d^3 staggered grid, '
-                               'Passot-Pouquet k0=4 u_rms=1, nu=%g, RK4, %s tol %g'
+                               'Passot-Pouquet k0=4 u_rms=1, nu=%g, RK4, %s tol %g, %s'
                                % (a.N, a.nu, 'BiCGStab' if a.solver == 'bicgstab' else
-                                  'BiCGStab in FP64 iterative refinement', tol),
+                                  'BiCGStab in FP64 iterative refinement', tol, a.dtype.upper()),
                    'grid': a.N, 'nu': a.nu, 'tol': tol, 'dt': r['dt'],
-                   'bicgstab_iters_mean': round(kbar, 2),
+                   'bicgstab_iters_mean': leg['bicgstab_iters_mean'],
                    'bicgstab_last_residual': float('%.3e' % r['res']),
                    'bicgstab_status': r['status'], 'parallelism': 'single GPU'},
-        'hbm_gbs_algorithmic': round(alg_bytes / r['wall'] / 1e9, 2),
-        'step_roofline_frac': round(alg_bytes / r['wall'] / 1e9 / HBM_PEAK_GBS, 4),
     }
-    if r['prof']:
-        per = {}
-        for name, wmul in (('KA', 6), ('KB', 2), ('KC', 7)):
-            tot, n = r['prof'][name]
-            if n:
-                avg = tot / n / 1e3
-                gbs = wmul * w * ncell / avg / 1e9
-                per[name] = {'avg_us': round(avg * 1e6, 3), 'launches': n,
-                             'bytes_per_launch': wmul * w * ncell, 'GB/s': round(gbs, 1)}
-        out['kernels'] = per
-        if 'KA' in per:
-            out['roofline'] = {'kernel': 'KA: p update, v = A p, <v, r_hat> (bicg_tile2_kernel / bicg_tile_kernel)', 'bound': 'hbm',
-                               'achieved': per['KA']['GB/s'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                               'frac': round(per['KA']['GB/s'] / HBM_PEAK_GBS, 4),
-                               'traffic': KA_TRAFFIC_BYTES.get((a.N, a.dtype)),
-                               'bytes_per_launch': per['KA']['bytes_per_launch']}
+    for k in ('hbm_gbs_algorithmic', 'step_roofline_frac', 'step_roofline_frac_17w',
+              'kernels', 'roofline', 'infinity_cache'):
+        if k in leg:
+            out[k] = leg[k]
+    cb = None
     if not a.no_cpu:
         u0, v0, w0 = r['u0']
         otol = 1e-10   # the oracle is the reference's FP64 path at the FP64 tolerance
         cb = cpu_baseline(u0, v0, w0, a.N, a.nu, r['dt'], otol, a.maxit, a.cpu_steps)
-        out['cpu_baseline'] = {'value': round(cb['value'], 5), 'unit': 'Mcell-steps/s',
-                               'cores': cb['cores'], 'kind': 'port',
-                               'sample': '%d time step(s) of the same %d^3 workload in FP64 from '
-                                         'the same initial field (CPU oracle, torch-CPU FP64, %d '
-                                         'threads; %d BiCGStab iterations in its last step, '
-                                         '%.1f s)' % (a.cpu_steps, a.N, cb['cores'], cb['iters'],
-                                                      cb['wall'])}
-        # field error of the GPU path against the oracle after the same steps
+        out['cpu_baseline'] = {
+            'value': round(cb['value'], 5), 'unit': 'Mcell-steps/s', 'cores': cb['cores'],
+            'kind': 'port',
+            'sample': '%d time step(s) of the same %d^3 workload in FP64 from the same initial '
+                      'field: the CPU oracle (torch-CPU FP64 restatement of the reference\'s op '
+                      'sequence) on %d threads (the process\'s CPU share; %d cores visible, %s); '
+                      '%d BiCGStab iterations in its last step, %.1f s. The oracle multiplies by '
+                      'the nine Laplace constants where the reference reads its 72 B/cell '
+                      'coefficient tensor, so it is faster than the reference\'s own CPU path: '
+                      'a conservative baseline'
+                      % (a.cpu_steps, a.N, cb['cores'], cb['visible'], cb['model'], cb['iters'],
+                         cb['wall'])}
         gf = gpu_fields(a.N, dtype, a.nu, tol, a.maxit, a.cpu_steps, a.seed, r['dt'], a.solver)
         errs = [float(torch.linalg.norm(x - y) / torch.linalg.norm(y))
                 for x, y in zip(gf, cb['fields'])]
         out['oracle_error'] = {'steps': a.cpu_steps, 'rel_l2_uvw': [float('%.3e' % e) for e in errs],
                                'gpu_tol': tol, 'oracle_tol': otol}
+    if a.dtype == 'f64' and not a.no_f32:
+        f32 = measure(a.N, 'f32', a.nu, 1e-8, a.maxit, a.steps, a.warmup, a.seed,
+                      0 if a.no_profile else a.profile_period, a.solver)
+        f32.pop('_run')
+        f32['tol'] = 1e-8
+        if cb is not None:
+            gf = gpu_fields(a.N, torch.float32, a.nu, 1e-8, a.maxit, a.cpu_steps, a.seed,
+                            r['dt'], a.solver)
+            errs = [float(torch.linalg.norm(x - y) / torch.linalg.norm(y))
+                    for x, y in zip(gf, cb['fields'])]
+            f32['oracle_error'] = {'steps': a.cpu_steps, 'vs': 'FP64 oracle, tol 1e-10',
+                                   'rel_l2_uvw': [float('%.3e' % e) for e in errs]}
+        out['f32'] = f32
+    if a.serving_requests > 0:
+        from pyflow_amd import serve_bench
+        sv = serve_bench.run(requests=a.serving_requests)
+        b, q = sv['batched'], sv['sequential']
+        keys = ('mcell_steps_per_s', 'step_latency_p50_ms', 'step_latency_p99_ms', 'wall_s')
+        out['serving'] = {
+            'workload': sv['workload'],
+            'batched': {k: round(b[k], 3) for k in keys},
+            'sequential': {k: round(q[k], 3) for k in keys},
+            'throughput_ratio': round(sv['throughput_ratio'], 3),
+            'bit_identical_requests': sv['bit_identical_requests'],
+            'max_resident_gb': round(b['max_resident_gb'], 2),
+            'mean_batch': round(b['mean_batch'], 2)}
     print(json.dumps(out))
 
 

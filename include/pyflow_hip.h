@@ -310,6 +310,16 @@ int pf_bicgstab_mpir_solve(pf_box* box, void* DP, void* p, void* source,
 int pf_mpir_configure(pf_box* box, double inner_tol, int inner_maxit, int outer_maxit);
 
 /*
+ * The last refinement solve's FP64 iterate x and right-hand side
+ * b = -vol*(rho/dt)*source (interior layout, nx*ny*nz doubles each; either
+ * pointer may be NULL), copied on `stream` into caller device memory.  No
+ * counterpart in the reference: for callers that need the solution beyond
+ * the box precision, or its residual max|b - A x|/max|b| recomputed on
+ * their side.
+ */
+int pf_mpir_solution(pf_box* box, double* x, double* b, void* stream);
+
+/*
  * B (interior) = (-vol*L7) A for a padded A whose overlap is already filled.
  * Replaces Pressure_Base.applyOperator (solver/pressure.py:54-84) after its
  * communicate_border call; same nine-term summation order.

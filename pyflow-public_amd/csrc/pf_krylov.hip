@@ -1732,6 +1732,18 @@ int pf_mpir_configure(pf_box* h, double inner_tol, int inner_maxit, int outer_ma
   return mp_reserve(h);
 }
 
+int pf_mpir_solution(pf_box* h, double* x, double* b, void* stream) {
+  if (int rc = check_box(h)) return rc;
+  if (!h->mp_x || !h->mp_b)
+    return fail(PF_EINVAL, "pf_mpir_solution: no refinement workspace (call pf_mpir_configure "
+                           "and solve first)");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)h->g.ncell * sizeof(double);
+  if (x) PF_HIP(hipMemcpyAsync(x, h->mp_x, n, hipMemcpyDeviceToDevice, s));
+  if (b) PF_HIP(hipMemcpyAsync(b, h->mp_b, n, hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
 int pf_jacobi_solve(pf_box* h, void* pOld, void* p, void* source, double rho, double dt, int nit,
                     double* out_res, void* stream) {
   if (int rc = check_box(h)) return rc;

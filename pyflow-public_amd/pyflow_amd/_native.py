@@ -86,6 +86,7 @@ SIGNATURES = [
     ('pf_sfs_smagorinsky', _I, [_P, _P, _P, _P, _D, _D, _D, _D, _P, _P]),
     ('pf_bicgstab_mpir_solve', _I, [_P, _P, _P, _P, _D, _D, _D, _I, _PD, ctypes.POINTER(_I), _P]),
     ('pf_mpir_configure', _I, [_P, _D, _I, _I]),
+    ('pf_mpir_solution', _I, [_P, _P, _P, _P]),
     ('pf_residual_history', _I, [_P, _PD, _I, ctypes.POINTER(ctypes.c_int), _P]),
     ('pf_source_inputs', _I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     ('pf_rhs_ns_src', _I, [_P, _P, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
@@ -367,6 +368,15 @@ class Box:
         which bicgstab_mpir and vel_corr with the refinement solver need."""
         check(lib().pf_mpir_configure(self.h, float(inner_tol), int(inner_maxit), int(outer_maxit)),
               'pf_mpir_configure')
+
+    def mpir_solution(self):
+        """(x, b) of the last refinement solve: FP64 interior tensors."""
+        x = torch.empty(self.interior_shape, dtype=torch.float64, device=self.device)
+        b = torch.empty_like(x)
+        check(lib().pf_mpir_solution(self.h, ctypes.c_void_p(x.data_ptr()),
+                                     ctypes.c_void_p(b.data_ptr()), self.stream()),
+              'pf_mpir_solution')
+        return x, b
 
     def jacobi(self, pOld, p, source, rho, dt, nit):
         res = ctypes.c_double(0.0)

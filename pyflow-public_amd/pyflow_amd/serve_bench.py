@@ -30,8 +30,51 @@ def _same_bits(a, b):
     return torch.equal(a.contiguous().view(torch.uint8), b.contiguous().view(torch.uint8))
 
 
-def main(argv=None):
+def run(requests=48, rate=50.0, seed=7, capacity=256, hbm_gb=200.0, tol=1e-6, dtype='f32',
+        sequential=True, batched=True):
+    """Serve one request stream batched and sequentially; returns the summary
+    dict that main() prints."""
     from . import serving
+    tdtype = torch.float32 if dtype == 'f32' else torch.float64
+    # warm both paths once (graph capture, kernel loads), untimed
+    warm = serving.make_requests(4, rate=1e9, steps=(2, 2), seed=seed + 1000, tol=tol)
+    serving.BatchServer(dtype=tdtype, capacity=8).serve(warm, realtime=False)
+    warm = serving.make_requests(4, rate=1e9, steps=(2, 2), seed=seed + 1000, tol=tol)
+    serving.SequentialRunner(dtype=tdtype).serve(warm, realtime=False)
+
+    out = {'workload': '%d requests, Poisson arrivals %.0f/s, N^3 in {32,48,64,96,128}, 10-200 '
+                       'steps, k0 in {2,4,8}, nu in [0.002,0.02], %s, BiCGStab %g'
+                       % (requests, rate, dtype, tol)}
+    reqs = serving.make_requests(requests, rate=rate, seed=seed, tol=tol)
+    if batched:
+        srv = serving.BatchServer(dtype=tdtype, capacity=capacity,
+                                  hbm_cap=int(hbm_gb * serving.GB))
+        sb = srv.serve(reqs, keep_results=True, realtime=True)
+        sb['max_resident_gb'] = srv.max_resident / serving.GB
+        sb['mean_batch'] = float(np.mean(srv.batch_sizes))
+        sb['server_steps'] = len(srv.step_times)
+        out['batched'] = sb
+    if sequential:
+        seq = serving.make_requests(requests, rate=rate, seed=seed, tol=tol)
+        ss = serving.SequentialRunner(dtype=tdtype).serve(seq, keep_results=True, realtime=True)
+        out['sequential'] = ss
+        if batched:
+            out['throughput_ratio'] = out['batched']['mcell_steps_per_s'] / ss['mcell_steps_per_s']
+            same = 0
+            for x, y in zip(reqs, seq):
+                if all(_same_bits(p, q) for p, q in zip(x.result, y.result)):
+                    same += 1
+            out['bit_identical_requests'] = '%d/%d' % (same, len(reqs))
+        out['nonfinite_requests'] = sum(
+            1 for r in seq if not all(bool(torch.isfinite(t).all()) for t in r.result))
+        for r in seq:
+            r.result = None
+    for r in reqs:
+        r.result = None
+    return out
+
+
+def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--requests', type=int, default=48)
     ap.add_argument('--rate', type=float, default=50.0)
@@ -43,40 +86,8 @@ def main(argv=None):
     ap.add_argument('--no-sequential', action='store_true')
     ap.add_argument('--no-batched', action='store_true')
     a = ap.parse_args(argv)
-    dtype = torch.float32 if a.dtype == 'f32' else torch.float64
-    # warm both paths once (graph capture, kernel loads), untimed
-    warm = serving.make_requests(4, rate=1e9, steps=(2, 2), seed=a.seed + 1000, tol=a.tol)
-    serving.BatchServer(dtype=dtype, capacity=8).serve(warm, realtime=False)
-    warm = serving.make_requests(4, rate=1e9, steps=(2, 2), seed=a.seed + 1000, tol=a.tol)
-    serving.SequentialRunner(dtype=dtype).serve(warm, realtime=False)
-
-    out = {'workload': 'Poisson arrivals %.0f/s, N^3 in {32,48,64,96,128}, 10-200 steps, '
-                       'k0 in {2,4,8}, nu in [0.002,0.02], %s, BiCGStab %g'
-                       % (a.rate, a.dtype, a.tol)}
-    reqs = serving.make_requests(a.requests, rate=a.rate, seed=a.seed, tol=a.tol)
-    if not a.no_batched:
-        srv = serving.BatchServer(dtype=dtype, capacity=a.capacity,
-                                  hbm_cap=int(a.hbm_gb * serving.GB))
-        sb = srv.serve(reqs, keep_results=True, realtime=True)
-        sb['max_resident_gb'] = srv.max_resident / serving.GB
-        sb['mean_batch'] = float(np.mean(srv.batch_sizes))
-        sb['server_steps'] = len(srv.step_times)
-        out['batched'] = sb
-    if not a.no_sequential:
-        seq = serving.make_requests(a.requests, rate=a.rate, seed=a.seed, tol=a.tol)
-        ss = serving.SequentialRunner(dtype=dtype).serve(seq, keep_results=True, realtime=True)
-        out['sequential'] = ss
-        if not a.no_batched:
-            out['throughput_ratio'] = out['batched']['mcell_steps_per_s'] / ss['mcell_steps_per_s']
-            same = 0
-            for x, y in zip(reqs, seq):
-                if all(_same_bits(p, q) for p, q in zip(x.result, y.result)):
-                    same += 1
-            out['bit_identical_requests'] = '%d/%d' % (same, len(reqs))
-        out['nonfinite_requests'] = sum(
-            1 for r in seq if not all(bool(torch.isfinite(t).all()) for t in r.result))
-    for r in reqs:
-        r.result = None
+    out = run(a.requests, a.rate, a.seed, a.capacity, a.hbm_gb, a.tol, a.dtype,
+              not a.no_sequential, not a.no_batched)
     print(json.dumps(out))
     return 0
 

@@ -116,6 +116,9 @@ class Slot:
         self.f['src'] = torch.zeros(self.box.interior_shape, dtype=dtype, device=device)
         self.bytes = self.box.device_bytes + sum(t.numel() * t.element_size()
                                                  for t in self.f.values())
+        # recorded on the serving stream after the last reads of a finished
+        # request's fields; the next load of the slot waits for it
+        self.released = None
 
     def load(self, req):
         """Initial state of `req` (fields, p = 0) and its fixed dt."""
@@ -200,6 +203,11 @@ def _finish(req, slot, keep_result):
     req.finished = time.perf_counter()
     if keep_result:
         req.result = [slot.f[n][2:-2, 2:-2, 2:-2].clone() for n in ('u', 'v', 'w', 'p')]
+    # the clones are queued on this thread's stream; a later load of the slot
+    # (on the loader's stream) must not overwrite the fields before they ran
+    ev = torch.cuda.Event()
+    ev.record()
+    slot.released = ev
 
 
 def _record(req, row, npts):
@@ -227,6 +235,8 @@ class _Loader(threading.Thread):
                 return
             req, slot = item
             try:
+                if slot.released is not None:
+                    self.stream.wait_event(slot.released)
                 with torch.cuda.stream(self.stream):
                     slot.load(req)
                 self.stream.synchronize()
