@@ -530,6 +530,15 @@ int pf_closure_vjp(pf_box* box, int kind, const void* u, const void* v, const vo
  *     fast box starts its next step while slow ones iterate.  out_chunks:
  *     graph replays in this call.  Ordered after the caller's work on
  *     `stream`; returns when the finishing boxes' steps are complete.
+ *   pf_batch_set_group_bytes: inside a chunk the resident boxes run in
+ *     groups whose Krylov vectors (nine interior vectors per box) together
+ *     take at most `bytes` (default 320 MiB, measured best for the 256 MB
+ *     Infinity Cache; a box larger than that is a group of its own; 0 = one
+ *     group of all boxes, with non-temporal output stores); each group runs
+ *     all chunk iterations before the next, so its Krylov vectors stay in
+ *     that cache from one kernel to the next.  Only the order of independent
+ *     boxes' launches changes, never a result.  Synchronous; re-captures the
+ *     graph.
  */
 typedef struct pf_batch pf_batch;
 int pf_batch_create(pf_batch** out, int capacity, int dtype);
@@ -539,6 +548,7 @@ int pf_batch_set(pf_batch* batch, int n, pf_box* const* boxes, void* const* fiel
 int pf_batch_advance(pf_batch* batch, const int* start, int* finished, double* out,
                      int* out_chunks, void* stream);
 int pf_batch_chunk_iterations(void);
+int pf_batch_set_group_bytes(pf_batch* batch, int64_t bytes);
 
 /*
  * Measurement support (no counterpart in the reference): with period > 0,

@@ -21,8 +21,18 @@
 // graph is captured once per composition (pf_batch_set) and replayed; it
 // does not depend on iteration counts.  Steps start at chunk boundaries,
 // which keeps every box's iteration parity aligned with the graph's.
+//
+// Inside a chunk the boxes run in cache-residency groups: boxes are packed,
+// by tile shape and then size, into groups whose Krylov vectors together fit
+// a byte budget sized for the Infinity Cache (MALL), and each group runs all
+// kChunk iterations before the next group starts.  A box alone on the GPU
+// keeps its Krylov set in that cache from one kernel to the next; sweeping
+// every resident box in each launch instead makes the sets of a mix stream
+// from HBM.  Grouping changes only the order of independent boxes' launches,
+// not any box's arithmetic.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <unordered_map>
 #include <vector>
@@ -32,6 +42,13 @@
 namespace {
 
 constexpr int kChunk = 16;    // BiCGStab iterations per graph replay (even: buffer parity)
+// Default cache-residency budget of a group's Krylov vectors.  The nine
+// vectors per box overstate what one kernel streams (KA, KB and KC touch six,
+// two and seven of them), so the best budget exceeds the 256 MB Infinity
+// Cache: on the 24-request serving stream (FP32) 0 (one group) gave 251,
+// 192 MiB 260, 256 MiB 268, 320 MiB 272, 384 MiB 258 and 512 MiB 256
+// Mcell-steps/s.
+constexpr size_t kGroupBytes = (size_t)320 << 20;
 
 struct List {
   std::vector<int> box, begin;
@@ -82,6 +99,8 @@ struct pf_batch {
   size_t list_cap;            // ints per region
   hipGraphExec_t g_chunk;
   int maxit_max;
+  size_t group_bytes;         // cache-residency budget of one group (0: one group)
+  std::vector<Lists> groups;  // tile lists of each group (list region 3)
 };
 
 using namespace pf;
@@ -197,10 +216,58 @@ int upload_lists(pf_batch* b, Lists& L, int region) {
   return 0;
 }
 
+// Bytes one box's BiCGStab iterations stream: the nine interior Krylov vectors.
+size_t krylov_set_bytes(const pf_box* h) { return (size_t)9 * (size_t)h->g.ncell * h->esize; }
+
+// Pack the resident boxes into cache-residency groups (greedy, boxes ordered
+// by tile shape, then Krylov set size) and upload the groups' tile lists.
+int build_groups(pf_batch* b) {
+  b->groups.clear();
+  const int n = b->n;
+  if (n == 0) return 0;
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  auto shape = [&](int i) {
+    const pf_box* h = b->boxes[i];
+    const TileGrid tg = make_tile_grid(h->g, (int)h->esize, h->tile_rows);
+    return tg.tz * 1000 + tg.ty;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    const int sa = shape(a), sc = shape(c);
+    if (sa != sc) return sa < sc;
+    return krylov_set_bytes(b->boxes[a]) < krylov_set_bytes(b->boxes[c]);
+  });
+  std::vector<std::vector<int>> ids(1);
+  size_t used = 0;
+  for (int i : order) {
+    const size_t by = krylov_set_bytes(b->boxes[i]);
+    if (b->group_bytes && !ids.back().empty() && used + by > b->group_bytes) {
+      ids.emplace_back();
+      used = 0;
+    }
+    ids.back().push_back(i);
+    used += by;
+  }
+  int* host = b->list_stage + (size_t)3 * b->list_cap;
+  int* dev = b->list_mem + (size_t)3 * b->list_cap;
+  size_t off = 0;
+  for (auto& g : ids) {
+    std::sort(g.begin(), g.end());
+    b->groups.emplace_back();
+    Lists& L = b->groups.back();
+    build_lists(b, g, L);
+    const size_t u = pack_lists(L, host + off, dev + off, b->list_cap - off);
+    if (off + u > b->list_cap) return fail(PF_EINVAL, "pf_batch: group lists exceed their region");
+    off += u;
+  }
+  if (off) PF_HIP(hipMemcpyAsync(dev, host, off * sizeof(int), hipMemcpyHostToDevice, b->st));
+  return 0;
+}
+
 int tiles_launch(pf_batch* b, Lists& L, int what, int parity) {
   for (const List& t : L.tiles)
     if (int rc = batch_krylov_launch(b->dtype, what, parity, b->desc, t.view(), t.total, t.tz,
-                                     t.ty, b->st))
+                                     t.ty, b->st, b->group_bytes == 0))
       return rc;
   return 0;
 }
@@ -233,12 +300,13 @@ int launch_post(pf_batch* b, Lists& L) {
 }
 
 int record_chunk(pf_batch* b) {
-  Lists& L = b->full;
-  for (int q = 0; q < kChunk; ++q) {
-    const int par = q & 1;
-    if (int rc = tiles_launch(b, L, kBatchKB, par)) return rc;
-    if (int rc = tiles_launch(b, L, kBatchKC, par)) return rc;
-    if (int rc = tiles_launch(b, L, kBatchKA, 1 - par)) return rc;
+  for (Lists& L : b->groups) {
+    for (int q = 0; q < kChunk; ++q) {
+      const int par = q & 1;
+      if (int rc = tiles_launch(b, L, kBatchKB, par)) return rc;
+      if (int rc = tiles_launch(b, L, kBatchKC, par)) return rc;
+      if (int rc = tiles_launch(b, L, kBatchKA, 1 - par)) return rc;
+    }
   }
   if (int rc = batch_ctl_launch(b->dtype, kBatchCtlFlags, b->desc, nullptr, b->n, b->flags_dev,
                                 b->st))
@@ -292,13 +360,15 @@ int pf_batch_create(pf_batch** out, int capacity, int dtype) {
   b->list_cap = list_ints(capacity);
   b->g_chunk = nullptr;
   b->maxit_max = 0;
+  b->group_bytes = kGroupBytes;
   hipError_t e = hipMalloc(&b->desc, b->desc_size * capacity);
   if (e == hipSuccess) e = hipMalloc((void**)&b->out_dev, sizeof(double) * 8 * capacity);
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->out_host, sizeof(double) * 8 * capacity);
   if (e == hipSuccess) e = hipMalloc((void**)&b->flags_dev, sizeof(int) * capacity);
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->flags_host, sizeof(int) * capacity);
-  if (e == hipSuccess) e = hipMalloc((void**)&b->list_mem, sizeof(int) * 3 * b->list_cap);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&b->list_stage, sizeof(int) * 3 * b->list_cap);
+  // list regions: all boxes, boxes starting a step, boxes finishing one, groups
+  if (e == hipSuccess) e = hipMalloc((void**)&b->list_mem, sizeof(int) * 4 * b->list_cap);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->list_stage, sizeof(int) * 4 * b->list_cap);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
   if (e != hipSuccess) {
@@ -382,6 +452,7 @@ int pf_batch_set(pf_batch* b, int n, pf_box* const* boxes, void* const* fields,
   for (int i = 0; i < n; ++i) all[i] = i;
   build_lists(b, all, b->full);
   if (int rc = upload_lists(b, b->full, 0)) return rc;
+  if (int rc = build_groups(b)) return rc;
   PF_HIP(hipStreamSynchronize(b->st));
   if (n > 0) return capture_chunk(b);
   if (b->g_chunk) hipGraphExecDestroy(b->g_chunk);
@@ -445,5 +516,16 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
 }
 
 int pf_batch_chunk_iterations(void) { return kChunk; }
+
+int pf_batch_set_group_bytes(pf_batch* b, int64_t bytes) {
+  if (!b) return fail(PF_EINVAL, "pf_batch_set_group_bytes: NULL batch");
+  if (bytes < 0) return fail(PF_EINVAL, "pf_batch_set_group_bytes: bytes must be >= 0");
+  PF_HIP(hipStreamSynchronize(b->st));
+  b->group_bytes = (size_t)bytes;
+  if (b->n == 0) return 0;
+  if (int rc = build_groups(b)) return rc;
+  PF_HIP(hipStreamSynchronize(b->st));
+  return capture_chunk(b);
+}
 
 }  // extern "C"

@@ -212,7 +212,7 @@ __device__ __forceinline__ int find_box(const BatchList& L, int bid) {
 int batch_stage_launch(int dtype, int stage, const void* desc, const BatchList& L, int nblocks,
                        hipStream_t s);
 int batch_krylov_launch(int dtype, int what, int parity, const void* desc, const BatchList& L,
-                        int nblocks, int tz, int ty, hipStream_t s);
+                        int nblocks, int tz, int ty, hipStream_t s, bool nts = true);
 int batch_ctl_launch(int dtype, int what, const void* desc, const int* ids, int n, int* flags,
                      hipStream_t s);
 

@@ -1062,7 +1062,7 @@ __global__ void __launch_bounds__(kBlock) batch_init_kernel(const BoxDesc<T>* __
 // j & 1): p[1-parity] from p[parity], v[parity] and r[parity]; KB and KC of
 // iteration j (launched with parity j & 1): s, t from r[parity] and
 // v[1-parity]; KC also reads p[1-parity] and writes r[1-parity].
-template <typename T, int MODE, int TZ, int TY, bool PAIR>
+template <typename T, int MODE, int TZ, int TY, bool PAIR, bool NTS>
 __global__ void __launch_bounds__(PAIR ? TZ / 2 * TY : TZ * TY) batch_tile_kernel(
     const BoxDesc<T>* __restrict__ D, BatchList L, int parity) {
   PF_BATCH_BOX;
@@ -1074,15 +1074,17 @@ __global__ void __launch_bounds__(PAIR ? TZ / 2 * TY : TZ * TY) batch_tile_kerne
   const double* pin = MODE == kModeA ? d.pC : (MODE == kModeB ? d.pA : d.pB);
   double* pout = MODE == kModeA ? d.pA : (MODE == kModeB ? d.pB : d.pC);
   const T* r = d.rk[parity];
-  // served boxes are small (short x chunks): the one-plane-ahead schedule;
-  // non-temporal stores, since the boxes of a batch together outgrow the
-  // Infinity Cache (48-request serving benchmark: 243.2 -> 246.2 Mcell-steps/s)
+  // served boxes are small (short x chunks): the one-plane-ahead schedule.
+  // NTS: non-temporal output stores, for launches whose boxes together
+  // outgrow the Infinity Cache (all resident boxes at once); launches over a
+  // cache-residency group keep their outputs cacheable for the group's next
+  // kernel.
   if (PAIR)
-    bicg_tile2_body<T, MODE, TZ, TY, true, true>(d.g, d.tg, d.tg.nblk, d.ctl, pin, pout, r, d.rh, a_old,
-                                     v_in, f_out, af_out, d.L, blk);
+    bicg_tile2_body<T, MODE, TZ, TY, true, NTS>(d.g, d.tg, d.tg.nblk, d.ctl, pin, pout, r, d.rh,
+                                                a_old, v_in, f_out, af_out, d.L, blk);
   else
-    bicg_tile_kernel_body<T, MODE, TZ, TY, true, true>(d.g, d.tg, d.tg.nblk, d.ctl, pin, pout, r, d.rh,
-                                           a_old, v_in, f_out, af_out, d.L, blk);
+    bicg_tile_kernel_body<T, MODE, TZ, TY, true, NTS>(d.g, d.tg, d.tg.nblk, d.ctl, pin, pout, r,
+                                                      d.rh, a_old, v_in, f_out, af_out, d.L, blk);
 }
 
 template <typename T>
@@ -1149,7 +1151,7 @@ __global__ void __launch_bounds__(kBlock) batch_ctl_kernel(const BoxDesc<T>* __r
 }
 
 int batch_krylov_launch(int dtype, int what, int parity, const void* desc, const BatchList& L,
-                        int nblocks, int tz, int ty, hipStream_t s) {
+                        int nblocks, int tz, int ty, hipStream_t s, bool nts) {
   if (nblocks <= 0) return 0;
   auto go = [&](auto zero) -> int {
     using T = decltype(zero);
@@ -1163,16 +1165,20 @@ int batch_krylov_launch(int dtype, int what, int parity, const void* desc, const
       case kBatchKA:
       case kBatchKB:
       case kBatchKC: {
-#define PF_BTILE(TZ, TY, PAIR)                                                                 \
+#define PF_BTILE_N(TZ, TY, PAIR, NT)                                                           \
   if (what == kBatchKA)                                                                          \
-    hipLaunchKernelGGL((batch_tile_kernel<T, kModeA, TZ, TY, PAIR>), gb,                         \
+    hipLaunchKernelGGL((batch_tile_kernel<T, kModeA, TZ, TY, PAIR, NT>), gb,                     \
                        dim3(PAIR ? TZ / 2 * TY : TZ * TY), 0, s, D, L, parity);                  \
   else if (what == kBatchKB)                                                                     \
-    hipLaunchKernelGGL((batch_tile_kernel<T, kModeB, TZ, TY, PAIR>), gb,                         \
+    hipLaunchKernelGGL((batch_tile_kernel<T, kModeB, TZ, TY, PAIR, NT>), gb,                     \
                        dim3(PAIR ? TZ / 2 * TY : TZ * TY), 0, s, D, L, parity);                  \
   else                                                                                           \
-    hipLaunchKernelGGL((batch_tile_kernel<T, kModeC, TZ, TY, PAIR>), gb,                         \
+    hipLaunchKernelGGL((batch_tile_kernel<T, kModeC, TZ, TY, PAIR, NT>), gb,                     \
                        dim3(PAIR ? TZ / 2 * TY : TZ * TY), 0, s, D, L, parity)
+#define PF_BTILE(TZ, TY, PAIR)                                                                 \
+  do {                                                                                           \
+    if (nts) { PF_BTILE_N(TZ, TY, PAIR, true); } else { PF_BTILE_N(TZ, TY, PAIR, false); }       \
+  } while (0)
         switch (tz * 1000 + ty) {
           case 64004: PF_BTILE(64, 4, false); break;
           case 32008: PF_BTILE(32, 8, false); break;
@@ -1184,6 +1190,7 @@ int batch_krylov_launch(int dtype, int what, int parity, const void* desc, const
           default: return fail(PF_EINVAL, "batch_krylov_launch: unsupported tile shape");
         }
 #undef PF_BTILE
+#undef PF_BTILE_N
         break;
       }
       default: return fail(PF_EINVAL, "batch_krylov_launch: unknown kernel");

@@ -77,6 +77,7 @@ SIGNATURES = [
     ('pf_batch_set', _I, [_P, _I, _PP, _PP, _PP, _PD]),
     ('pf_batch_advance', _I, [_P, _PI, _PI, _PD, _PI, _P]),
     ('pf_batch_chunk_iterations', _I, []),
+    ('pf_batch_set_group_bytes', _I, [_P, ctypes.c_int64]),
     ('pf_profile_enable', _I, [_P, _I]),
     ('pf_profile_read', _I, [_P, _I, _PD, ctypes.POINTER(ctypes.c_int64)]),
     ('pf_residual_history_enable', _I, [_P, _I]),
@@ -666,6 +667,10 @@ class Batch:
                                  ctypes.cast(srcs, _PP), ctypes.cast(prm, _PD)), 'pf_batch_set')
         self.n = n
         self._keep = members
+
+    def set_group_bytes(self, nbytes):
+        """Cache-residency budget of the chunk's box groups (0: one group)."""
+        check(lib().pf_batch_set_group_bytes(self.h, int(nbytes)), 'pf_batch_set_group_bytes')
 
     def advance(self, start, device=None):
         """Start a step for the boxes flagged in `start` (idle ones), run until

@@ -249,7 +249,8 @@ class BatchServer:
     """Continuous-batching server on one GPU.  hbm_cap: admission limit in
     bytes for resident slots (idle pooled slots are evicted to make room)."""
 
-    def __init__(self, dtype=torch.float32, device=None, capacity=256, hbm_cap=200 * GB):
+    def __init__(self, dtype=torch.float32, device=None, capacity=256, hbm_cap=200 * GB,
+                 group_bytes=None):
         _native.require_gpu()
         self.device = torch.device('cuda', torch.cuda.current_device()) if device is None \
             else torch.device(device)
@@ -258,6 +259,8 @@ class BatchServer:
         self.hbm_cap = int(hbm_cap)
         self.pool = SlotPool(dtype, self.device)
         self.batch = _native.Batch(self.capacity, dtype)
+        if group_bytes is not None:
+            self.batch.set_group_bytes(group_bytes)
         self.step_times = []
         self.batch_sizes = []
         self.max_resident = 0

@@ -74,3 +74,21 @@ def test_admission_holds_the_hbm_cap():
     # the third request is admitted only after one of the first two finished
     assert order[2].admitted >= min(order[0].finished, order[1].finished)
     assert all(r.done_steps == r.steps for r in reqs)
+
+
+def test_cache_groups_do_not_change_results():
+    """Grouping the boxes of a chunk for cache residency (and the store policy
+    that goes with it) reorders independent boxes' launches only: the same
+    bits with one group of all boxes, the default budget, and groups so small
+    that every box is its own."""
+    from pyflow_amd.serving import BatchServer
+    specs = [(64, 5, 71, 4.0, 0.01, 0.0), (32, 7, 72, 2.0, 0.005, 0.0),
+             (48, 6, 73, 8.0, 0.02, 0.0), (96, 3, 74, 4.0, 0.01, 0.0)]
+    runs = []
+    for gb in (0, None, 1):
+        r = _reqs(specs, 1e-8)
+        BatchServer(capacity=8, group_bytes=gb).serve(r, keep_results=True, realtime=False)
+        runs.append(r)
+    for a, b, c in zip(*runs):
+        for x, y, z in zip(a.result, b.result, c.result):
+            assert torch.equal(x, y) and torch.equal(x, z), a.rid
