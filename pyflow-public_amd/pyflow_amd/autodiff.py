@@ -197,7 +197,7 @@ def main(argv=None):
     """Time one closure-training window: K checkpointed steps forward and the
     reverse sweep back to u^0 and the MLP parameters, from a Passot-Pouquet
     field (k0 = 4, u_rms = 1, dt at CFL 0.3):
-    python -m pyflow_amd.autodiff --N 64 --steps 20 --dtype f32 --closure mlp"""
+    python -m pyflow_amd.autodiff --N 64 --steps 20 --dtype f32 --closure mlp --mlp-forward bf16"""
     import argparse
     import math
     import numpy as np
@@ -212,6 +212,8 @@ def main(argv=None):
     ap.add_argument('--tol', type=float, default=None,
                     help='Poisson tolerance (default 1e-6 in FP32, 1e-12 in FP64)')
     ap.add_argument('--reps', type=int, default=2)
+    ap.add_argument('--mlp-forward', choices=('fp32', 'bf16'), default='fp32',
+                    help='MLP forward arithmetic: FP32, or BF16 on MFMA (FP32 accumulation)')
     a = ap.parse_args(argv)
     dt_ = torch.float32 if a.dtype == 'f32' else torch.float64
     tol = a.tol if a.tol is not None else (1e-6 if a.dtype == 'f32' else 1e-12)
@@ -222,7 +224,7 @@ def main(argv=None):
     u0 = passot_pouquet(n, k0=4.0, urms=1.0, seed=3)
     dt = cfl_dt(*u0, h)
     box = _native.Box(n, n, n, h, h, h, dtype=dt_, device=dev)
-    st = StepVJP(box, a.nu, 1.0, dt, closure=closure, tol=tol)
+    st = StepVJP(box, a.nu, 1.0, dt, closure=closure, tol=tol, mlp_forward=a.mlp_forward)
     rng = np.random.Generator(np.random.PCG64(7))
     c = [torch.tensor(rng.standard_normal((n, n, n)), dtype=dt_, device=dev) for _ in range(3)]
     params = None
@@ -249,10 +251,10 @@ def main(argv=None):
     its = st.iterations
     nf = len(its) // 2
     f, r = min(fwd), min(rev)
-    print('closure-training window %d^3 %s closure=%s: %d steps, forward %.1f ms '
-          '(%.2f ms/step), reverse %.1f ms (%.2f ms/step), window %.1f ms; '
+    print('closure-training window %d^3 %s closure=%s (MLP forward %s): %d steps, forward '
+          '%.1f ms (%.2f ms/step), reverse %.1f ms (%.2f ms/step), window %.1f ms; '
           'Poisson iterations per solve: forward %.1f, reverse %.1f'
-          % (n, a.dtype, a.closure, K, f, f / K, r, r / K, f + r,
+          % (n, a.dtype, a.closure, a.mlp_forward, K, f, f / K, r, r / K, f + r,
              sum(its[:nf]) / max(nf, 1), sum(its[nf:]) / max(len(its) - nf, 1)), flush=True)
     return 0
 

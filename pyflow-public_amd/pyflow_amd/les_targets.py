@@ -9,7 +9,8 @@ the LES and writes, as NGA files the run loop reads (dataReader):
 * `<out>/config`: the LES grid (periodic, L = 2 pi);
 * `<out>/target_<it:08d>`: the filtered velocity (U, V, W and a zero P) at
   LES iteration it = 0, every, 2*every, ... steps; the LES time step is
-  ratio times the DNS one, so LES iteration it is DNS step ratio*it.
+  ratio times the DNS one, so LES iteration it is DNS step ratio*it;
+* `<out>/targets.json`: the time steps, sizes and file names.
 
 The file for it = 0 doubles as the LES initial condition (configName
 'restart', dataFileStr).  A run trains against these files with
@@ -24,6 +25,7 @@ divergence-free DNS gives a divergence-free LES field.
 """
 
 import argparse
+import json
 import math
 import os
 import sys
@@ -117,7 +119,11 @@ def generate(out, N=256, ratio=4, steps=20, every=5, seed=2024, k0=4.0, urms=1.0
                 print('LES it %d (DNS step %d): DNS TKE %.6e, filtered TKE %.6e -> %s'
                       % (it, it * ratio, st['TKE'],
                          0.5 * float((uc ** 2 + vc ** 2 + wc ** 2).mean()), fname), flush=True)
-    return dict(dt_les=dt_les, dt_dns=dt_dns, config=config, base=base, files=files)
+    meta = dict(dt_les=dt_les, dt_dns=dt_dns, config=config, base=base, files=files, N=N,
+                ratio=ratio, steps=steps, every=every, seed=seed, k0=k0, nu=nu)
+    with open(os.path.join(out, 'targets.json'), 'w') as f:
+        json.dump(meta, f, indent=1)
+    return meta
 
 
 def main(argv=None):
