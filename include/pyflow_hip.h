@@ -242,6 +242,30 @@ int64_t pf_sfs_ml_source_params(void);
 int pf_sfs_ml_source_reserve(pf_box* box);
 
 /*
+ * Training through the 'source' ML closure: the reverse of pf_sfs_ml_source,
+ * the autograd part of the reference's adjoint closure term
+ * (solver/adjoint.py:241-273, sfsmodel_ML.py:354-593 differentiated).
+ * For g = sum over all padded cells of GX*gx + GY*gy + GZ*gz with
+ * (GX, GY, GZ) = closure(ui, vi, wi): dui, dvi, dwi (FP32 padded, each may be
+ * NULL) receive dg/d(ui, vi, wi), and dparams (FP64, pf_sfs_ml_source_params()
+ * values in the packed order, may be NULL) accumulates dg/dparams, as the
+ * reference's g.backward() accumulates its model's gradients.  In FP32 like
+ * the reference's closure; summation order differs from torch's CPU
+ * autograd.  Needs the training workspace of pf_sfs_ml_source_train_reserve.
+ * pf_adj_closure_term, pf_adj_rk4_step and pf_adj_euler_step take this
+ * closure as kind 2: delta = -(face interpolant of dg/d var_i), with var_i the
+ * forward state's inputs as its RHS leaves them and (gx, gy, gz) the adjoint
+ * state's centre interpolants (adjoint.py:230-286).
+ */
+int pf_sfs_ml_source_vjp(pf_box* box, const float* params, const void* ui, const void* vi,
+                         const void* wi, const void* gx, const void* gy, const void* gz,
+                         float* dui, float* dvi, float* dwi, double* dparams, void* stream);
+/* Set-up call: reserve the reverse's workspace in the box (about 1.2 GB at
+ * 64^3, 3.6 GB at 128^3; allocates) and keep the 21-entry normalisation table
+ * (sfsmodel_ML.py:252-258) for the adjoint closure term.  Idempotent. */
+int pf_sfs_ml_source_train_reserve(pf_box* box, const float* norm21);
+
+/*
  * Smagorinsky eddy viscosity plus the molecular one, written to the padded
  * viscosity field: visc = mu + rho*(Cs*Delta)^2*|S| with
  * |S| = sqrt(S11^2+S22^2+S33^2 + 2(S12^2+S13^2+S23^2)) of the trace-free

@@ -157,6 +157,31 @@ def closure_adjoint_term(g, rho, fu, fv, fw, au, av, aw, closure):
     return [t.detach() for t in gr]
 
 
+def source_closure_term(g, closure, fu, fv, fw, au, av, aw, rhs, interp_SC):
+    """The closure term of rhs_adjPredictor.evaluate for a 'source' closure
+    (adjoint.py:230-286), run after the convective transposes: the forward
+    state's centre interpolants (overwriting var_i[:-1]; the last plane keeps
+    what the transposes' interpolations left), the closure on them with
+    autograd, the adjoint state's centre interpolants, g = sum of G . u_adj
+    over the padded arrays, g.backward(), then rhs -= the face interpolant of
+    dg/d var_i (taken to float32 first, :268-270).  The closure's trainable
+    weights accumulate their gradients, as the reference's model does."""
+    O.interp_u_xm(fu); O.interp_v_ym(fv); O.interp_w_zm(fw)               # :235-237
+    V = [f.var_i.detach().clone().requires_grad_(True) for f in (fu, fv, fw)]   # :241-243
+    closure.update(*V, requires_grad=True)                                 # :247
+    O.interp_u_xm(au); O.interp_v_ym(av); O.interp_w_zm(aw)                 # :250-252
+    A = [f.var_i.detach() for f in (au, av, aw)]                           # :254-256
+    gsum = (torch.sum(closure.GX[:, :, :, 0] * A[0]) + torch.sum(closure.GY[:, :, :, 0] * A[1])
+            + torch.sum(closure.GZ[:, :, :, 0] * A[2]))                    # :261-263
+    gsum.backward()                                                        # :268
+    gml = [v.grad.to(torch.float32) for v in V]                            # :271-273
+    core = (slice(g.imin, g.imax + 1), slice(g.jmin, g.jmax + 1), slice(g.kmin, g.kmax + 1))
+    for q, fn in enumerate((O.interp_sc_x, O.interp_sc_y, O.interp_sc_z)):  # :278-286
+        fn(g, gml[q], interp_SC)
+        rhs[q].sub_(interp_SC[core])
+    return gml
+
+
 def gradient_closure(g):
     """The gradient model as a closure callable for closure_adjoint_term."""
     def f(fu, fv, fw):
@@ -177,8 +202,9 @@ def mlp_closure(g, params):
 # ---------------------------------------------------------------------------
 class AdjointRHS:
     """rhs_adjPredictor.  ``fu, fv, fw`` are the forward-state Fields (held
-    fixed during an adjoint step); ``closure`` is None or a callable as for
-    closure_adjoint_term."""
+    fixed during an adjoint step); ``closure`` is None, a callable as for
+    closure_adjoint_term ('tensor' closures), or a 'source' closure object
+    such as pyflow_oracle.MLSource (source_closure_term)."""
 
     def __init__(self, g, rho, VISC, fu, fv, fw, closure=None):
         self.g = g
@@ -210,7 +236,10 @@ class AdjointRHS:
         O.interp_w_zm(fw); adj_conv_zz(g, aw, fw, rw)                     # 3c
         adj_conv_3d(g, au, fu, rw)                                        # 3d
         adj_conv_3e(g, av, fv, rw)                                        # 3e
-        if self.closure is not None:
+        if getattr(self.closure, 'modelType', None) == 'source':
+            source_closure_term(g, self.closure, fu, fv, fw, au, av, aw, (ru, rv, rw),
+                                self.ns.interp_SC)
+        elif self.closure is not None:
             d = closure_adjoint_term(g, self.rho, fu, fv, fw, au, av, aw, self.closure)
             ru.add_(d[0]); rv.add_(d[1]); rw.add_(d[2])
 

@@ -479,11 +479,25 @@ class MLSource:
                             1.2744681e+06, 9.6111162e+05, 1.2722050e+06, 1.2891108e+06,
                             1.2891108e+06, 9.6587112e+05])
 
-    def __init__(self, g, sd):
+    def __init__(self, g, sd, trainable=False):
         self.g = g
-        self.sd = {k: v.to(torch.float32) for k, v in sd.items()}
+        # trainable: the weights are autograd leaves, so a backward pass
+        # through update() accumulates their gradients (the reference's model
+        # parameters, whose .grad g.backward() fills in adjoint.py:265)
+        self.sd = {k: v.to(torch.float32).clone().requires_grad_(trainable)
+                   for k, v in sd.items()}
         self.norm = torch.tensor(self.NORM, dtype=torch.float32)
         self.last_output = None
+
+    def packed_grad(self):
+        """The weights' accumulated gradients in the packed order fc1, fcG,
+        fcG2, fc2, fc3, fc4 (weight, then bias)."""
+        parts = []
+        for n in ('fc1', 'fcG', 'fcG2', 'fc2', 'fc3', 'fc4'):
+            for k in ('.weight', '.bias'):
+                t = self.sd[n + k]
+                parts.append((t.grad if t.grad is not None else torch.zeros_like(t)).reshape(-1))
+        return torch.cat(parts).detach()
 
     def _d1(self, f, dim):
         return (torch.roll(f, -1, dim) - torch.roll(f, 1, dim)) / (2 * self.g.dx)

@@ -686,6 +686,7 @@ size_t adj_work_bytes(const pf_box* h) {
 int adj_closure_term(pf_box* h, int kind, const void* const U[3], const float* params,
                      const void* const A[3], void* work, void* const D[3], double* dparams,
                      hipStream_t s) {
+  if (kind == 2) return ml_source_adj_term(h, U, params, A, D, dparams, s);
   const size_t np = align256((size_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4) * h->esize);
   char* base = (char*)work;
   void* tb[6];
@@ -757,11 +758,11 @@ int pf_adj_closure_term(pf_box* h, int kind, const void* u, const void* v, const
                         const float* params, const void* ua, const void* va, const void* wa,
                         void* work, void* const* delta3, double* dparams, void* stream) {
   if (int rc = check_box(h)) return rc;
-  if (kind != 0 && kind != 1) return fail(PF_EINVAL, "pf_adj_closure_term: kind is 0 or 1");
-  if (!u || !v || !w || !ua || !va || !wa || !work || !delta3 || !delta3[0] || !delta3[1] ||
-      !delta3[2])
+  if (kind < 0 || kind > 2) return fail(PF_EINVAL, "pf_adj_closure_term: kind is 0, 1 or 2");
+  if (!u || !v || !w || !ua || !va || !wa || !delta3 || !delta3[0] || !delta3[1] || !delta3[2] ||
+      (kind < 2 && !work))
     return fail(PF_EINVAL, "pf_adj_closure_term: NULL pointer");
-  if (kind == 1 && !params) return fail(PF_EINVAL, "pf_adj_closure_term: the MLP needs params");
+  if (kind >= 1 && !params) return fail(PF_EINVAL, "pf_adj_closure_term: the network needs params");
   const void* U[3] = {u, v, w};
   const void* A[3] = {ua, va, wa};
   void* D[3] = {delta3[0], delta3[1], delta3[2]};

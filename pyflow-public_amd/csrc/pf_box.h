@@ -76,6 +76,13 @@ struct pf_box {
   // features and 9 network outputs per padded cell, FP32
   float* src_ws;
   size_t src_ws_bytes;
+  // its reverse (training through the closure), reserved by
+  // pf_sfs_ml_source_train_reserve: activations and adjoints of a chunk of
+  // cells, the input adjoints of every cell, parameter-gradient partials, the
+  // closure's input arrays; and the normalisation table the reverse divides by
+  void* srct_ws;
+  size_t srct_bytes;
+  float srct_norm[21];
   int chunk;                 // iterations per launch batch between host checks
   int tile_rows;             // block rows of the tiled Krylov kernels: 256*tile_rows threads
   int64_t bytes;             // device bytes owned by the handle
@@ -103,6 +110,11 @@ int launch_blocks(const Geom& g);   // grid size for flat one-cell-per-thread ke
 int adj_closure_term(pf_box* h, int kind, const void* const U[3], const float* params,
                      const void* const A[3], void* work, void* const D[3], double* dparams,
                      hipStream_t s);
+// The same term for the reference's 'source' ML closure (pf_source_train.hip,
+// adjoint.py:230-286): D = -(face interpolant of d g/d var_i) with
+// g = sum over padded cells of G(var_i(U)) . (centre interpolant of A).
+int ml_source_adj_term(pf_box* h, const void* const U[3], const float* params,
+                       const void* const A[3], void* const D[3], double* dparams, hipStream_t s);
 size_t adj_work_bytes(const pf_box* h);
 
 inline double* part_buf(pf_box* h, int which) {

@@ -141,7 +141,7 @@ int pf_box_destroy(pf_box* h) {
   void* dev[] = {h->x, h->r, h->rh, h->p[0], h->p[1], h->v[0], h->v[1], h->s, h->t,
                  h->acc[0], h->acc[1], h->acc[2], h->ypong[0], h->ypong[1], h->ypong[2],
                  h->part, h->ctl, h->stats_dev, h->hist, h->mp_x, h->mp_b, h->mp_hist,
-                 h->src_ws};
+                 h->src_ws, h->srct_ws};
   for (void* p : dev)
     if (p) hipFree(p);
   if (h->prof_ev) {

@@ -721,7 +721,7 @@ int launch_adj_stage(pf_box* h, hipStream_t s, const void* const Y[3], const voi
 
 // Closure term of one adjoint stage, when the step carries a closure.
 struct AdjClosure {
-  int kind;              // -1 none, 0 gradient model, 1 MLP
+  int kind;              // -1 none, 0 gradient model, 1 MLP, 2 'source' ML closure
   const float* params;
   void* work;
   void* delta[3];
@@ -1039,10 +1039,11 @@ static int adj_closure_args(pf_box* h, int kind, const float* params, void* work
   cl->dparams = dparams;
   for (int q = 0; q < 3; ++q) cl->delta[q] = delta3 ? delta3[q] : nullptr;
   if (kind < 0) return 0;
-  if (kind > 1) return fail(PF_EINVAL, std::string(what) + ": closure kind is -1, 0 or 1");
-  if (!work || !delta3 || !delta3[0] || !delta3[1] || !delta3[2])
+  if (kind > 2) return fail(PF_EINVAL, std::string(what) + ": closure kind is -1, 0, 1 or 2");
+  if ((kind < 2 && !work) || !delta3 || !delta3[0] || !delta3[1] || !delta3[2])
     return fail(PF_EINVAL, std::string(what) + ": a closure needs work and delta3");
-  if (kind == 1 && !params) return fail(PF_EINVAL, std::string(what) + ": the MLP needs params");
+  if (kind >= 1 && !params)
+    return fail(PF_EINVAL, std::string(what) + ": the network closure needs params");
   return 0;
 }
 

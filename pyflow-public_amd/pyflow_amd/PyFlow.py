@@ -127,7 +127,7 @@ def simulate(inputConfig, verbose=True):
     if adjointTraining and not adjointVerification and not hasattr(getattr(D, 'sfsmodel', None),
                                                                    'finalize'):
         raise Exception('adjoint training against target data needs a trainable closure '
-                        "(SFSmodel 'MLP')")
+                        "(SFSmodel 'MLP' or 'ML')")
     caseName = getattr(inputConfig, 'caseName', 'PyFlow')
     monitor = None
     if getattr(inputConfig, 'writeMonitor', True):

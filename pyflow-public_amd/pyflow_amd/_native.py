@@ -97,6 +97,8 @@ SIGNATURES = [
     ('pf_sfs_ml_source', _I, [_P, _P, ctypes.POINTER(ctypes.c_float), _P, _P, _P, _P, _P, _P, _P]),
     ('pf_sfs_ml_source_params', ctypes.c_int64, []),
     ('pf_sfs_ml_source_reserve', _I, [_P]),
+    ('pf_sfs_ml_source_train_reserve', _I, [_P, ctypes.POINTER(ctypes.c_float)]),
+    ('pf_sfs_ml_source_vjp', _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _PD, _P]),
     ('pf_rk4_step_noalloc', _I, [_I, _I, _I, _D, _D, _D, _I, _P, _P, _P, _P, _P, _P, _D, _D, _D,
                                  _PP, _P]),
     ('pf_rk4_step_noalloc_max_bytes', ctypes.c_int64, []),
@@ -335,6 +337,37 @@ class Box:
         """Reserve the ML closure's workspace in the box (allocates once)."""
         check(lib().pf_sfs_ml_source_reserve(self.h), 'pf_sfs_ml_source_reserve')
 
+    def sfs_ml_source_train_reserve(self, norm21):
+        """Reserve the closure reverse's workspace (allocates once) and keep
+        the normalisation table for the adjoint closure term."""
+        nrm = (ctypes.c_float * 21)(*[float(x) for x in norm21])
+        check(lib().pf_sfs_ml_source_train_reserve(self.h, nrm), 'pf_sfs_ml_source_train_reserve')
+
+    def sfs_ml_source_vjp(self, params, ui, vi, wi, G, dI=None, dparams=None):
+        """dg/d(ui, vi, wi) into dI (three FP32 padded arrays, optional) and
+        dg/dparams added to dparams (FP64, optional), for g = sum(G_closure * G)."""
+        n = int(lib().pf_sfs_ml_source_params())
+        if not isinstance(params, torch.Tensor) or params.dtype != torch.float32 or \
+                params.device.type != 'cuda' or params.numel() != n or not params.is_contiguous():
+            raise ValueError('params must be a contiguous float32 GPU tensor of %d values' % n)
+        g = [self.padded(a, name) for name, a in zip(('gx', 'gy', 'gz'), G)]
+        d = [None, None, None]
+        if dI is not None:
+            for q, a in enumerate(dI):
+                if a.dtype != torch.float32 or tuple(a.shape) != self.padded_shape or \
+                        not a.is_contiguous() or a.device.type != 'cuda':
+                    raise ValueError('dI must be contiguous float32 padded GPU arrays')
+                d[q] = ctypes.c_void_p(a.data_ptr())
+        dp = None
+        if dparams is not None:
+            if dparams.dtype != torch.float64 or dparams.numel() != n or not dparams.is_contiguous():
+                raise ValueError('dparams must be a contiguous float64 tensor of %d values' % n)
+            dp = ctypes.cast(ctypes.c_void_p(dparams.data_ptr()), _PD)
+        check(lib().pf_sfs_ml_source_vjp(self.h, ctypes.c_void_p(params.data_ptr()),
+                                         self.padded(ui, 'u'), self.padded(vi, 'v'),
+                                         self.padded(wi, 'w'), *g, *d, dp, self.stream()),
+              'pf_sfs_ml_source_vjp')
+
     def sfs_smagorinsky(self, u, v, w, rho, Cs, Delta, mu, visc):
         check(lib().pf_sfs_smagorinsky(self.h, self.padded(u, 'u'), self.padded(v, 'v'),
                                        self.padded(w, 'w'), float(rho), float(Cs), float(Delta),
@@ -476,13 +509,21 @@ class Box:
               'pf_sfs_gradient')
 
     # -- adjoint step and reverse-mode kernels ---------------------------------
-    def _params(self, params):
+    @staticmethod
+    def _param_count(kind):
+        """Packed parameters of closure kind 1 (MLP) or 2 ('source' ML closure)."""
+        if kind == 2:
+            return int(lib().pf_sfs_ml_source_params())
+        return int(lib().pf_sfs_mlp_param_count())
+
+    def _params(self, params, kind=1):
         if params is None:
             return None
         if params.dtype != torch.float32 or params.device.type != 'cuda' or not params.is_contiguous():
-            raise ValueError('MLP parameters must be a contiguous float32 GPU tensor')
-        if params.numel() != lib().pf_sfs_mlp_param_count():
-            raise ValueError('wrong number of MLP parameters')
+            raise ValueError('closure parameters must be a contiguous float32 GPU tensor')
+        if params.numel() != self._param_count(kind):
+            raise ValueError('closure kind %d takes %d parameters, got %d'
+                             % (kind, self._param_count(kind), params.numel()))
         return ctypes.c_void_p(params.data_ptr())
 
     def _ptrs(self, fields, name):
@@ -491,12 +532,13 @@ class Box:
         return ctypes.cast(arr, _PP), arr
 
     @staticmethod
-    def _dparams(d):
+    def _dparams(d, kind=1):
         if d is None:
             return None
         if d.dtype != torch.float64 or d.device.type != 'cuda' or not d.is_contiguous() or \
-                d.numel() != lib().pf_sfs_mlp_param_count():
-            raise ValueError('dparams must be a contiguous float64 GPU tensor of the MLP size')
+                d.numel() != Box._param_count(kind):
+            raise ValueError('dparams must be a contiguous float64 GPU tensor of the closure '
+                             "kind's parameter count")
         return ctypes.cast(ctypes.c_void_p(d.data_ptr()), _PD)
 
     def sfs_mlp_f32(self, u, v, w, params, sfs6):
@@ -524,10 +566,10 @@ class Box:
     def adj_closure_term(self, kind, u, v, w, params, ua, va, wa, work, delta3, dparams=None):
         da = self._ptrs(delta3, 'delta')
         check(lib().pf_adj_closure_term(self.h, int(kind), self.padded(u, 'u'), self.padded(v, 'v'),
-                                        self.padded(w, 'w'), self._params(params),
+                                        self.padded(w, 'w'), self._params(params, kind),
                                         self.padded(ua, 'ua'), self.padded(va, 'va'),
                                         self.padded(wa, 'wa'), self._work(work), da[0],
-                                        self._dparams(dparams), self.stream()),
+                                        self._dparams(dparams, kind), self.stream()),
               'pf_adj_closure_term')
 
     def _work(self, work):
@@ -546,8 +588,8 @@ class Box:
                 self.padded(ua, 'ua'), self.padded(va, 'va'), self.padded(wa, 'wa'),
                 self.padded(uta, 'uta'), self.padded(vta, 'vta'), self.padded(wta, 'wta'),
                 float(dt))
-        tail = (float(rho), int(closure), self._params(params), self._work(work),
-                da[0] if da else None, self._dparams(dparams), self.stream())
+        tail = (float(rho), int(closure), self._params(params, closure), self._work(work),
+                da[0] if da else None, self._dparams(dparams, closure), self.stream())
         if isinstance(mu, torch.Tensor):
             check(lib().pf_adj_rk4_step_visc(self.h, *head, self.padded(mu, 'VISC'), *tail),
                   'pf_adj_rk4_step_visc')
@@ -560,8 +602,8 @@ class Box:
         da = self._ptrs(delta3, 'delta') if delta3 is not None else None
         head = (self.padded(u, 'u'), self.padded(v, 'v'), self.padded(w, 'w'),
                 self.padded(ua, 'ua'), self.padded(va, 'va'), self.padded(wa, 'wa'), float(dt))
-        tail = (float(rho), int(closure), self._params(params), self._work(work),
-                da[0] if da else None, self._dparams(dparams), self.stream())
+        tail = (float(rho), int(closure), self._params(params, closure), self._work(work),
+                da[0] if da else None, self._dparams(dparams, closure), self.stream())
         if isinstance(mu, torch.Tensor):
             check(lib().pf_adj_euler_step_visc(self.h, *head, self.padded(mu, 'VISC'), *tail),
                   'pf_adj_euler_step_visc')

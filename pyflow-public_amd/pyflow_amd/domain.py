@@ -115,9 +115,6 @@ class Domain:
         elif SFSmodelName == 'ML':
             from . import sfsmodel_ML
             self.sfsmodel = sfsmodel_ML.residual_stress(inputConfig, decomp, geometry)
-            if adjointTraining:
-                raise NotImplementedError("adjoint training through the 'source' ML closure "
-                                          "(adjoint.py:241-286) is not part of this build")
         else:
             self.use_SFSmodel = False
             self.sfsmodel = sfsmodel_smagorinsky.stress_constCs(inputConfig, geometry)

@@ -12,13 +12,25 @@ modelDictName: a dict with 'model_state_dict', read with weights_only=True),
 or `mlSourceParams` (packed vector, the layout of pack_state_dict), or a
 seeded draw in torch.nn.Linear's default ranges (`mlSourceSeed`, default 0).
 
-Training through this closure (the reference differentiates it with
-autograd, adjoint.py:241-286) is not part of this build: update() with
-requires_grad=True and finalize() raise.
+Training (sfsmodel_ML.py:160-352, adjoint.py:241-286).  The reference
+differentiates the closure with autograd inside its adjoint RHS; here the
+adjoint kernels do it by hand (closure kind 2 of pf_adj_closure_term, the
+network's reverse pass on MFMA) and add d(sum G . u_A)/d theta to `grad`, an
+FP64 accumulator, on every adjoint stage.  finalize() then follows the
+reference step by step: scale by -simDt, add to the running sum, every
+`numSumDistr` windows replace the distributed gradient by the mean of the last
+`numSumDistr`, take one RMSprop step (torch's defaults) with the distributed
+gradient, and halve the step size every `LR_decay` windows from 4/5 of
+LR_decay on.  As in the reference, the RMSprop step size starts at 0.01 (or
+the checkpoint's optimizer state) whatever `LR` says: the reference builds
+its optimizer before it reads inputConfig.LR, which then only sets the
+reported rate.
 """
 
 import numpy as np
 import torch
+
+from .sfsmodel_mlp import GradientAverager, rmsprop_step
 
 # (name, outputs, inputs) of NeuralNetworkModel2's layers (sfsmodel_ML.py:76-87),
 # in the packed order pf_sfs_ml_source takes
@@ -102,31 +114,108 @@ class residual_stress:
             raise ValueError('sfsmodel_ML needs %d parameters, got %d'
                              % (N_PARAMS, self.params.numel()))
         self.norm = [float(x) for x in NORM]
+        # training state (sfsmodel_ML.py:160-252)
+        self.grad = torch.zeros(N_PARAMS, dtype=torch.float64, device=self.device)
+        self.square_avg = torch.zeros(N_PARAMS, dtype=torch.float32, device=self.device)
+        self.averager = GradientAverager(N_PARAMS, getattr(inputConfig, 'numSumDistr', 8),
+                                         self.device)
+        self.LR_decay = int(getattr(inputConfig, 'LR_decay', 2500))
+        self.modelDictSave = getattr(inputConfig, 'modelDictSave', None)
+        if self._ckpt is not None:
+            self._load_training_state(self._ckpt)
+        self._ckpt = None
+        if hasattr(inputConfig, 'LR'):
+            self.LR = float(inputConfig.LR)
+        if getattr(inputConfig, 'adjointTraining', False):
+            # the reverse pass's workspace (allocates once)
+            self.box.sfs_ml_source_train_reserve(self.norm)
+
+    # the RMSprop step size of the reference's optimizer (sfsmodel_ML.py:164-168)
+    step_lr = 0.01
+    _ckpt = None
 
     def load(self, path):
-        """A checkpoint in the reference's format (sfsmodel_ML.py:192-205)."""
+        """A checkpoint in the reference's format (sfsmodel_ML.py:176-205),
+        read with weights_only=True: epoch, LR, model_state_dict and, when
+        present, the RMSprop state of optimizer_state_dict and the
+        gradient-averaging lists (as tensors)."""
         d = torch.load(path, weights_only=True)
         self.epoch = int(d.get('epoch', 0))
         self.LR = float(d.get('LR', self.LR))
+        self._ckpt = d
         return pack_state_dict(d['model_state_dict'])
 
+    def _flat(self, tensors):
+        return torch.cat([torch.as_tensor(t).reshape(-1).to(torch.float64) for t in tensors]) \
+            .to(self.device)
+
+    def _load_training_state(self, d):
+        opt = d.get('optimizer_state_dict')
+        if opt:
+            groups = opt.get('param_groups') or [{}]
+            self.step_lr = float(groups[0].get('lr', self.step_lr))
+            st = opt.get('state') or {}
+            if len(st) == 2 * len(LAYERS):
+                self.square_avg.copy_(self._flat(st[i]['square_avg'] for i in range(len(st))))
+        for key, attr in (('outParamsList', 'out'), ('sumParamsList', 'sum')):
+            lst = d.get(key)
+            if lst:
+                setattr(self.averager, attr, self._flat(lst))
+
     def state_dict(self):
+        """The model's parameters with the reference model's names."""
         return unpack_state_dict(self.params)
 
+    def save(self):
+        """sfsmodel_ML.py:270-280: the reference's checkpoint keys; the
+        averaging lists are stored as tensors so weights_only loads read them."""
+        if not self.modelDictSave:
+            return
+        sd = unpack_state_dict(self.params)
+        names = list(sd)
+        sq = unpack_state_dict(self.square_avg)
+        state = {i: {'step': torch.tensor(float(self.epoch)), 'square_avg': sq[n]}
+                 for i, n in enumerate(names)}
+        group = {'lr': self.step_lr, 'alpha': 0.99, 'eps': 1e-08, 'weight_decay': 0,
+                 'momentum': 0, 'centered': False, 'params': list(range(len(names)))}
+
+        def split(v):
+            return list(unpack_state_dict(v.to(torch.float32)).values())
+        torch.save({'epoch': self.epoch, 'LR': self.LR, 'model_state_dict': sd,
+                    'optimizer_state_dict': {'state': state, 'param_groups': [group]},
+                    'outParamsList': split(self.averager.out),
+                    'sumParamsList': split(self.averager.sum)}, self.modelDictSave)
+
+    def param_grad(self):
+        """FP64 accumulator the adjoint kernels add d(sum G . u_A)/d theta to."""
+        return self.grad
+
     def update(self, u_P, v_P, w_P, requires_grad=False):
-        """sfsmodel_ML.py:354-404: GX, GY, GZ from the RHS's var_i arrays."""
-        if requires_grad:
-            raise NotImplementedError("gradients through the 'source' ML closure are not part "
-                                      "of this build")
+        """sfsmodel_ML.py:354-404: GX, GY, GZ from the RHS's var_i arrays.
+        requires_grad only says that the caller will differentiate; the
+        adjoint closure term does that by hand, so the evaluation is the same."""
         self.box.sfs_ml_source(self.params, self.norm, u_P, v_P, w_P, (self.GX, self.GY, self.GZ))
 
     def finalize(self, comms=None):
-        raise NotImplementedError("training the 'source' ML closure is not part of this build")
+        """sfsmodel_ML.py:284-352 on one process: the window's gradient times
+        -simDt, the running average over numSumDistr windows, one RMSprop step
+        with the distributed average, and the step-size decay."""
+        g = self.grad * (-self.simDt)
+        self.last_window_grad = g.clone()
+        avg = self.averager.push(g, self.epoch)
+        rmsprop_step(self.params, avg.to(torch.float32), self.square_avg, self.step_lr)
+        self.grad.zero_()
+        self.epoch += 1
+        if self.epoch >= 4 * self.LR_decay // 5 and self.epoch % self.LR_decay == 0:
+            self.LR *= 0.5
+            self.step_lr *= 0.5
 
 
 def main(argv=None):
-    """Time one closure evaluation on random padded fields:
-    python -m pyflow_amd.sfsmodel_ML --N 64 --dtype f32 --reps 5"""
+    """Time one closure evaluation on random padded fields, or with --reverse
+    its reverse pass (pf_sfs_ml_source_vjp: the forward recomputed, the
+    input- and weight-gradient products, three times the forward's):
+    python -m pyflow_amd.sfsmodel_ML --N 64 --dtype f32 --reps 5 [--reverse]"""
     import argparse
     import math
     import types
@@ -135,6 +224,7 @@ def main(argv=None):
     ap.add_argument('--N', type=int, default=64)
     ap.add_argument('--dtype', choices=('f32', 'f64'), default='f32')
     ap.add_argument('--reps', type=int, default=5)
+    ap.add_argument('--reverse', action='store_true')
     a = ap.parse_args(argv)
     dt = torch.float32 if a.dtype == 'f32' else torch.float64
     n = a.N
@@ -143,21 +233,30 @@ def main(argv=None):
     d = types.SimpleNamespace(prec=dt, device=torch.device('cuda'), box=box, nxo_=n + 4,
                               nyo_=n + 4, nzo_=n + 4)
     geo = types.SimpleNamespace(type='uniform', dx=h)
-    ml = residual_stress(types.SimpleNamespace(simDt=1e-3), d, geo)
+    ml = residual_stress(types.SimpleNamespace(simDt=1e-3, adjointTraining=a.reverse), d, geo)
     gen = torch.Generator(device='cuda').manual_seed(0)
     U = [torch.randn((n + 4,) * 3, dtype=dt, device='cuda', generator=gen) for _ in range(3)]
-    ml.update(*U)
+    if a.reverse:
+        Gb = [torch.randn((n + 4,) * 3, dtype=dt, device='cuda', generator=gen) for _ in range(3)]
+        dI = [torch.empty((n + 4,) * 3, dtype=torch.float32, device='cuda') for _ in range(3)]
+
+        def run():
+            box.sfs_ml_source_vjp(ml.params, *U, Gb, dI, ml.grad)
+    else:
+        def run():
+            ml.update(*U)
+    run()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(a.reps):
-        ml.update(*U)
+        run()
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / a.reps
-    flop = 2.0 * (n + 4) ** 3 * sum(o * i for _, o, i in LAYERS)
-    print('ML source closure %d^3 %s: %.3f ms per evaluation, %.1f TFLOP/s of the network'
-          % (n, a.dtype, ms, flop / ms / 1e9), flush=True)
+    flop = 2.0 * (n + 4) ** 3 * sum(o * i for _, o, i in LAYERS) * (3 if a.reverse else 1)
+    print('ML source closure %s %d^3 %s: %.3f ms per evaluation, %.1f TFLOP/s of the network'
+          % ('reverse' if a.reverse else 'forward', n, a.dtype, ms, flop / ms / 1e9), flush=True)
     return 0
 
 

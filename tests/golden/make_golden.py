@@ -26,7 +26,10 @@ functions that no closed form or textbook routine pins on its own:
   (source_stub.RecordingSource), and the arrays the reference hands it;
 * the reference's 'source' ML closure itself (sfsmodel/sfsmodel_ML.py:354-593,
   NeuralNetworkModel2 with the seeded weights of source_stub.ml_source_params)
-  on those arrays: GX, GY, GZ and the network's output channels 0-8.
+  on those arrays: GX, GY, GZ and the network's output channels 0-8;
+* rhs_adjPredictor.evaluate with that ML closure (the closure term of
+  adjoint.py:230-286 by the reference's own autograd): the adjoint RHS and
+  the parameter gradients the evaluation leaves in the model.
 
 The reference's MPI layer is replaced by a single-rank periodic stub with the
 semantics of library/parallel.py:301-414 on a periodic Cartesian communicator
@@ -422,6 +425,26 @@ def main():
     out['mlsrc_GX'], out['mlsrc_GY'], out['mlsrc_GZ'] = (
         ml.GX.numpy().copy(), ml.GY.numpy().copy(), ml.GZ.numpy().copy())
     out['mlsrc_p'] = ml.model.out[..., :9].numpy().copy()
+
+    # 13. the adjoint RHS with the ML closure: its closure term
+    #     (adjoint.py:230-286) and the parameter gradients g.backward() leaves
+    with contextlib.redirect_stdout(io.StringIO()):
+        ml2 = sfsmodel_ML.residual_stress(cfg, d, geo)
+    ml2.model.load_state_dict(ml_state_dict(ml_source_params()))
+    ml2.model.zero_grad()
+    su, sv, sw = states(u0, v0, w0)
+    au, av, aw = states(a0, b0, c0)
+    ar = adjoint.rhs_adjPredictor(d, metric, rho, VISC, ml2, su, sv, sw)
+    ar.evaluate(au, av, aw)
+    out['adjrhs_ml_u'], out['adjrhs_ml_v'], out['adjrhs_ml_w'] = (
+        ar.rhs_u.numpy().copy(), ar.rhs_v.numpy().copy(), ar.rhs_w.numpy().copy())
+    grads = []
+    for name, _, _ in (('fc1', 0, 0), ('fcG', 0, 0), ('fcG2', 0, 0), ('fc2', 0, 0),
+                       ('fc3', 0, 0), ('fc4', 0, 0)):
+        layer = getattr(ml2.model, name)
+        grads.append(layer.weight.grad.numpy().ravel().copy())
+        grads.append(layer.bias.grad.numpy().ravel().copy())
+    out['adjrhs_ml_dparams'] = np.concatenate(grads)
 
     np.savez_compressed(OUT, **out)
     print('wrote', OUT, 'keys:', len(out))

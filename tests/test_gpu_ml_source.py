@@ -120,8 +120,8 @@ def test_checkpoint_in_reference_format(tmp_path):
                                                            modelDictName=path), d, g)
     assert np.array_equal(ml.params.cpu().numpy(), flat)
     assert ml.epoch == 4 and ml.LR == 0.005
-    with pytest.raises(NotImplementedError):
-        ml.update(ml.GX[..., 0], ml.GY[..., 0], ml.GZ[..., 0], requires_grad=True)
+    # without an optimizer state the step size is the reference optimizer's default
+    assert ml.step_lr == 0.01
 
 
 def test_run_loop_with_ml_closure():

@@ -184,3 +184,36 @@ def test_adjoint_rhs_with_viscosity_field_golden(G):
     r.evaluate(*(O.Field(g, G[k]) for k in ('adj_u0', 'adj_v0', 'adj_w0')))
     for c in 'uvw':
         assert np.array_equal(getattr(r, 'rhs_' + c).numpy(), G['adjrhs_smag_' + c])
+
+
+def _ml_closure_rhs(G, g, trainable=True):
+    from helpers import ml_source_params
+    _, sd = ml_source_params()
+    fu, fv, fw = (O.Field(g, G[c + '0']) for c in 'uvw')
+    au, av, aw = (O.Field(g, G['adj_%s0' % c]) for c in 'uvw')
+    ml = O.MLSource(g, sd, trainable=trainable)
+    r = A.AdjointRHS(g, float(G['rho']), visc(g, float(G['mu'])), fu, fv, fw, closure=ml)
+    r.evaluate(au, av, aw)
+    return r, ml
+
+
+def test_source_ml_closure_adjoint_term_golden(G):
+    """rhs_adjPredictor.evaluate with the reference's 'source' ML closure
+    (adjoint.py:230-286): the closure's share of the adjoint RHS and the
+    parameter gradients its g.backward() leaves, against the reference's own
+    (golden vectors).  The oracle issues the same torch CPU ops; autograd may
+    accumulate the gradient contributions of a tensor in another order, so
+    the bar is FP32 rounding (measured below)."""
+    g = grid(G)
+    r, ml = _ml_closure_rhs(G, g)
+    for c, rr in zip('uvw', (r.rhs_u, r.rhs_v, r.rhs_w)):
+        got = rr.numpy() - G['adjrhs_' + c]
+        ref = G['adjrhs_ml_' + c] - G['adjrhs_' + c]
+        e = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        print('closure term %s: rel L2 %.3e (max|term| %.3e)' % (c, e, np.abs(ref).max()))
+        assert e < 1e-5
+    gp = ml.packed_grad().numpy().astype(np.float64)
+    ref = G['adjrhs_ml_dparams'].astype(np.float64)
+    e = np.linalg.norm(gp - ref) / np.linalg.norm(ref)
+    print('parameter gradient rel L2 %.3e' % e)
+    assert e < 1e-5

@@ -102,3 +102,104 @@ def test_gradient_averaging_cadence():
     assert seen[0] == [0.0, 0.0] and seen[1] == [0.0, 0.0]
     assert seen[2] == [2.0, 2.0] and seen[3] == [2.0, 2.0] and seen[4] == [2.0, 2.0]
     assert seen[5] == [5.0, 5.0] and seen[6] == [5.0, 5.0]
+
+
+class _HostBox:
+    """Stands in for the GPU box: the ML closure's constructor only reserves
+    workspaces through it, and finalize() is host-side tensor arithmetic."""
+
+    def __init__(self):
+        self.reserved = []
+
+    def sfs_ml_source_reserve(self):
+        self.reserved.append('forward')
+
+    def sfs_ml_source_train_reserve(self, norm):
+        self.reserved.append(('train', len(norm)))
+
+
+def _ml_closure(**cfg):
+    import types
+    from pyflow_amd import sfsmodel_ML
+    d = types.SimpleNamespace(prec=torch.float64, device=torch.device('cpu'), box=_HostBox(),
+                              nxo_=6, nyo_=6, nzo_=6)
+    geo = types.SimpleNamespace(type='uniform', dx=0.5)
+    return sfsmodel_ML.residual_stress(types.SimpleNamespace(**cfg), d, geo)
+
+
+def test_ml_closure_finalize_matches_reference_bookkeeping():
+    """sfsmodel_ML.py:284-352 with torch.optim.RMSprop over the model's twelve
+    parameter tensors: grad *= -simDt, running sums, the mean every
+    numSumDistr windows, the optimizer step, and the halving of the step size
+    at epoch LR_decay (>= 4/5 LR_decay).  The reference builds its optimizer
+    with lr 0.01 before it reads inputConfig.LR, so an LR override changes the
+    reported rate only.  The window gradients keep one sign per parameter:
+    this closure sums them in FP64 where the reference sums FP32 arrays, and
+    sums that cancel would turn that rounding into a visible difference of
+    the RMSprop step wherever the mean falls to the size of its eps."""
+    from pyflow_amd import sfsmodel_ML
+    simDt, nsum, decay = 2.5e-3, 2, 5
+    ml = _ml_closure(simDt=simDt, numSumDistr=nsum, LR_decay=decay, LR=0.003,
+                     mlSourceSeed=4, adjointTraining=True)
+    assert ml.box.reserved == ['forward', ('train', 21)]
+    sd = sfsmodel_ML.unpack_state_dict(ml.params.clone())
+    params = [torch.nn.Parameter(t.clone()) for t in sd.values()]
+    opt = torch.optim.RMSprop(params, lr=0.01)
+    out = [torch.zeros_like(p) for p in params]
+    acc = [torch.zeros_like(p) for p in params]
+    gen = torch.Generator().manual_seed(9)
+    sign = torch.sign(torch.randn(sfsmodel_ML.N_PARAMS, generator=gen))
+    for epoch in range(7):
+        mag = 0.5 + torch.rand(sfsmodel_ML.N_PARAMS, generator=gen)
+        g32 = sign * mag * 10.0 ** (epoch % 3)
+        ml.grad.copy_(g32.double())
+        ml.finalize()
+        # the reference's steps, on its per-tensor lists
+        o = 0
+        for i, p in enumerate(params):
+            n = p.numel()
+            gi = g32[o:o + n].reshape(p.shape) * -simDt
+            o += n
+            acc[i] += gi
+            if (epoch + 1) % nsum == 0:
+                out[i] = acc[i] / float(nsum)
+                acc[i] = torch.zeros_like(p)
+            p.grad = out[i].clone()
+        opt.step()
+        opt.zero_grad()
+        if epoch + 1 >= 4 * decay // 5 and (epoch + 1) % decay == 0:
+            for grp in opt.param_groups:
+                grp['lr'] *= 0.5
+        ref = torch.cat([p.detach().reshape(-1) for p in params])
+        err = float((ml.params - ref).abs().max() / ref.abs().max())
+        assert err < 1e-6, (epoch, err)
+        assert float(ml.grad.abs().max()) == 0.0
+    assert ml.epoch == 7
+    assert ml.step_lr == opt.param_groups[0]['lr'] == 0.005
+    assert ml.LR == 0.0015
+
+
+def test_ml_closure_checkpoint_roundtrip(tmp_path):
+    """save() writes the reference's keys (epoch, LR, model_state_dict,
+    optimizer_state_dict, outParamsList, sumParamsList); loading it with
+    weights_only=True restores the parameters, the RMSprop state, the
+    averaging sums and the step size."""
+    path = str(tmp_path / 'ml.pt')
+    a = _ml_closure(simDt=1e-3, numSumDistr=3, LR_decay=2, mlSourceSeed=2, modelDictSave=path)
+    gen = torch.Generator().manual_seed(1)
+    for _ in range(4):
+        a.grad.copy_(torch.randn(a.grad.numel(), generator=gen, dtype=torch.float64))
+        a.finalize()
+    a.save()
+    d = torch.load(path, weights_only=True)
+    assert set(d) == {'epoch', 'LR', 'model_state_dict', 'optimizer_state_dict',
+                      'outParamsList', 'sumParamsList'}
+    assert list(d['model_state_dict']) == ['fc1.weight', 'fc1.bias', 'fcG.weight', 'fcG.bias',
+                                           'fcG2.weight', 'fcG2.bias', 'fc2.weight', 'fc2.bias',
+                                           'fc3.weight', 'fc3.bias', 'fc4.weight', 'fc4.bias']
+    b = _ml_closure(simDt=1e-3, numSumDistr=3, LR_decay=2, loadModel=True, modelDictName=path)
+    assert b.epoch == a.epoch == 4 and b.step_lr == a.step_lr == 0.0025
+    assert torch.equal(b.params, a.params)
+    assert torch.equal(b.square_avg, a.square_avg)
+    assert torch.allclose(b.averager.out, a.averager.out, rtol=1e-6, atol=0)
+    assert torch.allclose(b.averager.sum, a.averager.sum, rtol=1e-6, atol=0)
