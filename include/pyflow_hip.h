@@ -25,7 +25,8 @@
  *     with periodic images, as state_P.update_border does.
  *   - Device memory is allocated only by pf_box_create and by the set-up
  *     calls that reserve optional workspace in a box (pf_mpir_configure,
- *     pf_sfs_ml_source_reserve, pf_residual_history_enable; pf_profile_enable
+ *     pf_sfs_ml_source_reserve, pf_sfs_ml_source_train_reserve,
+ *     pf_residual_history_enable; pf_profile_enable
  *     creates events).  Every other entry point allocates nothing and fails
  *     with PF_EINVAL when the workspace it needs was not reserved, so a box's
  *     pf_box_device_bytes is final once its set-up calls are done.  A box may
@@ -451,17 +452,19 @@ int pf_sfs_mlp_f32(pf_box* box, const void* u, const void* v, const void* w, con
  *   pf_rhs_adj: RHS into ru, rv, rw (interior arrays).
  *   pf_adj_rk4_step / pf_adj_euler_step: the reference's RK4 / Euler on the
  *     adjoint RHS (advancer.py:49-113); uta..wta receive the last stage state.
- *     closure = -1 (none), 0 (gradient model) or 1 (MLP with `params`): the
- *     closure term of every stage is recomputed from that stage's adjoint
- *     state with pf_adj_closure_term; then work (pf_adj_work_bytes bytes) and
- *     delta3 (three padded scratch arrays) are required and dparams (746
- *     doubles, may be NULL) accumulates the MLP parameter gradient of every
- *     stage (the reference accumulates its closure's parameter gradients
- *     over every adjoint RHS evaluation the same way, adjoint.py:266-270).
- *   pf_adj_closure_term: delta = (d tau/du)^T (d R/d tau)^T a for a 'tensor'
- *     closure tau at the forward state (the reference's closure term,
- *     adjoint.py:241-286, is this transpose for its 'source' closure),
- *     folded onto the interior with periodic overlap values.
+ *     closure = -1 (none), 0 (gradient model), 1 (MLP with `params`) or 2
+ *     (the reference's 'source' ML closure with `params`): the closure term
+ *     of every stage is recomputed from that stage's adjoint state with
+ *     pf_adj_closure_term; then delta3 (three padded scratch arrays) is
+ *     required, work (pf_adj_work_bytes bytes) too for kinds 0 and 1, and
+ *     dparams (the closure's packed parameter count of doubles, may be NULL)
+ *     accumulates the parameter gradient of every stage (the reference
+ *     accumulates its closure's parameter gradients over every adjoint RHS
+ *     evaluation the same way, adjoint.py:266-270).
+ *   pf_adj_closure_term: kinds 0 and 1: delta = (d tau/du)^T (d R/d tau)^T a
+ *     for a 'tensor' closure tau at the forward state, folded onto the
+ *     interior with periodic overlap values; kind 2: the reference's own
+ *     term (adjoint.py:230-286), see pf_sfs_ml_source_vjp.
  *   pf_adj_conv: one adj_conv_* method on padded arrays: for s in {0, e_s2},
  *     rhs -= ((a[m - e_d + s] - a[m + s]) * vi[m + s + vsh]) * coef with
  *     vsh = -e_d when centre != 0 (adj_conv_xx/yy/zz) and 0 otherwise.
