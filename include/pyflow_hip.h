@@ -262,7 +262,7 @@ int pf_sfs_ml_source_vjp(pf_box* box, const float* params, const void* ui, const
                          const void* wi, const void* gx, const void* gy, const void* gz,
                          float* dui, float* dvi, float* dwi, double* dparams, void* stream);
 /* Set-up call: reserve the reverse's workspace in the box (about 1.2 GB at
- * 64^3, 3.6 GB at 128^3; allocates) and keep the 21-entry normalisation table
+ * 64^3, 3.8 GB at 128^3; allocates) and keep the 21-entry normalisation table
  * (sfsmodel_ML.py:252-258) for the adjoint closure term.  Idempotent. */
 int pf_sfs_ml_source_train_reserve(pf_box* box, const float* norm21);
 

@@ -13,19 +13,25 @@
 // into an FP64 accumulator.
 //
 // The reverse runs as:
-//  1. the closure's input features of every padded cell (pf_source.h);
+//  1. the closure's input features of every padded cell (pf_source.h) and a
+//     copy of the weights in the products' padded layouts (pack_kernel);
 //  2. per chunk of cells: the 273 network inputs (the cell's features and
 //     those of its 12 neighbours), the forward activations of every layer,
 //     the output adjoints from the divergence's transpose, the reverse through
-//     the gated network, the input adjoints of the chunk's cells and the
-//     parameter-gradient sums; every layer is a GEMM over the chunk's cells
-//     (gemm_kernel: v_mfma_f32_16x16x4_f32, cells on M) followed by an
-//     elementwise kernel (bias, tanh, gates, tanh' = 1 - y*y);
+//     the gated network into the input adjoints of the chunk's cells, and the
+//     parameter-gradient sums.  Every layer is one tiled FP32 MFMA product
+//     (mm_kernel, v_mfma_f32_16x16x4_f32, cells on M) whose epilogue applies
+//     the layer's elementwise part (bias and tanh, the gates, tanh' = 1-y*y);
+//     fc1, fcG and fcG2 share their input, so their three products are one
+//     (624 = 3 x 208 columns), and so are their three input-adjoint products
+//     (K = 624);
 //  3. the feature adjoints of every cell (the 13 shifted copies gathered
 //     back, divided by the normalisation table) and the transpose of the
 //     finite differences: d g / d var_i.
-// Parameter gradients are split-K GEMMs over the chunk's cells into FP32
-// partials, summed in a fixed order into FP64: deterministic results.
+// Parameter gradients are split-K products over 1,024-cell ranges of the
+// chunk into FP32 partials laid out like the packed parameters (the bias
+// gradient is the product's column against a column of ones), summed in a
+// fixed order into FP64: deterministic results.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -41,9 +47,23 @@ extern "C" int pf_sfs_ml_source_reserve(pf_box* h);
 namespace {
 
 constexpr int kChunkRows = 65536;   // cells per chunk of the network reverse
-constexpr int kSplitK = 2048;       // cells per split of the parameter-gradient GEMMs
-constexpr int kXw = kSrcIn;         // network input width (273)
+constexpr int kSplitK = 1024;       // cells per split of the parameter-gradient products
+constexpr int kMaxSplits = kChunkRows / kSplitK;
 constexpr int kH = kSrcHid;         // hidden width (200)
+// Padded widths: every reduction length is a multiple of the 16-deep K
+// stage, so products never mask K except over cells; pad columns hold zeros.
+constexpr int kHp = 208;            // hidden width padded (13 x 16)
+constexpr int kCat = 3 * kHp;       // fc1 | fcG | fcG2 outputs (624)
+constexpr int kXld = 288;           // network input width padded (273 -> 18 x 16)
+constexpr int kOutp = 32;           // fc4 outputs padded (18 -> 2 x 16)
+constexpr int kTileM = 128;         // rows (cells) per product tile
+// packed weights: Wcat [624][288] (fc1, fcG, fcG2 rows at 0, 208, 416),
+// W2p, W3p [208][208], W4p [32][208]
+constexpr int64_t kPkCat = 0;
+constexpr int64_t kPkW2 = kPkCat + (int64_t)kCat * kXld;
+constexpr int64_t kPkW3 = kPkW2 + (int64_t)kHp * kHp;
+constexpr int64_t kPkW4 = kPkW3 + (int64_t)kHp * kHp;
+constexpr int64_t kPkFloats = kPkW4 + (int64_t)kOutp * kHp;
 
 inline size_t al256(size_t n) { return (n + 255) / 256 * 256; }
 
@@ -51,10 +71,10 @@ inline size_t al256(size_t n) { return (n + 255) / 256 * 256; }
 struct Srct {
   int64_t m;        // padded cells
   int C;            // rows per chunk
-  int splits;       // parameter-gradient splits of a full chunk
+  int Ca;           // rows allocated per chunk array (C rounded up to a tile)
   size_t off_fwd, off_adj;            // 3 padded box-dtype arrays each
-  size_t off_dX, off_dF, off_dvar;    // FP32: m x 273, 21 x m, 3 x m
-  size_t off_X, off_act, off_dout, off_adjb, off_ones, off_part;
+  size_t off_dX, off_dF, off_dvar;    // FP32: m x 288, 21 x m, 3 x m
+  size_t off_X, off_cat, off_h, off_dout, off_dcat, off_dl, off_pack, off_part;
   size_t bytes;
 };
 
@@ -62,7 +82,7 @@ Srct srct_layout(const pf_box* h) {
   Srct L;
   L.m = (int64_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4);
   L.C = L.m < kChunkRows ? (int)L.m : kChunkRows;
-  L.splits = (L.C + kSplitK - 1) / kSplitK;
+  L.Ca = (L.C + kTileM - 1) / kTileM * kTileM;
   size_t o = 0;
   auto take = [&](size_t n) {
     const size_t at = o;
@@ -70,144 +90,375 @@ Srct srct_layout(const pf_box* h) {
     return at;
   };
   const size_t pad = (size_t)L.m * h->esize;
+  const size_t ca = (size_t)L.Ca;
   L.off_fwd = take(3 * pad);
   L.off_adj = take(3 * pad);
-  L.off_dX = take((size_t)L.m * kXw * 4);
+  L.off_dX = take((size_t)L.m * kXld * 4);
   L.off_dF = take((size_t)L.m * kSrcFeat * 4);
   L.off_dvar = take((size_t)L.m * 3 * 4);
-  L.off_X = take((size_t)L.C * kXw * 4);
-  L.off_act = take((size_t)L.C * kH * 4 * 7);      // H1, G, Gate2, H2, H2G, H3, H4
-  L.off_dout = take((size_t)L.C * kSrcOut * 4);
-  L.off_adjb = take((size_t)L.C * kH * 4 * 6);     // dH, dL1, dLG, dLG2, dL2, dL4
-  L.off_ones = take((size_t)L.C * 4);
-  L.off_part = take((size_t)L.splits * kSrcParams * 4);
+  L.off_X = take(ca * kXld * 4);
+  L.off_cat = take(ca * kCat * 4);          // H1 | G | Gate2
+  L.off_h = take(ca * kHp * 4 * 4);         // H2, H2G, H3, H4
+  L.off_dout = take(ca * kOutp * 4);
+  L.off_dcat = take(ca * kCat * 4);         // dL1 | dLG | dLG2
+  L.off_dl = take(ca * kHp * 4 * 2);        // dL4, dL2
+  L.off_pack = take((size_t)kPkFloats * 4);
+  L.off_part = take((size_t)kMaxSplits * kSrcParams * 4);
   L.bytes = o;
   return L;
 }
 
+__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
 // ---------------------------------------------------------------------------
-// C (op) op(A) op(B): A is M x K, B is K x N after their transposes; row
-// major storage with leading dimensions.  TA: A stored as [K][M]; TB: B
-// stored as [N][K].  Split-K over blockIdx.z (K range kchunk, C offset cz).
-// act: 0 store, 1 store tanh(acc + bias), 2 accumulate, 3 store acc + bias.
-struct GemmOp {
+// C = A B over a tile grid, A M x K, B K x N (FP32, row-major, leading
+// dimensions in floats, every row 16-byte aligned).  ACOL: A stored as
+// [K][M]; else [M][K].  BCOL: B stored as [K][N]; else [N][K].  Split-K over
+// `splits` ranges of kchunk (only the cell dimension is split; it is the only
+// K that is masked).  Epilogues (EPI), with d = the tile's accumulator at
+// (row, col):
+//   kEpiStore     C = d (col < N)
+//   kEpiTanhCat   C = tanh(d + bias) over 3 groups of 208 columns (bias of
+//                 group col/208 at col%208 < 200, else 0)
+//   kEpiTanhGate  h = tanh(d + bias[col]); C = h; C2 = h * E1
+//   kEpiRevPair   C = (d*E1)*(1 - E2*E2); C2 = (d*E2)*(1 - E1*E1)
+//   kEpiRevTanh   C = d*(1 - E1*E1)
+//   kEpiPartial   the parameter-gradient partials of split z: row = group
+//                 row/rg, output o = row%rg (< nout); col < nin: weight
+//                 [o][col], col == nin: the bias (B's column nin is ones)
+enum { kEpiStore, kEpiTanhCat, kEpiTanhGate, kEpiRevPair, kEpiRevTanh, kEpiPartial };
+
+struct MmOp {
   const float* A;
   int64_t lda;
-  int ta;
   const float* B;
   int64_t ldb;
-  int tb;
+  int M, N, K;
+  int kchunk, splits;
+  int ones;                 // B column replaced by ones (-1: none)
+  int tiles_m, tiles_n;
   float* C;
   int64_t ldc;
-  int M, N, K;
-  int kchunk;
-  int64_t cz;
-  const float* bias;
-  int act;
+  float* C2;
+  int64_t ldc2;
+  const float* E1;
+  int64_t lde1;
+  const float* E2;
+  int64_t lde2;
+  const float* bias[3];
+  int rg, nout, nin;
+  float* part;              // slab z at part + z*kSrcParams
+  int64_t woff[3], boff[3];
 };
 
-constexpr int kGM = 64, kGN = 64, kGK = 16;
+// Up to three products of one kind in one launch (the small
+// parameter-gradient products share a launch to fill the GPU).
+struct MmBatch {
+  MmOp op[3];
+  int nops;
+  int start[4];             // first block of each product
+};
 
-// 256 threads, a 64 x 64 tile of C; each wave a 32 x 32 quarter as 2 x 2
-// MFMA tiles of 16 x 16 (lane l: A[l%16][l/16], B[l/16][l%16], C rows
-// 4*(l/16)+i, column l%16).  The K slice of each step is staged through LDS.
-__global__ void __launch_bounds__(256) gemm_kernel(GemmOp op) {
-  __shared__ float As[kGK][kGM + 4];
-  __shared__ float Bs[kGK][kGN + 4];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int m0 = blockIdx.y * kGM, n0 = blockIdx.x * kGN;
-  const int kb = blockIdx.z * op.kchunk;
+constexpr int kPitch = 20;  // LDS floats per row of a 16-deep K slice
+
+// 256 threads = 4 waves stacked along M; wave w owns rows 32w..32w+31 of the
+// 128-row tile (2 MFMA row tiles) and all TN*16 columns.  The K slice of a
+// stage sits in LDS as [row][k] for both operands (pitch 20: the 128-bit
+// reads of a pass land on distinct banks), so one 128-bit read gives a lane
+// its operands of four consecutive MFMAs (lane l: row l%16, k = 4*(l/16)+s
+// in MFMA s; both operands use the same K order).  Double-buffered: the next
+// stage's global loads are in flight in registers while the current one is
+// consumed; one barrier per stage.
+template <int TN, bool ACOL, bool BCOL, int EPI>
+__global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
+  constexpr int BM = kTileM, BN = TN * 16;
+  constexpr int kAF4 = BM * 16 / 4 / 256;              // float4 of A per thread (2)
+  constexpr int kBF4 = (BN * 16 / 4 + 255) / 256;       // float4 of B per thread
+  __shared__ __align__(16) float lds[2 * (BM + BN) * kPitch];
+
+  int oi = 0;
+  while (oi + 1 < bt.nops && (int)blockIdx.x >= bt.start[oi + 1]) ++oi;
+  const MmOp& op = bt.op[oi];
+  const int nb = bt.start[oi + 1] - bt.start[oi];
+  int b = (int)blockIdx.x - bt.start[oi];
+  // XCD-aware order: blocks are dealt to the 8 XCDs round robin; give each
+  // XCD a contiguous run of tiles, so the tiles of one row block (and of one
+  // split) share that XCD's L2
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  const int per = op.tiles_m * op.tiles_n;
+  const int z = b / per;
+  const int tt = b - z * per;
+  const int tm = tt / op.tiles_n, tn = tt - tm * op.tiles_n;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int kb = z * op.kchunk;
   const int ke = min(op.K, kb + op.kchunk);
-  float* C = op.C + (int64_t)blockIdx.z * op.cz;
-  const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
-  f32x4 acc[2][2];
+  const int nst = (ke - kb + 15) >> 4;
+
+  const int t = threadIdx.x, l = t & 63, w = t >> 6;
+  const int lr = l & 15, lg = l >> 4;
+  // the operands' fields, read once (op is indexed by a runtime product number)
+  const float* __restrict__ Ag = op.A;
+  const float* __restrict__ Bg = op.B;
+  const int64_t lda = op.lda, ldb = op.ldb;
+  const int Mv = op.M, Nv = op.N, ones = op.ones;
+
+  // Column-major operands ([K][M] or [K][N]): a thread loads four
+  // consecutive rows or columns at one k, with k the fastest index across
+  // lanes, so a wave's transposed LDS stores (row*20 + k) hit 64 distinct
+  // banks and its loads read 64-byte runs of 16 k-rows.
+  f32x4 ra[kAF4], rb[kBF4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < kAF4; ++e) {
+      const int idx = t + 256 * e;
+      f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (ACOL) {
+        const int k = idx & 15, mq = idx >> 4;
+        const int gk = k0 + k, gm = m0 + 4 * mq;
+        if (gk < ke && gm < Mv)
+          v = *reinterpret_cast<const f32x4*>(Ag + (int64_t)gk * lda + gm);
+      } else {
+        const int mr = idx >> 2, q = idx & 3;
+        v = *reinterpret_cast<const f32x4*>(Ag + (int64_t)(m0 + mr) * lda + k0 + 4 * q);
+      }
+      ra[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < kBF4; ++e) {
+      const int idx = t + 256 * e;
+      f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (idx < BN * 4) {
+        if (BCOL) {
+          const int k = idx & 15, nq = idx >> 4;
+          const int gk = k0 + k, gn = n0 + 4 * nq;
+          if (gk < ke && gn < Nv) {
+            v = *reinterpret_cast<const f32x4*>(Bg + (int64_t)gk * ldb + gn);
+            if (ones >= gn && ones < gn + 4) v[ones - gn] = 1.f;
+          }
+        } else {
+          const int nr = idx >> 2, q = idx & 3;
+          if (n0 + nr < Nv)
+            v = *reinterpret_cast<const f32x4*>(Bg + (int64_t)(n0 + nr) * ldb + k0 + 4 * q);
+        }
+      }
+      rb[e] = v;
+    }
+  };
+  auto lstore = [&](float* As) {
+    float* Bs = As + BM * kPitch;
+#pragma unroll
+    for (int e = 0; e < kAF4; ++e) {
+      const int idx = t + 256 * e;
+      if (ACOL) {
+        const int k = idx & 15, mq = idx >> 4;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) As[(4 * mq + c) * kPitch + k] = ra[e][c];
+      } else {
+        const int mr = idx >> 2, q = idx & 3;
+        *reinterpret_cast<f32x4*>(As + mr * kPitch + 4 * q) = ra[e];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < kBF4; ++e) {
+      const int idx = t + 256 * e;
+      if (idx < BN * 4) {
+        if (BCOL) {
+          const int k = idx & 15, nq = idx >> 4;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) Bs[(4 * nq + c) * kPitch + k] = rb[e][c];
+        } else {
+          const int nr = idx >> 2, q = idx & 3;
+          *reinterpret_cast<f32x4*>(Bs + nr * kPitch + 4 * q) = rb[e];
+        }
+      }
+    }
+  };
+
+  f32x4 acc[2][TN];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = kb; k0 < ke; k0 += kGK) {
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  constexpr int kBuf = (BM + BN) * kPitch;
+  gload(kb);
+  lstore(lds);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const bool more = st + 1 < nst;
+    if (more) gload(kb + 16 * (st + 1));
+    const float* As = lds + (st & 1) * kBuf;
+    const float* Bs = As + BM * kPitch;
+    const f32x4 a0 = *reinterpret_cast<const f32x4*>(As + (32 * w + lr) * kPitch + 4 * lg);
+    const f32x4 a1 = *reinterpret_cast<const f32x4*>(As + (32 * w + 16 + lr) * kPitch + 4 * lg);
+    f32x4 bv[TN];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = t * 4 + q;
-      const int m = op.ta ? (e & 63) : (e >> 4);
-      const int k = op.ta ? (e >> 6) : (e & 15);
-      const int gm = m0 + m, gk = k0 + k;
-      float v = 0.f;
-      if (gm < op.M && gk < ke)
-        v = op.ta ? op.A[(int64_t)gk * op.lda + gm] : op.A[(int64_t)gm * op.lda + gk];
-      As[k][m] = v;
-    }
+    for (int j = 0; j < TN; ++j)
+      bv[j] = *reinterpret_cast<const f32x4*>(Bs + (16 * j + lr) * kPitch + 4 * lg);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = t * 4 + q;
-      const int n = op.tb ? (e >> 4) : (e & 63);
-      const int k = op.tb ? (e & 15) : (e >> 6);
-      const int gn = n0 + n, gk = k0 + k;
-      float v = 0.f;
-      if (gn < op.N && gk < ke)
-        v = op.tb ? op.B[(int64_t)gn * op.ldb + gk] : op.B[(int64_t)gk * op.ldb + gn];
-      Bs[k][n] = v;
-    }
+    for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        acc[0][j] = mfma4(a0[s4], bv[j][s4], acc[0][j]);
+        acc[1][j] = mfma4(a1[s4], bv[j][s4], acc[1][j]);
+      }
+    if (more) lstore(lds + ((st + 1) & 1) * kBuf);
     __syncthreads();
+  }
+
+  // epilogue: lane l holds rows 4*(l/16)+r of each 16-row tile, column l%16.
+  // The operands it reads (bias, E1, E2) are loaded unconditionally, one
+  // column tile at a time, so their loads overlap: rows < tiles_m*128 and
+  // columns < 208 stay inside the chunk arrays for every product that has
+  // them; only the stores are masked.
+  const int rbase = m0 + 32 * w + 4 * lg;
+  // kernel-argument arrays indexed by a per-lane group are read once here
+  // (an indexed read inside the loop would be a memory round trip each)
+  const float* bias0 = op.bias[0];
+  const float* bias1 = op.bias[1];
+  const float* bias2 = op.bias[2];
+  const int64_t wo0 = op.woff[0], wo1 = op.woff[1], wo2 = op.woff[2];
+  const int64_t bo0 = op.boff[0], bo1 = op.boff[1], bo2 = op.boff[2];
+  float* slab = op.part + (int64_t)z * kSrcParams;
 #pragma unroll
-    for (int ks = 0; ks < kGK; ks += 4) {
-      const int kk = ks + (lane >> 4);
-      float a[2], b[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[kk][wm + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[kk][wn + j * 16 + (lane & 15)];
+  for (int j = 0; j < TN; ++j) {
+    const int col = n0 + 16 * j + lr;
+    const bool cok = col < op.N;
+    float e1[2][4], e2[2][4], bb = 0.f;
+    if (EPI == kEpiTanhCat) {
+      const int g = col / kHp, o = col - g * kHp;
+      const float* bp = g == 0 ? bias0 : (g == 1 ? bias1 : bias2);
+      const float bv = bp[o < kH ? o : kH - 1];
+      bb = o < kH ? bv : 0.f;
+    } else if (EPI == kEpiTanhGate) {
+      const float bv = bias0[col < kH ? col : kH - 1];
+      bb = col < kH ? bv : 0.f;
+    }
+    if (EPI == kEpiTanhGate || EPI == kEpiRevPair || EPI == kEpiRevTanh) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int r = 0; r < 4; ++r)
+          e1[i][r] = op.E1[(int64_t)(rbase + 16 * i + r) * op.lde1 + col];
     }
-    __syncthreads();
-  }
+    if (EPI == kEpiRevPair) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+        for (int r = 0; r < 4; ++r)
+          e2[i][r] = op.E2[(int64_t)(rbase + 16 * i + r) * op.lde2 + col];
+    }
+    // values first, without branches; then the masked stores
+    float v1[2][4], v2[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
-        const int col = n0 + wn + j * 16 + (lane & 15);
-        if (row >= op.M || col >= op.N) continue;
-        float* c = C + (int64_t)row * op.ldc + col;
-        const float v = acc[i][j][r];
-        if (op.act == 2) {
-          *c = *c + v;
-        } else if (op.act == 1) {
-          *c = tanhf(v + op.bias[col]);
-        } else if (op.act == 3) {
-          *c = v + op.bias[col];
+        const float d = acc[i][j][r];
+        if (EPI == kEpiTanhCat) {
+          v1[i][r] = tanhf(d + bb);
+        } else if (EPI == kEpiTanhGate) {
+          v1[i][r] = tanhf(d + bb);
+          v2[i][r] = v1[i][r] * e1[i][r];
+        } else if (EPI == kEpiRevPair) {
+          const float a1 = e1[i][r], a2 = e2[i][r];
+          v1[i][r] = (d * a1) * (1.f - a2 * a2);
+          v2[i][r] = (d * a2) * (1.f - a1 * a1);
+        } else if (EPI == kEpiRevTanh) {
+          const float a1 = e1[i][r];
+          v1[i][r] = d * (1.f - a1 * a1);
         } else {
-          *c = v;
+          v1[i][r] = d;
         }
       }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rbase + 16 * i + r;
+        if (row >= op.M || !cok) continue;
+        if (EPI == kEpiPartial) {
+          const int g = row / op.rg, o = row - g * op.rg;
+          if (o >= op.nout) continue;
+          const int64_t wo = g == 0 ? wo0 : (g == 1 ? wo1 : wo2);
+          const int64_t bo = g == 0 ? bo0 : (g == 1 ? bo1 : bo2);
+          if (col < op.nin)
+            slab[wo + (int64_t)o * op.nin + col] = v1[i][r];
+          else if (col == op.nin)
+            slab[bo + o] = v1[i][r];
+        } else {
+          op.C[(int64_t)row * op.ldc + col] = v1[i][r];
+          if (EPI == kEpiTanhGate || EPI == kEpiRevPair)
+            op.C2[(int64_t)row * op.ldc2 + col] = v2[i][r];
+        }
+      }
+  }
 }
 
-int gemm(const GemmOp& op, int splits, hipStream_t s) {
-  if (op.M <= 0 || op.N <= 0 || op.K <= 0) return 0;
-  const dim3 grid((op.N + kGN - 1) / kGN, (op.M + kGM - 1) / kGM, splits);
-  hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, op);
-  PF_LAUNCH_CHECK("gemm_kernel");
+template <int TN, bool ACOL, bool BCOL, int EPI>
+int mm_launch(MmBatch bt, hipStream_t s) {
+  int total = 0;
+  for (int i = 0; i < bt.nops; ++i) {
+    MmOp& o = bt.op[i];
+    o.tiles_m = (o.M + kTileM - 1) / kTileM;
+    o.tiles_n = (o.N + TN * 16 - 1) / (TN * 16);
+    bt.start[i] = total;
+    total += o.tiles_m * o.tiles_n * o.splits;
+  }
+  bt.start[bt.nops] = total;
+  if (total == 0) return 0;
+  hipLaunchKernelGGL((mm_kernel<TN, ACOL, BCOL, EPI>), dim3(total), dim3(256), 0, s, bt);
+  PF_LAUNCH_CHECK("mm_kernel");
   return 0;
 }
 
-GemmOp op_of(const float* A, int64_t lda, int ta, const float* B, int64_t ldb, int tb, float* C,
-             int64_t ldc, int M, int N, int K, int act, const float* bias = nullptr) {
-  GemmOp o;
-  o.A = A; o.lda = lda; o.ta = ta;
-  o.B = B; o.ldb = ldb; o.tb = tb;
-  o.C = C; o.ldc = ldc;
+MmOp mm_op(const float* A, int64_t lda, const float* B, int64_t ldb, int M, int N, int K,
+           float* C, int64_t ldc) {
+  MmOp o;
+  std::memset(&o, 0, sizeof(o));
+  o.A = A; o.lda = lda;
+  o.B = B; o.ldb = ldb;
   o.M = M; o.N = N; o.K = K;
   o.kchunk = K;
-  o.cz = 0;
-  o.bias = bias;
-  o.act = act;
+  o.splits = 1;
+  o.ones = -1;
+  o.C = C; o.ldc = ldc;
   return o;
+}
+
+MmBatch one(const MmOp& o) {
+  MmBatch b;
+  std::memset(&b, 0, sizeof(b));
+  b.op[0] = o;
+  b.nops = 1;
+  return b;
+}
+
+// The weights in the products' padded layouts (zeros in the pads).
+__global__ void __launch_bounds__(kBlock) pack_kernel(const float* __restrict__ prm,
+                                                      float* __restrict__ pk) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < kPkFloats;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    float v = 0.f;
+    if (e < kPkW2) {
+      const int r = (int)(e / kXld), c = (int)(e % kXld);
+      const int g = r / kHp, o = r % kHp;
+      const int64_t woff[3] = {kOffFc1W, kOffGW, kOffG2W};
+      if (o < kH && c < kSrcIn) v = prm[woff[g] + (int64_t)o * kSrcIn + c];
+    } else if (e < kPkW4) {
+      const int64_t q = e < kPkW3 ? e - kPkW2 : e - kPkW3;
+      const int r = (int)(q / kHp), c = (int)(q % kHp);
+      const int64_t off = e < kPkW3 ? kOffFc2W : kOffFc3W;
+      if (r < kH && c < kH) v = prm[off + (int64_t)r * kH + c];
+    } else {
+      const int64_t q = e - kPkW4;
+      const int r = (int)(q / kHp), c = (int)(q % kHp);
+      if (r < kSrcOut && c < kH) v = prm[kOffFc4W + (int64_t)r * kH + c];
+    }
+    pk[e] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -243,19 +494,10 @@ __global__ void __launch_bounds__(kBlock) gather_x_kernel(SrcGeom s, const float
     int i, j, k;
     decode_padded(s, c0 + r, i, j, k);
     const int64_t q = shifted(s, i, j, k, sh);
-    float* x = X + (int64_t)r * kXw + sh * kSrcFeat;
+    float* x = X + (int64_t)r * kXld + sh * kSrcFeat;
 #pragma unroll
     for (int f = 0; f < kSrcFeat; ++f) x[f] = Fn[(int64_t)f * s.m + q];
   }
-}
-
-// out = a * b over n elements (the gates: H2G = G*H2, H4 = H3*Gate2).
-__global__ void __launch_bounds__(kBlock) mul_kernel(int64_t n, const float* __restrict__ a,
-                                                     const float* __restrict__ b,
-                                                     float* __restrict__ out) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * blockDim.x)
-    out[e] = a[e] * b[e];
 }
 
 // Output adjoints from the divergence (:466-589): G_q = sum over axes a of
@@ -277,7 +519,7 @@ __global__ void __launch_bounds__(kBlock) dout_kernel(SrcGeom s, CVec3<T> gI,
         {(int64_t)wrap(i - 1, s.nxo) * sx + j * sy + k, (int64_t)wrap(i + 1, s.nxo) * sx + j * sy + k},
         {(int64_t)i * sx + wrap(j - 1, s.nyo) * sy + k, (int64_t)i * sx + wrap(j + 1, s.nyo) * sy + k},
         {(int64_t)i * sx + j * sy + wrap(k - 1, s.nzo), (int64_t)i * sx + j * sy + wrap(k + 1, s.nzo)}};
-    float* o = dout + r * kSrcOut;
+    float* o = dout + r * kOutp;
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
@@ -287,46 +529,6 @@ __global__ void __launch_bounds__(kBlock) dout_kernel(SrcGeom s, CVec3<T> gI,
       }
 #pragma unroll
     for (int c = kSrcUsed; c < kSrcOut; ++c) o[c] = 0.f;
-  }
-}
-
-// Reverse through H4 = H3*Gate2, H3 = tanh(L4), Gate2 = tanh(LG2):
-// dL4 = (dH4*Gate2)*(1 - H3*H3), dLG2 = (dH4*H3)*(1 - Gate2*Gate2).
-__global__ void __launch_bounds__(kBlock) rev_out_kernel(int64_t n, const float* __restrict__ dH,
-                                                         const float* __restrict__ H3,
-                                                         const float* __restrict__ Gate2,
-                                                         float* __restrict__ dL4,
-                                                         float* __restrict__ dLG2) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const float d = dH[e], h3 = H3[e], g2 = Gate2[e];
-    dL4[e] = (d * g2) * (1.f - h3 * h3);
-    dLG2[e] = (d * h3) * (1.f - g2 * g2);
-  }
-}
-
-// Reverse through H2G = G*H2, G = tanh(LG), H2 = tanh(L2).
-__global__ void __launch_bounds__(kBlock) rev_gate_kernel(int64_t n, const float* __restrict__ dH,
-                                                          const float* __restrict__ G,
-                                                          const float* __restrict__ H2,
-                                                          float* __restrict__ dLG,
-                                                          float* __restrict__ dL2) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const float d = dH[e], gv = G[e], h2 = H2[e];
-    dLG[e] = (d * h2) * (1.f - gv * gv);
-    dL2[e] = (d * gv) * (1.f - h2 * h2);
-  }
-}
-
-// Reverse through H1 = tanh(L1).
-__global__ void __launch_bounds__(kBlock) rev_tanh_kernel(int64_t n, const float* __restrict__ dH,
-                                                          const float* __restrict__ H,
-                                                          float* __restrict__ dL) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const float h = H[e];
-    dL[e] = dH[e] * (1.f - h * h);
   }
 }
 
@@ -350,7 +552,7 @@ __global__ void __launch_bounds__(kBlock) feat_adj_kernel(SrcGeom s, const float
     for (int f = 0; f < kSrcFeat; ++f) acc[f] = 0.f;
     for (int sh = 0; sh < kSrcShifts; ++sh) {
       const int64_t c = shifted(s, i, j, k, opposite(sh));
-      const float* x = dX + c * kXw + sh * kSrcFeat;
+      const float* x = dX + c * kXld + sh * kSrcFeat;
 #pragma unroll
       for (int f = 0; f < kSrcFeat; ++f) acc[f] += x[f] / nrm.v[f];
     }
@@ -422,12 +624,6 @@ __global__ void __launch_bounds__(kBlock) add_partials_kernel(double* __restrict
   }
 }
 
-__global__ void __launch_bounds__(kBlock) fill_kernel(int64_t n, float* __restrict__ a, float v) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * blockDim.x)
-    a[e] = v;
-}
-
 unsigned grid_of(int64_t n) {
   int64_t b = (n + kBlock - 1) / kBlock;
   if (b > 65536) b = 65536;
@@ -466,100 +662,126 @@ int ml_vjp_impl(pf_box* h, const float* prm, const void* const fwdI[3], const vo
   float* dX = (float*)(base + L.off_dX);
   float* dF = (float*)(base + L.off_dF);
   float* X = (float*)(base + L.off_X);
-  float* act = (float*)(base + L.off_act);
-  float* H1 = act;
-  float* Gt = act + (size_t)L.C * kH;
-  float* G2 = act + (size_t)2 * L.C * kH;
-  float* H2 = act + (size_t)3 * L.C * kH;
-  float* H2G = act + (size_t)4 * L.C * kH;
-  float* H3 = act + (size_t)5 * L.C * kH;
-  float* H4 = act + (size_t)6 * L.C * kH;
-  float* dout = (float*)(base + L.off_dout);
-  float* ab = (float*)(base + L.off_adjb);
-  float* dH = ab;
-  float* dL1 = ab + (size_t)L.C * kH;
-  float* dLG = ab + (size_t)2 * L.C * kH;
-  float* dLG2 = ab + (size_t)3 * L.C * kH;
-  float* dL2 = ab + (size_t)4 * L.C * kH;
-  float* dL4 = ab + (size_t)5 * L.C * kH;
-  float* ones = (float*)(base + L.off_ones);
+  float* cat = (float*)(base + L.off_cat);        // H1 | G | Gate2, ld 624
+  float* H1 = cat;
+  float* Gt = cat + kHp;
+  float* G2 = cat + 2 * kHp;
+  const size_t ca = (size_t)L.Ca * kHp;
+  float* H2 = (float*)(base + L.off_h);            // ld 208
+  float* H2G = H2 + ca;
+  float* H3 = H2 + 2 * ca;
+  float* H4 = H2 + 3 * ca;
+  float* dout = (float*)(base + L.off_dout);       // ld 32
+  float* dcat = (float*)(base + L.off_dcat);       // dL1 | dLG | dLG2, ld 624
+  float* dL4 = (float*)(base + L.off_dl);          // ld 208
+  float* dL2 = dL4 + ca;
+  float* pk = (float*)(base + L.off_pack);
   float* part = (float*)(base + L.off_part);
-  const float* W1 = prm + kOffFc1W;
-  const float* b1 = prm + kOffFc1B;
-  const float* WG = prm + kOffGW;
-  const float* bG = prm + kOffGB;
-  const float* WG2 = prm + kOffG2W;
-  const float* bG2 = prm + kOffG2B;
-  const float* W2 = prm + kOffFc2W;
-  const float* b2 = prm + kOffFc2B;
-  const float* W3 = prm + kOffFc3W;
-  const float* b3 = prm + kOffFc3B;
-  const float* W4 = prm + kOffFc4W;
+  const float* Wcat = pk + kPkCat;
+  const float* W2p = pk + kPkW2;
+  const float* W3p = pk + kPkW3;
+  const float* W4p = pk + kPkW4;
 
-  // 1. features of every padded cell
+  // 1. features of every padded cell; the weights in the padded layouts
   hipLaunchKernelGGL(src_features_kernel<T>, dim3(grid_of(s.m)), dim3(kBlock), 0, st, s,
                      (const T*)fwdI[0], (const T*)fwdI[1], (const T*)fwdI[2], Fn, nrm);
   PF_LAUNCH_CHECK("src_features_kernel");
-  PF_ELEM(fill_kernel, L.C, (int64_t)L.C, ones, 1.f);
+  PF_ELEM(pack_kernel, kPkFloats, prm, pk);
   CVec3<T> g3;
   for (int q = 0; q < 3; ++q) g3.a[q] = (const T*)gI[q];
 
   // 2. chunks of cells
   for (int64_t c0 = 0; c0 < s.m; c0 += L.C) {
     const int R = (int)((s.m - c0) < L.C ? (s.m - c0) : L.C);
-    const int64_t nRH = (int64_t)R * kH;
     PF_ELEM(gather_x_kernel, (int64_t)R * kSrcShifts, s, (const float*)Fn, X, c0, R);
-    // forward: L1, LG, LG2 from X; L2 from H1; L4 from G*H2; H4 = H3*Gate2
-    if (int rc = gemm(op_of(X, kXw, 0, W1, kXw, 1, H1, kH, R, kH, kXw, 1, b1), 1, st)) return rc;
-    if (int rc = gemm(op_of(X, kXw, 0, WG, kXw, 1, Gt, kH, R, kH, kXw, 1, bG), 1, st)) return rc;
-    if (int rc = gemm(op_of(X, kXw, 0, WG2, kXw, 1, G2, kH, R, kH, kXw, 1, bG2), 1, st)) return rc;
-    if (int rc = gemm(op_of(H1, kH, 0, W2, kH, 1, H2, kH, R, kH, kH, 1, b2), 1, st)) return rc;
-    PF_ELEM(mul_kernel, nRH, nRH, (const float*)Gt, (const float*)H2, H2G);
-    if (int rc = gemm(op_of(H2G, kH, 0, W3, kH, 1, H3, kH, R, kH, kH, 1, b3), 1, st)) return rc;
-    PF_ELEM(mul_kernel, nRH, nRH, (const float*)H3, (const float*)G2, H4);
-    // reverse: output adjoints, then back through fc4, fc3, the gates, fc2, fc1/fcG/fcG2
+    // forward: [H1 G Gate2] = tanh(X [fc1 fcG fcG2]^T + b) (:91-102)
+    {
+      MmOp o = mm_op(X, kXld, Wcat, kXld, R, kCat, kXld, cat, kCat);
+      o.bias[0] = prm + kOffFc1B;
+      o.bias[1] = prm + kOffGB;
+      o.bias[2] = prm + kOffG2B;
+      if (int rc = mm_launch<13, false, false, kEpiTanhCat>(one(o), st)) return rc;
+    }
+    // H2 = tanh(fc2 H1), H2G = G*H2 (:94-104)
+    {
+      MmOp o = mm_op(H1, kCat, W2p, kHp, R, kHp, kHp, H2, kHp);
+      o.bias[0] = prm + kOffFc2B;
+      o.C2 = H2G; o.ldc2 = kHp;
+      o.E1 = Gt; o.lde1 = kCat;
+      if (int rc = mm_launch<13, false, false, kEpiTanhGate>(one(o), st)) return rc;
+    }
+    // H3 = tanh(fc3 H2G), H4 = H3*Gate2 (:108-112)
+    {
+      MmOp o = mm_op(H2G, kHp, W3p, kHp, R, kHp, kHp, H3, kHp);
+      o.bias[0] = prm + kOffFc3B;
+      o.C2 = H4; o.ldc2 = kHp;
+      o.E1 = G2; o.lde1 = kCat;
+      if (int rc = mm_launch<13, false, false, kEpiTanhGate>(one(o), st)) return rc;
+    }
+    // reverse: output adjoints; dH4 = dout fc4 -> dL4, dLG2
     PF_ELEM(dout_kernel<T>, R, s, g3, dout, c0, R);
-    if (int rc = gemm(op_of(dout, kSrcOut, 0, W4, kH, 0, dH, kH, R, kH, kSrcOut, 0), 1, st))
-      return rc;
-    PF_ELEM(rev_out_kernel, nRH, nRH, (const float*)dH, (const float*)H3, (const float*)G2, dL4,
-            dLG2);
-    if (int rc = gemm(op_of(dL4, kH, 0, W3, kH, 0, dH, kH, R, kH, kH, 0), 1, st)) return rc;
-    PF_ELEM(rev_gate_kernel, nRH, nRH, (const float*)dH, (const float*)Gt, (const float*)H2, dLG,
-            dL2);
-    if (int rc = gemm(op_of(dL2, kH, 0, W2, kH, 0, dH, kH, R, kH, kH, 0), 1, st)) return rc;
-    PF_ELEM(rev_tanh_kernel, nRH, nRH, (const float*)dH, (const float*)H1, dL1);
-    float* dXc = dX + c0 * kXw;
-    if (int rc = gemm(op_of(dL1, kH, 0, W1, kXw, 0, dXc, kXw, R, kXw, kH, 0), 1, st)) return rc;
-    if (int rc = gemm(op_of(dLG, kH, 0, WG, kXw, 0, dXc, kXw, R, kXw, kH, 2), 1, st)) return rc;
-    if (int rc = gemm(op_of(dLG2, kH, 0, WG2, kXw, 0, dXc, kXw, R, kXw, kH, 2), 1, st))
-      return rc;
+    {
+      MmOp o = mm_op(dout, kOutp, W4p, kHp, R, kHp, kOutp, dL4, kHp);
+      o.C2 = dcat + 2 * kHp; o.ldc2 = kCat;
+      o.E1 = G2; o.lde1 = kCat;
+      o.E2 = H3; o.lde2 = kHp;
+      if (int rc = mm_launch<13, false, true, kEpiRevPair>(one(o), st)) return rc;
+    }
+    // dH2G = dL4 fc3 -> dLG, dL2
+    {
+      MmOp o = mm_op(dL4, kHp, W3p, kHp, R, kHp, kHp, dcat + kHp, kCat);
+      o.C2 = dL2; o.ldc2 = kHp;
+      o.E1 = H2; o.lde1 = kHp;
+      o.E2 = Gt; o.lde2 = kCat;
+      if (int rc = mm_launch<13, false, true, kEpiRevPair>(one(o), st)) return rc;
+    }
+    // dH1 = dL2 fc2 -> dL1
+    {
+      MmOp o = mm_op(dL2, kHp, W2p, kHp, R, kHp, kHp, dcat, kCat);
+      o.E1 = H1; o.lde1 = kCat;
+      if (int rc = mm_launch<13, false, true, kEpiRevTanh>(one(o), st)) return rc;
+    }
+    // input adjoints of the chunk: dX = [dL1 dLG dLG2] [fc1; fcG; fcG2]
+    {
+      MmOp o = mm_op(dcat, kCat, Wcat, kXld, R, kSrcIn, kCat, dX + c0 * kXld, kXld);
+      if (int rc = mm_launch<9, false, true, kEpiStore>(one(o), st)) return rc;
+    }
     if (!dparams) continue;
-    // parameter gradients of this chunk: dW = dL^T act and db = dL^T 1 over
-    // its cells, split over kSplitK-cell ranges into FP32 partials laid out
-    // like the packed parameters, then summed in order into FP64
+    // parameter gradients of this chunk: [dW | db] = dL^T [act | 1] over its
+    // cells, split over kSplitK-cell ranges into FP32 partials laid out like
+    // the packed parameters, then summed in order into FP64
     const int splits = (R + kSplitK - 1) / kSplitK;
-    struct Pg {
-      const float* dL;
-      int out;
-      const float* a;
-      int in;
-      int64_t w, b;
-    };
-    const Pg pg[6] = {{dL1, kH, X, kXw, kOffFc1W, kOffFc1B},
-                      {dLG, kH, X, kXw, kOffGW, kOffGB},
-                      {dLG2, kH, X, kXw, kOffG2W, kOffG2B},
-                      {dL2, kH, H1, kH, kOffFc2W, kOffFc2B},
-                      {dL4, kH, H2G, kH, kOffFc3W, kOffFc3B},
-                      {dout, kSrcOut, H4, kH, kOffFc4W, kOffFc4B}};
-    for (const Pg& q : pg) {
-      GemmOp o = op_of(q.dL, q.out, 1, q.a, q.in, 0, part + q.w, q.in, q.out, q.in, R, 0);
-      o.kchunk = kSplitK;
-      o.cz = kSrcParams;
-      if (int rc = gemm(o, splits, st)) return rc;
-      GemmOp ob = op_of(ones, 1, 1, q.dL, q.out, 0, part + q.b, q.out, 1, q.out, R, 0);
-      ob.kchunk = kSplitK;
-      ob.cz = kSrcParams;
-      if (int rc = gemm(ob, splits, st)) return rc;
+    {
+      MmOp o = mm_op(dcat, kCat, X, kXld, kCat, kSrcIn + 1, R, nullptr, 0);
+      o.kchunk = kSplitK; o.splits = splits; o.ones = kSrcIn;
+      o.part = part; o.rg = kHp; o.nout = kH; o.nin = kSrcIn;
+      o.woff[0] = kOffFc1W; o.woff[1] = kOffGW; o.woff[2] = kOffG2W;
+      o.boff[0] = kOffFc1B; o.boff[1] = kOffGB; o.boff[2] = kOffG2B;
+      if (int rc = mm_launch<9, true, true, kEpiPartial>(one(o), st)) return rc;
+    }
+    {
+      MmBatch bt;
+      std::memset(&bt, 0, sizeof(bt));
+      struct Pg {
+        const float* dL;
+        int64_t ldl;
+        int M, nout;
+        const float* act;
+        int64_t lda;
+        int64_t w, b;
+      };
+      const Pg pg[3] = {{dL2, kHp, kHp, kH, H1, kCat, kOffFc2W, kOffFc2B},
+                        {dL4, kHp, kHp, kH, H2G, kHp, kOffFc3W, kOffFc3B},
+                        {dout, kOutp, kOutp, kSrcOut, H4, kHp, kOffFc4W, kOffFc4B}};
+      for (int i = 0; i < 3; ++i) {
+        MmOp o = mm_op(pg[i].dL, pg[i].ldl, pg[i].act, pg[i].lda, pg[i].M, kH + 1, R, nullptr, 0);
+        o.kchunk = kSplitK; o.splits = splits; o.ones = kH;
+        o.part = part; o.rg = pg[i].M; o.nout = pg[i].nout; o.nin = kH;
+        o.woff[0] = pg[i].w; o.boff[0] = pg[i].b;
+        bt.op[i] = o;
+      }
+      bt.nops = 3;
+      if (int rc = mm_launch<13, true, true, kEpiPartial>(bt, st)) return rc;
     }
     PF_ELEM(add_partials_kernel, kSrcParams, dparams, (const float*)part, (int64_t)kSrcParams,
             splits, (int64_t)kSrcParams);
