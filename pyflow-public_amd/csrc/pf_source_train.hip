@@ -164,19 +164,32 @@ struct MmBatch {
 constexpr int kPitch = 20;  // LDS floats per row of a 16-deep K slice
 
 // 256 threads = 4 waves stacked along M; wave w owns rows 32w..32w+31 of the
-// 128-row tile (2 MFMA row tiles) and all TN*16 columns.  The K slice of a
-// stage sits in LDS as [row][k] for both operands (pitch 20: the 128-bit
-// reads of a pass land on distinct banks), so one 128-bit read gives a lane
-// its operands of four consecutive MFMAs (lane l: row l%16, k = 4*(l/16)+s
-// in MFMA s; both operands use the same K order).  Double-buffered: the next
-// stage's global loads are in flight in registers while the current one is
-// consumed; one barrier per stage.
+// 128-row tile (2 MFMA row tiles) and all TN*16 columns.  Lane l supplies
+// row (or column) l%16 and k = 4*(l/16) + s to MFMA s of a stage, for both
+// operands.  A row-major operand ([M][K] or [N][K]) sits in LDS as
+// [row][k] (pitch 20: the 128-bit reads of a pass land on distinct banks),
+// so one 128-bit read gives a lane its operands of four MFMAs.  A
+// column-major one ([K][M] or [K][N]) sits as it arrives, [k][row] with
+// pitch rows + 4 (the four k-rows of a 32-bit read pass start 16 banks
+// apart), read 32 bits per MFMA -- except the B operand of 208-column tiles,
+// whose 52 such reads per stage cost more than transposing it into [row][k]
+// on the way in (a thread's four columns at one k, k fastest across lanes,
+// so a wave's 32-bit stores hit distinct banks).  Measured at 64^3: k-major
+// 316 vs 358 us for the 144-column weight-gradient product, 353 vs 277 us
+// for the 208-column one.  Double-buffered: the next stage's global loads
+// are in flight in registers while the current one is consumed; one barrier
+// per stage.
 template <int TN, bool ACOL, bool BCOL, int EPI>
 __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
   constexpr int BM = kTileM, BN = TN * 16;
   constexpr int kAF4 = BM * 16 / 4 / 256;              // float4 of A per thread (2)
   constexpr int kBF4 = (BN * 16 / 4 + 255) / 256;       // float4 of B per thread
-  __shared__ __align__(16) float lds[2 * (BM + BN) * kPitch];
+  constexpr bool KMB = BCOL && TN <= 9;                 // B k-major in LDS
+  constexpr int kPA = BM + 4, kPB = BN + 4;             // k-major pitches
+  constexpr int kAFl = ACOL ? 16 * kPA : BM * kPitch;   // LDS floats of an A slice
+  constexpr int kBFl = KMB ? 16 * kPB : BN * kPitch;
+  constexpr int kBuf = kAFl + kBFl;
+  __shared__ __align__(16) float lds[2 * kBuf];
 
   int oi = 0;
   while (oi + 1 < bt.nops && (int)blockIdx.x >= bt.start[oi + 1]) ++oi;
@@ -204,10 +217,6 @@ __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
   const int64_t lda = op.lda, ldb = op.ldb;
   const int Mv = op.M, Nv = op.N, ones = op.ones;
 
-  // Column-major operands ([K][M] or [K][N]): a thread loads four
-  // consecutive rows or columns at one k, with k the fastest index across
-  // lanes, so a wave's transposed LDS stores (row*20 + k) hit 64 distinct
-  // banks and its loads read 64-byte runs of 16 k-rows.
   f32x4 ra[kAF4], rb[kBF4];
   auto gload = [&](int k0) {
 #pragma unroll
@@ -215,7 +224,7 @@ __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
       const int idx = t + 256 * e;
       f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
       if (ACOL) {
-        const int k = idx & 15, mq = idx >> 4;
+        const int k = idx / (BM / 4), mq = idx - k * (BM / 4);
         const int gk = k0 + k, gm = m0 + 4 * mq;
         if (gk < ke && gm < Mv)
           v = *reinterpret_cast<const f32x4*>(Ag + (int64_t)gk * lda + gm);
@@ -231,7 +240,8 @@ __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
       f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
       if (idx < BN * 4) {
         if (BCOL) {
-          const int k = idx & 15, nq = idx >> 4;
+          const int k = KMB ? idx / (BN / 4) : (idx & 15);
+          const int nq = KMB ? idx - k * (BN / 4) : (idx >> 4);
           const int gk = k0 + k, gn = n0 + 4 * nq;
           if (gk < ke && gn < Nv) {
             v = *reinterpret_cast<const f32x4*>(Bg + (int64_t)gk * ldb + gn);
@@ -247,14 +257,13 @@ __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
     }
   };
   auto lstore = [&](float* As) {
-    float* Bs = As + BM * kPitch;
+    float* Bs = As + kAFl;
 #pragma unroll
     for (int e = 0; e < kAF4; ++e) {
       const int idx = t + 256 * e;
       if (ACOL) {
-        const int k = idx & 15, mq = idx >> 4;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) As[(4 * mq + c) * kPitch + k] = ra[e][c];
+        const int k = idx / (BM / 4), mq = idx - k * (BM / 4);
+        *reinterpret_cast<f32x4*>(As + k * kPA + 4 * mq) = ra[e];
       } else {
         const int mr = idx >> 2, q = idx & 3;
         *reinterpret_cast<f32x4*>(As + mr * kPitch + 4 * q) = ra[e];
@@ -264,7 +273,10 @@ __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
     for (int e = 0; e < kBF4; ++e) {
       const int idx = t + 256 * e;
       if (idx < BN * 4) {
-        if (BCOL) {
+        if (KMB) {
+          const int k = idx / (BN / 4), nq = idx - k * (BN / 4);
+          *reinterpret_cast<f32x4*>(Bs + k * kPB + 4 * nq) = rb[e];
+        } else if (BCOL) {
           const int k = idx & 15, nq = idx >> 4;
 #pragma unroll
           for (int c = 0; c < 4; ++c) Bs[(4 * nq + c) * kPitch + k] = rb[e][c];
@@ -282,7 +294,6 @@ __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  constexpr int kBuf = (BM + BN) * kPitch;
   gload(kb);
   lstore(lds);
   __syncthreads();
@@ -290,13 +301,28 @@ __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
     const bool more = st + 1 < nst;
     if (more) gload(kb + 16 * (st + 1));
     const float* As = lds + (st & 1) * kBuf;
-    const float* Bs = As + BM * kPitch;
-    const f32x4 a0 = *reinterpret_cast<const f32x4*>(As + (32 * w + lr) * kPitch + 4 * lg);
-    const f32x4 a1 = *reinterpret_cast<const f32x4*>(As + (32 * w + 16 + lr) * kPitch + 4 * lg);
+    const float* Bs = As + kAFl;
+    f32x4 a0, a1;
+    if (ACOL) {
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        a0[s4] = As[(4 * lg + s4) * kPA + 32 * w + lr];
+        a1[s4] = As[(4 * lg + s4) * kPA + 32 * w + 16 + lr];
+      }
+    } else {
+      a0 = *reinterpret_cast<const f32x4*>(As + (32 * w + lr) * kPitch + 4 * lg);
+      a1 = *reinterpret_cast<const f32x4*>(As + (32 * w + 16 + lr) * kPitch + 4 * lg);
+    }
     f32x4 bv[TN];
 #pragma unroll
-    for (int j = 0; j < TN; ++j)
-      bv[j] = *reinterpret_cast<const f32x4*>(Bs + (16 * j + lr) * kPitch + 4 * lg);
+    for (int j = 0; j < TN; ++j) {
+      if (KMB) {
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) bv[j][s4] = Bs[(4 * lg + s4) * kPB + 16 * j + lr];
+      } else {
+        bv[j] = *reinterpret_cast<const f32x4*>(Bs + (16 * j + lr) * kPitch + 4 * lg);
+      }
+    }
 #pragma unroll
     for (int s4 = 0; s4 < 4; ++s4)
 #pragma unroll
