@@ -262,9 +262,22 @@ int pf_sfs_ml_source_vjp(pf_box* box, const float* params, const void* ui, const
                          const void* wi, const void* gx, const void* gy, const void* gz,
                          float* dui, float* dvi, float* dwi, double* dparams, void* stream);
 /* Set-up call: reserve the reverse's workspace in the box (about 1.2 GB at
- * 64^3, 3.8 GB at 128^3; allocates) and keep the 21-entry normalisation table
+ * 64^3, 3.8 GB at 128^3, plus the all-cell cache below when it fits its
+ * budget; allocates) and keep the 21-entry normalisation table
  * (sfsmodel_ML.py:252-258) for the adjoint closure term.  Idempotent. */
 int pf_sfs_ml_source_train_reserve(pf_box* box, const float* norm21);
+/* Set-up call, before pf_sfs_ml_source_train_reserve: the bytes the
+ * closure's all-cell cache may take (default 48 GiB; negative = default).
+ * With the cache (11.3 KB per padded cell: 3.5 GB at 64^3, 26 GB at 128^3)
+ * the workspace holds the network's forward activations of every cell, and
+ * pf_adj_rk4_step with closure 2 evaluates the network on the forward state
+ * once per step (the reference's four adjoint stages see the same forward
+ * state) and its parameter gradient as one product of the stage-summed
+ * layer adjoints; without it every stage evaluates the whole term.  The
+ * adjoint fields are the same bits either way; the parameter gradient
+ * differs by FP32 rounding (sum then product instead of product then sum).
+ * No counterpart in the reference. */
+int pf_sfs_ml_source_cache_budget(pf_box* box, int64_t bytes);
 
 /*
  * Smagorinsky eddy viscosity plus the molecular one, written to the padded

@@ -78,11 +78,14 @@ struct pf_box {
   size_t src_ws_bytes;
   // its reverse (training through the closure), reserved by
   // pf_sfs_ml_source_train_reserve: activations and adjoints of a chunk of
-  // cells, the input adjoints of every cell, parameter-gradient partials, the
-  // closure's input arrays; and the normalisation table the reverse divides by
+  // cells (or the activations of every cell and stage sums of the layer
+  // adjoints, when they fit), the input adjoints of every cell,
+  // parameter-gradient partials, the closure's input arrays; and the
+  // normalisation table the reverse divides by
   void* srct_ws;
   size_t srct_bytes;
   float srct_norm[21];
+  int64_t srct_budget;       // bytes the all-cell cache may take (-1: default)
   int chunk;                 // iterations per launch batch between host checks
   int tile_rows;             // block rows of the tiled Krylov kernels: 256*tile_rows threads
   int64_t bytes;             // device bytes owned by the handle
@@ -115,6 +118,16 @@ int adj_closure_term(pf_box* h, int kind, const void* const U[3], const float* p
 // g = sum over padded cells of G(var_i(U)) . (centre interpolant of A).
 int ml_source_adj_term(pf_box* h, const void* const U[3], const float* params,
                        const void* const A[3], void* const D[3], double* dparams, hipStream_t s);
+// The same over the four stages of an adjoint RK4 step, when the box's
+// training workspace holds the forward activations of every cell
+// (ml_source_step_cached): _begin evaluates the network on U once, _term
+// gives stage `stage`'s D and adds its layer adjoints to the stage sums,
+// _end adds the step's parameter gradient (from the sums) to dparams.
+bool ml_source_step_cached(const pf_box* h);
+int ml_source_step_begin(pf_box* h, const void* const U[3], const float* params, hipStream_t s);
+int ml_source_step_term(pf_box* h, const void* const A[3], void* const D[3], int stage,
+                        hipStream_t s);
+int ml_source_step_end(pf_box* h, double* dparams, hipStream_t s);
 size_t adj_work_bytes(const pf_box* h);
 
 inline double* part_buf(pf_box* h, int which) {

@@ -79,6 +79,7 @@ int pf_box_create(pf_box** out, int nx, int ny, int nz, double dx, double dy, do
 
   pf_box* h = new pf_box();
   memset(h, 0, sizeof(*h));
+  h->srct_budget = -1;
   h->nx = nx; h->ny = ny; h->nz = nz;
   h->dx = dx; h->dy = dy; h->dz = dz;
   h->dtype = dtype;

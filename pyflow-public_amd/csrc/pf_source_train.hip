@@ -67,22 +67,45 @@ constexpr int64_t kPkFloats = kPkW4 + (int64_t)kOutp * kHp;
 
 inline size_t al256(size_t n) { return (n + 255) / 256 * 256; }
 
-// Layout of the box's training workspace.
+// Layout of the box's training workspace.  The network's forward
+// activations (X, H1 | G | Gate2, H2, H2G, H3, H4) are kept either for one
+// chunk of cells (rows = Ca) or, when the box's cache fits its budget
+// (kCacheBudget unless pf_sfs_ml_source_cache_budget set one), for
+// every cell (rows = Ma), together with the sums of the layer adjoints over
+// the stages of an adjoint RK4 step: the forward state is the same in all
+// four stages, so the step evaluates the forward once and takes the weight
+// gradient of its four stages as one product of the summed adjoints with
+// the (stage-independent) layer inputs.
+constexpr size_t kCacheBudget = (size_t)48 << 30;
+
 struct Srct {
   int64_t m;        // padded cells
   int C;            // rows per chunk
   int Ca;           // rows allocated per chunk array (C rounded up to a tile)
+  int64_t Ma;       // m rounded up to a tile
+  bool full;        // forward activations and stage sums for every cell
+  int64_t Mr;       // rows of the forward-activation arrays (Ma or Ca)
   size_t off_fwd, off_adj;            // 3 padded box-dtype arrays each
   size_t off_dX, off_dF, off_dvar;    // FP32: m x 288, 21 x m, 3 x m
-  size_t off_X, off_cat, off_h, off_dout, off_dcat, off_dl, off_pack, off_part;
+  size_t off_X, off_cat, off_h;       // forward activations, Mr rows
+  size_t off_dout, off_dcat, off_dl;  // a chunk's layer adjoints
+  size_t off_sout, off_scat, off_sl;  // stage sums of the layer adjoints (full only)
+  size_t off_pack, off_part;
   size_t bytes;
 };
+
+// Floats per row of the full-cache arrays: forward activations and sums.
+constexpr size_t kCacheRowFloats = kXld + kCat + 4 * kHp + kOutp + kCat + 2 * kHp;
 
 Srct srct_layout(const pf_box* h) {
   Srct L;
   L.m = (int64_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4);
   L.C = L.m < kChunkRows ? (int)L.m : kChunkRows;
   L.Ca = (L.C + kTileM - 1) / kTileM * kTileM;
+  L.Ma = (L.m + kTileM - 1) / kTileM * kTileM;
+  const size_t budget = h->srct_budget < 0 ? kCacheBudget : (size_t)h->srct_budget;
+  L.full = (size_t)L.Ma * kCacheRowFloats * 4 <= budget;
+  L.Mr = L.full ? L.Ma : L.Ca;
   size_t o = 0;
   auto take = [&](size_t n) {
     const size_t at = o;
@@ -90,18 +113,22 @@ Srct srct_layout(const pf_box* h) {
     return at;
   };
   const size_t pad = (size_t)L.m * h->esize;
-  const size_t ca = (size_t)L.Ca;
+  const size_t ca = (size_t)L.Ca, mr = (size_t)L.Mr;
   L.off_fwd = take(3 * pad);
   L.off_adj = take(3 * pad);
   L.off_dX = take((size_t)L.m * kXld * 4);
   L.off_dF = take((size_t)L.m * kSrcFeat * 4);
   L.off_dvar = take((size_t)L.m * 3 * 4);
-  L.off_X = take(ca * kXld * 4);
-  L.off_cat = take(ca * kCat * 4);          // H1 | G | Gate2
-  L.off_h = take(ca * kHp * 4 * 4);         // H2, H2G, H3, H4
+  L.off_X = take(mr * kXld * 4);
+  L.off_cat = take(mr * kCat * 4);          // H1 | G | Gate2
+  L.off_h = take(mr * kHp * 4 * 4);         // H2, H2G, H3, H4
   L.off_dout = take(ca * kOutp * 4);
   L.off_dcat = take(ca * kCat * 4);         // dL1 | dLG | dLG2
   L.off_dl = take(ca * kHp * 4 * 2);        // dL4, dL2
+  const size_t ms = L.full ? (size_t)L.Ma : 0;
+  L.off_sout = take(ms * kOutp * 4);
+  L.off_scat = take(ms * kCat * 4);
+  L.off_sl = take(ms * kHp * 4 * 2);
   L.off_pack = take((size_t)kPkFloats * 4);
   L.off_part = take((size_t)kMaxSplits * kSrcParams * 4);
   L.bytes = o;
@@ -128,6 +155,7 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
 //   kEpiPartial   the parameter-gradient partials of split z: row = group
 //                 row/rg, output o = row%rg (< nout); col < nin: weight
 //                 [o][col], col == nin: the bias (B's column nin is ones)
+// SUMS (kEpiRevPair, kEpiRevTanh): C and C2 are also added to S1 and S2.
 enum { kEpiStore, kEpiTanhCat, kEpiTanhGate, kEpiRevPair, kEpiRevTanh, kEpiPartial };
 
 struct MmOp {
@@ -148,6 +176,11 @@ struct MmOp {
   const float* E2;
   int64_t lde2;
   const float* bias[3];
+  float* S1;                // stage sums of C and C2 (SUMS): S += value, or
+  int64_t lds1;             // S = value when sacc == 0
+  float* S2;
+  int64_t lds2;
+  int sacc;
   int rg, nout, nin;
   float* part;              // slab z at part + z*kSrcParams
   int64_t woff[3], boff[3];
@@ -179,7 +212,7 @@ constexpr int kPitch = 20;  // LDS floats per row of a 16-deep K slice
 // for the 208-column one.  Double-buffered: the next stage's global loads
 // are in flight in registers while the current one is consumed; one barrier
 // per stage.
-template <int TN, bool ACOL, bool BCOL, int EPI>
+template <int TN, bool ACOL, bool BCOL, int EPI, bool SUMS = false>
 __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
   constexpr int BM = kTileM, BN = TN * 16;
   constexpr int kAF4 = BM * 16 / 4 / 256;              // float4 of A per thread (2)
@@ -376,6 +409,16 @@ __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
         for (int r = 0; r < 4; ++r)
           e2[i][r] = op.E2[(int64_t)(rbase + 16 * i + r) * op.lde2 + col];
     }
+    float s1[2][4], s2[2][4];
+    if (SUMS) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          s1[i][r] = op.S1[(int64_t)(rbase + 16 * i + r) * op.lds1 + col];
+          if (EPI == kEpiRevPair) s2[i][r] = op.S2[(int64_t)(rbase + 16 * i + r) * op.lds2 + col];
+        }
+    }
     // values first, without branches; then the masked stores
     float v1[2][4], v2[2][4];
 #pragma unroll
@@ -418,12 +461,17 @@ __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
           op.C[(int64_t)row * op.ldc + col] = v1[i][r];
           if (EPI == kEpiTanhGate || EPI == kEpiRevPair)
             op.C2[(int64_t)row * op.ldc2 + col] = v2[i][r];
+          if (SUMS) {
+            op.S1[(int64_t)row * op.lds1 + col] = op.sacc ? s1[i][r] + v1[i][r] : v1[i][r];
+            if (EPI == kEpiRevPair)
+              op.S2[(int64_t)row * op.lds2 + col] = op.sacc ? s2[i][r] + v2[i][r] : v2[i][r];
+          }
         }
       }
   }
 }
 
-template <int TN, bool ACOL, bool BCOL, int EPI>
+template <int TN, bool ACOL, bool BCOL, int EPI, bool SUMS = false>
 int mm_launch(MmBatch bt, hipStream_t s) {
   int total = 0;
   for (int i = 0; i < bt.nops; ++i) {
@@ -435,7 +483,7 @@ int mm_launch(MmBatch bt, hipStream_t s) {
   }
   bt.start[bt.nops] = total;
   if (total == 0) return 0;
-  hipLaunchKernelGGL((mm_kernel<TN, ACOL, BCOL, EPI>), dim3(total), dim3(256), 0, s, bt);
+  hipLaunchKernelGGL((mm_kernel<TN, ACOL, BCOL, EPI, SUMS>), dim3(total), dim3(256), 0, s, bt);
   PF_LAUNCH_CHECK("mm_kernel");
   return 0;
 }
@@ -534,7 +582,8 @@ __global__ void __launch_bounds__(kBlock) gather_x_kernel(SrcGeom s, const float
 template <typename T>
 __global__ void __launch_bounds__(kBlock) dout_kernel(SrcGeom s, CVec3<T> gI,
                                                       float* __restrict__ dout, int64_t c0,
-                                                      int rows) {
+                                                      int rows, float* __restrict__ dsum,
+                                                      int sacc) {
   const int64_t sy = s.nzo, sx = (int64_t)s.nyo * s.nzo;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
        r += (int64_t)gridDim.x * blockDim.x) {
@@ -555,6 +604,11 @@ __global__ void __launch_bounds__(kBlock) dout_kernel(SrcGeom s, CVec3<T> gI,
       }
 #pragma unroll
     for (int c = kSrcUsed; c < kSrcOut; ++c) o[c] = 0.f;
+    if (dsum) {
+      float* q = dsum + r * kOutp;
+#pragma unroll
+      for (int c = 0; c < kSrcUsed; ++c) q[c] = sacc ? q[c] + o[c] : o[c];
+    }
   }
 }
 
@@ -674,184 +728,319 @@ SrcGeom src_geom(const pf_box* h) {
     PF_LAUNCH_CHECK(#kernel);                                                                \
   } while (0)
 
+// Device arrays of the reverse, from the workspace layout.
+struct Bufs {
+  Srct L;
+  SrcGeom s;
+  Norm21 nrm;
+  float *Fn, *dX, *dF, *dvar;
+  float *X, *cat, *H2, *H2G, *H3, *H4;      // forward activations (Mr rows)
+  float *dout, *dcat, *dL4, *dL2;           // a chunk's layer adjoints (Ca rows)
+  float *sout, *scat, *sL4, *sL2;           // stage sums (Ma rows, full only)
+  float *pk, *part;
+  void *fwd[3], *adj[3];
+};
+
+Bufs bufs_of(pf_box* h) {
+  Bufs B;
+  B.L = srct_layout(h);
+  B.s = src_geom(h);
+  for (int f = 0; f < kSrcFeat; ++f) B.nrm.v[f] = h->srct_norm[f];
+  char* base = (char*)h->srct_ws;
+  const Srct& L = B.L;
+  auto F = [&](size_t off) { return (float*)(base + off); };
+  B.Fn = h->src_ws;
+  B.dX = F(L.off_dX);
+  B.dF = F(L.off_dF);
+  B.dvar = F(L.off_dvar);
+  B.X = F(L.off_X);
+  B.cat = F(L.off_cat);
+  const size_t mr = (size_t)L.Mr * kHp, ca = (size_t)L.Ca * kHp;
+  B.H2 = F(L.off_h);
+  B.H2G = B.H2 + mr;
+  B.H3 = B.H2 + 2 * mr;
+  B.H4 = B.H2 + 3 * mr;
+  B.dout = F(L.off_dout);
+  B.dcat = F(L.off_dcat);
+  B.dL4 = F(L.off_dl);
+  B.dL2 = B.dL4 + ca;
+  const size_t ma = (size_t)L.Ma * kHp;
+  B.sout = L.full ? F(L.off_sout) : nullptr;
+  B.scat = L.full ? F(L.off_scat) : nullptr;
+  B.sL4 = L.full ? F(L.off_sl) : nullptr;
+  B.sL2 = L.full ? B.sL4 + ma : nullptr;
+  B.pk = F(L.off_pack);
+  B.part = F(L.off_part);
+  const size_t pad = (size_t)L.m * h->esize;
+  for (int q = 0; q < 3; ++q) {
+    B.fwd[q] = base + L.off_fwd + q * pad;
+    B.adj[q] = base + L.off_adj + q * pad;
+  }
+  return B;
+}
+
+// Features of the closure's inputs and the weights in the padded layouts.
+template <typename T>
+int prep(const Bufs& B, const float* prm, const void* const fwdI[3], hipStream_t st) {
+  hipLaunchKernelGGL(src_features_kernel<T>, dim3(grid_of(B.s.m)), dim3(kBlock), 0, st, B.s,
+                     (const T*)fwdI[0], (const T*)fwdI[1], (const T*)fwdI[2], B.Fn, B.nrm);
+  PF_LAUNCH_CHECK("src_features_kernel");
+  PF_ELEM(pack_kernel, kPkFloats, prm, B.pk);
+  return 0;
+}
+
+// Row offset of chunk c0 in the forward-activation arrays.
+inline int64_t frow(const Bufs& B, int64_t c0) { return B.L.full ? c0 : 0; }
+
+// Forward of chunk c0 (R cells): inputs and activations of every layer.
+int fwd_chunk(const Bufs& B, const float* prm, int64_t c0, int R, hipStream_t st) {
+  const int64_t f = frow(B, c0);
+  float* X = B.X + f * kXld;
+  float* cat = B.cat + f * kCat;
+  float* H2 = B.H2 + f * kHp;
+  float* H2G = B.H2G + f * kHp;
+  float* H3 = B.H3 + f * kHp;
+  float* H4 = B.H4 + f * kHp;
+  PF_ELEM(gather_x_kernel, (int64_t)R * kSrcShifts, B.s, (const float*)B.Fn, X, c0, R);
+  // [H1 G Gate2] = tanh(X [fc1 fcG fcG2]^T + b) (:91-102)
+  {
+    MmOp o = mm_op(X, kXld, B.pk + kPkCat, kXld, R, kCat, kXld, cat, kCat);
+    o.bias[0] = prm + kOffFc1B;
+    o.bias[1] = prm + kOffGB;
+    o.bias[2] = prm + kOffG2B;
+    if (int rc = mm_launch<13, false, false, kEpiTanhCat>(one(o), st)) return rc;
+  }
+  // H2 = tanh(fc2 H1), H2G = G*H2 (:94-104)
+  {
+    MmOp o = mm_op(cat, kCat, B.pk + kPkW2, kHp, R, kHp, kHp, H2, kHp);
+    o.bias[0] = prm + kOffFc2B;
+    o.C2 = H2G; o.ldc2 = kHp;
+    o.E1 = cat + kHp; o.lde1 = kCat;
+    if (int rc = mm_launch<13, false, false, kEpiTanhGate>(one(o), st)) return rc;
+  }
+  // H3 = tanh(fc3 H2G), H4 = H3*Gate2 (:108-112)
+  {
+    MmOp o = mm_op(H2G, kHp, B.pk + kPkW3, kHp, R, kHp, kHp, H3, kHp);
+    o.bias[0] = prm + kOffFc3B;
+    o.C2 = H4; o.ldc2 = kHp;
+    o.E1 = cat + 2 * kHp; o.lde1 = kCat;
+    if (int rc = mm_launch<13, false, false, kEpiTanhGate>(one(o), st)) return rc;
+  }
+  return 0;
+}
+
+// Reverse of chunk c0 from the output adjoints of gI into the chunk's layer
+// adjoints and the input adjoints dX; with sums, the layer adjoints are also
+// added to (sacc) or stored in the stage sums.
+template <typename T, bool SUMS>
+int rev_chunk(const Bufs& B, const void* const gI[3], int64_t c0, int R, int sacc,
+              hipStream_t st) {
+  const int64_t f = frow(B, c0);
+  const float* cat = B.cat + f * kCat;
+  const float* H2 = B.H2 + f * kHp;
+  const float* H3 = B.H3 + f * kHp;
+  CVec3<T> g3;
+  for (int q = 0; q < 3; ++q) g3.a[q] = (const T*)gI[q];
+  float* sout = SUMS ? B.sout + c0 * kOutp : nullptr;
+  float* scat = SUMS ? B.scat + c0 * kCat : nullptr;
+  float* sL4 = SUMS ? B.sL4 + c0 * kHp : nullptr;
+  float* sL2 = SUMS ? B.sL2 + c0 * kHp : nullptr;
+  PF_ELEM(dout_kernel<T>, R, B.s, g3, B.dout, c0, R, sout, sacc);
+  // dH4 = dout fc4 -> dL4, dLG2
+  {
+    MmOp o = mm_op(B.dout, kOutp, B.pk + kPkW4, kHp, R, kHp, kOutp, B.dL4, kHp);
+    o.C2 = B.dcat + 2 * kHp; o.ldc2 = kCat;
+    o.E1 = cat + 2 * kHp; o.lde1 = kCat;
+    o.E2 = H3; o.lde2 = kHp;
+    o.S1 = sL4; o.lds1 = kHp;
+    o.S2 = scat + (SUMS ? 2 * kHp : 0); o.lds2 = kCat;
+    o.sacc = sacc;
+    if (int rc = mm_launch<13, false, true, kEpiRevPair, SUMS>(one(o), st)) return rc;
+  }
+  // dH2G = dL4 fc3 -> dLG, dL2
+  {
+    MmOp o = mm_op(B.dL4, kHp, B.pk + kPkW3, kHp, R, kHp, kHp, B.dcat + kHp, kCat);
+    o.C2 = B.dL2; o.ldc2 = kHp;
+    o.E1 = H2; o.lde1 = kHp;
+    o.E2 = cat + kHp; o.lde2 = kCat;
+    o.S1 = scat + (SUMS ? kHp : 0); o.lds1 = kCat;
+    o.S2 = sL2; o.lds2 = kHp;
+    o.sacc = sacc;
+    if (int rc = mm_launch<13, false, true, kEpiRevPair, SUMS>(one(o), st)) return rc;
+  }
+  // dH1 = dL2 fc2 -> dL1
+  {
+    MmOp o = mm_op(B.dL2, kHp, B.pk + kPkW2, kHp, R, kHp, kHp, B.dcat, kCat);
+    o.E1 = cat; o.lde1 = kCat;
+    o.S1 = scat; o.lds1 = kCat;
+    o.sacc = sacc;
+    if (int rc = mm_launch<13, false, true, kEpiRevTanh, SUMS>(one(o), st)) return rc;
+  }
+  // input adjoints of the chunk: dX = [dL1 dLG dLG2] [fc1; fcG; fcG2]
+  {
+    MmOp o = mm_op(B.dcat, kCat, B.pk + kPkCat, kXld, R, kSrcIn, kCat, B.dX + c0 * kXld, kXld);
+    if (int rc = mm_launch<9, false, true, kEpiStore>(one(o), st)) return rc;
+  }
+  return 0;
+}
+
+// Parameter gradients of chunk c0 added to dparams: [dW | db] = dL^T [act | 1]
+// over its cells, split over kSplitK-cell ranges into FP32 partials laid out
+// like the packed parameters, then summed in order into FP64.  The layer
+// adjoints are the chunk's (dcat, dL2, dL4, dout; rows from 0) or the stage
+// sums (rows from c0).
+int wgrad_chunk(const Bufs& B, const float* dcat, const float* dL2, const float* dL4,
+                const float* dout, int64_t c0, int R, double* dparams, hipStream_t st) {
+  const int64_t f = frow(B, c0);
+  const float* X = B.X + f * kXld;
+  const float* H1 = B.cat + f * kCat;
+  const float* H2G = B.H2G + f * kHp;
+  const float* H4 = B.H4 + f * kHp;
+  const int splits = (R + kSplitK - 1) / kSplitK;
+  {
+    MmOp o = mm_op(dcat, kCat, X, kXld, kCat, kSrcIn + 1, R, nullptr, 0);
+    o.kchunk = kSplitK; o.splits = splits; o.ones = kSrcIn;
+    o.part = B.part; o.rg = kHp; o.nout = kH; o.nin = kSrcIn;
+    o.woff[0] = kOffFc1W; o.woff[1] = kOffGW; o.woff[2] = kOffG2W;
+    o.boff[0] = kOffFc1B; o.boff[1] = kOffGB; o.boff[2] = kOffG2B;
+    if (int rc = mm_launch<9, true, true, kEpiPartial>(one(o), st)) return rc;
+  }
+  {
+    MmBatch bt;
+    std::memset(&bt, 0, sizeof(bt));
+    struct Pg {
+      const float* dL;
+      int64_t ldl;
+      int M, nout;
+      const float* act;
+      int64_t lda;
+      int64_t w, b;
+    };
+    const Pg pg[3] = {{dL2, kHp, kHp, kH, H1, kCat, kOffFc2W, kOffFc2B},
+                      {dL4, kHp, kHp, kH, H2G, kHp, kOffFc3W, kOffFc3B},
+                      {dout, kOutp, kOutp, kSrcOut, H4, kHp, kOffFc4W, kOffFc4B}};
+    for (int i = 0; i < 3; ++i) {
+      MmOp o = mm_op(pg[i].dL, pg[i].ldl, pg[i].act, pg[i].lda, pg[i].M, kH + 1, R, nullptr, 0);
+      o.kchunk = kSplitK; o.splits = splits; o.ones = kH;
+      o.part = B.part; o.rg = pg[i].M; o.nout = pg[i].nout; o.nin = kH;
+      o.woff[0] = pg[i].w; o.boff[0] = pg[i].b;
+      bt.op[i] = o;
+    }
+    bt.nops = 3;
+    if (int rc = mm_launch<13, true, true, kEpiPartial>(bt, st)) return rc;
+  }
+  PF_ELEM(add_partials_kernel, kSrcParams, dparams, (const float*)B.part, (int64_t)kSrcParams,
+          splits, (int64_t)kSrcParams);
+  return 0;
+}
+
+// Input adjoints of every cell -> d g / d var_i (dvar).
+int finish_inputs(const Bufs& B, hipStream_t st) {
+  PF_ELEM(feat_adj_kernel, B.s.m, B.s, (const float*)B.dX, B.dF, B.nrm);
+  PF_ELEM(var_adj_kernel, B.s.m, B.s, (const float*)B.dF, B.dvar);
+  return 0;
+}
+
 // d g / d var_i into dvar (3 x m FP32) for g = sum(G(fwdI) . gI), and the
 // parameter gradient added to dparams (FP64, may be NULL).
 template <typename T>
 int ml_vjp_impl(pf_box* h, const float* prm, const void* const fwdI[3], const void* const gI[3],
-                float* dvar, double* dparams, hipStream_t st) {
-  const Srct L = srct_layout(h);
-  const SrcGeom s = src_geom(h);
-  char* base = (char*)h->srct_ws;
-  Norm21 nrm;
-  for (int f = 0; f < kSrcFeat; ++f) nrm.v[f] = h->srct_norm[f];
-  float* Fn = h->src_ws;
-  float* dX = (float*)(base + L.off_dX);
-  float* dF = (float*)(base + L.off_dF);
-  float* X = (float*)(base + L.off_X);
-  float* cat = (float*)(base + L.off_cat);        // H1 | G | Gate2, ld 624
-  float* H1 = cat;
-  float* Gt = cat + kHp;
-  float* G2 = cat + 2 * kHp;
-  const size_t ca = (size_t)L.Ca * kHp;
-  float* H2 = (float*)(base + L.off_h);            // ld 208
-  float* H2G = H2 + ca;
-  float* H3 = H2 + 2 * ca;
-  float* H4 = H2 + 3 * ca;
-  float* dout = (float*)(base + L.off_dout);       // ld 32
-  float* dcat = (float*)(base + L.off_dcat);       // dL1 | dLG | dLG2, ld 624
-  float* dL4 = (float*)(base + L.off_dl);          // ld 208
-  float* dL2 = dL4 + ca;
-  float* pk = (float*)(base + L.off_pack);
-  float* part = (float*)(base + L.off_part);
-  const float* Wcat = pk + kPkCat;
-  const float* W2p = pk + kPkW2;
-  const float* W3p = pk + kPkW3;
-  const float* W4p = pk + kPkW4;
-
-  // 1. features of every padded cell; the weights in the padded layouts
-  hipLaunchKernelGGL(src_features_kernel<T>, dim3(grid_of(s.m)), dim3(kBlock), 0, st, s,
-                     (const T*)fwdI[0], (const T*)fwdI[1], (const T*)fwdI[2], Fn, nrm);
-  PF_LAUNCH_CHECK("src_features_kernel");
-  PF_ELEM(pack_kernel, kPkFloats, prm, pk);
-  CVec3<T> g3;
-  for (int q = 0; q < 3; ++q) g3.a[q] = (const T*)gI[q];
-
-  // 2. chunks of cells
-  for (int64_t c0 = 0; c0 < s.m; c0 += L.C) {
-    const int R = (int)((s.m - c0) < L.C ? (s.m - c0) : L.C);
-    PF_ELEM(gather_x_kernel, (int64_t)R * kSrcShifts, s, (const float*)Fn, X, c0, R);
-    // forward: [H1 G Gate2] = tanh(X [fc1 fcG fcG2]^T + b) (:91-102)
-    {
-      MmOp o = mm_op(X, kXld, Wcat, kXld, R, kCat, kXld, cat, kCat);
-      o.bias[0] = prm + kOffFc1B;
-      o.bias[1] = prm + kOffGB;
-      o.bias[2] = prm + kOffG2B;
-      if (int rc = mm_launch<13, false, false, kEpiTanhCat>(one(o), st)) return rc;
-    }
-    // H2 = tanh(fc2 H1), H2G = G*H2 (:94-104)
-    {
-      MmOp o = mm_op(H1, kCat, W2p, kHp, R, kHp, kHp, H2, kHp);
-      o.bias[0] = prm + kOffFc2B;
-      o.C2 = H2G; o.ldc2 = kHp;
-      o.E1 = Gt; o.lde1 = kCat;
-      if (int rc = mm_launch<13, false, false, kEpiTanhGate>(one(o), st)) return rc;
-    }
-    // H3 = tanh(fc3 H2G), H4 = H3*Gate2 (:108-112)
-    {
-      MmOp o = mm_op(H2G, kHp, W3p, kHp, R, kHp, kHp, H3, kHp);
-      o.bias[0] = prm + kOffFc3B;
-      o.C2 = H4; o.ldc2 = kHp;
-      o.E1 = G2; o.lde1 = kCat;
-      if (int rc = mm_launch<13, false, false, kEpiTanhGate>(one(o), st)) return rc;
-    }
-    // reverse: output adjoints; dH4 = dout fc4 -> dL4, dLG2
-    PF_ELEM(dout_kernel<T>, R, s, g3, dout, c0, R);
-    {
-      MmOp o = mm_op(dout, kOutp, W4p, kHp, R, kHp, kOutp, dL4, kHp);
-      o.C2 = dcat + 2 * kHp; o.ldc2 = kCat;
-      o.E1 = G2; o.lde1 = kCat;
-      o.E2 = H3; o.lde2 = kHp;
-      if (int rc = mm_launch<13, false, true, kEpiRevPair>(one(o), st)) return rc;
-    }
-    // dH2G = dL4 fc3 -> dLG, dL2
-    {
-      MmOp o = mm_op(dL4, kHp, W3p, kHp, R, kHp, kHp, dcat + kHp, kCat);
-      o.C2 = dL2; o.ldc2 = kHp;
-      o.E1 = H2; o.lde1 = kHp;
-      o.E2 = Gt; o.lde2 = kCat;
-      if (int rc = mm_launch<13, false, true, kEpiRevPair>(one(o), st)) return rc;
-    }
-    // dH1 = dL2 fc2 -> dL1
-    {
-      MmOp o = mm_op(dL2, kHp, W2p, kHp, R, kHp, kHp, dcat, kCat);
-      o.E1 = H1; o.lde1 = kCat;
-      if (int rc = mm_launch<13, false, true, kEpiRevTanh>(one(o), st)) return rc;
-    }
-    // input adjoints of the chunk: dX = [dL1 dLG dLG2] [fc1; fcG; fcG2]
-    {
-      MmOp o = mm_op(dcat, kCat, Wcat, kXld, R, kSrcIn, kCat, dX + c0 * kXld, kXld);
-      if (int rc = mm_launch<9, false, true, kEpiStore>(one(o), st)) return rc;
-    }
-    if (!dparams) continue;
-    // parameter gradients of this chunk: [dW | db] = dL^T [act | 1] over its
-    // cells, split over kSplitK-cell ranges into FP32 partials laid out like
-    // the packed parameters, then summed in order into FP64
-    const int splits = (R + kSplitK - 1) / kSplitK;
-    {
-      MmOp o = mm_op(dcat, kCat, X, kXld, kCat, kSrcIn + 1, R, nullptr, 0);
-      o.kchunk = kSplitK; o.splits = splits; o.ones = kSrcIn;
-      o.part = part; o.rg = kHp; o.nout = kH; o.nin = kSrcIn;
-      o.woff[0] = kOffFc1W; o.woff[1] = kOffGW; o.woff[2] = kOffG2W;
-      o.boff[0] = kOffFc1B; o.boff[1] = kOffGB; o.boff[2] = kOffG2B;
-      if (int rc = mm_launch<9, true, true, kEpiPartial>(one(o), st)) return rc;
-    }
-    {
-      MmBatch bt;
-      std::memset(&bt, 0, sizeof(bt));
-      struct Pg {
-        const float* dL;
-        int64_t ldl;
-        int M, nout;
-        const float* act;
-        int64_t lda;
-        int64_t w, b;
-      };
-      const Pg pg[3] = {{dL2, kHp, kHp, kH, H1, kCat, kOffFc2W, kOffFc2B},
-                        {dL4, kHp, kHp, kH, H2G, kHp, kOffFc3W, kOffFc3B},
-                        {dout, kOutp, kOutp, kSrcOut, H4, kHp, kOffFc4W, kOffFc4B}};
-      for (int i = 0; i < 3; ++i) {
-        MmOp o = mm_op(pg[i].dL, pg[i].ldl, pg[i].act, pg[i].lda, pg[i].M, kH + 1, R, nullptr, 0);
-        o.kchunk = kSplitK; o.splits = splits; o.ones = kH;
-        o.part = part; o.rg = pg[i].M; o.nout = pg[i].nout; o.nin = kH;
-        o.woff[0] = pg[i].w; o.boff[0] = pg[i].b;
-        bt.op[i] = o;
-      }
-      bt.nops = 3;
-      if (int rc = mm_launch<13, true, true, kEpiPartial>(bt, st)) return rc;
-    }
-    PF_ELEM(add_partials_kernel, kSrcParams, dparams, (const float*)part, (int64_t)kSrcParams,
-            splits, (int64_t)kSrcParams);
+                double* dparams, hipStream_t st) {
+  const Bufs B = bufs_of(h);
+  if (int rc = prep<T>(B, prm, fwdI, st)) return rc;
+  for (int64_t c0 = 0; c0 < B.s.m; c0 += B.L.C) {
+    const int R = (int)((B.s.m - c0) < B.L.C ? (B.s.m - c0) : B.L.C);
+    if (int rc = fwd_chunk(B, prm, c0, R, st)) return rc;
+    if (int rc = rev_chunk<T, false>(B, gI, c0, R, 0, st)) return rc;
+    if (dparams)
+      if (int rc = wgrad_chunk(B, B.dcat, B.dL2, B.dL4, B.dout, c0, R, dparams, st)) return rc;
   }
-  // 3. feature adjoints, then d g / d var_i
-  PF_ELEM(feat_adj_kernel, s.m, s, (const float*)dX, dF, nrm);
-  PF_ELEM(var_adj_kernel, s.m, s, (const float*)dF, dvar);
+  return finish_inputs(B, st);
+}
+
+// The forward state's var_i as the reference's adjoint RHS leaves them
+// before its closure block (adjoint.py:170-237: the same interpolations as
+// the forward RHS, so the same stale last planes).
+template <typename T>
+int fwd_inputs(pf_box* h, const Bufs& B, const void* const U[3], hipStream_t st) {
+  return pf_source_inputs(h, U[0], U[1], U[2], B.fwd[0], B.fwd[1], B.fwd[2], st);
+}
+
+// The adjoint state's centre interpolants.
+template <typename T>
+int adj_inputs(const Bufs& B, const void* const A[3], hipStream_t st) {
+  CVec3<T> a3;
+  Vec3<T> o3;
+  for (int q = 0; q < 3; ++q) {
+    a3.a[q] = (const T*)A[q];
+    o3.a[q] = (T*)B.adj[q];
+  }
+  hipLaunchKernelGGL(centre_interp_kernel<T>, dim3(grid_of(B.s.m)), dim3(kBlock), 0, st, B.s,
+                     a3, o3);
+  PF_LAUNCH_CHECK("centre_interp_kernel");
+  return 0;
+}
+
+template <typename T>
+int delta_out(pf_box* h, const Bufs& B, void* const D[3], hipStream_t st) {
+  Vec3<T> d3;
+  for (int q = 0; q < 3; ++q) d3.a[q] = (T*)D[q];
+  hipLaunchKernelGGL(delta_kernel<T>, dim3(launch_blocks(h->g)), dim3(kBlock), 0, st, h->g,
+                     (const float*)B.dvar, B.L.m, d3);
+  PF_LAUNCH_CHECK("delta_kernel");
   return 0;
 }
 
 template <typename T>
 int ml_adj_term_impl(pf_box* h, const void* const U[3], const float* prm, const void* const A[3],
                      void* const D[3], double* dparams, hipStream_t st) {
-  const Srct L = srct_layout(h);
-  const SrcGeom s = src_geom(h);
-  char* base = (char*)h->srct_ws;
-  const size_t pad = (size_t)L.m * sizeof(T);
-  void* fwd[3];
-  void* adj[3];
-  for (int q = 0; q < 3; ++q) {
-    fwd[q] = base + L.off_fwd + q * pad;
-    adj[q] = base + L.off_adj + q * pad;
+  const Bufs B = bufs_of(h);
+  if (int rc = fwd_inputs<T>(h, B, U, st)) return rc;
+  if (int rc = adj_inputs<T>(B, A, st)) return rc;
+  const void* const fc[3] = {B.fwd[0], B.fwd[1], B.fwd[2]};
+  const void* const ac[3] = {B.adj[0], B.adj[1], B.adj[2]};
+  if (int rc = ml_vjp_impl<T>(h, prm, fc, ac, dparams, st)) return rc;
+  return delta_out<T>(h, B, D, st);
+}
+
+// One adjoint RK4 step with the closure (full cache): the forward once ...
+template <typename T>
+int ml_step_begin_impl(pf_box* h, const void* const U[3], const float* prm, hipStream_t st) {
+  const Bufs B = bufs_of(h);
+  if (int rc = fwd_inputs<T>(h, B, U, st)) return rc;
+  const void* const fc[3] = {B.fwd[0], B.fwd[1], B.fwd[2]};
+  if (int rc = prep<T>(B, prm, fc, st)) return rc;
+  for (int64_t c0 = 0; c0 < B.s.m; c0 += B.L.C) {
+    const int R = (int)((B.s.m - c0) < B.L.C ? (B.s.m - c0) : B.L.C);
+    if (int rc = fwd_chunk(B, prm, c0, R, st)) return rc;
   }
-  // the forward state's var_i as the reference's adjoint RHS leaves them
-  // before its closure block (adjoint.py:170-237: the same interpolations as
-  // the forward RHS, so the same stale last planes)
-  if (int rc = pf_source_inputs(h, U[0], U[1], U[2], fwd[0], fwd[1], fwd[2], st)) return rc;
-  CVec3<T> a3;
-  Vec3<T> o3;
-  for (int q = 0; q < 3; ++q) {
-    a3.a[q] = (const T*)A[q];
-    o3.a[q] = (T*)adj[q];
+  return 0;
+}
+
+// ... the reverse of each stage, its layer adjoints summed over the stages ...
+template <typename T>
+int ml_step_term_impl(pf_box* h, const void* const A[3], void* const D[3], int stage,
+                      hipStream_t st) {
+  const Bufs B = bufs_of(h);
+  if (int rc = adj_inputs<T>(B, A, st)) return rc;
+  const void* const ac[3] = {B.adj[0], B.adj[1], B.adj[2]};
+  for (int64_t c0 = 0; c0 < B.s.m; c0 += B.L.C) {
+    const int R = (int)((B.s.m - c0) < B.L.C ? (B.s.m - c0) : B.L.C);
+    if (int rc = rev_chunk<T, true>(B, ac, c0, R, stage > 0, st)) return rc;
   }
-  hipLaunchKernelGGL(centre_interp_kernel<T>, dim3(grid_of(s.m)), dim3(kBlock), 0, st, s, a3, o3);
-  PF_LAUNCH_CHECK("centre_interp_kernel");
-  float* dvar = (float*)(base + L.off_dvar);
-  const void* const fc[3] = {fwd[0], fwd[1], fwd[2]};
-  const void* const ac[3] = {adj[0], adj[1], adj[2]};
-  if (int rc = ml_vjp_impl<T>(h, prm, fc, ac, dvar, dparams, st)) return rc;
-  Vec3<T> d3;
-  for (int q = 0; q < 3; ++q) d3.a[q] = (T*)D[q];
-  hipLaunchKernelGGL(delta_kernel<T>, dim3(launch_blocks(h->g)), dim3(kBlock), 0, st, h->g,
-                     (const float*)dvar, L.m, d3);
-  PF_LAUNCH_CHECK("delta_kernel");
+  if (int rc = finish_inputs(B, st)) return rc;
+  return delta_out<T>(h, B, D, st);
+}
+
+// ... and the step's parameter gradient from the sums.
+int ml_step_end_impl(pf_box* h, double* dparams, hipStream_t st) {
+  const Bufs B = bufs_of(h);
+  for (int64_t c0 = 0; c0 < B.s.m; c0 += B.L.C) {
+    const int R = (int)((B.s.m - c0) < B.L.C ? (B.s.m - c0) : B.L.C);
+    if (int rc = wgrad_chunk(B, B.scat + c0 * kCat, B.sL2 + c0 * kHp, B.sL4 + c0 * kHp,
+                             B.sout + c0 * kOutp, c0, R, dparams, st))
+      return rc;
+  }
   return 0;
 }
 
@@ -871,6 +1060,27 @@ int ml_source_adj_term(pf_box* h, const void* const U[3], const float* params,
   if (!params) return fail(PF_EINVAL, "adjoint closure term: NULL params");
   return h->dtype == PF_F64 ? ml_adj_term_impl<double>(h, U, params, A, D, dparams, s)
                             : ml_adj_term_impl<float>(h, U, params, A, D, dparams, s);
+}
+
+bool ml_source_step_cached(const pf_box* h) {
+  return h->srct_ws && h->srct_bytes >= srct_layout(h).bytes && srct_layout(h).full;
+}
+
+int ml_source_step_begin(pf_box* h, const void* const U[3], const float* params, hipStream_t s) {
+  if (int rc = check_reserved(h, "adjoint closure term of the 'source' ML closure")) return rc;
+  if (!params) return fail(PF_EINVAL, "adjoint closure term: NULL params");
+  return h->dtype == PF_F64 ? ml_step_begin_impl<double>(h, U, params, s)
+                            : ml_step_begin_impl<float>(h, U, params, s);
+}
+
+int ml_source_step_term(pf_box* h, const void* const A[3], void* const D[3], int stage,
+                        hipStream_t s) {
+  return h->dtype == PF_F64 ? ml_step_term_impl<double>(h, A, D, stage, s)
+                            : ml_step_term_impl<float>(h, A, D, stage, s);
+}
+
+int ml_source_step_end(pf_box* h, double* dparams, hipStream_t s) {
+  return dparams ? ml_step_end_impl(h, dparams, s) : 0;
 }
 
 }  // namespace pf
@@ -911,6 +1121,15 @@ int pf_sfs_ml_source_train_reserve(pf_box* h, const float* norm21) {
   return 0;
 }
 
+int pf_sfs_ml_source_cache_budget(pf_box* h, int64_t bytes) {
+  if (int rc = check_box(h)) return rc;
+  if (h->srct_ws)
+    return fail(PF_EINVAL, "pf_sfs_ml_source_cache_budget: call it before "
+                           "pf_sfs_ml_source_train_reserve");
+  h->srct_budget = bytes < 0 ? -1 : bytes;
+  return 0;
+}
+
 int pf_sfs_ml_source_vjp(pf_box* h, const float* params, const void* ui, const void* vi,
                          const void* wi, const void* gx, const void* gy, const void* gz,
                          float* dui, float* dvi, float* dwi, double* dparams, void* stream) {
@@ -923,8 +1142,8 @@ int pf_sfs_ml_source_vjp(pf_box* h, const float* params, const void* ui, const v
   float* dvar = (float*)((char*)h->srct_ws + L.off_dvar);
   const void* const F[3] = {ui, vi, wi};
   const void* const G[3] = {gx, gy, gz};
-  const int rc = h->dtype == PF_F64 ? ml_vjp_impl<double>(h, params, F, G, dvar, dparams, s)
-                                    : ml_vjp_impl<float>(h, params, F, G, dvar, dparams, s);
+  const int rc = h->dtype == PF_F64 ? ml_vjp_impl<double>(h, params, F, G, dparams, s)
+                                    : ml_vjp_impl<float>(h, params, F, G, dparams, s);
   if (rc) return rc;
   float* out[3] = {dui, dvi, dwi};
   for (int q = 0; q < 3; ++q)

@@ -728,14 +728,21 @@ struct AdjClosure {
   double* dparams;
 };
 
+// stage >= 0: a stage of an adjoint RK4 step whose 'source' ML closure term
+// runs from the step's cached forward (ml_source_step_begin ... _end).
 static int adj_term(pf_box* h, const AdjClosure& cl, const void* const U[3],
-                    const void* const Y[3], hipStream_t s, const void* const** D) {
+                    const void* const Y[3], hipStream_t s, const void* const** D,
+                    int stage = -1) {
   if (cl.kind < 0) {
     *D = nullptr;
     return 0;
   }
-  if (int rc = adj_closure_term(h, cl.kind, U, cl.params, Y, cl.work, cl.delta, cl.dparams, s))
+  if (stage >= 0) {
+    if (int rc = ml_source_step_term(h, Y, cl.delta, stage, s)) return rc;
+  } else if (int rc = adj_closure_term(h, cl.kind, U, cl.params, Y, cl.work, cl.delta,
+                                       cl.dparams, s)) {
     return rc;
+  }
   *D = (const void* const*)cl.delta;
   return 0;
 }
@@ -754,18 +761,25 @@ int adj_rk4_impl(pf_box* h, const void* const U[3], void* const Ua[3], void* con
   const void* const Bc[3] = {B[0], B[1], B[2]};
   const void* const* D;
   int rc;
-  if ((rc = adj_term(h, cl, U, Uc, s, &D))) return rc;
+  // the 'source' ML closure: the forward state is the same in the four
+  // stages, so its network forward runs once and the step's parameter
+  // gradient is one product of the stage-summed layer adjoints (when the
+  // box's cache fits; else each stage evaluates the whole term)
+  const bool mlc = cl.kind == 2 && ml_source_step_cached(h);
+  if (mlc && (rc = ml_source_step_begin(h, U, cl.params, s))) return rc;
+  if ((rc = adj_term(h, cl, U, Uc, s, &D, mlc ? 0 : -1))) return rc;
   if ((rc = launch_adj_stage<T, kStage1>(h, s, Uc, U, D, Uc, A, nullptr, k, visc, rho)))
     return rc;
-  if ((rc = adj_term(h, cl, U, Ac, s, &D))) return rc;
+  if ((rc = adj_term(h, cl, U, Ac, s, &D, mlc ? 1 : -1))) return rc;
   if ((rc = launch_adj_stage<T, kStage2>(h, s, Ac, U, D, Uc, B, nullptr, k, visc, rho)))
     return rc;
-  if ((rc = adj_term(h, cl, U, Bc, s, &D))) return rc;
+  if ((rc = adj_term(h, cl, U, Bc, s, &D, mlc ? 2 : -1))) return rc;
   if ((rc = launch_adj_stage<T, kStage3>(h, s, Bc, U, D, Uc, A, nullptr, k, visc, rho)))
     return rc;
-  if ((rc = adj_term(h, cl, U, Ac, s, &D))) return rc;
+  if ((rc = adj_term(h, cl, U, Ac, s, &D, mlc ? 3 : -1))) return rc;
   if ((rc = launch_adj_stage<T, kStage4>(h, s, Ac, U, D, Uc, nullptr, Ua, k, visc, rho)))
     return rc;
+  if (mlc && (rc = ml_source_step_end(h, cl.dparams, s))) return rc;
   return 0;
 }
 

@@ -98,6 +98,7 @@ SIGNATURES = [
     ('pf_sfs_ml_source_params', ctypes.c_int64, []),
     ('pf_sfs_ml_source_reserve', _I, [_P]),
     ('pf_sfs_ml_source_train_reserve', _I, [_P, ctypes.POINTER(ctypes.c_float)]),
+    ('pf_sfs_ml_source_cache_budget', _I, [_P, ctypes.c_int64]),
     ('pf_sfs_ml_source_vjp', _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _PD, _P]),
     ('pf_rk4_step_noalloc', _I, [_I, _I, _I, _D, _D, _D, _I, _P, _P, _P, _P, _P, _P, _D, _D, _D,
                                  _PP, _P]),
@@ -342,6 +343,12 @@ class Box:
         the normalisation table for the adjoint closure term."""
         nrm = (ctypes.c_float * 21)(*[float(x) for x in norm21])
         check(lib().pf_sfs_ml_source_train_reserve(self.h, nrm), 'pf_sfs_ml_source_train_reserve')
+
+    def sfs_ml_source_cache_budget(self, nbytes):
+        """Bytes the closure's all-cell training cache may take (before
+        sfs_ml_source_train_reserve; negative = the default, 48 GiB)."""
+        check(lib().pf_sfs_ml_source_cache_budget(self.h, int(nbytes)),
+              'pf_sfs_ml_source_cache_budget')
 
     def sfs_ml_source_vjp(self, params, ui, vi, wi, G, dI=None, dparams=None):
         """dg/d(ui, vi, wi) into dI (three FP32 padded arrays, optional) and

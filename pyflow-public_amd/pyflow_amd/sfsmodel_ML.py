@@ -127,7 +127,11 @@ class residual_stress:
         if hasattr(inputConfig, 'LR'):
             self.LR = float(inputConfig.LR)
         if getattr(inputConfig, 'adjointTraining', False):
-            # the reverse pass's workspace (allocates once)
+            # the reverse pass's workspace (allocates once); its all-cell
+            # cache lets each adjoint RK4 step evaluate the network once
+            budget = getattr(inputConfig, 'mlSourceCacheBytes', None)
+            if budget is not None:
+                self.box.sfs_ml_source_cache_budget(budget)
             self.box.sfs_ml_source_train_reserve(self.norm)
 
     # the RMSprop step size of the reference's optimizer (sfsmodel_ML.py:164-168)
@@ -212,10 +216,13 @@ class residual_stress:
 
 
 def main(argv=None):
-    """Time one closure evaluation on random padded fields, or with --reverse
+    """Time one closure evaluation on random padded fields, with --reverse
     its reverse pass (pf_sfs_ml_source_vjp: the forward recomputed, the
-    input- and weight-gradient products, three times the forward's):
-    python -m pyflow_amd.sfsmodel_ML --N 64 --dtype f32 --reps 5 [--reverse]"""
+    input- and weight-gradient products, three times the forward's), or with
+    --adjoint-step one adjoint RK4 step with the closure's term (closure
+    kind 2 of pf_adj_rk4_step, BiCGStab-free) beside four separate
+    evaluations of the term:
+    python -m pyflow_amd.sfsmodel_ML --N 64 --dtype f32 --reps 5 [--reverse | --adjoint-step]"""
     import argparse
     import math
     import types
@@ -225,7 +232,10 @@ def main(argv=None):
     ap.add_argument('--dtype', choices=('f32', 'f64'), default='f32')
     ap.add_argument('--reps', type=int, default=5)
     ap.add_argument('--reverse', action='store_true')
+    ap.add_argument('--adjoint-step', action='store_true')
     a = ap.parse_args(argv)
+    if a.adjoint_step:
+        a.reverse = True
     dt = torch.float32 if a.dtype == 'f32' else torch.float64
     n = a.N
     h = 2 * math.pi / n
@@ -236,6 +246,31 @@ def main(argv=None):
     ml = residual_stress(types.SimpleNamespace(simDt=1e-3, adjointTraining=a.reverse), d, geo)
     gen = torch.Generator(device='cuda').manual_seed(0)
     U = [torch.randn((n + 4,) * 3, dtype=dt, device='cuda', generator=gen) for _ in range(3)]
+    if a.adjoint_step:
+        A = [0.01 * torch.randn((n + 4,) * 3, dtype=dt, device='cuda', generator=gen)
+             for _ in range(3)]
+        At = [torch.zeros_like(x) for x in A]
+        delta = [torch.zeros_like(x) for x in A]
+
+        def step():
+            box.adj_rk4_step(*U, *A, *At, 1e-3, 0.01, 1.0, 2, ml.params, None, delta, ml.grad)
+
+        def terms():
+            for _ in range(4):
+                box.adj_closure_term(2, *U, ml.params, *A, None, delta, ml.grad)
+        for name, fn in (('adjoint RK4 step (forward once, stage sums)', step),
+                         ('four separate closure terms', terms)):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(a.reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            print('ML source closure %d^3 %s: %s %.3f ms'
+                  % (n, a.dtype, name, e0.elapsed_time(e1) / a.reps), flush=True)
+        return 0
     if a.reverse:
         Gb = [torch.randn((n + 4,) * 3, dtype=dt, device='cuda', generator=gen) for _ in range(3)]
         dI = [torch.empty((n + 4,) * 3, dtype=torch.float32, device='cuda') for _ in range(3)]

@@ -12,7 +12,12 @@ closure kind 2.
   of the RHS and the gradient to rel L2 1e-4;
 * a two-step window of Domain.forwardStep / adjointStep (RK4 + BiCGStab at
   1e-12) against the oracle's window: adjoint velocity to rel L2 1e-9 and the
-  accumulated parameter gradient (eight adjoint RHS evaluations) to 1e-4.
+  accumulated parameter gradient (eight adjoint RHS evaluations) to 1e-4;
+* the adjoint RK4 step that evaluates the network once per step (all-cell
+  cache, stage-summed weight gradient) against the one that evaluates the
+  whole term in each stage, on a box of two chunks: adjoint fields bit for
+  bit, parameter gradient to FP32 rounding (1e-4; measured 2.0e-5: FP32
+  sums over four stages, then one product, instead of four products).
 
 The bars are FP32 rounding of a network whose 273 -> 200 -> 200 -> 18
 products are summed in another order; the measured values are printed.
@@ -198,3 +203,40 @@ def test_adjoint_window_with_ml_closure_against_oracle(G):
     e = rel_l2(cpu(D.sfsmodel.grad), train.packed_grad().numpy())
     print('window parameter gradient (8 evaluations) rel L2 %.2e' % e)
     assert e < TOL_PARAMS
+
+
+def test_cached_adjoint_step_matches_per_stage_terms():
+    """pf_adj_rk4_step with closure 2 on a 38^3 box (42^3 = 74,088 padded
+    cells: two chunks of the reverse) with and without the all-cell cache."""
+    from pyflow_amd import _native
+    from pyflow_amd.sfsmodel_ML import NORM
+    n = 38
+    h = 2 * math.pi / n
+    flat, _ = ml_source_params(5)
+    params = torch.tensor(flat, dtype=torch.float32, device='cuda')
+    gen = torch.Generator(device='cuda').manual_seed(17)
+    shape = (n + 4,) * 3
+    U = [torch.randn(shape, dtype=torch.float64, device='cuda', generator=gen) for _ in range(3)]
+    A0 = [1e-2 * torch.randn(shape, dtype=torch.float64, device='cuda', generator=gen)
+          for _ in range(3)]
+    out = []
+    for budget in (-1, 0):
+        box = _native.Box(n, n, n, h, h, h, dtype=torch.float64)
+        box.sfs_ml_source_cache_budget(budget)
+        box.sfs_ml_source_train_reserve(NORM)
+        A = [a.clone() for a in A0]
+        At = [torch.zeros_like(a) for a in A0]
+        delta = [torch.zeros_like(a) for a in A0]
+        grad = torch.zeros(params.numel(), dtype=torch.float64, device='cuda')
+        for _ in range(2):
+            box.adj_rk4_step(*U, *A, *At, 2e-3, 0.01, 1.0, 2, params, None, delta, grad)
+        torch.cuda.synchronize()
+        out.append(([cpu(a) for a in A], cpu(grad), box.device_bytes))
+    (Ac, gc, bc), (Ap, gp, bp) = out
+    print('device bytes: cached %.2f GB, per-stage %.2f GB' % (bc / 1e9, bp / 1e9))
+    assert bc > bp
+    for a, b in zip(Ac, Ap):
+        assert np.array_equal(a, b)
+    e = rel_l2(gc, gp)
+    print('parameter gradient, cached vs per-stage: rel L2 %.2e' % e)
+    assert 0 < np.abs(gc).max() and e < TOL_PARAMS
