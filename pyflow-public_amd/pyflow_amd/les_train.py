@@ -1,11 +1,12 @@
-"""Closure training against filtered-DNS targets: the counterpart, for the
-closure MLP, of the reference's adjoint-training driver
-(examples/adjoint_training/run_PyFlow_adjoint_training.py:43-141, which
-restarts from a target file and trains its closure with adjointTraining,
-numCheckpointIt and getTargetFileName).
+"""Closure training against filtered-DNS targets: the reference's
+adjoint-training driver (examples/adjoint_training/
+run_PyFlow_adjoint_training.py:43-141, which restarts from a target file and
+trains its closure with adjointTraining, numCheckpointIt and
+getTargetFileName), with the closure MLP or the reference's own 'source' ML
+closure (--closure ml).
 
     python -m pyflow_amd.les_train --targets DIR --make --N 256 --ratio 4 \\
-        --window 20 --windows 2 --dtype f32
+        --window 20 --windows 2 --dtype f32 [--closure ml]
 
 --make first writes the targets (pyflow_amd.les_targets: a Passot-Pouquet
 DNS on N^3, box-filtered ratio x onto the LES grid, one file every `window`
@@ -29,15 +30,17 @@ import numpy as np
 import torch
 
 
-def config(meta, window, windows, dtype, params, LR=0.01, tol=None):
+def config(meta, window, windows, dtype, params, LR=0.01, tol=None, closure='mlp'):
     """The run's input configuration (attribute names of the reference's
     inputConfigClass)."""
     f32 = dtype == torch.float32
     base = meta['base']
+    sfs = {'SFSmodel': 'MLP', 'mlpParams': params} if closure == 'mlp' else \
+        {'SFSmodel': 'ML', 'mlSourceParams': params}
     cfg = types.SimpleNamespace(
         configName='restart', configFileStr=meta['config'], dataFileStr=meta['files'][0],
         dataFileType='restart', nproc_x=1, nproc_y=1, nproc_z=1, mu=meta['nu'], rho=1.0,
-        simDt=meta['dt_les'], numIt=window * windows, SFSmodel='MLP', mlpParams=params,
+        simDt=meta['dt_les'], numIt=window * windows, **sfs,
         advancerName='RK4', equationMode='NS', pSolverMode='bicgstab',
         min_pressure_residual=tol if tol is not None else (1e-6 if f32 else 1e-12),
         max_pressure_iterations=3000, dtypeTorch=dtype,
@@ -61,7 +64,9 @@ def main(argv=None):
     ap.add_argument('--window', type=int, default=20, help='LES steps per training window')
     ap.add_argument('--windows', type=int, default=2)
     ap.add_argument('--dtype', choices=('f32', 'f64'), default='f32')
-    ap.add_argument('--mlp-seed', type=int, default=13)
+    ap.add_argument('--mlp-seed', type=int, default=13, help='seed of the closure weights')
+    ap.add_argument('--closure', choices=('mlp', 'ml'), default='mlp',
+                    help="the closure MLP or the reference's 'source' ML closure")
     a = ap.parse_args(argv)
     dtype = torch.float32 if a.dtype == 'f32' else torch.float64
     if a.make:
@@ -76,12 +81,18 @@ def main(argv=None):
         raise SystemExit('the targets hold one file every %d steps up to %d; the run needs '
                          'every %d up to %d' % (meta['every'], meta['steps'], a.window,
                                                 a.window * a.windows))
-    cfg = config(meta, a.window, a.windows, dtype, init_params(a.mlp_seed))
+    if a.closure == 'ml':
+        from .sfsmodel_ML import init_params as init_ml
+        params = init_ml(a.mlp_seed)
+    else:
+        params = init_params(a.mlp_seed)
+    cfg = config(meta, a.window, a.windows, dtype, params, closure=a.closure)
     torch.cuda.synchronize()
     res = PyFlow.simulate(cfg, verbose=False)
     torch.cuda.synchronize()
     m = meta['N'] // meta['ratio']
-    out = {'les_grid': m, 'dns_grid': meta['N'], 'dtype': a.dtype, 'window_steps': a.window,
+    out = {'closure': a.closure, 'les_grid': m, 'dns_grid': meta['N'], 'dtype': a.dtype,
+           'window_steps': a.window,
            'windows': a.windows, 'wall_s': round(res.wall, 3),
            's_per_window': round(res.wall / a.windows, 3),
            'forward_plus_adjoint_mcell_steps_per_s':
