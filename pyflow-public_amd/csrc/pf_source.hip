@@ -22,12 +22,16 @@
 // on M, so a layer's accumulator tile (lane l holds neurons 4*(l/16)+i of cell
 // l%16) is, element by element, the next layer's B operand: the K index of
 // k-step (t, i) in lane group g is neuron 16 t + 4 g + i, and the weights (A
-// operand) are read with the same permutation.  Activations never leave the
-// registers.  The first layer's K runs over (shift, feature) with the 21
-// features of a shift padded to 24, so each k-step reads one neighbour.
-// Weights reach the MFMAs through LDS: each block stages one chunk of input
-// columns at a time for its four waves, the next chunk's loads in flight
-// while the current one is consumed; a 128-bit LDS read feeds four MFMAs.
+// operand) are read with the same permutation.  Two waves share 16 cells,
+// each computing half of every layer's 13 neuron tiles (7 and 6), so that a
+// wave's activations fit two waves per SIMD; before each hidden layer the
+// halves meet in LDS.  The first layer's K runs over (shift, feature) with
+// the 21 features of a shift padded to 24, so each k-step reads one
+// neighbour.  Weights reach the MFMAs through LDS: each block stages one
+// chunk of input columns at a time for its eight waves, the next chunk's
+// loads in flight while the current one is consumed; a 128-bit LDS read
+// feeds four MFMAs.  Measured at 64^3: 2.96 ms per evaluation with one wave
+// per 16 cells (498 registers, one wave per SIMD), 2.50 ms with the halves.
 // Summation order differs from the reference's CPU GEMM, so results agree to
 // FP32 rounding, not bit for bit.
 #include <hip/hip_runtime.h>
@@ -47,21 +51,21 @@ __device__ __forceinline__ f32x4 mfma(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// tanh(acc + b) in accumulator layout: element i of tile t is neuron
-// 16 t + 4 g + i.
-__device__ __forceinline__ void bias_tanh(f32x4 (&a)[kSrcTiles], const float* __restrict__ b,
-                                          int g) {
+// tanh(acc + b) over tiles O0..O0+NT-1 in accumulator layout: element i of
+// tile t is neuron 16 t + 4 g + i.
+template <int O0, int NT>
+__device__ __forceinline__ void bias_tanh(f32x4 (&a)[NT], const float* __restrict__ b, int g) {
 #pragma unroll
-  for (int t = 0; t < kSrcTiles; ++t)
+  for (int o = 0; o < NT; ++o)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int n = 16 * t + 4 * g + i;
-      a[t][i] = n < kSrcHid ? tanhf(a[t][i] + b[n]) : 0.f;
+      const int n = 16 * (O0 + o) + 4 * g + i;
+      a[o][i] = n < kSrcHid ? tanhf(a[o][i] + b[n]) : 0.f;
     }
 }
 
 // Weights go through LDS: a block stages a chunk of input columns of a
-// weight matrix once for its four waves, row-major (chunk[neuron][column]),
+// weight matrix once for its eight waves, row-major (chunk[neuron][column]),
 // so one 128-bit LDS read gives a lane the A operands of four consecutive
 // k-steps.  Pitches (in floats) put the 8 lanes of each pass of a 128-bit
 // read on distinct banks: 20 for the 16-column chunks of the hidden layers,
@@ -72,6 +76,10 @@ constexpr int kStageCols = 24;                             // 21 features of a s
 constexpr int kRows = 16 * kSrcTiles;                      // 208 neurons
 constexpr int kMatIn = kRows * kPitchIn;                   // one input-layer chunk
 constexpr int kStageFloats = 2 * kMatIn;                   // >= kRows * kPitchHid
+constexpr int kNetThreads = 512;                           // 4 groups of 16 cells x 2 halves
+// A wave computes one half of a layer's 13 neuron tiles for its 16 cells:
+// tiles 0-6 (half 0) or 7-12 (half 1).
+constexpr int kHalf0 = 7;
 
 typedef float f32x2v __attribute__((ext_vector_type(2)));
 
@@ -83,13 +91,13 @@ typedef float f32x2v __attribute__((ext_vector_type(2)));
 template <int PITCH, int NC, int NMAT>
 struct Stager {
   static constexpr int kPer = NMAT * NC * kRows;
-  static constexpr int kN = (kPer + 255) / 256;
+  static constexpr int kN = (kPer + kNetThreads - 1) / kNetThreads;
   float v[kN];
   __device__ __forceinline__ void load(const float* __restrict__ W0, const float* __restrict__ W1,
                                        int ld, int nrows, int c0, int nvalid) {
 #pragma unroll
     for (int e = 0; e < kN; ++e) {
-      const int q = threadIdx.x + 256 * e;
+      const int q = threadIdx.x + kNetThreads * e;
       const int mat = q / (NC * kRows);
       const int qq = q - mat * (NC * kRows);
       const int r = qq / NC, j = qq - r * NC;
@@ -100,7 +108,7 @@ struct Stager {
   __device__ __forceinline__ void store(float* __restrict__ chunk) const {
 #pragma unroll
     for (int e = 0; e < kN; ++e) {
-      const int q = threadIdx.x + 256 * e;
+      const int q = threadIdx.x + kNetThreads * e;
       if (q < kPer) {
         const int mat = q / (NC * kRows);
         const int qq = q - mat * (NC * kRows);
@@ -111,30 +119,31 @@ struct Stager {
   }
 };
 
-// One hidden layer on register-resident activations: out = W x over the
-// permuted K order (see the file comment), one 16-column chunk of W per t,
-// double-buffered in LDS (buf0, buf1); one barrier per chunk.  A lane reads
-// W[16 o + col][16 t + 4 g .. + 3] with one 128-bit read, one tile ahead.
+// One hidden layer: out (tiles O0..O0+NT-1) = W x over the permuted K order
+// (see the file comment), x the whole previous layer (13 tiles, in
+// registers), one 16-column chunk of W per t, double-buffered in LDS (buf0,
+// buf1); one barrier per chunk.  A lane reads W[16 o + col][16 t + 4 g .. +
+// 3] with one 128-bit read, one tile ahead.
+template <int O0, int NT>
 __device__ __forceinline__ void hidden_layer(const f32x4 (&x)[kSrcTiles],
                                              const float* __restrict__ W, float* buf0,
-                                             float* buf1, int col, int g,
-                                             f32x4 (&out)[kSrcTiles]) {
+                                             float* buf1, int col, int g, f32x4 (&out)[NT]) {
   auto nvalid = [](int t) { return t < kSrcTiles - 1 ? 16 : kSrcHid - 16 * t; };
   Stager<kPitchHid, 16, 1> st;
   st.load(W, W, kSrcHid, kSrcHid, 0, nvalid(0));
   st.store(buf0);
   __syncthreads();
 #pragma unroll
-  for (int o = 0; o < kSrcTiles; ++o) out[o] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int o = 0; o < NT; ++o) out[o] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int t = 0; t < kSrcTiles; ++t) {
     if (t + 1 < kSrcTiles) st.load(W, W, kSrcHid, kSrcHid, 16 * (t + 1), nvalid(t + 1));
-    const float* base = ((t & 1) ? buf1 : buf0) + col * kPitchHid + 4 * g;
+    const float* base = ((t & 1) ? buf1 : buf0) + (16 * O0 + col) * kPitchHid + 4 * g;
     f32x4 a = *reinterpret_cast<const f32x4*>(base);
 #pragma unroll
-    for (int o = 0; o < kSrcTiles; ++o) {
+    for (int o = 0; o < NT; ++o) {
       f32x4 an = a;
-      if (o + 1 < kSrcTiles) an = *reinterpret_cast<const f32x4*>(base + 16 * (o + 1) * kPitchHid);
+      if (o + 1 < NT) an = *reinterpret_cast<const f32x4*>(base + 16 * (o + 1) * kPitchHid);
 #pragma unroll
       for (int i = 0; i < 4; ++i) out[o] = mfma(a[i], x[t][i], out[o]);
       a = an;
@@ -145,17 +154,18 @@ __device__ __forceinline__ void hidden_layer(const f32x4 (&x)[kSrcTiles],
 }
 
 // First-layer product of one or two 273-input matrices over the gathered
-// input, one chunk of 21 columns (one shift) per matrix and step, double-
-// buffered like hidden_layer.  K order within a shift: four k-steps i with
-// lane group g on feature 4 g + i (one 128-bit read), then two with
-// 16 + 2 g + i (one 64-bit read; features 21-23 are zero).
-template <int NMAT>
+// input (output tiles O0..O0+NT-1), one chunk of 21 columns (one shift) per
+// matrix and step, double-buffered like hidden_layer.  K order within a
+// shift: four k-steps i with lane group g on feature 4 g + i (one 128-bit
+// read), then two with 16 + 2 g + i (one 64-bit read; features 21-23 are
+// zero).
+template <int NMAT, int O0, int NT>
 __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __restrict__ Fn,
                                             bool valid, int ci, int cj, int ck,
                                             const float* __restrict__ W0,
                                             const float* __restrict__ W1, float* buf0,
-                                            float* buf1, int col, int g,
-                                            f32x4 (&a0)[kSrcTiles], f32x4 (&a1)[kSrcTiles]) {
+                                            float* buf1, int col, int g, f32x4 (&a0)[NT],
+                                            f32x4 (&a1)[NT]) {
   auto gather = [&](int sh, float (&b)[6]) {
     const int64_t nb = valid ? shifted(s, ci, cj, ck, sh) : 0;
 #pragma unroll
@@ -171,7 +181,7 @@ __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __res
   gather(0, b);
   __syncthreads();
 #pragma unroll
-  for (int o = 0; o < kSrcTiles; ++o) {
+  for (int o = 0; o < NT; ++o) {
     a0[o] = f32x4{0.f, 0.f, 0.f, 0.f};
     if (NMAT > 1) a1[o] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
@@ -181,7 +191,7 @@ __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __res
       st.load(W0, W1, kSrcIn, kSrcHid, (sh + 1) * kSrcFeat, kSrcFeat);
       gather(sh + 1, bn);
     }
-    const float* c0 = ((sh & 1) ? buf1 : buf0) + col * kPitchIn;
+    const float* c0 = ((sh & 1) ? buf1 : buf0) + (16 * O0 + col) * kPitchIn;
     const float* c1 = c0 + (NMAT > 1 ? kMatIn : 0);
     // the operands of tile o + 1 are read while tile o's MFMAs run
     struct Ops {
@@ -203,9 +213,9 @@ __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __res
     };
     Ops nx = read_ops(0);
 #pragma unroll
-    for (int o = 0; o < kSrcTiles; ++o) {
+    for (int o = 0; o < NT; ++o) {
       const Ops cu = nx;
-      if (o + 1 < kSrcTiles) nx = read_ops(o + 1);
+      if (o + 1 < NT) nx = read_ops(o + 1);
       const f32x4 wa = cu.wa, va = cu.va;
       const f32x2v wb = cu.wb, vb = cu.vb;
 #pragma unroll
@@ -228,60 +238,111 @@ __device__ __forceinline__ void input_layer(const SrcGeom& s, const float* __res
   }
 }
 
-// 2. the network: 64 cells per 256-thread block, 16 per wave; P[c * m + cell]
-// for channels c = 0..8.
-__global__ void __launch_bounds__(256) src_network_kernel(SrcGeom s, const float* __restrict__ Fn,
-                                                          const float* __restrict__ prm,
-                                                          float* __restrict__ P) {
-  __shared__ __align__(16) float chunks[2 * kStageFloats];
-  float* buf0 = chunks;
-  float* buf1 = chunks + kStageFloats;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int col = lane & 15, g = lane >> 4;
-  const int64_t cell = ((int64_t)blockIdx.x * 4 + wave) * 16 + col;
-  const bool valid = cell < s.m;
-  int ci = 0, cj = 0, ck = 0;
-  if (valid) decode_padded(s, cell, ci, cj, ck);
+// The two halves of a layer meet in LDS: each wave of a cell group writes
+// its tiles (one 128-bit slot per lane and tile), and after a barrier reads
+// its partner's, so that both hold the whole layer as the next layer's B
+// operand.
+template <int O0, int NT>
+__device__ __forceinline__ void exchange(float* __restrict__ xb, int lane,
+                                         const f32x4 (&mine)[NT], f32x4 (&all)[kSrcTiles]) {
+#pragma unroll
+  for (int o = 0; o < NT; ++o) {
+    *reinterpret_cast<f32x4*>(xb + ((O0 + o) * 64 + lane) * 4) = mine[o];
+    all[O0 + o] = mine[o];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < kSrcTiles; ++t)
+    if (t < O0 || t >= O0 + NT) all[t] = *reinterpret_cast<const f32x4*>(xb + (t * 64 + lane) * 4);
+}
 
-  f32x4 h1[kSrcTiles], gt[kSrcTiles];
+// The network for one half (H) of a wave pair: 16 cells, neuron tiles
+// O0..O0+NT-1 of every layer.
+template <int H>
+__device__ __forceinline__ void net_half(const SrcGeom& s, const float* __restrict__ Fn,
+                                         const float* __restrict__ prm, float* __restrict__ P,
+                                         float* buf0, float* buf1, float* xb, float* red,
+                                         int lane, int64_t cell, bool valid, int ci, int cj,
+                                         int ck) {
+  constexpr int O0 = H ? kHalf0 : 0;
+  constexpr int NT = H ? kSrcTiles - kHalf0 : kHalf0;
+  const int col = lane & 15, g = lane >> 4;
+  f32x4 all[kSrcTiles];
+  f32x4 h1[NT], gt[NT];
   // L1 = tanh(fc1 x), G = tanh(fcG x)  (:91-100)
-  input_layer<2>(s, Fn, valid, ci, cj, ck, prm + kOffFc1W, prm + kOffGW, buf0, buf1, col, g,
-                 h1, gt);
-  bias_tanh(h1, prm + kOffFc1B, g);
-  bias_tanh(gt, prm + kOffGB, g);
+  input_layer<2, O0, NT>(s, Fn, valid, ci, cj, ck, prm + kOffFc1W, prm + kOffGW, buf0, buf1,
+                         col, g, h1, gt);
+  bias_tanh<O0, NT>(h1, prm + kOffFc1B, g);
+  bias_tanh<O0, NT>(gt, prm + kOffGB, g);
+  exchange<O0, NT>(xb, lane, h1, all);
   // H2 = tanh(fc2 L1); H2_G = G * H2  (:94-104)
-  f32x4 hg[kSrcTiles];
-  hidden_layer(h1, prm + kOffFc2W, buf0, buf1, col, g, hg);
-  bias_tanh(hg, prm + kOffFc2B, g);
+  f32x4 hg[NT];
+  hidden_layer<O0, NT>(all, prm + kOffFc2W, buf0, buf1, col, g, hg);
+  bias_tanh<O0, NT>(hg, prm + kOffFc2B, g);
 #pragma unroll
-  for (int t = 0; t < kSrcTiles; ++t) hg[t] = gt[t] * hg[t];
+  for (int o = 0; o < NT; ++o) hg[o] = gt[o] * hg[o];
   // Gate2 = tanh(fcG2 x)  (:102)
-  f32x4 g2[kSrcTiles];
-  input_layer<1>(s, Fn, valid, ci, cj, ck, prm + kOffG2W, prm + kOffG2W, buf0, buf1, col, g,
-                 g2, g2);
-  bias_tanh(g2, prm + kOffG2B, g);
+  f32x4 g2[NT];
+  input_layer<1, O0, NT>(s, Fn, valid, ci, cj, ck, prm + kOffG2W, prm + kOffG2W, buf0, buf1,
+                         col, g, g2, g2);
+  bias_tanh<O0, NT>(g2, prm + kOffG2B, g);
+  exchange<O0, NT>(xb, lane, hg, all);
   // H3 = tanh(fc3 H2_G); H4 = H3 * Gate2  (:108-112)
-  f32x4 h3[kSrcTiles];
-  hidden_layer(hg, prm + kOffFc3W, buf0, buf1, col, g, h3);
-  bias_tanh(h3, prm + kOffFc3B, g);
+  f32x4 h3[NT];
+  hidden_layer<O0, NT>(all, prm + kOffFc3W, buf0, buf1, col, g, h3);
+  bias_tanh<O0, NT>(h3, prm + kOffFc3B, g);
 #pragma unroll
-  for (int t = 0; t < kSrcTiles; ++t) h3[t] = h3[t] * g2[t];
-  // out = fc4 H4  (:114); one 16-neuron tile holds channels 0..15
+  for (int o = 0; o < NT; ++o) h3[o] = h3[o] * g2[o];
+  // out = fc4 H4  (:114): each half sums its K tiles, one 16-neuron tile of
+  // channels 0..15; the halves' sums meet in LDS
   f32x4 o4 = f32x4{0.f, 0.f, 0.f, 0.f};
   const float* W4 = prm + kOffFc4W;
 #pragma unroll
-  for (int t = 0; t < kSrcTiles; ++t)
+  for (int o = 0; o < NT; ++o)
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      o4 = mfma(wload(W4, kSrcHid, kSrcOut, kSrcHid, col, 16 * t + 4 * g + i), h3[t][i], o4);
-  if (valid) {
+      o4 = mfma(wload(W4, kSrcHid, kSrcOut, kSrcHid, col, 16 * (O0 + o) + 4 * g + i), h3[o][i],
+                o4);
+  if (H) *reinterpret_cast<f32x4*>(red + lane * 4) = o4;
+  __syncthreads();
+  if (!H && valid) {
+    const f32x4 o1 = *reinterpret_cast<const f32x4*>(red + lane * 4);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int c = 4 * g + i;
-      if (c < kSrcUsed) P[(int64_t)c * s.m + cell] = o4[i] + prm[kOffFc4B + c];
+      if (c < kSrcUsed) P[(int64_t)c * s.m + cell] = (o4[i] + o1[i]) + prm[kOffFc4B + c];
     }
   }
+}
+
+// 2. the network: 64 cells per 512-thread block, 16 per pair of waves, each
+// wave of a pair on half of every layer's neurons (one wave per SIMD would
+// otherwise hold the whole layer: 13 tiles per activation); P[c * m + cell]
+// for channels c = 0..8.
+__global__ void __launch_bounds__(kNetThreads) src_network_kernel(SrcGeom s,
+                                                                  const float* __restrict__ Fn,
+                                                                  const float* __restrict__ prm,
+                                                                  float* __restrict__ P) {
+  __shared__ __align__(16) float chunks[2 * kStageFloats];
+  __shared__ __align__(16) float xchg[4 * kSrcTiles * 64 * 4];   // per cell group
+  __shared__ __align__(16) float red[4 * 64 * 4];
+  float* buf0 = chunks;
+  float* buf1 = chunks + kStageFloats;
+  const int lane = threadIdx.x & 63;
+  // wave-uniform (kept in a scalar register, so the halves branch, not mask)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int grp = wave >> 1, half = wave & 1;
+  const int col = lane & 15;
+  const int64_t cell = ((int64_t)blockIdx.x * 4 + grp) * 16 + col;
+  const bool valid = cell < s.m;
+  int ci = 0, cj = 0, ck = 0;
+  if (valid) decode_padded(s, cell, ci, cj, ck);
+  float* xb = xchg + grp * kSrcTiles * 64 * 4;
+  float* rb = red + grp * 64 * 4;
+  if (half)
+    net_half<1>(s, Fn, prm, P, buf0, buf1, xb, rb, lane, cell, valid, ci, cj, ck);
+  else
+    net_half<0>(s, Fn, prm, P, buf0, buf1, xb, rb, lane, cell, valid, ci, cj, ck);
 }
 
 // 3. GX, GY, GZ from the network's channels 0-8
@@ -337,7 +398,7 @@ int ml_source_impl(pf_box* h, const float* prm, const Norm21& nrm, const void* c
                      (const T*)U[0], (const T*)U[1], (const T*)U[2], Fn, nrm);
   PF_LAUNCH_CHECK("src_features_kernel");
   const int64_t nnet = (s.m + 63) / 64;
-  hipLaunchKernelGGL(src_network_kernel, dim3((unsigned)nnet), dim3(256), 0, st, s,
+  hipLaunchKernelGGL(src_network_kernel, dim3((unsigned)nnet), dim3(kNetThreads), 0, st, s,
                      (const float*)Fn, prm, P);
   PF_LAUNCH_CHECK("src_network_kernel");
   hipLaunchKernelGGL(src_divergence_kernel<T>, dim3((unsigned)nb), dim3(kBlock), 0, st, s,
