@@ -267,12 +267,29 @@ struct alignas(2 * sizeof(T)) Pair {
 // (512^3 FP32: KA 674 -> 659, KB 230 -> 217, KC 691 -> 674 us); on the
 // short-chunk grids (128^3 FP64, whose set fits) it was slower (KA 20.6 ->
 // 22.5 us), the next kernel's reads no longer finding the data there.
+// Otherwise, in FP32, the stores write through the XCD's L2 at device
+// scope (relaxed agent-scope stores: global_store ... sc1), so that no dirty
+// lines of the output are left in the eight L2s for the end of the kernel
+// to write back before the next kernel, on any XCD, may read them (128^3
+// FP32: KA 13.6 -> 12.6 us, KC 11.9 -> 11.1 us, the step 191 -> 199
+// Mcell-steps/s).  FP64 keeps plain 16-byte stores: as two 8-byte
+// write-through stores per pair its 128^3 kernels ran half again slower,
+// and as one 16-byte write-through buffer store no faster.
+#ifndef PF_KRYLOV_WT
+#define PF_KRYLOV_WT 1
+#endif
+
 template <bool NT, typename T>
 __device__ __forceinline__ void store2(T* p, const Pair<T>& v) {
   if constexpr (NT) {
     typedef T v2 __attribute__((ext_vector_type(2)));
     const v2 w = {v.a, v.b};
     __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+  } else if constexpr (PF_KRYLOV_WT && sizeof(T) == 4) {
+    uint64_t w;
+    __builtin_memcpy(&w, &v, 8);
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(p), w, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   } else {
     *reinterpret_cast<Pair<T>*>(p) = v;
   }
@@ -282,6 +299,8 @@ template <bool NT, typename T>
 __device__ __forceinline__ void store1(T* p, T v) {
   if constexpr (NT)
     __builtin_nontemporal_store(v, p);
+  else if constexpr (PF_KRYLOV_WT && sizeof(T) == 4)
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   else
     *p = v;
 }
