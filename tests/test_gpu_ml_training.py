@@ -151,16 +151,19 @@ def test_adjoint_rhs_with_ml_closure_against_reference(G):
     assert e < TOL_PARAMS
 
 
-def test_adjoint_window_with_ml_closure_against_oracle(G):
+@pytest.mark.parametrize('adv', ['RK4', 'Euler'])
+def test_adjoint_window_with_ml_closure_against_oracle(G, adv):
     """Two forward steps with checkpoints, then two adjoint steps each on its
-    restored forward state (PyFlow.py:333-367, 548-550), RK4 + BiCGStab at
-    1e-12, the 'source' ML closure in both directions; against the oracle's
-    window with the closure's weights as autograd leaves."""
+    restored forward state (PyFlow.py:333-367, 548-550), BiCGStab at 1e-12,
+    the 'source' ML closure in both directions; against the oracle's window
+    with the closure's weights as autograd leaves.  RK4 runs the adjoint
+    step's closure work from one network forward per step (all-cell cache);
+    Euler evaluates the whole term in its single stage."""
     from pyflow_amd import domain
     shape, L, d, g, m = env(G)
     flat, sd = ml_source_params()
     dt, mu, rho = float(G['dt']), float(G['mu']), float(G['rho'])
-    cfg = types.SimpleNamespace(simDt=dt, equationMode='NS', advancerName='RK4', mu=mu, rho=rho,
+    cfg = types.SimpleNamespace(simDt=dt, equationMode='NS', advancerName=adv, mu=mu, rho=rho,
                                 dtypeTorch=torch.float64, SFSmodel='ML', mlSourceParams=flat,
                                 pSolverMode='bicgstab', min_pressure_residual=1e-12,
                                 max_pressure_iterations=500, adjointTraining=True,
@@ -170,8 +173,8 @@ def test_adjoint_window_with_ml_closure_against_oracle(G):
                       g, m, ['U', 'V', 'W', 'P'], 0.0)
     torch.set_num_threads(1)
     og = O.Grid(*shape, Lx=L[0], Ly=L[1], Lz=L[2])
-    B = O.Box(og, G['u0'], G['v0'], G['w0'], mu=mu, rho=rho, dt=dt, tol=1e-12, maxit=500,
-              sfs=O.MLSource(og, sd))
+    B = O.Box(og, G['u0'], G['v0'], G['w0'], mu=mu, rho=rho, dt=dt, advancer=adv, tol=1e-12,
+              maxit=500, sfs=O.MLSource(og, sd))
     ck = []
     D.checkpoint(0)
     ck.append([f.var.clone() for f in (B.u, B.v, B.w)])
@@ -198,10 +201,10 @@ def test_adjoint_window_with_ml_closure_against_oracle(G):
     for c, st, f in (('u', D.state_u_adj_P, AB.au), ('v', D.state_v_adj_P, AB.av),
                      ('w', D.state_w_adj_P, AB.aw)):
         e = rel_l2(cpu(st.var), f.var.numpy())
-        print('adjoint %s after the window: rel L2 %.2e' % (c, e))
+        print('%s adjoint %s after the window: rel L2 %.2e' % (adv, c, e))
         assert e < 1e-9, (c, e)
     e = rel_l2(cpu(D.sfsmodel.grad), train.packed_grad().numpy())
-    print('window parameter gradient (8 evaluations) rel L2 %.2e' % e)
+    print('%s window parameter gradient rel L2 %.2e' % (adv, e))
     assert e < TOL_PARAMS
 
 
