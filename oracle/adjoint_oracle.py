@@ -10,11 +10,12 @@ Restated from the reference (paths relative to its ``src/``):
   (metric/metric_staggered.py:529-800);
 * ``rhs_adjPredictor.evaluate`` (solver/adjoint.py:94-298): the viscous
   operator of the forward RHS applied to the adjoint field, then the
-  convective transposes in the reference's order.  The closure term of the
-  reference (adjoint.py:241-286) belongs to its 'source' ML closure; for the
-  'tensor' closures of this build the term is the exact transpose of the
-  closure's contribution to the forward RHS, computed here with torch
-  autograd through this module's forward restatement;
+  convective transposes in the reference's order, then the closure term:
+  for the reference's 'source' ML closure (adjoint.py:230-286) its own
+  recipe, autograd through pyflow_oracle.MLSource (source_closure_term);
+  for the 'tensor' closures of this build the exact transpose of the
+  closure's contribution to the forward RHS, computed with torch autograd
+  through this module's forward restatement;
 * ``Domain.adjointStep`` (data/domain.py:409-438): adjoint projection with
   +dt/rho, then the advancer on the adjoint RHS with the forward state held
   at the checkpoint;
@@ -24,8 +25,10 @@ Restated from the reference (paths relative to its ``src/``):
 
 How each part is pinned: golden vectors written by importing the reference
 (tests/golden/make_golden.py: rhs_adjPredictor.evaluate and
-Domain.adjointStep), and the transpose identity against the forward RHS,
-whose restatement is itself pinned bit-exactly (tests/test_oracle_adjoint.py).
+Domain.adjointStep, and the evaluation with the 'source' ML closure: the
+RHS and the parameter gradients its g.backward() leaves), and the transpose
+identity against the forward RHS, whose restatement is itself pinned
+bit-exactly (tests/test_oracle_adjoint.py).
 """
 
 import torch
