@@ -20,7 +20,9 @@ metric ride along: `f32` (the same workload with FP32 storage and FP64
 Krylov reductions at tolerance 1e-8, its error against the FP64 oracle) and
 `serving` (a bounded Poisson stream of mixed-size requests served with
 continuous batching and one after another: throughput, p50/p99 step latency,
-requests with bit-identical results).
+requests with bit-identical results).  `ml_closure` times the reference's
+'source' ML closure at 64^3 FP32 (the network forward, its reverse pass, the
+closure work of one adjoint RK4 step) against the FP32 MFMA peak.
 """
 
 import argparse
@@ -249,6 +251,8 @@ def main():
                     help='skip the FP32-storage leg of the default FP64 run')
     ap.add_argument('--serving-requests', type=int, default=24,
                     help='requests of the serving leg (0 = skip it)')
+    ap.add_argument('--no-ml-closure', action='store_true',
+                    help="skip the leg that times the reference's ML closure at 64^3")
     ap.add_argument('--no-profile', action='store_true')
     ap.add_argument('--profile-period', type=int, default=32,
                     help='time every n-th launch of each Krylov kernel with dispatch events')
@@ -333,6 +337,21 @@ def main():
             'bit_identical_requests': sv['bit_identical_requests'],
             'max_resident_gb': round(b['max_resident_gb'], 2),
             'mean_batch': round(b['mean_batch'], 2)}
+    if not a.no_ml_closure:
+        from pyflow_amd import sfsmodel_ML
+        t = sfsmodel_ML.timings(64, torch.float32, 5)
+        f = t['flop_forward']
+        out['ml_closure'] = {
+            'workload': "the reference's 'source' ML closure (NeuralNetworkModel2, 0.495 MFLOP "
+                        "per padded cell) on a 64^3 FP32 box (68^3 padded cells), random fields",
+            'forward_ms': round(t['forward'], 3),
+            'forward_tflops': round(f / t['forward'] / 1e9, 1),
+            'reverse_ms': round(t['reverse'], 3),
+            'reverse_tflops': round(3 * f / t['reverse'] / 1e9, 1),
+            'adjoint_rk4_step_closure_ms': round(t['adjoint_step'], 3),
+            'four_separate_terms_ms': round(t['four_terms'], 3),
+            'peak_tflops_fp32_mfma': 157.3,
+            'forward_frac': round(f / t['forward'] / 1e9 / 157.3, 4)}
     print(json.dumps(out))
 
 

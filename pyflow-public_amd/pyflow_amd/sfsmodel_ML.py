@@ -215,83 +215,99 @@ class residual_stress:
             self.step_lr *= 0.5
 
 
-def main(argv=None):
-    """Time one closure evaluation on random padded fields, with --reverse
-    its reverse pass (pf_sfs_ml_source_vjp: the forward recomputed, the
-    input- and weight-gradient products, three times the forward's), or with
-    --adjoint-step one adjoint RK4 step with the closure's term (closure
-    kind 2 of pf_adj_rk4_step, BiCGStab-free) beside four separate
-    evaluations of the term:
-    python -m pyflow_amd.sfsmodel_ML --N 64 --dtype f32 --reps 5 [--reverse | --adjoint-step]"""
-    import argparse
+def timings(N=64, dtype=torch.float32, reps=5,
+            which=('forward', 'reverse', 'adjoint_step', 'four_terms')):
+    """Milliseconds per call on random padded fields of an N^3 box: the
+    closure's forward evaluation, its reverse pass (pf_sfs_ml_source_vjp:
+    the forward recomputed, the input- and weight-gradient products, three
+    times the forward's flops), the closure work of one adjoint RK4 step
+    (closure kind 2 of pf_adj_rk4_step: one network forward and four stage
+    reverses, BiCGStab-free) and four separate closure terms; plus the
+    network's flops per forward evaluation."""
     import math
     import types
     from . import _native
+    n = N
+    h = 2 * math.pi / n
+    box = _native.Box(n, n, n, h, h, h, dtype=dtype, device=torch.device('cuda'))
+    d = types.SimpleNamespace(prec=dtype, device=torch.device('cuda'), box=box, nxo_=n + 4,
+                              nyo_=n + 4, nzo_=n + 4)
+    geo = types.SimpleNamespace(type='uniform', dx=h)
+    ml = residual_stress(types.SimpleNamespace(simDt=1e-3, adjointTraining=True), d, geo)
+    gen = torch.Generator(device='cuda').manual_seed(0)
+    shape = (n + 4,) * 3
+    U = [torch.randn(shape, dtype=dtype, device='cuda', generator=gen) for _ in range(3)]
+    Gb = [torch.randn(shape, dtype=dtype, device='cuda', generator=gen) for _ in range(3)]
+    dI = [torch.empty(shape, dtype=torch.float32, device='cuda') for _ in range(3)]
+    A = [0.01 * torch.randn(shape, dtype=dtype, device='cuda', generator=gen) for _ in range(3)]
+    At = [torch.zeros_like(x) for x in A]
+    delta = [torch.zeros_like(x) for x in A]
+
+    def forward():
+        ml.update(*U)
+
+    def reverse():
+        box.sfs_ml_source_vjp(ml.params, *U, Gb, dI, ml.grad)
+
+    def adjoint_step():
+        box.adj_rk4_step(*U, *A, *At, 1e-3, 0.01, 1.0, 2, ml.params, None, delta, ml.grad)
+
+    def four_terms():
+        for _ in range(4):
+            box.adj_closure_term(2, *U, ml.params, *A, None, delta, ml.grad)
+
+    out = {}
+    for name, fn in (('forward', forward), ('reverse', reverse), ('adjoint_step', adjoint_step),
+                     ('four_terms', four_terms)):
+        if name not in which:
+            continue
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = e0.elapsed_time(e1) / reps
+    out['flop_forward'] = 2.0 * (n + 4) ** 3 * sum(o * i for _, o, i in LAYERS)
+    box.close()
+    return out
+
+
+def main(argv=None):
+    """python -m pyflow_amd.sfsmodel_ML --N 64 --dtype f32 --reps 5
+    [--reverse | --adjoint-step | --all]: the closure's forward evaluation
+    (default), its reverse pass, or the closure work of one adjoint RK4 step
+    beside four separate terms (timings()), as ms and TFLOP/s."""
+    import argparse
     ap = argparse.ArgumentParser()
     ap.add_argument('--N', type=int, default=64)
     ap.add_argument('--dtype', choices=('f32', 'f64'), default='f32')
     ap.add_argument('--reps', type=int, default=5)
     ap.add_argument('--reverse', action='store_true')
     ap.add_argument('--adjoint-step', action='store_true')
+    ap.add_argument('--all', action='store_true')
     a = ap.parse_args(argv)
-    if a.adjoint_step:
-        a.reverse = True
-    dt = torch.float32 if a.dtype == 'f32' else torch.float64
-    n = a.N
-    h = 2 * math.pi / n
-    box = _native.Box(n, n, n, h, h, h, dtype=dt, device=torch.device('cuda'))
-    d = types.SimpleNamespace(prec=dt, device=torch.device('cuda'), box=box, nxo_=n + 4,
-                              nyo_=n + 4, nzo_=n + 4)
-    geo = types.SimpleNamespace(type='uniform', dx=h)
-    ml = residual_stress(types.SimpleNamespace(simDt=1e-3, adjointTraining=a.reverse), d, geo)
-    gen = torch.Generator(device='cuda').manual_seed(0)
-    U = [torch.randn((n + 4,) * 3, dtype=dt, device='cuda', generator=gen) for _ in range(3)]
-    if a.adjoint_step:
-        A = [0.01 * torch.randn((n + 4,) * 3, dtype=dt, device='cuda', generator=gen)
-             for _ in range(3)]
-        At = [torch.zeros_like(x) for x in A]
-        delta = [torch.zeros_like(x) for x in A]
-
-        def step():
-            box.adj_rk4_step(*U, *A, *At, 1e-3, 0.01, 1.0, 2, ml.params, None, delta, ml.grad)
-
-        def terms():
-            for _ in range(4):
-                box.adj_closure_term(2, *U, ml.params, *A, None, delta, ml.grad)
-        for name, fn in (('adjoint RK4 step (forward once, stage sums)', step),
-                         ('four separate closure terms', terms)):
-            fn()
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(a.reps):
-                fn()
-            e1.record()
-            torch.cuda.synchronize()
-            print('ML source closure %d^3 %s: %s %.3f ms'
-                  % (n, a.dtype, name, e0.elapsed_time(e1) / a.reps), flush=True)
-        return 0
+    which = ('forward',)
     if a.reverse:
-        Gb = [torch.randn((n + 4,) * 3, dtype=dt, device='cuda', generator=gen) for _ in range(3)]
-        dI = [torch.empty((n + 4,) * 3, dtype=torch.float32, device='cuda') for _ in range(3)]
-
-        def run():
-            box.sfs_ml_source_vjp(ml.params, *U, Gb, dI, ml.grad)
-    else:
-        def run():
-            ml.update(*U)
-    run()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(a.reps):
-        run()
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / a.reps
-    flop = 2.0 * (n + 4) ** 3 * sum(o * i for _, o, i in LAYERS) * (3 if a.reverse else 1)
-    print('ML source closure %s %d^3 %s: %.3f ms per evaluation, %.1f TFLOP/s of the network'
-          % ('reverse' if a.reverse else 'forward', n, a.dtype, ms, flop / ms / 1e9), flush=True)
+        which = ('reverse',)
+    if a.adjoint_step:
+        which = ('adjoint_step', 'four_terms')
+    if a.all:
+        which = ('forward', 'reverse', 'adjoint_step', 'four_terms')
+    t = timings(a.N, torch.float32 if a.dtype == 'f32' else torch.float64, a.reps, which)
+    f = t['flop_forward']
+    head = 'ML source closure %d^3 %s:' % (a.N, a.dtype)
+    if 'forward' in t:
+        print('%s forward %.3f ms per evaluation, %.1f TFLOP/s of the network'
+              % (head, t['forward'], f / t['forward'] / 1e9), flush=True)
+    if 'reverse' in t:
+        print('%s reverse %.3f ms per evaluation, %.1f TFLOP/s of the network'
+              % (head, t['reverse'], 3 * f / t['reverse'] / 1e9), flush=True)
+    if 'adjoint_step' in t:
+        print('%s adjoint RK4 step (forward once, stage sums) %.3f ms; four separate closure '
+              'terms %.3f ms' % (head, t['adjoint_step'], t['four_terms']), flush=True)
     return 0
 
 
