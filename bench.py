@@ -50,6 +50,9 @@ KRYLOV_TRAFFIC_BYTES = {
     (128, 'f64', 'KA'): int((38403.6 / 0.50003 + 32099.1) * 1024),
     (128, 'f64', 'KB'): int((18283.6 / 0.50003 + 15.7) * 1024),
     (128, 'f64', 'KC'): int((44520.7 / 0.50003 + 32205.1) * 1024),
+    (128, 'f32', 'KA'): int((17221.6 / 0.50003 + 14685.6) * 1024),
+    (128, 'f32', 'KB'): int((8967.5 / 0.50003 + 28.7) * 1024),
+    (128, 'f32', 'KC'): int((20348.6 / 0.50003 + 14726.9) * 1024),
 }
 KERNEL_NAMES = {'KA': 'KA: p update, v = A p, <v, r_hat> (bicg_tile2_kernel mode 0)',
                 'KB': 'KB: s = r - alpha v, t = A s, <s,t>, <t,t> (bicg_tile2_kernel mode 1)',
