@@ -31,7 +31,8 @@
 // chunk of input columns at a time for its eight waves, the next chunk's
 // loads in flight while the current one is consumed; a 128-bit LDS read
 // feeds four MFMAs.  Measured at 64^3: 2.96 ms per evaluation with one wave
-// per 16 cells (498 registers, one wave per SIMD), 2.50 ms with the halves.
+// per 16 cells (498 registers, one wave per SIMD), 2.50 ms with the halves,
+// 2.34 ms with the two halves of a cell group on one SIMD.
 // Summation order differs from the reference's CPU GEMM, so results agree to
 // FP32 rounding, not bit for bit.
 #include <hip/hip_runtime.h>
@@ -329,9 +330,13 @@ __global__ void __launch_bounds__(kNetThreads) src_network_kernel(SrcGeom s,
   float* buf0 = chunks;
   float* buf1 = chunks + kStageFloats;
   const int lane = threadIdx.x & 63;
-  // wave-uniform (kept in a scalar register, so the halves branch, not mask)
+  // wave-uniform (kept in a scalar register, so the halves branch, not mask).
+  // The waves of a block are dealt to the CU's four SIMDs in turn, so waves w
+  // and w + 4 share a SIMD: making them the two halves of one cell group puts
+  // 7 + 6 tiles of MFMA work on every SIMD (with half = w & 1 two SIMDs
+  // would carry 7 + 7 and the other two 6 + 6).
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int grp = wave >> 1, half = wave & 1;
+  const int grp = wave & 3, half = wave >> 2;
   const int col = lane & 15;
   const int64_t cell = ((int64_t)blockIdx.x * 4 + grp) * 16 + col;
   const bool valid = cell < s.m;
