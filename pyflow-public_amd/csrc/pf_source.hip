@@ -32,7 +32,9 @@
 // loads in flight while the current one is consumed; a 128-bit LDS read
 // feeds four MFMAs.  Measured at 64^3: 2.96 ms per evaluation with one wave
 // per 16 cells (498 registers, one wave per SIMD), 2.50 ms with the halves,
-// 2.34 ms with the two halves of a cell group on one SIMD.
+// 2.34 ms with the two halves of a cell group on one SIMD.  Packing the
+// first layer's 273 columns into 12 dense chunks instead of 13 padded ones
+// (two neighbour indices per chunk) spilled 114 VGPRs: 2.64 ms.
 // Summation order differs from the reference's CPU GEMM, so results agree to
 // FP32 rounding, not bit for bit.
 #include <hip/hip_runtime.h>
