@@ -368,106 +368,149 @@ __global__ void __launch_bounds__(256, 2) mm_kernel(MmBatch bt) {
   }
 
   // epilogue: lane l holds rows 4*(l/16)+r of each 16-row tile, column l%16.
-  // The operands it reads (bias, E1, E2) are loaded unconditionally, one
-  // column tile at a time, so their loads overlap: rows < tiles_m*128 and
-  // columns < 208 stay inside the chunk arrays for every product that has
-  // them; only the stores are masked.
+  // The operands it reads (bias, E1, E2, S1, S2) are loaded unconditionally,
+  // one column tile at a time and one tile ahead: the loads of tile j + 1 are
+  // in flight while tile j's values are formed and stored (rows <
+  // tiles_m*128 and columns < 208 stay inside the chunk arrays for every
+  // product that has them; only the stores are masked).
+  constexpr bool HE1 = EPI == kEpiTanhGate || EPI == kEpiRevPair || EPI == kEpiRevTanh;
+  constexpr bool HE2 = EPI == kEpiRevPair;
+  constexpr bool HS1 = SUMS, HS2 = SUMS && EPI == kEpiRevPair;
   const int rbase = m0 + 32 * w + 4 * lg;
-  // kernel-argument arrays indexed by a per-lane group are read once here
-  // (an indexed read inside the loop would be a memory round trip each)
+  // kernel-argument fields are read once here (an indexed read of the batch
+  // inside the loop would be a memory round trip each)
   const float* bias0 = op.bias[0];
   const float* bias1 = op.bias[1];
   const float* bias2 = op.bias[2];
   const int64_t wo0 = op.woff[0], wo1 = op.woff[1], wo2 = op.woff[2];
   const int64_t bo0 = op.boff[0], bo1 = op.boff[1], bo2 = op.boff[2];
   float* slab = op.part + (int64_t)z * kSrcParams;
+  const float* __restrict__ E1p = op.E1;
+  const float* __restrict__ E2p = op.E2;
+  float* __restrict__ Cp = op.C;
+  float* __restrict__ C2p = op.C2;
+  float* __restrict__ S1p = op.S1;
+  float* __restrict__ S2p = op.S2;
+  const int64_t lde1 = op.lde1, lde2 = op.lde2, ldc = op.ldc, ldc2 = op.ldc2;
+  const int64_t lds1 = op.lds1, lds2 = op.lds2;
+  const int Nn = op.N, sacc = op.sacc;
+  if constexpr (HE1 || HS1) {
+    // prefetch unit u: UI 16-row tiles of column tile u*UI/2 (one where four
+    // operands are read, so that two units' operands fit in registers).
+    // Unit u's operands sit in buffer u & 1; the loop takes units in pairs,
+    // so every buffer is named statically and no copy waits on a load.
+    constexpr int UI = HS2 ? 1 : 2, NU = TN * 2 / UI;
+    struct EOps {
+      float e1[UI][4], e2[UI][4], s1[UI][4], s2[UI][4], bb;
+    };
+    EOps ob0, ob1;
+    auto eload = [&](int u, EOps& o) {
+      const int j = u * UI / 2, i0 = u * UI - 2 * j;
+      const int col = n0 + 16 * j + lr;
+      o.bb = 0.f;
+      if (EPI == kEpiTanhGate) {
+        const float bv = bias0[col < kH ? col : kH - 1];
+        o.bb = col < kH ? bv : 0.f;
+      }
 #pragma unroll
-  for (int j = 0; j < TN; ++j) {
-    const int col = n0 + 16 * j + lr;
-    const bool cok = col < op.N;
-    float e1[2][4], e2[2][4], bb = 0.f;
-    if (EPI == kEpiTanhCat) {
-      const int g = col / kHp, o = col - g * kHp;
-      const float* bp = g == 0 ? bias0 : (g == 1 ? bias1 : bias2);
-      const float bv = bp[o < kH ? o : kH - 1];
-      bb = o < kH ? bv : 0.f;
-    } else if (EPI == kEpiTanhGate) {
-      const float bv = bias0[col < kH ? col : kH - 1];
-      bb = col < kH ? bv : 0.f;
+      for (int ii = 0; ii < UI; ++ii)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = rbase + 16 * (i0 + ii) + r;
+          if (HE1) o.e1[ii][r] = E1p[row * lde1 + col];
+          if (HE2) o.e2[ii][r] = E2p[row * lde2 + col];
+          if (HS1) o.s1[ii][r] = S1p[row * lds1 + col];
+          if (HS2) o.s2[ii][r] = S2p[row * lds2 + col];
+        }
+    };
+    auto unit = [&](int u, const EOps& o) {
+      const int j = u * UI / 2, i0 = u * UI - 2 * j;
+      const int col = n0 + 16 * j + lr;
+      const bool cok = col < Nn;
+      // values first, without branches; then the masked stores
+      float v1[UI][4], v2[UI][4];
+#pragma unroll
+      for (int ii = 0; ii < UI; ++ii)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float d = acc[i0 + ii][j][r];
+          if (EPI == kEpiTanhGate) {
+            v1[ii][r] = tanhf(d + o.bb);
+            v2[ii][r] = v1[ii][r] * o.e1[ii][r];
+          } else if (EPI == kEpiRevPair) {
+            const float a1 = o.e1[ii][r], a2 = o.e2[ii][r];
+            v1[ii][r] = (d * a1) * (1.f - a2 * a2);
+            v2[ii][r] = (d * a2) * (1.f - a1 * a1);
+          } else {  // kEpiRevTanh
+            const float a1 = o.e1[ii][r];
+            v1[ii][r] = d * (1.f - a1 * a1);
+          }
+        }
+#pragma unroll
+      for (int ii = 0; ii < UI; ++ii)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rbase + 16 * (i0 + ii) + r;
+          if (row >= Mv || !cok) continue;
+          Cp[(int64_t)row * ldc + col] = v1[ii][r];
+          if (EPI == kEpiTanhGate || EPI == kEpiRevPair)
+            C2p[(int64_t)row * ldc2 + col] = v2[ii][r];
+          if (HS1) S1p[(int64_t)row * lds1 + col] = sacc ? o.s1[ii][r] + v1[ii][r] : v1[ii][r];
+          if (HS2) S2p[(int64_t)row * lds2 + col] = sacc ? o.s2[ii][r] + v2[ii][r] : v2[ii][r];
+        }
+    };
+    eload(0, ob0);
+#pragma unroll
+    for (int pu = 0; pu < (NU + 1) / 2; ++pu) {
+      const int u = 2 * pu;
+      if (u + 1 < NU) eload(u + 1, ob1);
+      unit(u, ob0);
+      if (u + 1 < NU) {
+        if (u + 2 < NU) eload(u + 2, ob0);
+        unit(u + 1, ob1);
+      }
     }
-    if (EPI == kEpiTanhGate || EPI == kEpiRevPair || EPI == kEpiRevTanh) {
+  } else {
+    // kEpiStore, kEpiTanhCat, kEpiPartial: no operands beyond the bias
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          e1[i][r] = op.E1[(int64_t)(rbase + 16 * i + r) * op.lde1 + col];
-    }
-    if (EPI == kEpiRevPair) {
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          e2[i][r] = op.E2[(int64_t)(rbase + 16 * i + r) * op.lde2 + col];
-    }
-    float s1[2][4], s2[2][4];
-    if (SUMS) {
+    for (int j = 0; j < TN; ++j) {
+      const int col = n0 + 16 * j + lr;
+      const bool cok = col < Nn;
+      float bb = 0.f;
+      if (EPI == kEpiTanhCat) {
+        const int g = col / kHp, o = col - g * kHp;
+        const float* bp = g == 0 ? bias0 : (g == 1 ? bias1 : bias2);
+        const float bv = bp[o < kH ? o : kH - 1];
+        bb = o < kH ? bv : 0.f;
+      }
+      float v1[2][4];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          s1[i][r] = op.S1[(int64_t)(rbase + 16 * i + r) * op.lds1 + col];
-          if (EPI == kEpiRevPair) s2[i][r] = op.S2[(int64_t)(rbase + 16 * i + r) * op.lds2 + col];
+          const float d = acc[i][j][r];
+          v1[i][r] = EPI == kEpiTanhCat ? tanhf(d + bb) : d;
         }
-    }
-    // values first, without branches; then the masked stores
-    float v1[2][4], v2[2][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float d = acc[i][j][r];
-        if (EPI == kEpiTanhCat) {
-          v1[i][r] = tanhf(d + bb);
-        } else if (EPI == kEpiTanhGate) {
-          v1[i][r] = tanhf(d + bb);
-          v2[i][r] = v1[i][r] * e1[i][r];
-        } else if (EPI == kEpiRevPair) {
-          const float a1 = e1[i][r], a2 = e2[i][r];
-          v1[i][r] = (d * a1) * (1.f - a2 * a2);
-          v2[i][r] = (d * a2) * (1.f - a1 * a1);
-        } else if (EPI == kEpiRevTanh) {
-          const float a1 = e1[i][r];
-          v1[i][r] = d * (1.f - a1 * a1);
-        } else {
-          v1[i][r] = d;
-        }
-      }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rbase + 16 * i + r;
-        if (row >= op.M || !cok) continue;
-        if (EPI == kEpiPartial) {
-          const int g = row / op.rg, o = row - g * op.rg;
-          if (o >= op.nout) continue;
-          const int64_t wo = g == 0 ? wo0 : (g == 1 ? wo1 : wo2);
-          const int64_t bo = g == 0 ? bo0 : (g == 1 ? bo1 : bo2);
-          if (col < op.nin)
-            slab[wo + (int64_t)o * op.nin + col] = v1[i][r];
-          else if (col == op.nin)
-            slab[bo + o] = v1[i][r];
-        } else {
-          op.C[(int64_t)row * op.ldc + col] = v1[i][r];
-          if (EPI == kEpiTanhGate || EPI == kEpiRevPair)
-            op.C2[(int64_t)row * op.ldc2 + col] = v2[i][r];
-          if (SUMS) {
-            op.S1[(int64_t)row * op.lds1 + col] = op.sacc ? s1[i][r] + v1[i][r] : v1[i][r];
-            if (EPI == kEpiRevPair)
-              op.S2[(int64_t)row * op.lds2 + col] = op.sacc ? s2[i][r] + v2[i][r] : v2[i][r];
+        for (int r = 0; r < 4; ++r) {
+          const int row = rbase + 16 * i + r;
+          if (row >= Mv || !cok) continue;
+          if (EPI == kEpiPartial) {
+            const int g = row / op.rg, o = row - g * op.rg;
+            if (o >= op.nout) continue;
+            const int64_t wo = g == 0 ? wo0 : (g == 1 ? wo1 : wo2);
+            const int64_t bo = g == 0 ? bo0 : (g == 1 ? bo1 : bo2);
+            if (col < op.nin)
+              slab[wo + (int64_t)o * op.nin + col] = v1[i][r];
+            else if (col == op.nin)
+              slab[bo + o] = v1[i][r];
+          } else {
+            Cp[(int64_t)row * ldc + col] = v1[i][r];
           }
         }
-      }
+    }
   }
 }
 

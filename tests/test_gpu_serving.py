@@ -5,7 +5,8 @@
 * A request's result is the same bits alone and in a mix that it joins
   while other requests are already running.
 * Admission holds the resident HBM footprint under the cap: with a cap of
-  two slots, later arrivals wait and are admitted as earlier ones finish.
+  two slots, later arrivals wait and are admitted as earlier ones finish;
+  the results kept from reused slots are those of a server with room for all.
 """
 
 import numpy as np
@@ -74,6 +75,27 @@ def test_admission_holds_the_hbm_cap():
     # the third request is admitted only after one of the first two finished
     assert order[2].admitted >= min(order[0].finished, order[1].finished)
     assert all(r.done_steps == r.steps for r in reqs)
+
+
+def test_reused_slots_keep_each_result():
+    """With room for two boxes, five requests of one size pass through two
+    slots: a finished request's result is copied out while the loader may
+    already write the next request's initial fields into the same slot.
+    Every kept result equals the one from a server with room for all five."""
+    from pyflow_amd.serving import BatchServer, SlotPool
+    pool = SlotPool(torch.float32, torch.device('cuda'))
+    one = pool.take(32)
+    cap = 2 * one.bytes + 1
+    pool.give(one)
+    specs = [(32, 3 + i % 3, 51 + i, 4.0, 0.01, 0.0) for i in range(5)]
+    tight, roomy = _reqs(specs), _reqs(specs)
+    srv = BatchServer(capacity=8, hbm_cap=cap)
+    srv.serve(tight, keep_results=True, realtime=False)
+    assert max(srv.batch_sizes) == 2
+    BatchServer(capacity=8).serve(roomy, keep_results=True, realtime=False)
+    for a, b in zip(tight, roomy):
+        for x, y in zip(a.result, b.result):
+            assert torch.equal(x, y), a.rid
 
 
 def test_cache_groups_do_not_change_results():
