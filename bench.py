@@ -58,7 +58,12 @@ KERNEL_NAMES = {'KA': 'KA: p update, v = A p, <v, r_hat> (bicg_tile2_kernel mode
                 'KB': 'KB: s = r - alpha v, t = A s, <s,t>, <t,t> (bicg_tile2_kernel mode 1)',
                 'KC': 'KC: s, t = A s again, x and r updates, <r,r_hat>, max|r| '
                       '(bicg_tile2_kernel mode 2)'}
+KERNEL_NAMES['KP'] = ('KP: the whole BiCGStab solve as one persistent cooperative kernel, '
+                      'vectors on chip (bicg_persist_kernel)')
 KERNEL_WORDS = {'KA': 6, 'KB': 2, 'KC': 7}   # algorithmic vector passes per launch
+# HBM-side bytes per launch of the persistent solve (rocprofv3 --pmc FETCH_SIZE,
+# WRITE_SIZE, scaled as above); filled in from the profile of this workload.
+PERSIST_TRAFFIC_BYTES = {}
 
 
 def make_config(N, dtype, nu, tol, maxit, seed, solver='bicgstab'):
@@ -73,7 +78,8 @@ def make_config(N, dtype, nu, tol, maxit, seed, solver='bicgstab'):
         writeMonitor=False)
 
 
-def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True, solver='bicgstab'):
+def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True, solver='bicgstab',
+            persistent=True):
     import torch
     from pyflow_amd import PyFlow
     from pyflow_amd.initial_conditions import passot_pouquet, cfl_dt
@@ -82,10 +88,11 @@ def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True, solver=
     cfg = make_config(N, dtype, nu, tol, maxit, seed, solver)
     cfg.simDt = dt
     comms, decomp, geometry, metric, D, t0 = PyFlow.setup(cfg)
+    box = decomp.box
+    box.set_persistent(persistent)
     for _ in range(warmup):
         D.forwardStep(dt)
         D.stats()
-    box = decomp.box
     if profile:
         box.profile_enable(profile)
     torch.cuda.synchronize()
@@ -104,16 +111,17 @@ def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True, solver=
         box.profile_enable(False)
     st = D.stats()
     return dict(wall=wall, its=its, prof=prof, dt=dt, stats=st, u0=(u0, v0, w0),
-                res=D.max_resP, status={k: status.count(k) for k in set(status)})
+                res=D.max_resP, status={k: status.count(k) for k in set(status)},
+                persistent=box.last_solve_persistent)
 
 
-def measure(N, dt_name, nu, tol, maxit, steps, warmup, seed, period, solver):
+def measure(N, dt_name, nu, tol, maxit, steps, warmup, seed, period, solver, persistent=True):
     """Timed steps of one workload: throughput, per-kernel bandwidth and the
     roofline of the Krylov kernel with the largest measured time."""
     import torch
     dtype = torch.float64 if dt_name == 'f64' else torch.float32
     w = 8 if dt_name == 'f64' else 4
-    r = gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, period, solver)
+    r = gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, period, solver, persistent)
     ncell = N ** 3
     value = ncell * steps / r['wall'] / 1e6
     kbar = sum(r['its']) / len(r['its'])
@@ -133,7 +141,27 @@ def measure(N, dt_name, nu, tol, maxit, steps, warmup, seed, period, solver):
     out['infinity_cache'] = {'krylov_set_bytes': krylov_set,
                              'cache_bytes': INFINITY_CACHE_BYTES,
                              'set_fits': krylov_set <= INFINITY_CACHE_BYTES}
-    if r['prof']:
+    out['krylov_path'] = 'persistent' if r['persistent'] else 'launch per reduction'
+    if r['prof'] and r['prof']['KP'][1]:
+        # the whole solve is one cooperative launch (bicg_persist_kernel): its
+        # algorithmic bytes are those of the iterations it ran, 15 vector
+        # passes each (what KA, KB and KC move), although the vectors stay on
+        # chip and HBM sees only the tile faces
+        ms, n, itp = r['prof']['KP']
+        avg = ms / n / 1e3
+        nbytes = 15 * w * ncell * itp / n
+        gbs = nbytes / avg / 1e9
+        out['kernels'] = {'KP': {'avg_us': round(avg * 1e6, 1), 'timed_launches': n,
+                                 'iterations_per_launch': round(itp / n, 2),
+                                 'bytes_per_launch': int(nbytes), 'GB/s': round(gbs, 1),
+                                 'us_per_iteration': round(avg * 1e6 * n / itp, 3)}}
+        out['roofline'] = {'kernel': KERNEL_NAMES['KP'], 'bound': 'hbm', 'achieved': round(gbs, 1),
+                           'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                           'frac': round(gbs / HBM_PEAK_GBS, 4),
+                           'traffic': PERSIST_TRAFFIC_BYTES.get((N, dt_name)),
+                           'bytes_per_launch': int(nbytes),
+                           'chosen_by': 'the solve is one launch: the largest kernel by time'}
+    elif r['prof']:
         per = {}
         for name in ('KA', 'KB', 'KC'):
             tot, n = r['prof'][name]
@@ -257,6 +285,9 @@ def main():
     ap.add_argument('--no-ml-closure', action='store_true',
                     help="skip the leg that times the reference's ML closure at 64^3")
     ap.add_argument('--no-profile', action='store_true')
+    ap.add_argument('--no-launch-path', action='store_true',
+                    help='skip the leg that runs the same workload with the launch-per-'
+                         'reduction Krylov kernels instead of the persistent solve')
     ap.add_argument('--profile-period', type=int, default=32,
                     help='time every n-th launch of each Krylov kernel with dispatch events')
     a = ap.parse_args()
@@ -292,6 +323,16 @@ def main():
               'kernels', 'roofline', 'infinity_cache'):
         if k in leg:
             out[k] = leg[k]
+    if 'krylov_path' in leg:
+        out['config']['krylov_path'] = leg['krylov_path']
+    if not a.no_launch_path:
+        # the same workload with the launch-per-reduction kernels (KA, KB, KC)
+        lp = measure(a.N, a.dtype, a.nu, tol, a.maxit, a.steps, a.warmup, a.seed,
+                     0 if a.no_profile else a.profile_period, a.solver, persistent=False)
+        lp.pop('_run')
+        out['launch_path'] = {k: lp[k] for k in ('value', 'ms_per_step', 'bicgstab_iters_mean',
+                                                 'step_roofline_frac', 'kernels', 'roofline')
+                              if k in lp}
     cb = None
     if not a.no_cpu:
         u0, v0, w0 = r['u0']

@@ -106,6 +106,19 @@ int64_t pf_box_device_bytes(const pf_box* box);
  * geometry in PyFlow.run (core/PyFlow.py:81-104), so the box learns dx, dy,
  * dz from geometry.uniform. */
 int pf_box_set_spacing(pf_box* box, double dx, double dy, double dz);
+/* Whether the box's BiCGStab solves (pf_bicgstab_solve, pf_vel_corr, the
+ * refinement's inner solves) may run as one persistent cooperative kernel that
+ * keeps the Krylov vectors on chip (default on).  Same iterates and bits as
+ * the launch-per-reduction kernels, which run when it is off, when the box's
+ * tile grid is not one the persistent kernel handles, and when the GPU cannot
+ * hold all of its blocks at once.
+ * Returns 1 if the box has the persistent kernel's buffers, 0 if not, or an
+ * error.  No counterpart in the reference (a tuning switch of
+ * solver_bicgstab.solve, solver/pressure.py:196-297). */
+int pf_box_set_persistent(pf_box* box, int on);
+/* 1 if the box's last BiCGStab solve ran as the persistent kernel, 0 if as
+ * launches per reduction (or an error).  No counterpart in the reference. */
+int pf_solve_last_persistent(const pf_box* box);
 
 /*
  * Periodic overlap fill of a padded field.
@@ -597,10 +610,18 @@ int pf_batch_set_group_bytes(pf_batch* batch, int64_t bytes);
  * updates + <r,r_hat>, max|r|) carries a start/stop hipEvent pair recorded by
  * the dispatch itself; pf_profile_read returns the accumulated milliseconds
  * and the number of timed launches that did work.  period = 0 turns it off.
+ * Solves that run as the persistent kernel have no per-kernel launches; see
+ * pf_profile_read_persistent.
  * Enabling resets the accumulators.  Creates events on first use.
  */
 int pf_profile_enable(pf_box* box, int period);
 int pf_profile_read(pf_box* box, int kernel, double* total_ms, int64_t* launches);
+/* The same for solves that ran as the persistent kernel (pf_box_set_persistent):
+ * while profiling is on, each such launch carries a start/stop event pair;
+ * returns their accumulated milliseconds, the number of launches and the
+ * BiCGStab iterations they performed. */
+int pf_profile_read_persistent(pf_box* box, double* total_ms, int64_t* launches,
+                               int64_t* iterations);
 
 /*
  * Residual history of the BiCGStab solves of one box: the value the

@@ -88,6 +88,11 @@ struct pf_box {
   int64_t srct_budget;       // bytes the all-cell cache may take (-1: default)
   int chunk;                 // iterations per launch batch between host checks
   int tile_rows;             // block rows of the tiled Krylov kernels: 256*tile_rows threads
+  // persistent BiCGStab (pf_persist.hip): face, slot and flag buffers, when
+  // the box's tile grid has a shape that kernel handles (else NULL)
+  void* pers;
+  int persist_on;            // 1: solves take the persistent kernel when it fits
+  int last_persist;          // the last BiCGStab solve ran as the persistent kernel
   int64_t bytes;             // device bytes owned by the handle
   // optional per-launch timing of the Krylov kernels (measurement only)
   int prof_on;               // 0 = off, else time every prof_on-th launch
@@ -96,6 +101,10 @@ struct pf_box {
   hipEvent_t* prof_ev;       // [3][prof_cap][2]
   double prof_ms[3];         // accumulated milliseconds of KA, KB, KC
   int64_t prof_n[3];         // timed launches that did work
+  hipEvent_t prof_pev[2];    // start/stop of the persistent solve's launch
+  double prof_pms;           // accumulated milliseconds of persistent solves
+  int64_t prof_pn;           // timed persistent solves
+  int64_t prof_pit;          // BiCGStab iterations inside them
 };
 
 namespace pf {

@@ -220,4 +220,17 @@ enum { kBatchInit = 0, kBatchKA = 1, kBatchKB = 2, kBatchKC = 3, kBatchScatter =
        kBatchProject = 5, kBatchStats = 6 };
 enum { kBatchCtlInit = 0, kBatchCtlFlags = 1, kBatchFinalize = 2 };
 
+// Persistent BiCGStab (pf_persist.hip).  persist_bytes: size of the buffers
+// box h needs for it (0: its tile grid is not a shape the kernel handles).
+// persist_solve: after the init kernel, run the whole solve as one
+// cooperative launch; *used = false when it was not launched (the caller
+// then runs the launch-per-reduction path); ev, if not NULL, is an event
+// pair recorded around the launch.  persist_aborted: whether the
+// last persistent solve gave up waiting for a block.
+size_t persist_bytes(const pf_box* h);
+template <typename T>
+int persist_solve(pf_box* h, hipStream_t s, const TileGrid& tg, const Lap<T>& L, const T* r0,
+                  const T* rh, T* x, int npart_init, bool* used, hipEvent_t* ev);
+int persist_aborted(pf_box* h, hipStream_t s, bool* aborted);
+
 }  // namespace pf

@@ -1316,6 +1316,36 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
     launch_tile<T, kModeA>(tg, s, e0, e1, g, nbA, h->ctl, pC, pA, Rr[j & 1], rh, P[old], Vv[old],
                            P[nw], Vv[nw], L);
   };
+  {
+    // the whole solve as one cooperative launch when the box's vectors fit
+    // on chip (pf_persist.hip): same iterates, same bits
+    bool used = false;
+    hipEvent_t* pe = h->prof_on ? h->prof_pev : nullptr;
+    if (int rc = persist_solve<T>(h, s, tg, L, Rr[0], rh, x, nb, &used, pe)) return rc;
+    h->last_persist = used ? 1 : 0;
+    if (used) {
+      PF_HIP(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
+      bool aborted = false;
+      if (int rc = persist_aborted(h, s, &aborted)) return rc;   // synchronises s
+      if (aborted)
+        return fail(PF_EHIP, "bicgstab: a block of the persistent solve stopped waiting for the "
+                             "others (grid sync timed out)");
+      const KrylovCtl c = *h->ctl_host;
+      if (pe) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe[0], pe[1]) == hipSuccess) {
+          h->prof_pms += ms;
+          h->prof_pn += 1;
+          h->prof_pit += c.iter;
+        }
+      }
+      h->last_iters = c.iter;
+      if (out_res) *out_res = c.max_res;
+      if (out_iters) *out_iters = c.iter;
+      return c.status;
+    }
+  }
+  h->last_persist = 0;
   launch(0, 0, [&](hipEvent_t e0, hipEvent_t e1) { ka(0, 0, 1, e0, e1); });
   PF_LAUNCH_CHECK("bicg_tile_kernel");
   const int cap = maxit < 0 ? 0 : maxit;
@@ -1812,6 +1842,8 @@ int pf_profile_enable(pf_box* h, int on) {
     h->prof_ev = new hipEvent_t[(size_t)3 * cap * 2];
     for (int q = 0; q < 3 * cap * 2; ++q) PF_HIP(hipEventCreate(&h->prof_ev[q]));
     h->prof_cap = cap;
+    PF_HIP(hipEventCreate(&h->prof_pev[0]));
+    PF_HIP(hipEventCreate(&h->prof_pev[1]));
   }
   h->prof_on = on > 0 ? on : 0;
   for (int k = 0; k < 3; ++k) h->prof_seq[k] = 0;
@@ -1819,6 +1851,17 @@ int pf_profile_enable(pf_box* h, int on) {
     h->prof_ms[k] = 0.0;
     h->prof_n[k] = 0;
   }
+  h->prof_pms = 0.0;
+  h->prof_pn = 0;
+  h->prof_pit = 0;
+  return 0;
+}
+int pf_profile_read_persistent(pf_box* h, double* total_ms, int64_t* launches,
+                               int64_t* iterations) {
+  if (int rc = check_box(h)) return rc;
+  if (total_ms) *total_ms = h->prof_pms;
+  if (launches) *launches = h->prof_pn;
+  if (iterations) *iterations = h->prof_pit;
   return 0;
 }
 

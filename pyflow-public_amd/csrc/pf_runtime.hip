@@ -122,6 +122,11 @@ int pf_box_create(pf_box** out, int nx, int ny, int nz, double dx, double dy, do
   if (!rc) rc = alloc((void**)&h->ctl, sizeof(KrylovCtl));
   if (!rc) rc = alloc((void**)&h->stats_dev, sizeof(double) * 8);
   if (!rc) {
+    const size_t pb = persist_bytes(h);
+    if (pb) rc = alloc(&h->pers, pb);
+    h->persist_on = 1;
+  }
+  if (!rc) {
     hipError_t e = hipHostMalloc((void**)&h->ctl_host, sizeof(KrylovCtl), hipHostMallocDefault);
     if (e != hipSuccess) rc = check_hip(e, "hipHostMalloc");
   }
@@ -142,12 +147,14 @@ int pf_box_destroy(pf_box* h) {
   void* dev[] = {h->x, h->r, h->rh, h->p[0], h->p[1], h->v[0], h->v[1], h->s, h->t,
                  h->acc[0], h->acc[1], h->acc[2], h->ypong[0], h->ypong[1], h->ypong[2],
                  h->part, h->ctl, h->stats_dev, h->hist, h->mp_x, h->mp_b, h->mp_hist,
-                 h->src_ws, h->srct_ws};
+                 h->src_ws, h->srct_ws, h->pers};
   for (void* p : dev)
     if (p) hipFree(p);
   if (h->prof_ev) {
     for (int q = 0; q < 3 * h->prof_cap * 2; ++q) hipEventDestroy(h->prof_ev[q]);
     delete[] h->prof_ev;
+    hipEventDestroy(h->prof_pev[0]);
+    hipEventDestroy(h->prof_pev[1]);
   }
   if (h->ctl_host) hipHostFree(h->ctl_host);
   if (h->stats_host) hipHostFree(h->stats_host);
@@ -156,6 +163,17 @@ int pf_box_destroy(pf_box* h) {
 }
 
 int64_t pf_box_device_bytes(const pf_box* h) { return h ? h->bytes : 0; }
+
+int pf_box_set_persistent(pf_box* h, int on) {
+  if (int rc = check_box(h)) return rc;
+  h->persist_on = on ? 1 : 0;
+  return h->pers ? 1 : 0;
+}
+
+int pf_solve_last_persistent(const pf_box* h) {
+  if (int rc = check_box(h)) return rc;
+  return h->last_persist;
+}
 
 int pf_box_set_spacing(pf_box* h, double dx, double dy, double dz) {
   if (int rc = check_box(h)) return rc;

@@ -13,7 +13,7 @@ LIB_DIR = os.path.join(HERE, '_lib')
 LIB_NAME = 'libpyflow_hip.so'
 LIB_PATH = os.path.join(LIB_DIR, LIB_NAME)
 
-SOURCES = ['pf_runtime.hip', 'pf_step.hip', 'pf_krylov.hip', 'pf_ops.hip', 'pf_closure.hip',
+SOURCES = ['pf_runtime.hip', 'pf_step.hip', 'pf_krylov.hip', 'pf_persist.hip', 'pf_ops.hip', 'pf_closure.hip',
            'pf_adjoint.hip', 'pf_batch.hip', 'pf_source.hip', 'pf_source_train.hip']
 
 ARCH = os.environ.get('PYFLOW_HIP_ARCH', 'gfx950')

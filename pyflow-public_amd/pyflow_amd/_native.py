@@ -35,6 +35,8 @@ SIGNATURES = [
     ('pf_box_destroy', _I, [_P]),
     ('pf_box_device_bytes', ctypes.c_int64, [_P]),
     ('pf_box_set_spacing', _I, [_P, _D, _D, _D]),
+    ('pf_box_set_persistent', _I, [_P, _I]),
+    ('pf_solve_last_persistent', _I, [_P]),
     ('pf_fill_halo', _I, [_P, _P, _P]),
     ('pf_rk4_step', _I, [_P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _PP, _P]),
     ('pf_euler_step', _I, [_P, _P, _P, _P, _D, _D, _D, _PP, _P]),
@@ -80,6 +82,8 @@ SIGNATURES = [
     ('pf_batch_set_group_bytes', _I, [_P, ctypes.c_int64]),
     ('pf_profile_enable', _I, [_P, _I]),
     ('pf_profile_read', _I, [_P, _I, _PD, ctypes.POINTER(ctypes.c_int64)]),
+    ('pf_profile_read_persistent', _I, [_P, _PD, ctypes.POINTER(ctypes.c_int64),
+                                        ctypes.POINTER(ctypes.c_int64)]),
     ('pf_residual_history_enable', _I, [_P, _I]),
     ('pf_rk4_step_visc', _I, [_P, _P, _P, _P, _P, _P, _P, _D, _P, _D, _P, _P]),
     ('pf_euler_step_visc', _I, [_P, _P, _P, _P, _D, _P, _D, _P, _P]),
@@ -179,6 +183,18 @@ class Box:
         check(lib().pf_box_set_spacing(self.h, float(dx), float(dy), float(dz)),
               'pf_box_set_spacing')
         self.spacing = (float(dx), float(dy), float(dz))
+
+    def set_persistent(self, on):
+        """Allow (or forbid) the persistent on-chip BiCGStab kernel for this
+        box's solves; returns whether the box has that kernel's buffers."""
+        rc = lib().pf_box_set_persistent(self.h, 1 if on else 0)
+        check(rc, 'pf_box_set_persistent')
+        return rc == 1
+
+    @property
+    def last_solve_persistent(self):
+        """Whether the last BiCGStab solve ran as the persistent kernel."""
+        return check(lib().pf_solve_last_persistent(self.h), 'pf_solve_last_persistent') == 1
 
     @property
     def device_bytes(self):
@@ -476,13 +492,20 @@ class Box:
         check(lib().pf_profile_enable(self.h, int(period)), 'pf_profile_enable')
 
     def profile_read(self):
-        """{'KA': (ms_total, launches), 'KB': ..., 'KC': ...}"""
+        """{'KA': (ms_total, launches), 'KB': ..., 'KC': ...,
+        'KP': (ms_total, launches, iterations) of persistent solves}"""
         out = {}
         for k, name in enumerate(('KA', 'KB', 'KC')):
             ms = ctypes.c_double(0.0)
             n = ctypes.c_int64(0)
             check(lib().pf_profile_read(self.h, k, ctypes.byref(ms), ctypes.byref(n)), 'pf_profile_read')
             out[name] = (ms.value, n.value)
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        it = ctypes.c_int64(0)
+        check(lib().pf_profile_read_persistent(self.h, ctypes.byref(ms), ctypes.byref(n),
+                                               ctypes.byref(it)), 'pf_profile_read_persistent')
+        out['KP'] = (ms.value, n.value, it.value)
         return out
 
     def residual_history_enable(self, capacity):
