@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace pf {
 
 constexpr int kNover = 2;
@@ -130,8 +132,35 @@ __device__ __forceinline__ Nbr nbr(const Geom& g, int i, int j, int k) {
 // fixed assignment and tree.  Results therefore depend only on the data and
 // the (fixed) block count, never on scheduling.
 
+// Sum over the wave as the xor butterfly v += shfl_xor(v, off), off = 32,
+// 16, ..., 1, with the same pairs in the same order and so the same bits,
+// without LDS: the 32- and 16-lane steps swap halves of the wave and of each
+// 32-lane half between two copies (v_permlane32_swap / v_permlane16_swap,
+// one copy then holds a lane's own value and the other its partner's, and
+// addition commutes); the 8- and 4-lane steps rotate each 16-lane row, whose
+// source lane holds the same value as the xor partner once the wider steps
+// are done; the 2- and 1-lane steps are quad permutations.
 __device__ __forceinline__ double wave_sum(double v) {
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  auto swap_add = [](double x, bool wide) -> double {
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+    const auto L = wide ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false)
+                        : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto H = wide ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false)
+                        : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return __hiloint2double((int)H[0], (int)L[0]) + __hiloint2double((int)H[1], (int)L[1]);
+  };
+  auto dpp_add = [](double x, auto ctrl) -> double {
+    constexpr int c = decltype(ctrl)::value;
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), c, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), c, 0xf, 0xf, false);
+    return x + __hiloint2double(hi, lo);
+  };
+  v = swap_add(v, true);
+  v = swap_add(v, false);
+  v = dpp_add(v, std::integral_constant<int, 0x128>{});   // row_ror:8
+  v = dpp_add(v, std::integral_constant<int, 0x124>{});   // row_ror:4
+  v = dpp_add(v, std::integral_constant<int, 0x4E>{});    // quad_perm [2,3,0,1]
+  v = dpp_add(v, std::integral_constant<int, 0xB1>{});    // quad_perm [1,0,3,2]
   return v;
 }
 // NaN-propagating max (torch.max propagates NaN; fmax would drop it).
@@ -152,8 +181,39 @@ __device__ __forceinline__ double sqrt_rn(double x) {
 }
 __device__ __forceinline__ float sqrt_rn(float x) { return (float)sqrt_rn((double)x); }
 
+// Max over the wave as the xor butterfly v = nmax(v, shfl_xor(v, off)), off
+// = 32, ..., 1, in the same way as wave_sum (the lane's own value stays the
+// first operand; a row rotation's source lane holds the xor partner's value
+// whenever nmax of two values does not depend on their order, i.e. except
+// for two NaNs of different payloads or two zeros of opposite signs - the
+// maxima taken here are of absolute values).
 __device__ __forceinline__ double wave_max(double v) {
-  for (int off = kWave / 2; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off, kWave));
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  auto swap_max = [&](double x, bool wide) -> double {
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+    const auto L = wide ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false)
+                        : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto H = wide ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false)
+                        : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const double d0 = __hiloint2double((int)H[0], (int)L[0]);
+    const double d1 = __hiloint2double((int)H[1], (int)L[1]);
+    // d0 holds the own value in the lower half of each swapped pair of
+    // halves, d1 in the upper half
+    const bool upper = (lane & (wide ? 32 : 16)) != 0;
+    return upper ? nmax(d1, d0) : nmax(d0, d1);
+  };
+  auto dpp_max = [](double x, auto ctrl) -> double {
+    constexpr int c = decltype(ctrl)::value;
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), c, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), c, 0xf, 0xf, false);
+    return nmax(x, __hiloint2double(hi, lo));
+  };
+  v = swap_max(v, true);
+  v = swap_max(v, false);
+  v = dpp_max(v, std::integral_constant<int, 0x128>{});   // row_ror:8
+  v = dpp_max(v, std::integral_constant<int, 0x124>{});   // row_ror:4
+  v = dpp_max(v, std::integral_constant<int, 0x4E>{});    // quad_perm [2,3,0,1]
+  v = dpp_max(v, std::integral_constant<int, 0xB1>{});    // quad_perm [1,0,3,2]
   return v;
 }
 

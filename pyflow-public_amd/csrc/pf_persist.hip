@@ -52,13 +52,18 @@ constexpr int kSpin = 1 << 21;        // polls before a block gives up
 constexpr int kPT = 256;              // threads per block
 constexpr int kPZ = 64, kPY = 8;      // tile extents (z, y): two z cells per thread
 
+// Device-scope accesses through global-address-space pointers: a generic
+// (flat) access would also count against the LDS wait counter, so every LDS
+// wait of the stencils would wait for the face stores in flight as well.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
 template <typename T>
 __device__ __forceinline__ T ld_dev(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load((gptr<T>)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ void st_dev(T* p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store((gptr<T>)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // Fused multiply-add in the operands' own precision (one rounding).
 __device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -152,12 +157,16 @@ namespace {
 // blocks vid[v] (mine[v*NV + q]) for sync j, wait for every virtual block's,
 // and reduce them as reduce_partials<NV, 256> reduces a kernel's partials
 // (thread t takes virtual blocks t, t+256, ... in order, then the block
-// tree).  Returns false (block-uniform) when the wait was abandoned.
-template <int NV, int V>
+// tree).  As soon as the six neighbours' slots nbv (one virtual block of
+// each) are filled, their faces are complete: early() then issues this
+// thread's face loads, which travel while the wait for the other blocks
+// goes on.  Returns false (block-uniform) when the wait was abandoned.
+template <int NV, int V, typename Early>
 __device__ __forceinline__ bool grid_sync(double* __restrict__ slots, unsigned* flag, int nblk,
                                           const int (&vid)[V], int j,
                                           const double (&mine)[V * NV], double (&red)[NV],
-                                          const int (&op)[NV]) {
+                                          const int (&op)[NV], const int (&nbv)[6],
+                                          Early&& early) {
   double* set = slots + (size_t)(j % kSets) * nblk * 2;
   wait_mem();        // this thread's face stores (and loads) are complete
   __syncthreads();   // ... and every other thread's of the block
@@ -170,7 +179,18 @@ __device__ __forceinline__ bool grid_sync(double* __restrict__ slots, unsigned* 
   constexpr int MB = 4;   // up to 1024 virtual blocks
   double got[MB][NV];
   bool ok = false;
-  for (int it = 0; it < kSpin; ++it) {
+  // first the neighbours (their faces), then everyone (the partials)
+  bool nb = false;
+  for (int it = 0; it < kSpin && !nb; ++it) {
+    nb = true;
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      nb = nb && ld_dev(reinterpret_cast<const unsigned long long*>(set + 2 * nbv[q])) !=
+                     kSlotEmpty;
+    if ((it & 63) == 63 && ld_dev(flag) != 0u) break;
+  }
+  early();   // (after a timeout too: in-bounds loads whose values go unused)
+  for (int it = 0; it < kSpin && nb; ++it) {
     ok = true;
 #pragma unroll
     for (int u = 0; u < MB; ++u) {
@@ -238,6 +258,15 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
   int vid[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) vid[v] = block_of_tile(tg, nblk, tzi, tyi, cg * V + v);
+  // one virtual block of each neighbour block: its partial slot filled means
+  // the neighbour's faces are complete
+  int nbv[6];
+  nbv[0] = block_of_tile(tg, nblk, tzi, tyi, (cg - 1) * V);
+  nbv[1] = block_of_tile(tg, nblk, tzi, tyi, (cg + 1) * V);
+  nbv[2] = block_of_tile(tg, nblk, tzi, tyi - 1, cg * V);
+  nbv[3] = block_of_tile(tg, nblk, tzi, tyi + 1, cg * V);
+  nbv[4] = block_of_tile(tg, nblk, tzi - 1, tyi, cg * V);
+  nbv[5] = block_of_tile(tg, nblk, tzi + 1, tyi, cg * V);
   auto phys_of = [&](int tz, int ty, int c) -> int {
     tz = (tz + tg.ntz) % tg.ntz;
     ty = (ty + tg.nty) % tg.nty;
@@ -301,25 +330,6 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
     H.z = hasz ? A.r0[gidx(x0 + t / 16, y0 + (t & 7), ((t / 8) & 1) ? z0 + kPZ : z0 - 1)]
                : (T)0;
   };
-  // faces of a field this block publishes
-  auto publish = [&](T* base, auto&& F) {
-    T* reg = base + (size_t)b * FW;
-    st2_dev(reg + FM::xlo + ly * kPZ + 2 * lz, F(0));
-    st2_dev(reg + FM::xhi + ly * kPZ + 2 * lz, F(XC - 1));
-    if (ly == 0 || ly == kPY - 1) {
-      T* yr = reg + (ly == 0 ? FM::ylo : FM::yhi) + 2 * lz;
-#pragma unroll
-      for (int x = 0; x < XC; ++x) st2_dev(yr + x * kPZ, F(x));
-    }
-    if (lz == 0) {
-#pragma unroll
-      for (int x = 0; x < XC; ++x) st_dev(reg + FM::zlo + x * kPY + ly, F(x).a);
-    } else if (lz == kPZ / 2 - 1) {
-#pragma unroll
-      for (int x = 0; x < XC; ++x) st_dev(reg + FM::zhi + x * kPY + ly, F(x).b);
-    }
-  };
-
   __shared__ T s_rh[XC][kPY][kPZ];
   __shared__ T s_x[XC][kPY][kPZ];
   // LDS planes of the stencil's field with their halo: staged GP at a time
@@ -373,6 +383,47 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
     }
   };
 
+  // faces of a field this block publishes.  The x faces (every thread's
+  // column) go straight out; the y faces (two rows of threads) and z faces
+  // (two lanes per wave) are gathered in the idle LDS plane buffer first, so
+  // that all threads store them with full-wave, contiguous stores (the face
+  // region from row y0 on is [y0 row][y0+7 row][z0 column][z0+63 column]).
+  auto publish = [&](T* base, auto&& F) {
+    T* reg = base + (size_t)b * FW;
+    st2_dev(reg + FM::xlo + ly * kPZ + 2 * lz, F(0));
+    st2_dev(reg + FM::xhi + ly * kPZ + 2 * lz, F(XC - 1));
+    constexpr int NYZ = FM::size - FM::ylo;
+    static_assert(NYZ <= GP * (kPY + 2) * (kPZ + 2), "face staging exceeds the plane buffer");
+    T* stg = &plane[0][0][0];
+    __syncthreads();   // the plane buffer is free
+    if (ly == 0 || ly == kPY - 1) {
+      T* d = stg + (ly == 0 ? 0 : FM::yhi - FM::ylo) + 2 * lz;
+#pragma unroll
+      for (int x = 0; x < XC; ++x) {
+        const Pr f = F(x);
+        d[x * kPZ] = f.a;
+        d[x * kPZ + 1] = f.b;
+      }
+    }
+    if (lz == 0) {
+#pragma unroll
+      for (int x = 0; x < XC; ++x) stg[FM::zlo - FM::ylo + x * kPY + ly] = F(x).a;
+    } else if (lz == kPZ / 2 - 1) {
+#pragma unroll
+      for (int x = 0; x < XC; ++x) stg[FM::zhi - FM::ylo + x * kPY + ly] = F(x).b;
+    }
+    __syncthreads();
+    if constexpr (sizeof(T) == 4) {
+      static_assert(NYZ % 2 == 0, "");
+      for (int i = t; i < NYZ / 2; i += kPT)
+        st_dev(reinterpret_cast<uint64_t*>(reg + FM::ylo) + i,
+               reinterpret_cast<const uint64_t*>(stg)[i]);
+    } else {
+      for (int i = t; i < NYZ; i += kPT) st_dev(reg + FM::ylo + i, stg[i]);
+    }
+    __syncthreads();   // the plane buffer may be written again
+  };
+
   // own cells: p, v, r, t in registers; r_hat and x in LDS
   Pr P[XC], V_[XC], R[XC], Tt[XC];
   const Pr zero2 = {(T)0, (T)0};
@@ -398,6 +449,9 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
   double rho1, rho2 = c0.rho2, alpha = c0.alpha, omega = c0.omega, res0, maxr = 0.0;
   int iter = 0, j = 0;
   int status = PF_SOLVE_CONVERGED;
+  // halo values fetched inside the syncs: p, v, r for the next KA (during
+  // the sync after KC), v and r for KB (during the sync after KA)
+  Halo hPo, hVo, hRa, hVb, hRb;
   {
     double red[2];
     const int ops[2] = {0, 1};
@@ -419,7 +473,7 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
       const double max_res = maxr / res0;                  // pressure.py:279-280
       if (lead) {
         A.ctl->max_res = max_res;
-        if (iter <= c0.hist_cap) c0.hist[iter - 1] = max_res;
+        if (iter <= c0.hist_cap) *(gptr<double>)(c0.hist + iter - 1) = max_res;
       }
       const bool cap = iter >= c0.maxit;                   // pressure.py:283-287
       const bool conv = max_res <= c0.tol;
@@ -454,17 +508,13 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
         for (int u = 0; u < NY; ++u) H.y[u] = pupd(zero2, zero2, R0.y[u]);
         H.z = (fmx((T)0, c1, (T)0) * c2) + R0.z;
       } else {
-        Halo Po, Vo, Rc;
-        fetch(fpar[1 - par], Po);
-        fetch(fvar[1 - par], Vo);
-        fetch(fr, Rc);
 #pragma unroll
         for (int x = 0; x < XC; ++x) P[x] = pupd(P[x], V_[x], R[x]);
 #pragma unroll
-        for (int s = 0; s < 2; ++s) H.x[s] = pupd(Po.x[s], Vo.x[s], Rc.x[s]);
+        for (int s = 0; s < 2; ++s) H.x[s] = pupd(hPo.x[s], hVo.x[s], hRa.x[s]);
 #pragma unroll
-        for (int u = 0; u < NY; ++u) H.y[u] = pupd(Po.y[u], Vo.y[u], Rc.y[u]);
-        H.z = (fmx(Vo.z, c1, Po.z) * c2) + Rc.z;
+        for (int u = 0; u < NY; ++u) H.y[u] = pupd(hPo.y[u], hVo.y[u], hRa.y[u]);
+        H.z = (fmx(hVo.z, c1, hPo.z) * c2) + hRa.z;
       }
       publish(fpar[par], [&](int x) -> Pr { return P[x]; });
       double acc[V];
@@ -487,7 +537,15 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
         for (int v = 0; v < V; ++v) opv[v] = 0;
         block_reduce<V, kPT>(acc, opv);
       }
-      if (!grid_sync<1, V>(A.slots, A.flag, nblk, vid, j++, acc, gam, ops)) return;
+      const bool first = iter == 0;
+      if (!grid_sync<1, V>(A.slots, A.flag, nblk, vid, j++, acc, gam, ops, nbv, [&]() {
+            fetch(fvar[par], hVb);
+            if (first)
+              fetch_r0(hRb);   // no r face has been published yet
+            else
+              fetch(fr, hRb);
+          }))
+        return;
       if (gam[0] == 0.0) {                                    // pressure.py:247-248
         status = PF_SOLVE_BREAKDOWN_A;
         break;
@@ -502,19 +560,14 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
       return Pr{fmx(V_[x].a, na, R[x].a), fmx(V_[x].b, na, R[x].b)};
     };
     {
-      Halo Vn, Rc, S;
-      fetch(fvar[par], Vn);
-      if (iter > 0)
-        fetch(fr, Rc);
-      else
-        fetch_r0(Rc);
+      Halo S;
 #pragma unroll
       for (int s = 0; s < 2; ++s)
-        S.x[s] = Pr{fmx(Vn.x[s].a, na, Rc.x[s].a), fmx(Vn.x[s].b, na, Rc.x[s].b)};
+        S.x[s] = Pr{fmx(hVb.x[s].a, na, hRb.x[s].a), fmx(hVb.x[s].b, na, hRb.x[s].b)};
 #pragma unroll
       for (int u = 0; u < NY; ++u)
-        S.y[u] = Pr{fmx(Vn.y[u].a, na, Rc.y[u].a), fmx(Vn.y[u].b, na, Rc.y[u].b)};
-      S.z = fmx(Vn.z, na, Rc.z);
+        S.y[u] = Pr{fmx(hVb.y[u].a, na, hRb.y[u].a), fmx(hVb.y[u].b, na, hRb.y[u].b)};
+      S.z = fmx(hVb.z, na, hRb.z);
       double ab[2 * V];
 #pragma unroll
       for (int q = 0; q < 2 * V; ++q) ab[q] = 0.0;
@@ -537,7 +590,8 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
         for (int q = 0; q < 2 * V; ++q) opv[q] = 0;
         block_reduce<2 * V, kPT>(ab, opv);
       }
-      if (!grid_sync<2, V>(A.slots, A.flag, nblk, vid, j++, ab, st, ops)) return;
+      if (!grid_sync<2, V>(A.slots, A.flag, nblk, vid, j++, ab, st, ops, nbv, []() {}))
+        return;
       if (st[1] == 0) {                                     // pressure.py:251, 266-267
 #pragma unroll
         for (int x = 0; x < XC; ++x) {
@@ -589,7 +643,13 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
         for (int q = 0; q < 2 * V; ++q) opv[q] = q & 1;
         block_reduce<2 * V, kPT>(cc, opv);
       }
-      if (!grid_sync<2, V>(A.slots, A.flag, nblk, vid, j++, cc, rc, opc)) return;
+      const int pn = par;   // the faces the next KA reads: this iteration's p and v
+      if (!grid_sync<2, V>(A.slots, A.flag, nblk, vid, j++, cc, rc, opc, nbv, [&]() {
+            fetch(fpar[pn], hPo);
+            fetch(fvar[pn], hVo);
+            fetch(fr, hRa);
+          }))
+        return;
       rho1 = rc[0];
       maxr = rc[1];
     }
