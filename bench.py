@@ -8,11 +8,17 @@ a relative L-inf residual of 1e-10 -- the reference's forward loop
 PyFlow.setup.  A step is one full time step of that loop.
 
 Prints one JSON line.  `value` is Mcell-steps/s of the timed FP64 steps.
-The roofline object is for the Krylov kernel with the largest measured time
-(KA, KB and KC run once per iteration each; their average launch durations
-come from start/stop hipEvents recorded by the dispatch of every 32nd launch
-of each kernel during the timed steps, a sample that keeps the events' own
-cost out of `value`).  `cpu_baseline` times the CPU oracle (a torch-CPU FP64
+The Poisson solve of a step runs as one persistent cooperative kernel that
+keeps the Krylov vectors on chip (KP, bicg_persist_kernel) when the box
+fits; the roofline object is then for KP: the algorithmic bytes of the
+iterations it ran over its duration (start/stop hipEvents around each
+launch), which can exceed the HBM peak because the vectors never travel,
+with its measured HBM traffic (PMC) beside it.  `launch_path` is the same
+workload with the launch-per-reduction kernels (KA, KB, KC once per
+iteration each; their average durations come from start/stop hipEvents
+recorded by the dispatch of every 32nd launch of each kernel during the
+timed steps, a sample that keeps the events' own cost out of its value),
+and its roofline object is for the one of them with the largest total.  `cpu_baseline` times the CPU oracle (a torch-CPU FP64
 restatement of the reference's op sequence) on a bounded sample of the same
 workload on this host, and `oracle_error` is the relative L2 error of the GPU
 velocity against the oracle's after those steps.  The other legs of the
@@ -61,9 +67,11 @@ KERNEL_NAMES = {'KA': 'KA: p update, v = A p, <v, r_hat> (bicg_tile2_kernel mode
 KERNEL_NAMES['KP'] = ('KP: the whole BiCGStab solve as one persistent cooperative kernel, '
                       'vectors on chip (bicg_persist_kernel)')
 KERNEL_WORDS = {'KA': 6, 'KB': 2, 'KC': 7}   # algorithmic vector passes per launch
-# HBM-side bytes per launch of the persistent solve (rocprofv3 --pmc FETCH_SIZE,
-# WRITE_SIZE, scaled as above); filled in from the profile of this workload.
-PERSIST_TRAFFIC_BYTES = {}
+# HBM-side bytes per BiCGStab iteration of the persistent solve (rocprofv3
+# --pmc FETCH_SIZE, WRITE_SIZE on this bench's default workload, scaled as
+# above, over the iterations of the profiled solves): tile faces and partial
+# sums only.
+PERSIST_TRAFFIC_PER_ITER = {(128, 'f64'): (3029198.3 / 0.50003 + 7815144.1) * 1024 / 296.5}
 
 
 def make_config(N, dtype, nu, tol, maxit, seed, solver='bicgstab'):
@@ -158,7 +166,8 @@ def measure(N, dt_name, nu, tol, maxit, steps, warmup, seed, period, solver, per
         out['roofline'] = {'kernel': KERNEL_NAMES['KP'], 'bound': 'hbm', 'achieved': round(gbs, 1),
                            'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                            'frac': round(gbs / HBM_PEAK_GBS, 4),
-                           'traffic': PERSIST_TRAFFIC_BYTES.get((N, dt_name)),
+                           'traffic': (int(PERSIST_TRAFFIC_PER_ITER[(N, dt_name)] * itp / n)
+                                       if (N, dt_name) in PERSIST_TRAFFIC_PER_ITER else None),
                            'bytes_per_launch': int(nbytes),
                            'chosen_by': 'the solve is one launch: the largest kernel by time'}
     elif r['prof']:
@@ -285,6 +294,9 @@ def main():
     ap.add_argument('--no-ml-closure', action='store_true',
                     help="skip the leg that times the reference's ML closure at 64^3")
     ap.add_argument('--no-profile', action='store_true')
+    ap.add_argument('--krylov', choices=['persistent', 'launches'], default='persistent',
+                    help='Poisson solves of the main leg: one persistent cooperative kernel '
+                         'when the box fits (default), or the launch-per-reduction kernels')
     ap.add_argument('--no-launch-path', action='store_true',
                     help='skip the leg that runs the same workload with the launch-per-'
                          'reduction Krylov kernels instead of the persistent solve')
@@ -303,7 +315,8 @@ def main():
     tol = a.tol if a.tol is not None else (1e-10 if a.dtype == 'f64' else 1e-8)
     metric = 'Mcell-steps/s + HBM GB/s at 128^3 FP64/FP32; served p50/p99 step latency'
     leg = measure(a.N, a.dtype, a.nu, tol, a.maxit, a.steps, a.warmup, a.seed,
-                  0 if a.no_profile else a.profile_period, a.solver)
+                  0 if a.no_profile else a.profile_period, a.solver,
+                  persistent=a.krylov == 'persistent')
     r = leg.pop('_run')
     out = {
         'metric': metric, 'value': leg['value'], 'unit': 'Mcell-steps/s', 'n_gpus': 1,
@@ -357,7 +370,8 @@ def main():
                                'gpu_tol': tol, 'oracle_tol': otol}
     if a.dtype == 'f64' and not a.no_f32:
         f32 = measure(a.N, 'f32', a.nu, 1e-8, a.maxit, a.steps, a.warmup, a.seed,
-                      0 if a.no_profile else a.profile_period, a.solver)
+                      0 if a.no_profile else a.profile_period, a.solver,
+                      persistent=a.krylov == 'persistent')
         f32.pop('_run')
         f32['tol'] = 1e-8
         if cb is not None:
