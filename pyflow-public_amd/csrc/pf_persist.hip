@@ -449,9 +449,11 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
   double rho1, rho2 = c0.rho2, alpha = c0.alpha, omega = c0.omega, res0, maxr = 0.0;
   int iter = 0, j = 0;
   int status = PF_SOLVE_CONVERGED;
-  // halo values fetched inside the syncs: p, v, r for the next KA (during
-  // the sync after KC), v and r for KB (during the sync after KA)
-  Halo hPo, hVo, hRa, hVb, hRb;
+  // halo values fetched inside the syncs: v and r for KB (during the sync
+  // after KA; v is kept for the next KA's p), p for the next KA (during the
+  // sync after KB: KA published it before the first sync), r for the next
+  // KA (during the sync after KC)
+  Halo hPo, hRa, hVb, hRb;
   {
     double red[2];
     const int ops[2] = {0, 1};
@@ -511,10 +513,10 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
 #pragma unroll
         for (int x = 0; x < XC; ++x) P[x] = pupd(P[x], V_[x], R[x]);
 #pragma unroll
-        for (int s = 0; s < 2; ++s) H.x[s] = pupd(hPo.x[s], hVo.x[s], hRa.x[s]);
+        for (int s = 0; s < 2; ++s) H.x[s] = pupd(hPo.x[s], hVb.x[s], hRa.x[s]);
 #pragma unroll
-        for (int u = 0; u < NY; ++u) H.y[u] = pupd(hPo.y[u], hVo.y[u], hRa.y[u]);
-        H.z = (fmx(hVo.z, c1, hPo.z) * c2) + hRa.z;
+        for (int u = 0; u < NY; ++u) H.y[u] = pupd(hPo.y[u], hVb.y[u], hRa.y[u]);
+        H.z = (fmx(hVb.z, c1, hPo.z) * c2) + hRa.z;
       }
       publish(fpar[par], [&](int x) -> Pr { return P[x]; });
       double acc[V];
@@ -590,7 +592,8 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
         for (int q = 0; q < 2 * V; ++q) opv[q] = 0;
         block_reduce<2 * V, kPT>(ab, opv);
       }
-      if (!grid_sync<2, V>(A.slots, A.flag, nblk, vid, j++, ab, st, ops, nbv, []() {}))
+      if (!grid_sync<2, V>(A.slots, A.flag, nblk, vid, j++, ab, st, ops, nbv,
+                           [&]() { fetch(fpar[par], hPo); }))
         return;
       if (st[1] == 0) {                                     // pressure.py:251, 266-267
 #pragma unroll
@@ -643,10 +646,7 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
         for (int q = 0; q < 2 * V; ++q) opv[q] = q & 1;
         block_reduce<2 * V, kPT>(cc, opv);
       }
-      const int pn = par;   // the faces the next KA reads: this iteration's p and v
       if (!grid_sync<2, V>(A.slots, A.flag, nblk, vid, j++, cc, rc, opc, nbv, [&]() {
-            fetch(fpar[pn], hPo);
-            fetch(fvar[pn], hVo);
             fetch(fr, hRa);
           }))
         return;
