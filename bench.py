@@ -13,7 +13,10 @@ keeps the Krylov vectors on chip (KP, bicg_persist_kernel) when the box
 fits; the roofline object is then for KP: the algorithmic bytes of the
 iterations it ran over its duration (start/stop hipEvents around each
 launch), which can exceed the HBM peak because the vectors never travel,
-with its measured HBM traffic (PMC) beside it.  `launch_path` is the same
+with its HBM traffic beside it (`roofline.traffic`: bytes per launch from two
+rocprofv3 --pmc passes, FETCH_SIZE then WRITE_SIZE, over a short child run
+before the timed legs, each counter scaled by its reading for a plain
+device copy in the same pass; --no-traffic skips them).  `launch_path` is the same
 workload with the launch-per-reduction kernels (KA, KB, KC once per
 iteration each; their average durations come from start/stop hipEvents
 recorded by the dispatch of every 32nd launch of each kernel during the
@@ -47,19 +50,6 @@ for p in (ROOT, os.path.join(ROOT, 'pyflow-public_amd')):
 HBM_PEAK_GBS = 8000.0
 INFINITY_CACHE_BYTES = 256 << 20   # MI355X last-level (MALL) cache
 
-# HBM-side bytes per launch of each Krylov kernel measured with rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE on this bench's default workload (one pass
-# per counter), each scaled by the counter's reading for a plain 512 MiB
-# device copy (FETCH 0.500, WRITE 1.000): FETCH/0.500 + WRITE KiB per
-# dispatch.  Keyed by (grid, dtype, kernel); other workloads report null.
-KRYLOV_TRAFFIC_BYTES = {
-    (128, 'f64', 'KA'): int((38403.6 / 0.50003 + 32099.1) * 1024),
-    (128, 'f64', 'KB'): int((18283.6 / 0.50003 + 15.7) * 1024),
-    (128, 'f64', 'KC'): int((44520.7 / 0.50003 + 32205.1) * 1024),
-    (128, 'f32', 'KA'): int((17221.6 / 0.50003 + 14685.6) * 1024),
-    (128, 'f32', 'KB'): int((8967.5 / 0.50003 + 28.7) * 1024),
-    (128, 'f32', 'KC'): int((20348.6 / 0.50003 + 14726.9) * 1024),
-}
 KERNEL_NAMES = {'KA': 'KA: p update, v = A p, <v, r_hat> (bicg_tile2_kernel mode 0)',
                 'KB': 'KB: s = r - alpha v, t = A s, <s,t>, <t,t> (bicg_tile2_kernel mode 1)',
                 'KC': 'KC: s, t = A s again, x and r updates, <r,r_hat>, max|r| '
@@ -67,11 +57,9 @@ KERNEL_NAMES = {'KA': 'KA: p update, v = A p, <v, r_hat> (bicg_tile2_kernel mode
 KERNEL_NAMES['KP'] = ('KP: the whole BiCGStab solve as one persistent cooperative kernel, '
                       'vectors on chip (bicg_persist_kernel)')
 KERNEL_WORDS = {'KA': 6, 'KB': 2, 'KC': 7}   # algorithmic vector passes per launch
-# HBM-side bytes per BiCGStab iteration of the persistent solve (rocprofv3
-# --pmc FETCH_SIZE, WRITE_SIZE on this bench's default workload, scaled as
-# above, over the iterations of the profiled solves): tile faces and partial
-# sums only.
-PERSIST_TRAFFIC_PER_ITER = {(128, 'f64'): (3029198.3 / 0.50003 + 7815144.1) * 1024 / 296.5}
+KERNEL_MATCH = {'KP': 'bicg_persist_kernel<%s,', 'KA': 'bicg_tile2_kernel<%s, 0,',
+                'KB': 'bicg_tile2_kernel<%s, 1,', 'KC': 'bicg_tile2_kernel<%s, 2,'}
+COPY_MIB = 512      # plain device copy that calibrates the byte counters
 
 
 def make_config(N, dtype, nu, tol, maxit, seed, solver='bicgstab'):
@@ -166,10 +154,10 @@ def measure(N, dt_name, nu, tol, maxit, steps, warmup, seed, period, solver, per
         out['roofline'] = {'kernel': KERNEL_NAMES['KP'], 'bound': 'hbm', 'achieved': round(gbs, 1),
                            'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                            'frac': round(gbs / HBM_PEAK_GBS, 4),
-                           'traffic': (int(PERSIST_TRAFFIC_PER_ITER[(N, dt_name)] * itp / n)
-                                       if (N, dt_name) in PERSIST_TRAFFIC_PER_ITER else None),
+                           'traffic': None,   # measured by the --pmc passes at the end
                            'bytes_per_launch': int(nbytes),
                            'chosen_by': 'the solve is one launch: the largest kernel by time'}
+        out['_top'] = 'KP'
     elif r['prof']:
         per = {}
         for name in ('KA', 'KB', 'KC'):
@@ -189,9 +177,10 @@ def measure(N, dt_name, nu, tol, maxit, steps, warmup, seed, period, solver, per
             out['roofline'] = {'kernel': KERNEL_NAMES[top], 'bound': 'hbm',
                                'achieved': p['GB/s'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                                'frac': round(p['GB/s'] / HBM_PEAK_GBS, 4),
-                               'traffic': KRYLOV_TRAFFIC_BYTES.get((N, dt_name, top)),
+                               'traffic': None,
                                'bytes_per_launch': p['bytes_per_launch'],
                                'chosen_by': 'largest measured total time of KA, KB, KC'}
+            out['_top'] = top
     return out
 
 
@@ -269,6 +258,132 @@ def calibrate_copy(mib):
                       'GB/s_read_plus_write': round(2 * n / ms / 1e6, 1)}))
 
 
+def traffic_child(a):
+    """The program a --pmc pass profiles: a plain device copy of COPY_MIB MiB
+    (5 dispatches: the counters' reference) and two steps of each leg named in
+    --traffic-child ("f64:p" = FP64 with the persistent solve, "f64:l" = with
+    the launch-per-reduction kernels, "f32:p"); prints the legs' BiCGStab
+    iteration counts, one per dispatch of the persistent kernel."""
+    import torch
+    n = COPY_MIB << 20
+    dst = torch.empty(n, dtype=torch.uint8, device='cuda')
+    src = torch.ones(n, dtype=torch.uint8, device='cuda')
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del dst, src
+    legs = []
+    for spec in a.traffic_child.split(','):
+        dt_name, path = spec.split(':')
+        dtype = torch.float64 if dt_name == 'f64' else torch.float32
+        tol = 1e-10 if dt_name == 'f64' else 1e-8
+        t0 = time.perf_counter()
+        r = gpu_run(a.N, dtype, a.nu, tol, a.maxit, 2, 0, a.seed, 0, a.solver, path == 'p')
+        legs.append({'leg': spec, 'its': r['its'], 'persistent': r['persistent']})
+        print('traffic child: %s, 2 steps, %s iterations, %.1f s'
+              % (spec, r['its'], time.perf_counter() - t0), flush=True)
+    print('TRAFFIC_CHILD ' + json.dumps({'copy_bytes': n, 'legs': legs}), flush=True)
+
+
+def pmc_pass(a, counter, specs, timeout):
+    """One rocprofv3 --pmc pass of traffic_child: (its report, rows of
+    (kernel name, counter value in KiB) in dispatch order)."""
+    import csv
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    import threading
+    exe = shutil.which('rocprofv3')
+    if exe is None:
+        raise RuntimeError('rocprofv3 not found')
+    d = tempfile.mkdtemp(prefix='pf_pmc_')
+    try:
+        cmd = [exe, '--pmc', counter, '--output-format', 'csv', '-d', d, '-o', 'run', '--',
+               sys.executable, os.path.abspath(__file__), '--traffic-child', ','.join(specs),
+               '--N', str(a.N), '--nu', repr(a.nu), '--maxit', str(a.maxit),
+               '--seed', str(a.seed), '--solver', a.solver]
+        proc = subprocess.Popen(cmd, cwd=d, env=dict(os.environ, TMPDIR=d),
+                                stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                start_new_session=True)
+        lines = []
+
+        def pump():   # the child's progress lines go on to stderr as they come
+            for raw in proc.stdout:
+                ln = raw.decode(errors='replace').rstrip('\n')
+                lines.append(ln)
+                if ln.startswith('traffic child'):
+                    print('%s pass: %s' % (counter, ln), file=sys.stderr, flush=True)
+
+        th = threading.Thread(target=pump, daemon=True)
+        th.start()
+        try:
+            proc.wait(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.wait()
+            raise RuntimeError('%s pass did not finish in %d s' % (counter, timeout))
+        th.join(10)
+        text = '\n'.join(lines)
+        rep = [ln for ln in lines if ln.startswith('TRAFFIC_CHILD ')]
+        path = os.path.join(d, 'run_counter_collection.csv')
+        # the profiler may end with a signal after the program has finished
+        # and its files are written: the report line and the file decide
+        if not rep or not os.path.exists(path):
+            raise RuntimeError('%s pass failed (exit %s): %s'
+                               % (counter, proc.returncode, text[-300:]))
+        rows = []
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row['Counter_Name'] == counter:
+                    rows.append((row['Kernel_Name'], float(row['Counter_Value'])))
+        return json.loads(rep[-1][len('TRAFFIC_CHILD '):]), rows
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def measure_traffic(a, specs, timeout=300):
+    """HBM-side bytes of the Krylov kernels, measured in this run: two
+    rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over traffic_child,
+    each counter scaled by what it reads for the plain copy of COPY_MIB MiB in
+    the same pass (bytes = FETCH_KiB / f_fetch + WRITE_KiB / f_write, times
+    1024).  Returns {(dtype, kernel): bytes per launch} for KA, KB, KC, and
+    bytes per BiCGStab iteration for KP, plus the scale factors."""
+    got = {}
+    factors = {}
+    for counter in ('FETCH_SIZE', 'WRITE_SIZE'):
+        t0 = time.perf_counter()
+        rep, rows = pmc_pass(a, counter, specs, timeout)
+        n = rep['copy_bytes']
+        copies = [v for k, v in rows if 'copyBuffer' in k and v * 1024 > 0.1 * n]
+        if not copies:
+            raise RuntimeError('%s pass: no device copy of %d bytes found' % (counter, n))
+        copies.sort()
+        f = copies[len(copies) // 2] * 1024 / n
+        factors[counter] = round(f, 5)
+        for leg in rep['legs']:
+            dt_name, path = leg['leg'].split(':')
+            tname = 'double' if dt_name == 'f64' else 'float'
+            if path == 'p':
+                if not leg['persistent']:
+                    continue
+                vals = [v for k, v in rows if KERNEL_MATCH['KP'] % tname in k]
+                if len(vals) != len(leg['its']):
+                    raise RuntimeError('%s pass: %d persistent dispatches for %d solves'
+                                       % (counter, len(vals), len(leg['its'])))
+                got.setdefault((dt_name, 'KP'), 0.0)
+                got[(dt_name, 'KP')] += sum(vals) * 1024 / f / sum(leg['its'])
+            else:
+                for kn in ('KA', 'KB', 'KC'):
+                    vals = [v for k, v in rows if KERNEL_MATCH[kn] % tname in k]
+                    if vals:
+                        got.setdefault((dt_name, kn), 0.0)
+                        got[(dt_name, kn)] += sum(vals) / len(vals) * 1024 / f
+        print('traffic: %s pass %.1f s' % (counter, time.perf_counter() - t0), file=sys.stderr,
+              flush=True)
+    return got, factors
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -302,6 +417,10 @@ def main():
                          'reduction Krylov kernels instead of the persistent solve')
     ap.add_argument('--profile-period', type=int, default=32,
                     help='time every n-th launch of each Krylov kernel with dispatch events')
+    ap.add_argument('--no-traffic', action='store_true',
+                    help='skip the two rocprofv3 --pmc passes that measure the Krylov kernels\' '
+                         'HBM traffic (roofline.traffic is then null)')
+    ap.add_argument('--traffic-child', default='', help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU: this build runs one periodic box per GPU and has no '
@@ -310,6 +429,23 @@ def main():
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
         return
+    if a.traffic_child:
+        traffic_child(a)
+        return
+    # the --pmc passes run first, before this process opens the GPU: the
+    # profiled child then has the device to itself
+    traffic = None
+    if not a.no_traffic and not a.no_profile:
+        specs = ['%s:%s' % (a.dtype, 'p' if a.krylov == 'persistent' else 'l')]
+        if not a.no_launch_path:
+            specs.append('%s:l' % a.dtype)
+        if a.dtype == 'f64' and not a.no_f32:
+            specs.append('f32:%s' % ('p' if a.krylov == 'persistent' else 'l'))
+        specs = sorted(set(specs))
+        try:
+            traffic = measure_traffic(a, specs) + (specs,)
+        except Exception as e:   # the line still prints, with the reason
+            traffic = str(e)[:400]
     import torch
     dtype = torch.float64 if a.dtype == 'f64' else torch.float32
     tol = a.tol if a.tol is not None else (1e-10 if a.dtype == 'f64' else 1e-8)
@@ -318,6 +454,7 @@ def main():
                   0 if a.no_profile else a.profile_period, a.solver,
                   persistent=a.krylov == 'persistent')
     r = leg.pop('_run')
+    legs = {'main': (a.dtype, leg.pop('_top', None), leg, leg.get('krylov_path') == 'persistent')}
     out = {
         'metric': metric, 'value': leg['value'], 'unit': 'Mcell-steps/s', 'n_gpus': 1,
         'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': leg['ms_per_step'],
@@ -343,9 +480,11 @@ def main():
         lp = measure(a.N, a.dtype, a.nu, tol, a.maxit, a.steps, a.warmup, a.seed,
                      0 if a.no_profile else a.profile_period, a.solver, persistent=False)
         lp.pop('_run')
+        top = lp.pop('_top', None)
         out['launch_path'] = {k: lp[k] for k in ('value', 'ms_per_step', 'bicgstab_iters_mean',
                                                  'step_roofline_frac', 'kernels', 'roofline')
                               if k in lp}
+        legs['launch_path'] = (a.dtype, top, out['launch_path'], False)
     cb = None
     if not a.no_cpu:
         u0, v0, w0 = r['u0']
@@ -373,6 +512,7 @@ def main():
                       0 if a.no_profile else a.profile_period, a.solver,
                       persistent=a.krylov == 'persistent')
         f32.pop('_run')
+        legs['f32'] = ('f32', f32.pop('_top', None), f32, f32.get('krylov_path') == 'persistent')
         f32['tol'] = 1e-8
         if cb is not None:
             gf = gpu_fields(a.N, torch.float32, a.nu, 1e-8, a.maxit, a.cpu_steps, a.seed,
@@ -410,6 +550,29 @@ def main():
             'four_separate_terms_ms': round(t['four_terms'], 3),
             'peak_tflops_fp32_mfma': 157.3,
             'forward_frac': round(f / t['forward'] / 1e9 / 157.3, 4)}
+    if isinstance(traffic, str):
+        out['traffic_measurement'] = {'error': traffic}
+    elif traffic is not None:
+        got, factors, specs = traffic
+        for dt, top, d, pers in legs.values():
+            if top is None or (dt, top) not in got:
+                continue
+            per = got[(dt, top)]
+            if top == 'KP':
+                d['kernels']['KP']['hbm_bytes_per_iteration'] = int(per)
+                per *= d['kernels']['KP']['iterations_per_launch']
+            d['roofline']['traffic'] = int(per)
+            d['roofline']['traffic_vs_algorithmic'] = round(
+                per / d['roofline']['bytes_per_launch'], 4)
+            for kn, kd in d['kernels'].items():
+                if kn != 'KP' and (dt, kn) in got:
+                    kd['hbm_bytes_measured'] = int(got[(dt, kn)])
+        out['traffic_measurement'] = {
+            'method': 'rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE (one pass each) over 2 steps '
+                      'of each leg in a child run before the timed legs, KiB per dispatch '
+                      'scaled by the same counter over a plain %d MiB device copy in that pass'
+                      % COPY_MIB,
+            'copy_scale': factors, 'legs': specs}
     print(json.dumps(out))
 
 
