@@ -129,7 +129,8 @@ inline TileGrid tile_grid_with(const Geom& g, int tz, int pair, int rows, int mi
 // exactly) and chunks of at least 64 bytes per column (8 planes in FP64, 4
 // in FP32) - measured best at 128^3.  Grids that would get fewer than 512
 // blocks that way (below two per CU) take 4-plane chunks, then one cell per
-// thread.
+// thread - except 64 x 8 tiles on 4-plane chunks that cover the box exactly
+// (64^3: 128 blocks), which the persistent solve holds on chip.
 inline TileGrid make_tile_grid(const Geom& g, int esize, int rows = 1) {
   const int tz = tile_extent_z(g.nz);
   const bool can_pair = rows == 1 && tz >= 16 && g.nz % tz == 0;
@@ -141,6 +142,7 @@ inline TileGrid make_tile_grid(const Geom& g, int esize, int rows = 1) {
       t = tile_grid_with(g, tz, 2, rows, 4);
       if (t.nblk >= 512) return t;
     }
+    if (t.tz == 64 && t.ty == 8 && t.xc == 4 && g.nx % 4 == 0 && g.ny % 8 == 0) return t;
   }
   return tile_grid_with(g, tz, 1, rows, 4);
 }

@@ -704,9 +704,13 @@ bool persist_shape(const pf_box* h, const TileGrid& tg, PersistShape& sh) {
   // per thread, so that one block's waits are the other's compute (128^3
   // FP32: 27.1 -> 23.3 us per iteration)
   sh.v = (h->esize == 4 && tg.xc == 4) ? 2 : 16 / tg.xc;
+  // grids of at most 256 tiles (64^3: 128): one chunk per block, so that
+  // the solve spreads over as many CUs as the grid has tiles
+  if (tg.nblk <= 256) sh.v = 1;
   if (tg.ncx % sh.v) return false;
   sh.nphys = tg.nblk / sh.v;
-  sh.face_elems = sh.v * sh.xv == 16 ? FaceMap<16>::size : FaceMap<8>::size;
+  const int xc = sh.v * sh.xv;
+  sh.face_elems = xc == 16 ? FaceMap<16>::size : xc == 8 ? FaceMap<8>::size : FaceMap<4>::size;
   return true;
 }
 
@@ -790,6 +794,11 @@ int persist_solve(pf_box* h, hipStream_t s, const TileGrid& tg, const Lap<T>& L,
   if (!h->pers || !h->persist_on) return 0;
   PersistShape sh;
   if (!persist_shape(h, tg, sh)) return 0;
+  if (sh.v == 1) {
+    if (sh.xv == 8)
+      return persist_launch<T, 8, 1, 1>(h, s, tg, sh, L, r0, rh, x, npart_init, used, ev);
+    return persist_launch<T, 4, 1, 1>(h, s, tg, sh, L, r0, rh, x, npart_init, used, ev);
+  }
   if (sh.xv == 8)
     return persist_launch<T, 8, 2, 1>(h, s, tg, sh, L, r0, rh, x, npart_init, used, ev);
   if constexpr (sizeof(T) == 4) {

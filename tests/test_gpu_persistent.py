@@ -11,8 +11,9 @@ iterates within a few iterations, so bit identity over a few hundred
 iterations is the test of the synchronisation.
 
 Shapes: 128^3 (the headline box: 8-plane chunks in FP64, two virtual blocks
-per kernel block; 4-plane chunks in FP32, four) and 128 x 128 x 64 (4-plane
-chunks in FP64).  The pressure update p += DP (pressure.py:293-294) and the
+per kernel block; 4-plane chunks in FP32, two per block at two blocks per
+CU), 128 x 128 x 64 (4-plane chunks in FP64, four per block) and 64^3
+(4-plane chunks, one per block: 128 blocks).  The pressure update p += DP (pressure.py:293-294) and the
 velocity projection are covered through Domain.velCorr.
 """
 
@@ -29,7 +30,9 @@ pytestmark = pytest.mark.gpu
 RHO, DT = 1.0, 1e-3
 CASES = [((128, 128, 128), torch.float64, 1e-10),
          ((128, 128, 128), torch.float32, 1e-6),
-         ((128, 128, 64), torch.float64, 1e-10)]
+         ((128, 128, 64), torch.float64, 1e-10),
+         ((64, 64, 64), torch.float64, 1e-10),
+         ((64, 64, 64), torch.float32, 1e-6)]
 
 
 def source(shape, seed=5):
@@ -51,7 +54,7 @@ def solve(box, shape, dtype, tol, persistent, hist_cap=4096):
 
 
 @pytest.mark.parametrize('shape,dtype,tol', CASES,
-                         ids=['128c-f64', '128c-f32', '128x128x64-f64'])
+                         ids=['128c-f64', '128c-f32', '128x128x64-f64', '64c-f64', '64c-f32'])
 def test_persistent_solve_bit_identical(shape, dtype, tol):
     from pyflow_amd import _native
     L = (2 * math.pi, 2 * math.pi, 2 * math.pi * shape[2] / shape[0])
