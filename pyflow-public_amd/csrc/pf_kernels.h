@@ -127,16 +127,27 @@ inline TileGrid tile_grid_with(const Geom& g, int tz, int pair, int rows, int mi
 
 // Preferred: two z-neighbour cells per thread (when the tiles cover z
 // exactly) and chunks of at least 64 bytes per column (8 planes in FP64, 4
-// in FP32) - measured best at 128^3.  Grids that would get fewer than 512
-// blocks that way (below two per CU) take 4-plane chunks, then one cell per
-// thread - except 64 x 8 tiles on 4-plane chunks that cover the box exactly
-// (64^3: 128 blocks), which the persistent solve holds on chip.
+// in FP32) - measured best at 128^3 for the launch-per-reduction kernels.
+// Grids that would get fewer than 512 blocks that way (below two per CU)
+// take 4-plane chunks, then one cell per thread - except 64 x 8 tiles on
+// 4-plane chunks that cover the box exactly (64^3: 128 blocks), which the
+// persistent solve holds on chip.  FP64 boxes whose 64 x 8 tiles on
+// 16-plane chunks cover them exactly in 128 to 256 tiles (128^3: 256) take
+// those: the persistent solve then holds one chunk per block on at least
+// half the CUs, with half the partial-sum slots of 8-plane chunks per grid
+// sync (128^3 FP64: 28.5 -> 27.1 us per iteration; its launch kernels, the
+// fallback, 139.8 -> 135.5 Mcell-steps/s).
 inline TileGrid make_tile_grid(const Geom& g, int esize, int rows = 1) {
   const int tz = tile_extent_z(g.nz);
   const bool can_pair = rows == 1 && tz >= 16 && g.nz % tz == 0;
   const int minp = esize >= 8 ? 8 : 4;
   if (can_pair) {
-    TileGrid t = tile_grid_with(g, tz, 2, rows, minp);
+    TileGrid t;
+    if (esize >= 8 && tz == 64 && g.nx % 16 == 0 && g.ny % 8 == 0) {
+      t = tile_grid_with(g, tz, 2, rows, 16);
+      if (t.xc == 16 && t.nblk >= 128 && t.nblk <= 256) return t;
+    }
+    t = tile_grid_with(g, tz, 2, rows, minp);
     if (t.nblk >= 512) return t;
     if (minp > 4) {
       t = tile_grid_with(g, tz, 2, rows, 4);

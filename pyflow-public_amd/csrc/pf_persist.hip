@@ -733,7 +733,7 @@ struct PersistShape {
 
 bool persist_shape(const pf_box* h, const TileGrid& tg, PersistShape& sh) {
   if (tg.tz != kPZ || tg.ty != kPY) return false;
-  if (!(tg.xc == 4 || tg.xc == 8)) return false;
+  if (!(tg.xc == 4 || tg.xc == 8 || (tg.xc == 16 && h->esize == 8))) return false;
   if (h->g.nx % tg.xc || h->g.ny % kPY || h->g.nz % kPZ) return false;
   if (tg.nblk > 1024) return false;
   sh.xv = tg.xc;
@@ -742,10 +742,17 @@ bool persist_shape(const pf_box* h, const TileGrid& tg, PersistShape& sh) {
   // per thread, so that one block's waits are the other's compute (128^3
   // FP32: 27.1 -> 23.3 us per iteration)
   sh.v = (h->esize == 4 && tg.xc == 4) ? 2 : 16 / tg.xc;
-  // grids of at most 256 tiles (64^3: 128): one chunk per block, so that
-  // the solve spreads over as many CUs as the grid has tiles
+  // grids of at most 256 tiles (128^3 FP64 on 16-plane chunks: 256; 64^3:
+  // 128): one chunk per block, so that the solve spreads over as many CUs
+  // as the grid has tiles
   if (tg.nblk <= 256) sh.v = 1;
   if (tg.ncx % sh.v) return false;
+  // the kernels built: one chunk per block of 16 planes (FP64) or 4; two
+  // 8-plane chunks per block; two 4-plane chunks per block at two waves per
+  // SIMD (FP32)
+  const bool built = sh.v == 1 ? (sh.xv == 4 || (sh.xv == 16 && h->esize == 8))
+                               : sh.v == 2 && (sh.xv == 8 || (sh.xv == 4 && h->esize == 4));
+  if (!built) return false;
   sh.nphys = tg.nblk / sh.v;
   const int xc = sh.v * sh.xv;
   sh.face_elems = xc == 16 ? FaceMap<16>::size : xc == 8 ? FaceMap<8>::size : FaceMap<4>::size;
@@ -833,19 +840,20 @@ int persist_solve(pf_box* h, hipStream_t s, const TileGrid& tg, const Lap<T>& L,
   PersistShape sh;
   if (!persist_shape(h, tg, sh)) return 0;
   if (sh.v == 1) {
-    if (sh.xv == 8)
-      return persist_launch<T, 8, 1, 1>(h, s, tg, sh, L, r0, rh, x, npart_init, used, ev);
-    return persist_launch<T, 4, 1, 1>(h, s, tg, sh, L, r0, rh, x, npart_init, used, ev);
+    // one chunk per block: 16 planes (FP64 on 16-plane chunks) or 4
+    if constexpr (sizeof(T) == 8)
+      if (sh.xv == 16)
+        return persist_launch<T, 16, 1, 1>(h, s, tg, sh, L, r0, rh, x, npart_init, used, ev);
+    if (sh.xv == 4)
+      return persist_launch<T, 4, 1, 1>(h, s, tg, sh, L, r0, rh, x, npart_init, used, ev);
+    return 0;
   }
   if (sh.xv == 8)
     return persist_launch<T, 8, 2, 1>(h, s, tg, sh, L, r0, rh, x, npart_init, used, ev);
-  if constexpr (sizeof(T) == 4) {
-    if (sh.v != 2) return 0;
-    return persist_launch<T, 4, 2, 2>(h, s, tg, sh, L, r0, rh, x, npart_init, used, ev);
-  } else {
-    if (sh.v != 4) return 0;
-    return persist_launch<T, 4, 4, 1>(h, s, tg, sh, L, r0, rh, x, npart_init, used, ev);
-  }
+  if constexpr (sizeof(T) == 4)
+    if (sh.v == 2)
+      return persist_launch<T, 4, 2, 2>(h, s, tg, sh, L, r0, rh, x, npart_init, used, ev);
+  return 0;   // other shapes: the launch-per-reduction kernels
 }
 
 template int persist_solve<double>(pf_box*, hipStream_t, const TileGrid&, const Lap<double>&,

@@ -1,9 +1,9 @@
 """The tiled Krylov kernels on the decompositions the other solver tests
 do not reach:
 
-* 128 x 128 x 64: two z-neighbour cells per thread (64 x 8 tiles), taken
-  when a box gives at least 512 blocks that way, here with x chunks of 4
-  planes - the one-plane-ahead load schedule of short chunks;
+* 128 x 128 x 64: two z-neighbour cells per thread (64 x 8 tiles), here
+  with x chunks of 16 planes in FP64 (128 tiles) and 4 in FP32 (512) - the
+  one-plane-ahead load schedule of short chunks;
 * 136 x 1024 x 64: the same tiles on chunks of 17 planes - the rotating
   load schedule and non-temporal stores of long chunks (512^3 takes it);
 * 85 x 1024 x 45: one cell per thread (45 is no multiple of the 16-wide

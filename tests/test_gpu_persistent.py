@@ -10,10 +10,10 @@ stale face or partial (a sync that lets a block run ahead) would change the
 iterates within a few iterations, so bit identity over a few hundred
 iterations is the test of the synchronisation.
 
-Shapes: 128^3 (the headline box: 8-plane chunks in FP64, two virtual blocks
-per kernel block; 4-plane chunks in FP32, two per block at two blocks per
-CU), 128 x 128 x 64 (4-plane chunks in FP64, four per block) and 64^3
-(4-plane chunks, one per block: 128 blocks).  The pressure update p += DP (pressure.py:293-294) and the
+Shapes: 128^3 (the headline box: 16-plane chunks in FP64, one per kernel
+block, 256 blocks; 4-plane chunks in FP32, two per block at two blocks per
+CU), 128 x 128 x 64 (16-plane chunks in FP64, 128 blocks) and 64^3 (4-plane
+chunks, one per block: 128 blocks).  The pressure update p += DP (pressure.py:293-294) and the
 velocity projection are covered through Domain.velCorr.
 """
 
