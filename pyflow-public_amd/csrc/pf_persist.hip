@@ -49,6 +49,11 @@ namespace {
 
 constexpr unsigned long long kSlotEmpty = 0x7FF4DEADBEEF0123ull;
 constexpr int kSets = 4;              // slot sets in rotation (sync j uses set j % 4)
+// Replicas of each slot set: a block writes its partials into every replica
+// and polls replica (b mod 8) mod kReps, so that each replica's cache lines
+// carry a quarter of the polls (a bare all-to-all round at 256 blocks: 3.45
+// us with one replica, 2.43 with four; 512 blocks x 2 slots: 4.84 -> 4.29)
+constexpr int kReps = 4;
 constexpr int kSpin = 1 << 21;        // polls before a block gives up
 constexpr int kPT = 256;              // threads per block
 constexpr int kPZ = 64, kPY = 8;      // tile extents (z, y): two z cells per thread
@@ -147,7 +152,7 @@ struct PersistArgs {
   const T* rh;
   T* x;                      // solution out
   T* faces;                  // [5][blocks][face elems]: p, v (even / odd iterations), r
-  double* slots;             // [kSets][virtual blocks][2], kSlotEmpty where not yet written
+  double* slots;             // [kSets][kReps][virtual blocks][2], kSlotEmpty where not yet written
   unsigned* flag;            // [0] abort
   Lap<T> L;
 };
@@ -168,14 +173,16 @@ __device__ __forceinline__ bool grid_sync(double* __restrict__ slots, unsigned* 
                                           const double (&mine)[V * NV], double (&red)[NV],
                                           const int (&op)[NV], const int (&nbv)[6],
                                           Early&& early) {
-  double* set = slots + (size_t)(j % kSets) * nblk * 2;
+  const size_t stride = (size_t)nblk * 2;   // one replica of a slot set
+  double* set = slots + ((size_t)(j % kSets) * kReps + (blockIdx.x & 7) % kReps) * stride;
   wait_mem();        // this thread's face stores (and loads) are complete
   __syncthreads();   // ... and every other thread's of the block
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < kReps) {   // thread r writes replica r
+    double* ws = slots + ((size_t)(j % kSets) * kReps + threadIdx.x) * stride;
 #pragma unroll
     for (int v = 0; v < V; ++v)
 #pragma unroll
-      for (int q = 0; q < NV; ++q) st_dev(set + 2 * vid[v] + q, mine[v * NV + q]);
+      for (int q = 0; q < NV; ++q) st_dev(ws + 2 * vid[v] + q, mine[v * NV + q]);
   }
   constexpr int MB = 4;   // up to 1024 virtual blocks
   double got[MB][NV];
@@ -264,8 +271,8 @@ __device__ __forceinline__ bool grid_sync(double* __restrict__ slots, unsigned* 
   block_reduce<NV, kPT>(red, op);
   // every block has now published sync j, so all have read sync j-1: its
   // slots can be emptied for reuse at sync j + kSets - 1
-  if (threadIdx.x == 0 && j > 0) {
-    double* prev = slots + (size_t)((j - 1) % kSets) * nblk * 2;
+  if (threadIdx.x < kReps && j > 0) {
+    double* prev = slots + ((size_t)((j - 1) % kSets) * kReps + threadIdx.x) * stride;
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       st_dev(reinterpret_cast<unsigned long long*>(prev + 2 * vid[v]), kSlotEmpty);
@@ -760,7 +767,8 @@ bool persist_shape(const pf_box* h, const TileGrid& tg, PersistShape& sh) {
 }
 
 size_t shape_bytes(const PersistShape& sh, size_t esize, int nblk) {
-  return 5 * (size_t)sh.nphys * sh.face_elems * esize + sizeof(double) * kSets * 2 * nblk + 256;
+  return 5 * (size_t)sh.nphys * sh.face_elems * esize + sizeof(double) * kSets * kReps * 2 * nblk +
+         256;
 }
 
 }  // namespace
@@ -778,7 +786,7 @@ namespace {
 
 unsigned* flag_ptr(pf_box* h, const TileGrid& tg, const PersistShape& sh) {
   const char* slots = (const char*)h->pers + 5 * (size_t)sh.nphys * sh.face_elems * h->esize;
-  return (unsigned*)(slots + sizeof(double) * kSets * 2 * tg.nblk);
+  return (unsigned*)(slots + sizeof(double) * kSets * kReps * 2 * tg.nblk);
 }
 
 template <typename T, int XV, int V, int WPE>
@@ -813,7 +821,7 @@ int persist_launch(pf_box* h, hipStream_t s, const TileGrid& tg, const PersistSh
   a.slots = (double*)((char*)h->pers + 5 * (size_t)sh.nphys * sh.face_elems * sizeof(T));
   a.flag = flag_ptr(h, tg, sh);
   a.L = L;
-  const int ns = kSets * 2 * tg.nblk;
+  const int ns = kSets * kReps * 2 * tg.nblk;
   hipLaunchKernelGGL(persist_reset_kernel, dim3((ns + 255) / 256), dim3(256), 0, s, a.slots, ns,
                      a.flag);
   PF_LAUNCH_CHECK("persist_reset_kernel");
