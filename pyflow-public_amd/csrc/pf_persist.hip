@@ -496,7 +496,13 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
   int iter = 0, j = 0;
   int status = PF_SOLVE_CONVERGED;
   // halo values fetched inside the syncs: p, v, r for the next KA (during
-  // the sync after KC), v and r for KB (during the sync after KA)
+  // the sync after KC), v and r for KB (during the sync after KA).
+  // KEEP (FP32, whose halos fit beside its state in registers): a block keeps
+  // the p halo it formed in KA (the neighbours' p faces, bit for bit), the v
+  // halo of the sync after KA and the r halo of the sync after KC for the
+  // next use instead of fetching them again, so the sync after KA fetches v
+  // only, the sync after KC r only, and no p face is published
+  constexpr bool KEEP = sizeof(T) == 4;
   Halo hPo, hVo, hRa, hVb, hRb;
   {
     double red[2];
@@ -553,6 +559,7 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
 #pragma unroll
         for (int u = 0; u < NY; ++u) H.y[u] = pupd(zero2, zero2, R0.y[u]);
         H.z = (fmx((T)0, c1, (T)0) * c2) + R0.z;
+        if constexpr (KEEP) hRa = R0;
       } else {
 #pragma unroll
         for (int x = 0; x < XC; ++x) P[x] = pupd(P[x], V_[x], R[x]);
@@ -562,7 +569,10 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
         for (int u = 0; u < NY; ++u) H.y[u] = pupd(hPo.y[u], hVo.y[u], hRa.y[u]);
         H.z = (fmx(hVo.z, c1, hPo.z) * c2) + hRa.z;
       }
-      publish(fpar[par], [&](int x) -> Pr { return P[x]; });
+      if constexpr (KEEP)
+        hPo = H;
+      else
+        publish(fpar[par], [&](int x) -> Pr { return P[x]; });
       double acc[V];
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] = 0.0;
@@ -586,10 +596,12 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
       const bool first = iter == 0;
       if (!grid_sync<1, V, sizeof(T) == 4>(A.slots, A.flag, nblk, vid, j++, acc, gam, ops, nbv, [&]() {
             fetch(fvar[par], hVb);
-            if (first)
-              fetch_r0(hRb);   // no r face has been published yet
-            else
-              fetch(fr, hRb);
+            if constexpr (!KEEP) {
+              if (first)
+                fetch_r0(hRb);   // no r face has been published yet
+              else
+                fetch(fr, hRb);
+            }
           }))
         return;
       if (gam[0] == 0.0) {                                    // pressure.py:247-248
@@ -607,13 +619,15 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
     };
     {
       Halo S;
+      const Halo& hR = KEEP ? hRa : hRb;
 #pragma unroll
       for (int s = 0; s < 2; ++s)
-        S.x[s] = Pr{fmx(hVb.x[s].a, na, hRb.x[s].a), fmx(hVb.x[s].b, na, hRb.x[s].b)};
+        S.x[s] = Pr{fmx(hVb.x[s].a, na, hR.x[s].a), fmx(hVb.x[s].b, na, hR.x[s].b)};
 #pragma unroll
       for (int u = 0; u < NY; ++u)
-        S.y[u] = Pr{fmx(hVb.y[u].a, na, hRb.y[u].a), fmx(hVb.y[u].b, na, hRb.y[u].b)};
-      S.z = fmx(hVb.z, na, hRb.z);
+        S.y[u] = Pr{fmx(hVb.y[u].a, na, hR.y[u].a), fmx(hVb.y[u].b, na, hR.y[u].b)};
+      S.z = fmx(hVb.z, na, hR.z);
+      if constexpr (KEEP) hVo = hVb;
       double ab[2 * V];
 #pragma unroll
       for (int q = 0; q < 2 * V; ++q) ab[q] = 0.0;
@@ -691,8 +705,10 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
       }
       const int pn = par;   // the faces the next KA reads: this iteration's p and v
       if (!grid_sync<2, V, sizeof(T) == 4>(A.slots, A.flag, nblk, vid, j++, cc, rc, opc, nbv, [&]() {
-            fetch(fpar[pn], hPo);
-            fetch(fvar[pn], hVo);
+            if constexpr (!KEEP) {
+              fetch(fpar[pn], hPo);
+              fetch(fvar[pn], hVo);
+            }
             fetch(fr, hRa);
           }))
         return;
