@@ -151,7 +151,8 @@ struct PersistArgs {
   const T* r0;               // r = r_hat = b (interior layout)
   const T* rh;
   T* x;                      // solution out
-  T* faces;                  // [5][blocks][face elems]: p, v (even / odd iterations), r
+  T* faces;                  // FP64: [5][blocks][face elems]: p, v (even / odd iterations), r;
+                             // FP32: [2][blocks][face elems]: v, r
   double* slots;             // [kSets][kReps][virtual blocks][2], kSlotEmpty where not yet written
   unsigned* flag;            // [0] abort
   Lap<T> L;
@@ -501,7 +502,9 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
   // the p halo it formed in KA (the neighbours' p faces, bit for bit), the v
   // halo of the sync after KA and the r halo of the sync after KC for the
   // next use instead of fetching them again, so the sync after KA fetches v
-  // only, the sync after KC r only, and no p face is published
+  // only, the sync after KC r only, and no p face is published.  (FP64 keeping
+  // them in an L2-resident scratch slice per thread instead ran slower: 27.1
+  // -> 27.8 us per iteration at 128^3.)
   constexpr bool KEEP = sizeof(T) == 4;
   Halo hPo, hVo, hRa, hVb, hRb;
   {
@@ -512,9 +515,14 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
     res0 = red[1];
     if (lead) A.ctl->res0 = res0;
   }
+  // FP64: p and v faces double-buffered (a neighbour may still read
+  // iteration i's during the sync after KC while this block writes i+1's);
+  // with KEEP, v faces are read in the sync after KA only, which every block
+  // has finished before the next KA's writes, and no p faces exist
   T* const fpar[2] = {A.faces, A.faces + FB};            // p faces, even / odd iterations
   T* const fvar[2] = {A.faces + 2 * FB, A.faces + 3 * FB};
-  T* const fr = A.faces + 4 * FB;
+  T* const fv = A.faces;                                 // KEEP: v faces
+  T* const fr = A.faces + (KEEP ? 1 : 4) * FB;
 
   for (;;) {
     // keep per-thread index arithmetic inside the loop instead of holding
@@ -559,7 +567,7 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
 #pragma unroll
         for (int u = 0; u < NY; ++u) H.y[u] = pupd(zero2, zero2, R0.y[u]);
         H.z = (fmx((T)0, c1, (T)0) * c2) + R0.z;
-        if constexpr (KEEP) hRa = R0;
+        hRa = R0;
       } else {
 #pragma unroll
         for (int x = 0; x < XC; ++x) P[x] = pupd(P[x], V_[x], R[x]);
@@ -584,7 +592,10 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
 #pragma unroll
         for (int v = 0; v < V; ++v) pin(acc[v]);
       });
-      publish(fvar[par], [&](int x) -> Pr { return V_[x]; });
+      if constexpr (KEEP)
+        publish(fv, [&](int x) -> Pr { return V_[x]; });
+      else
+        publish(fvar[par], [&](int x) -> Pr { return V_[x]; });
       double gam[1];
       const int ops[1] = {0};
       {
@@ -595,8 +606,10 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
       }
       const bool first = iter == 0;
       if (!grid_sync<1, V, sizeof(T) == 4>(A.slots, A.flag, nblk, vid, j++, acc, gam, ops, nbv, [&]() {
-            fetch(fvar[par], hVb);
-            if constexpr (!KEEP) {
+            if constexpr (KEEP) {
+              fetch(fv, hVb);
+            } else {
+              fetch(fvar[par], hVb);
               if (first)
                 fetch_r0(hRb);   // no r face has been published yet
               else
@@ -782,9 +795,14 @@ bool persist_shape(const pf_box* h, const TileGrid& tg, PersistShape& sh) {
   return true;
 }
 
+// Persistent-solve buffers of a box: [faces][partial-sum slots][abort flag];
+// face fields: FP64 p, v (two each), r; FP32 v, r (it keeps its p halo)
+size_t faces_bytes(const PersistShape& sh, size_t esize) {
+  return (esize == 8 ? 5 : 2) * (size_t)sh.nphys * sh.face_elems * esize;
+}
+
 size_t shape_bytes(const PersistShape& sh, size_t esize, int nblk) {
-  return 5 * (size_t)sh.nphys * sh.face_elems * esize + sizeof(double) * kSets * kReps * 2 * nblk +
-         256;
+  return faces_bytes(sh, esize) + sizeof(double) * kSets * kReps * 2 * nblk + 256;
 }
 
 }  // namespace
@@ -801,7 +819,7 @@ size_t persist_bytes(const pf_box* h) {
 namespace {
 
 unsigned* flag_ptr(pf_box* h, const TileGrid& tg, const PersistShape& sh) {
-  const char* slots = (const char*)h->pers + 5 * (size_t)sh.nphys * sh.face_elems * h->esize;
+  const char* slots = (const char*)h->pers + faces_bytes(sh, h->esize);
   return (unsigned*)(slots + sizeof(double) * kSets * kReps * 2 * tg.nblk);
 }
 
@@ -834,7 +852,7 @@ int persist_launch(pf_box* h, hipStream_t s, const TileGrid& tg, const PersistSh
   a.rh = rh;
   a.x = x;
   a.faces = (T*)h->pers;
-  a.slots = (double*)((char*)h->pers + 5 * (size_t)sh.nphys * sh.face_elems * sizeof(T));
+  a.slots = (double*)((char*)h->pers + faces_bytes(sh, sizeof(T)));
   a.flag = flag_ptr(h, tg, sh);
   a.L = L;
   const int ns = kSets * kReps * 2 * tg.nblk;
