@@ -497,16 +497,17 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
   int iter = 0, j = 0;
   int status = PF_SOLVE_CONVERGED;
   // halo values fetched inside the syncs: p, v, r for the next KA (during
-  // the sync after KC), v and r for KB (during the sync after KA).
-  // KEEP (FP32, whose halos fit beside its state in registers): a block keeps
-  // the p halo it formed in KA (the neighbours' p faces, bit for bit), the v
-  // halo of the sync after KA and the r halo of the sync after KC for the
-  // next use instead of fetching them again, so the sync after KA fetches v
-  // only, the sync after KC r only, and no p face is published.  (FP64 keeping
-  // them in an L2-resident scratch slice per thread instead ran slower: 27.1
-  // -> 27.8 us per iteration at 128^3.)
+  // the sync after KC), v for KB (during the sync after KA).
+  // Every block keeps the r halo fetched during the sync after KC for KB's
+  // s halo, so the sync after KA fetches v faces only.  KEEP (FP32, whose
+  // halos fit beside its state in registers): a block also keeps the p halo
+  // it formed in KA (the neighbours' p faces, bit for bit) and the v halo
+  // of the sync after KA for the next KA instead of fetching them again, so
+  // the sync after KC fetches r faces only and no p face is published.
+  // (FP64 keeping those two in an L2-resident scratch slice per thread
+  // instead ran slower: 27.1 -> 27.8 us per iteration at 128^3.)
   constexpr bool KEEP = sizeof(T) == 4;
-  Halo hPo, hVo, hRa, hVb, hRb;
+  Halo hPo, hVo, hRa, hVb;
   {
     double red[2];
     const int ops[2] = {0, 1};
@@ -604,16 +605,11 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
         for (int v = 0; v < V; ++v) opv[v] = 0;
         block_reduce<V, kPT>(acc, opv);
       }
-      const bool first = iter == 0;
       if (!grid_sync<1, V, sizeof(T) == 4>(A.slots, A.flag, nblk, vid, j++, acc, gam, ops, nbv, [&]() {
             if constexpr (KEEP) {
               fetch(fv, hVb);
             } else {
               fetch(fvar[par], hVb);
-              if (first)
-                fetch_r0(hRb);   // no r face has been published yet
-              else
-                fetch(fr, hRb);
             }
           }))
         return;
@@ -632,7 +628,7 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
     };
     {
       Halo S;
-      const Halo& hR = KEEP ? hRa : hRb;
+      const Halo& hR = hRa;
 #pragma unroll
       for (int s = 0; s < 2; ++s)
         S.x[s] = Pr{fmx(hVb.x[s].a, na, hR.x[s].a), fmx(hVb.x[s].b, na, hR.x[s].b)};
