@@ -13,10 +13,11 @@
 //     per thread (two z cells each) at one wave per SIMD, so that its state
 //     fits the 512 VGPRs a lone wave may hold; in FP32 8 planes per thread at
 //     two waves per SIMD;
-//   * every block publishes the faces of p (after the p update), of v (after
-//     v = A p) and of r (after r = s - omega t) that its neighbours' stencils
-//     read; a block computes p and s on its halo from the neighbours' faces
-//     with the same operations as their owners, so the same bits;
+//   * every block publishes the faces of v (after v = A p) and of r (after
+//     r = s - omega t) that its neighbours read; a block computes p and s on
+//     its halo from those faces and the p halo it formed itself one
+//     iteration before, with the same operations as their owners, so the
+//     same bits;
 //   * a grid sync is an all-to-all of partial sums: every block writes its
 //     virtual blocks' partials into their slots of the sync's slot set
 //     (initially an empty marker, a signalling-NaN bit pattern that no
@@ -151,8 +152,7 @@ struct PersistArgs {
   const T* r0;               // r = r_hat = b (interior layout)
   const T* rh;
   T* x;                      // solution out
-  T* faces;                  // FP64: [5][blocks][face elems]: p, v (even / odd iterations), r;
-                             // FP32: [2][blocks][face elems]: v, r
+  T* faces;                  // [2][blocks][face elems]: v, r
   double* slots;             // [kSets][kReps][virtual blocks][2], kSlotEmpty where not yet written
   unsigned* flag;            // [0] abort
   Lap<T> L;
@@ -496,17 +496,17 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
   double rho1, rho2 = c0.rho2, alpha = c0.alpha, omega = c0.omega, res0, maxr = 0.0;
   int iter = 0, j = 0;
   int status = PF_SOLVE_CONVERGED;
-  // halo values fetched inside the syncs: p, v, r for the next KA (during
-  // the sync after KC), v for KB (during the sync after KA).
-  // Every block keeps the r halo fetched during the sync after KC for KB's
-  // s halo, so the sync after KA fetches v faces only.  KEEP (FP32, whose
-  // halos fit beside its state in registers): a block also keeps the p halo
-  // it formed in KA (the neighbours' p faces, bit for bit) and the v halo
-  // of the sync after KA for the next KA instead of fetching them again, so
-  // the sync after KC fetches r faces only and no p face is published.
-  // (FP64 keeping those two in an L2-resident scratch slice per thread
-  // instead ran slower: 27.1 -> 27.8 us per iteration at 128^3.)
-  constexpr bool KEEP = sizeof(T) == 4;
+  // Halos.  A block keeps the p halo it formed in KA (the neighbours' p
+  // faces, bit for bit: the same operations on the same values) and the v
+  // halo fetched during the sync after KA for the next KA, and the r halo
+  // fetched during the sync after KC for KB's s halo, instead of fetching any
+  // of them again: each sync fetches one face field (v after KA, r after KC)
+  // and no p face is published.  hPo, hVo: p and v of the previous
+  // iteration's KA on the halo; hRa: r on the halo; hVb: v on the halo for KB.
+  // (Fetching p, v and r after KC and v and r after KA, with p and v faces
+  // double-buffered: 128^3 FP64 26.5 us per iteration, FP32 21.9; keeping
+  // FP64's halos in an L2-resident scratch slice per thread instead of
+  // registers: 27.8.)
   Halo hPo, hVo, hRa, hVb;
   {
     double red[2];
@@ -516,14 +516,12 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
     res0 = red[1];
     if (lead) A.ctl->res0 = res0;
   }
-  // FP64: p and v faces double-buffered (a neighbour may still read
-  // iteration i's during the sync after KC while this block writes i+1's);
-  // with KEEP, v faces are read in the sync after KA only, which every block
-  // has finished before the next KA's writes, and no p faces exist
-  T* const fpar[2] = {A.faces, A.faces + FB};            // p faces, even / odd iterations
-  T* const fvar[2] = {A.faces + 2 * FB, A.faces + 3 * FB};
-  T* const fv = A.faces;                                 // KEEP: v faces
-  T* const fr = A.faces + (KEEP ? 1 : 4) * FB;
+  // v faces are read in the sync after KA only, which every block has
+  // finished before any block passes the sync after KB, so before the next
+  // KA writes them; r faces are read in the sync after KC only, finished
+  // before the next KC writes them
+  T* const fv = A.faces;
+  T* const fr = A.faces + FB;
 
   for (;;) {
     // keep per-thread index arithmetic inside the loop instead of holding
@@ -549,7 +547,6 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
     }
     const double beta = alpha * rho1 / (rho2 * omega);     // pressure.py:233
     if (lead) A.ctl->rho1 = rho1;
-    const int par = iter & 1;
     {
       const T c1 = (T)(-omega), c2 = (T)beta;
       auto pupd = [&](const Pr& pp, const Pr& vv, const Pr& rr) -> Pr {
@@ -578,10 +575,7 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
         for (int u = 0; u < NY; ++u) H.y[u] = pupd(hPo.y[u], hVo.y[u], hRa.y[u]);
         H.z = (fmx(hVo.z, c1, hPo.z) * c2) + hRa.z;
       }
-      if constexpr (KEEP)
-        hPo = H;
-      else
-        publish(fpar[par], [&](int x) -> Pr { return P[x]; });
+      hPo = H;
       double acc[V];
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] = 0.0;
@@ -593,10 +587,7 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
 #pragma unroll
         for (int v = 0; v < V; ++v) pin(acc[v]);
       });
-      if constexpr (KEEP)
-        publish(fv, [&](int x) -> Pr { return V_[x]; });
-      else
-        publish(fvar[par], [&](int x) -> Pr { return V_[x]; });
+      publish(fv, [&](int x) -> Pr { return V_[x]; });
       double gam[1];
       const int ops[1] = {0};
       {
@@ -606,11 +597,7 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
         block_reduce<V, kPT>(acc, opv);
       }
       if (!grid_sync<1, V, sizeof(T) == 4>(A.slots, A.flag, nblk, vid, j++, acc, gam, ops, nbv, [&]() {
-            if constexpr (KEEP) {
-              fetch(fv, hVb);
-            } else {
-              fetch(fvar[par], hVb);
-            }
+            fetch(fv, hVb);
           }))
         return;
       if (gam[0] == 0.0) {                                    // pressure.py:247-248
@@ -636,7 +623,7 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
       for (int u = 0; u < NY; ++u)
         S.y[u] = Pr{fmx(hVb.y[u].a, na, hR.y[u].a), fmx(hVb.y[u].b, na, hR.y[u].b)};
       S.z = fmx(hVb.z, na, hR.z);
-      if constexpr (KEEP) hVo = hVb;
+      hVo = hVb;
       double ab[2 * V];
 #pragma unroll
       for (int q = 0; q < 2 * V; ++q) ab[q] = 0.0;
@@ -712,12 +699,7 @@ __global__ void __launch_bounds__(kPT, WPE) bicg_persist_kernel(PersistArgs<T> A
         for (int q = 0; q < 2 * V; ++q) opv[q] = q & 1;
         block_reduce<2 * V, kPT>(cc, opv);
       }
-      const int pn = par;   // the faces the next KA reads: this iteration's p and v
       if (!grid_sync<2, V, sizeof(T) == 4>(A.slots, A.flag, nblk, vid, j++, cc, rc, opc, nbv, [&]() {
-            if constexpr (!KEEP) {
-              fetch(fpar[pn], hPo);
-              fetch(fvar[pn], hVo);
-            }
             fetch(fr, hRa);
           }))
         return;
@@ -791,10 +773,9 @@ bool persist_shape(const pf_box* h, const TileGrid& tg, PersistShape& sh) {
   return true;
 }
 
-// Persistent-solve buffers of a box: [faces][partial-sum slots][abort flag];
-// face fields: FP64 p, v (two each), r; FP32 v, r (it keeps its p halo)
+// Persistent-solve buffers of a box: [faces: v, r][partial-sum slots][abort flag]
 size_t faces_bytes(const PersistShape& sh, size_t esize) {
-  return (esize == 8 ? 5 : 2) * (size_t)sh.nphys * sh.face_elems * esize;
+  return 2 * (size_t)sh.nphys * sh.face_elems * esize;
 }
 
 size_t shape_bytes(const PersistShape& sh, size_t esize, int nblk) {
