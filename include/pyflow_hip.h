@@ -592,6 +592,13 @@ int pf_closure_vjp(pf_box* box, int kind, const void* u, const void* v, const vo
  *     that cache from one kernel to the next.  Only the order of independent
  *     boxes' launches changes, never a result.  Synchronous; re-captures the
  *     graph.
+ *   pf_batch_set_persistent: on (the default), a resident box whose
+ *     persistent solve fills the GPU (pf_box_set_persistent; 128^3 FP32 and
+ *     FP64) solves each step's Poisson problem as that one cooperative
+ *     launch when the step starts, instead of on the chunk kernels (same
+ *     iterates, same bits); off: every box runs on the chunk kernels.
+ *     Returns 1 if a resident box now takes the persistent solve, else 0.
+ *     Synchronous; re-captures the graph.
  */
 typedef struct pf_batch pf_batch;
 int pf_batch_create(pf_batch** out, int capacity, int dtype);
@@ -602,6 +609,7 @@ int pf_batch_advance(pf_batch* batch, const int* start, int* finished, double* o
                      int* out_chunks, void* stream);
 int pf_batch_chunk_iterations(void);
 int pf_batch_set_group_bytes(pf_batch* batch, int64_t bytes);
+int pf_batch_set_persistent(pf_batch* batch, int on);
 
 /*
  * Measurement support (no counterpart in the reference): with period > 0,

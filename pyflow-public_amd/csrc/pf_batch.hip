@@ -22,6 +22,14 @@
 // does not depend on iteration counts.  Steps start at chunk boundaries,
 // which keeps every box's iteration parity aligned with the graph's.
 //
+// A box whose persistent solve fills the GPU (pf_persist.hip: 128^3 FP32
+// takes 512 blocks) solves its Poisson problem as that one cooperative
+// launch when its step starts, right after the batched Krylov start, in
+// place of its share of the chunk kernels: its control block then says done
+// and the next chunk boundary finishes its step.  The persistent solve's
+// iterates are the chunk kernels' bits, so results stay bit-identical.  A
+// box whose cooperative launch cannot start runs on the chunk kernels.
+//
 // Inside a chunk the boxes run in cache-residency groups: boxes are packed,
 // by tile shape and then size, into groups whose Krylov vectors together fit
 // a byte budget sized for the Infinity Cache (MALL), and each group runs all
@@ -49,6 +57,10 @@ constexpr int kChunk = 16;    // BiCGStab iterations per graph replay (even: buf
 // 192 MiB 260, 256 MiB 268, 320 MiB 272, 384 MiB 258 and 512 MiB 256
 // Mcell-steps/s.
 constexpr size_t kGroupBytes = (size_t)320 << 20;
+// Boxes whose persistent solve takes at least this many blocks (every CU
+// busy) solve persistently in a batch; smaller ones gain more from sharing
+// the chunk kernels with the other boxes (64^3 FP32 alone: 128 blocks).
+constexpr int kPersistMinBlocks = 256;
 
 struct List {
   std::vector<int> box, begin;
@@ -87,14 +99,16 @@ struct pf_batch {
   void* desc;                 // device BoxDesc<T>[capacity]
   std::vector<pf_box*> boxes;
   std::vector<char> running;  // per box: inside a time step
+  std::vector<char> pers;     // per box: solves with the persistent kernel
+  int persist_on;             // big boxes take the persistent solve (default 1)
   double* out_dev;            // 8 doubles per box
   double* out_host;           // pinned
   int* flags_dev;             // done flag per box
   int* flags_host;            // pinned
   hipStream_t st;
   hipEvent_t ev;
-  Lists full, pre, post;
-  int* list_mem;              // device lists: full, then pre, then post regions
+  Lists full, pre, post, ka;
+  int* list_mem;              // device lists: full, pre, post, groups, first-KA regions
   int* list_stage;            // pinned staging of the three regions
   size_t list_cap;            // ints per region
   hipGraphExec_t g_chunk;
@@ -232,16 +246,21 @@ int build_groups(pf_batch* b) {
     const TileGrid tg = make_tile_grid(h->g, (int)h->esize, h->tile_rows);
     return tg.tz * 1000 + tg.ty;
   };
+  // boxes that solve persistently go last, in groups of their own: the chunk
+  // kernels only pass over them (unless their cooperative launch failed)
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    if (b->pers[a] != b->pers[c]) return b->pers[a] < b->pers[c];
     const int sa = shape(a), sc = shape(c);
     if (sa != sc) return sa < sc;
     return krylov_set_bytes(b->boxes[a]) < krylov_set_bytes(b->boxes[c]);
   });
   std::vector<std::vector<int>> ids(1);
   size_t used = 0;
-  for (int i : order) {
+  for (size_t k = 0; k < order.size(); ++k) {
+    const int i = order[k];
     const size_t by = krylov_set_bytes(b->boxes[i]);
-    if (b->group_bytes && !ids.back().empty() && used + by > b->group_bytes) {
+    const bool split = k > 0 && b->pers[i] != b->pers[order[k - 1]];
+    if (!ids.back().empty() && (split || (b->group_bytes && used + by > b->group_bytes))) {
       ids.emplace_back();
       used = 0;
     }
@@ -279,10 +298,20 @@ int launch_pre(pf_batch* b, Lists& L) {
   for (int s = 1; s <= 4; ++s)
     if (int rc = batch_stage_launch(b->dtype, s, b->desc, L.flat.view(), L.flat.total, b->st))
       return rc;
-  if (int rc = batch_krylov_launch(b->dtype, kBatchInit, 0, b->desc, L.red.view(), L.red.total,
-                                   0, 0, b->st))
-    return rc;
-  return tiles_launch(b, L, kBatchKA, 0);
+  return batch_krylov_launch(b->dtype, kBatchInit, 0, b->desc, L.red.view(), L.red.total, 0, 0,
+                             b->st);
+}
+
+// The whole Poisson solve of box i as one persistent launch on the batch's
+// stream (after its batched Krylov start); *used = false if it did not start.
+int persist_start(pf_batch* b, int i, bool* used) {
+  pf_box* h = b->boxes[i];
+  const TileGrid tg = make_tile_grid(h->g, (int)h->esize, h->tile_rows);
+  if (b->dtype == PF_F64)
+    return persist_solve<double>(h, b->st, tg, make_lap<double>(h), (const double*)h->r,
+                                 (const double*)h->rh, (double*)h->x, h->nblk, used, nullptr);
+  return persist_solve<float>(h, b->st, tg, make_lap<float>(h), (const float*)h->r,
+                              (const float*)h->rh, (float*)h->x, h->nblk, used, nullptr);
 }
 
 int launch_post(pf_batch* b, Lists& L) {
@@ -361,14 +390,16 @@ int pf_batch_create(pf_batch** out, int capacity, int dtype) {
   b->g_chunk = nullptr;
   b->maxit_max = 0;
   b->group_bytes = kGroupBytes;
+  b->persist_on = 1;
   hipError_t e = hipMalloc(&b->desc, b->desc_size * capacity);
   if (e == hipSuccess) e = hipMalloc((void**)&b->out_dev, sizeof(double) * 8 * capacity);
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->out_host, sizeof(double) * 8 * capacity);
   if (e == hipSuccess) e = hipMalloc((void**)&b->flags_dev, sizeof(int) * capacity);
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->flags_host, sizeof(int) * capacity);
-  // list regions: all boxes, boxes starting a step, boxes finishing one, groups
-  if (e == hipSuccess) e = hipMalloc((void**)&b->list_mem, sizeof(int) * 4 * b->list_cap);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&b->list_stage, sizeof(int) * 4 * b->list_cap);
+  // list regions: all boxes, boxes starting a step, boxes finishing one,
+  // groups, boxes starting a step on the chunk kernels
+  if (e == hipSuccess) e = hipMalloc((void**)&b->list_mem, sizeof(int) * 5 * b->list_cap);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->list_stage, sizeof(int) * 5 * b->list_cap);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
   if (e != hipSuccess) {
@@ -445,6 +476,9 @@ int pf_batch_set(pf_batch* b, int n, pf_box* const* boxes, void* const* fields,
   }
   b->boxes = bx;
   b->running = run;
+  b->pers.assign(n, 0);
+  for (int i = 0; i < n; ++i)
+    b->pers[i] = b->persist_on && persist_blocks(bx[i]) >= kPersistMinBlocks;
   b->n = n;
   b->maxit_max = maxit;
   if (n > 0) PF_HIP(hipMemcpy(b->desc, host.data(), b->desc_size * n, hipMemcpyHostToDevice));
@@ -475,10 +509,23 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
   std::vector<int> ids;
   for (int i = 0; i < b->n; ++i)
     if (start && start[i]) ids.push_back(i);
+  std::vector<int> psolved;   // boxes whose solve ran persistently in this call
   if (!ids.empty()) {
     build_lists(b, ids, b->pre);
     if (int rc = upload_lists(b, b->pre, 1)) return rc;
     if (int rc = launch_pre(b, b->pre)) return rc;
+    std::vector<int> ka;
+    for (int i : ids) {
+      bool used = false;
+      if (b->pers[i])
+        if (int rc = persist_start(b, i, &used)) return rc;
+      (used ? psolved : ka).push_back(i);
+    }
+    if (!ka.empty()) {   // the first KA of the others
+      build_lists(b, ka, b->ka);
+      if (int rc = upload_lists(b, b->ka, 4)) return rc;
+      if (int rc = tiles_launch(b, b->ka, kBatchKA, 0)) return rc;
+    }
     for (int i : ids) b->running[i] = 1;
   }
   bool any = false;
@@ -500,6 +547,13 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
     if (chunks > max_chunks)
       return fail(PF_EHIP, "pf_batch_advance: no solve finished within the iteration cap");
   }
+  for (int i : psolved) {
+    bool aborted = false;
+    if (int rc = persist_aborted(b->boxes[i], b->st, &aborted)) return rc;
+    if (aborted)
+      return fail(PF_EHIP, "pf_batch_advance: a block of a persistent solve stopped waiting for "
+                           "the others (grid sync timed out)");
+  }
   build_lists(b, fin, b->post);
   if (int rc = upload_lists(b, b->post, 2)) return rc;
   if (int rc = launch_post(b, b->post)) return rc;
@@ -516,6 +570,23 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
 }
 
 int pf_batch_chunk_iterations(void) { return kChunk; }
+
+int pf_batch_set_persistent(pf_batch* b, int on) {
+  if (!b) return fail(PF_EINVAL, "pf_batch_set_persistent: NULL batch");
+  PF_HIP(hipStreamSynchronize(b->st));
+  b->persist_on = on ? 1 : 0;
+  int any = 0;
+  for (int i = 0; i < b->n; ++i) {
+    b->pers[i] = b->persist_on && persist_blocks(b->boxes[i]) >= kPersistMinBlocks;
+    any |= b->pers[i];
+  }
+  if (b->n > 0) {
+    if (int rc = build_groups(b)) return rc;
+    PF_HIP(hipStreamSynchronize(b->st));
+    if (int rc = capture_chunk(b)) return rc;
+  }
+  return any;
+}
 
 int pf_batch_set_group_bytes(pf_batch* b, int64_t bytes) {
   if (!b) return fail(PF_EINVAL, "pf_batch_set_group_bytes: NULL batch");

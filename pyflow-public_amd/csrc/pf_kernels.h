@@ -235,12 +235,14 @@ enum { kBatchCtlInit = 0, kBatchCtlFlags = 1, kBatchFinalize = 2 };
 
 // Persistent BiCGStab (pf_persist.hip).  persist_bytes: size of the buffers
 // box h needs for it (0: its tile grid is not a shape the kernel handles).
+// persist_blocks: blocks of box h's persistent solve (0: none).
 // persist_solve: after the init kernel, run the whole solve as one
 // cooperative launch; *used = false when it was not launched (the caller
 // then runs the launch-per-reduction path); ev, if not NULL, is an event
 // pair recorded around the launch.  persist_aborted: whether the
 // last persistent solve gave up waiting for a block.
 size_t persist_bytes(const pf_box* h);
+int persist_blocks(const pf_box* h);
 template <typename T>
 int persist_solve(pf_box* h, hipStream_t s, const TileGrid& tg, const Lap<T>& L, const T* r0,
                   const T* rh, T* x, int npart_init, bool* used, hipEvent_t* ev);

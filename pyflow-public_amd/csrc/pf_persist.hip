@@ -793,6 +793,15 @@ size_t persist_bytes(const pf_box* h) {
   return shape_bytes(sh, h->esize, tg.nblk);
 }
 
+// Blocks of the persistent solve of box h (0 if its grid has no persistent
+// kernel, its buffers are missing or the box has it switched off).
+int persist_blocks(const pf_box* h) {
+  if (!h->pers || !h->persist_on) return 0;
+  const TileGrid tg = make_tile_grid(h->g, (int)h->esize, h->tile_rows);
+  PersistShape sh;
+  return persist_shape(h, tg, sh) ? sh.nphys : 0;
+}
+
 namespace {
 
 unsigned* flag_ptr(pf_box* h, const TileGrid& tg, const PersistShape& sh) {

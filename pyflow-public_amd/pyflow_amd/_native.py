@@ -80,6 +80,7 @@ SIGNATURES = [
     ('pf_batch_advance', _I, [_P, _PI, _PI, _PD, _PI, _P]),
     ('pf_batch_chunk_iterations', _I, []),
     ('pf_batch_set_group_bytes', _I, [_P, ctypes.c_int64]),
+    ('pf_batch_set_persistent', _I, [_P, _I]),
     ('pf_profile_enable', _I, [_P, _I]),
     ('pf_profile_read', _I, [_P, _I, _PD, ctypes.POINTER(ctypes.c_int64)]),
     ('pf_profile_read_persistent', _I, [_P, _PD, ctypes.POINTER(ctypes.c_int64),
@@ -743,6 +744,14 @@ class Batch:
     def set_group_bytes(self, nbytes):
         """Cache-residency budget of the chunk's box groups (0: one group)."""
         check(lib().pf_batch_set_group_bytes(self.h, int(nbytes)), 'pf_batch_set_group_bytes')
+
+    def set_persistent(self, on):
+        """Big boxes (a persistent solve that fills the GPU) solve each step as
+        one cooperative launch (on, the default) or on the chunk kernels; True
+        if a resident box now takes the persistent solve."""
+        rc = lib().pf_batch_set_persistent(self.h, 1 if on else 0)
+        check(rc if rc < 0 else 0, 'pf_batch_set_persistent')
+        return rc == 1
 
     def advance(self, start, device=None):
         """Start a step for the boxes flagged in `start` (idle ones), run until

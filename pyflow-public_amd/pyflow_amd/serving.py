@@ -250,7 +250,7 @@ class BatchServer:
     bytes for resident slots (idle pooled slots are evicted to make room)."""
 
     def __init__(self, dtype=torch.float32, device=None, capacity=256, hbm_cap=200 * GB,
-                 group_bytes=None):
+                 group_bytes=None, persistent=True):
         _native.require_gpu()
         self.device = torch.device('cuda', torch.cuda.current_device()) if device is None \
             else torch.device(device)
@@ -261,6 +261,8 @@ class BatchServer:
         self.batch = _native.Batch(self.capacity, dtype)
         if group_bytes is not None:
             self.batch.set_group_bytes(group_bytes)
+        # big boxes (128^3) solve each step as one persistent launch
+        self.batch.set_persistent(persistent)
         self.step_times = []
         self.batch_sizes = []
         self.max_resident = 0

@@ -1,7 +1,8 @@
 """The benchmark configurations at their own sizes on the GPU.
 
 * 128^3 decaying turbulence (nu = 0.005, Passot-Pouquet k0 = 4, the bench's
-  seed and Krylov decomposition: 64x8 tiles, 8-plane chunks, 512 blocks):
+  seed and Krylov decomposition: 64x8 tiles on 16-plane chunks in FP64, 256
+  blocks, 4-plane chunks in FP32, 1024 blocks; the persistent solve):
   - FP64, BiCGStab 1e-10 on both sides: velocity relative L2 <= 1e-10 after
     10 steps against the oracle; after the last projection
     max|div u| <= 2e-10 * max|div u*| (the solver's relative residual);
@@ -18,7 +19,9 @@
 * Serving mix with a 128^3 request of 12 steps joining running requests of
   32^3-96^3: every request's fields are the same bits as its own run on the
   single-box path (batched = single-box is exact; see test_gpu_serving.py);
-  a slot reused by later requests with results kept gives the same bits too.
+  the batch solves a 128^3 box's steps as persistent launches, with the same
+  bits as on its chunk kernels; a slot reused by later requests with results
+  kept gives the same bits too.
 """
 
 import math
@@ -241,6 +244,25 @@ def test_serving_mix_with_a_128_request_equals_alone():
     assert running >= 1
     assert max(srv.batch_sizes) >= 3
     for ra, rb in zip(mix, alone):
+        assert ra.iters == rb.iters, ra.rid
+        for x, y in zip(ra.result, rb.result):
+            assert torch.equal(x, y), ra.rid
+
+
+def test_serving_persistent_big_boxes_equal_chunk_kernels():
+    """The batch solves a 128^3 box's Poisson problem as one persistent
+    launch at the start of each step (pf_batch_set_persistent); with that
+    switched off it runs on the chunk kernels.  Same iterations, same bits."""
+    from pyflow_amd.serving import BatchServer
+    specs = [(128, 6, 71, 4.0, 0.005, 0.0), (64, 10, 72, 2.0, 0.01, 0.0),
+             (32, 14, 73, 8.0, 0.015, 0.0)]
+    out = []
+    for persistent in (True, False):
+        reqs = _reqs(specs, 1e-6)
+        srv = BatchServer(capacity=4, persistent=persistent)
+        srv.serve(reqs, keep_results=True, realtime=False)
+        out.append(reqs)
+    for ra, rb in zip(*out):
         assert ra.iters == rb.iters, ra.rid
         for x, y in zip(ra.result, rb.result):
             assert torch.equal(x, y), ra.rid
