@@ -107,6 +107,7 @@ struct pf_batch {
   int* flags_host;            // pinned
   hipStream_t st;
   hipEvent_t ev;
+  hipEvent_t wev;             // completion marker the chunk loop's waits spin on
   Lists full, pre, post, ka;
   int* list_mem;              // device lists: full, pre, post, groups, first-KA regions
   int* list_stage;            // pinned staging of the three regions
@@ -384,6 +385,7 @@ int pf_batch_create(pf_batch** out, int capacity, int dtype) {
   b->flags_host = nullptr;
   b->st = nullptr;
   b->ev = nullptr;
+  b->wev = nullptr;
   b->list_mem = nullptr;
   b->list_stage = nullptr;
   b->list_cap = list_ints(capacity);
@@ -402,6 +404,7 @@ int pf_batch_create(pf_batch** out, int capacity, int dtype) {
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->list_stage, sizeof(int) * 5 * b->list_cap);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->wev, hipEventDisableTiming);
   if (e != hipSuccess) {
     pf_batch_destroy(b);
     return check_hip(e, "pf_batch_create");
@@ -422,6 +425,7 @@ int pf_batch_destroy(pf_batch* b) {
   if (b->list_mem) hipFree(b->list_mem);
   if (b->list_stage) hipHostFree(b->list_stage);
   if (b->ev) hipEventDestroy(b->ev);
+  if (b->wev) hipEventDestroy(b->wev);
   if (b->st) hipStreamDestroy(b->st);
   delete b;
   return 0;
@@ -540,7 +544,7 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
   for (;;) {
     PF_HIP(hipGraphLaunch(b->g_chunk, b->st));
     ++chunks;
-    PF_HIP(hipStreamSynchronize(b->st));
+    if (int rc = wait_stream(b->st, b->wev)) return rc;
     for (int i = 0; i < b->n; ++i)
       if (b->running[i] && b->flags_host[i]) fin.push_back(i);
     if (!fin.empty()) break;

@@ -1247,7 +1247,7 @@ int finalize(pf_box* h, hipStream_t s, const double* part, int nv, const int* op
   }
   PF_LAUNCH_CHECK("finalize_kernel");
   PF_HIP(hipMemcpyAsync(h->stats_host, h->stats_dev, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
-  PF_HIP(hipStreamSynchronize(s));
+  if (int rc = wait_stream(s, h->wait_ev)) return rc;
   for (int q = 0; q < nv; ++q) out[q] = h->stats_host[q];
   return 0;
 }
@@ -1373,7 +1373,7 @@ int bicgstab_impl(pf_box* h, hipStream_t s, T* src, const T* U, const T* V, cons
     }
     PF_LAUNCH_CHECK("bicgstab batch");
     PF_HIP(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
-    PF_HIP(hipStreamSynchronize(s));
+    if (int rc = wait_stream(s, h->wait_ev)) return rc;
     const KrylovCtl cb = *h->ctl_host;
     for (int q = 0; q < nrec; ++q) {
       const Rec& rr = recs[q];

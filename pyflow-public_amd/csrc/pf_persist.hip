@@ -899,7 +899,7 @@ int persist_aborted(pf_box* h, hipStream_t s, bool* aborted) {
   if (!persist_shape(h, tg, sh)) return 0;
   unsigned* host = reinterpret_cast<unsigned*>(h->stats_host);
   PF_HIP(hipMemcpyAsync(host, flag_ptr(h, tg, sh), sizeof(unsigned), hipMemcpyDeviceToHost, s));
-  PF_HIP(hipStreamSynchronize(s));
+  if (int rc = wait_stream(s, h->wait_ev)) return rc;
   *aborted = host[0] != 0;
   return 0;
 }

@@ -1,6 +1,8 @@
 // Box handle lifetime, error reporting and overlap fill.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -17,6 +19,26 @@ void set_error(const std::string& msg) { g_err = msg; }
 int fail(int code, const std::string& msg) {
   set_error(msg);
   return code;
+}
+
+int wait_stream(hipStream_t s, hipEvent_t ev) {
+  if (!ev) {
+    const hipError_t e = hipStreamSynchronize(s);
+    return e == hipSuccess ? 0 : check_hip(e, "hipStreamSynchronize");
+  }
+  hipError_t e = hipEventRecord(ev, s);
+  if (e != hipSuccess) return check_hip(e, "hipEventRecord");
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    e = hipEventQuery(ev);
+    if (e != hipErrorNotReady) break;
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+      e = hipEventSynchronize(ev);
+      break;
+    }
+    __builtin_ia32_pause();
+  }
+  return e == hipSuccess ? 0 : check_hip(e, "wait_stream");
 }
 
 int check_hip(hipError_t e, const char* what) {
@@ -134,6 +156,10 @@ int pf_box_create(pf_box** out, int nx, int ny, int nz, double dx, double dy, do
     hipError_t e = hipHostMalloc((void**)&h->stats_host, sizeof(double) * 8, hipHostMallocDefault);
     if (e != hipSuccess) rc = check_hip(e, "hipHostMalloc");
   }
+  if (!rc) {
+    hipError_t e = hipEventCreateWithFlags(&h->wait_ev, hipEventDisableTiming);
+    if (e != hipSuccess) rc = check_hip(e, "hipEventCreateWithFlags");
+  }
   if (rc) {
     pf_box_destroy(h);
     return rc;
@@ -158,6 +184,7 @@ int pf_box_destroy(pf_box* h) {
   }
   if (h->ctl_host) hipHostFree(h->ctl_host);
   if (h->stats_host) hipHostFree(h->stats_host);
+  if (h->wait_ev) hipEventDestroy(h->wait_ev);
   delete h;
   return 0;
 }
