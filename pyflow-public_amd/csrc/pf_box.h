@@ -115,9 +115,9 @@ int fail(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);
 int check_box(const pf_box* h);
 // Wait until stream s has done its work: spin on event ev (recorded here)
-// for up to 50 ms, then block.  The runtime's own waits spin for a few
-// microseconds and then sleep until an interrupt, whose wake-up took
-// milliseconds on some hosts (128^3 FP32 steps: 6.6 -> 8.6-10.1 ms).
+// for up to 50 ms, then block.  The runtime's own waits spin only briefly
+// and then sleep until an interrupt, whose wake-up took milliseconds on
+// some hosts (128^3 FP32 steps: 6.6 -> 8.6-10.1 ms).
 // ev == NULL: hipStreamSynchronize.
 int wait_stream(hipStream_t s, hipEvent_t ev);
 int launch_blocks(const Geom& g);   // grid size for flat one-cell-per-thread kernels
