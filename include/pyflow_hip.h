@@ -592,13 +592,16 @@ int pf_closure_vjp(pf_box* box, int kind, const void* u, const void* v, const vo
  *     that cache from one kernel to the next.  Only the order of independent
  *     boxes' launches changes, never a result.  Synchronous; re-captures the
  *     graph.
- *   pf_batch_set_persistent: on (the default), a resident box whose
- *     persistent solve fills the GPU (pf_box_set_persistent; 128^3 FP32 and
- *     FP64) solves each step's Poisson problem as that one cooperative
- *     launch when the step starts, instead of on the chunk kernels (same
- *     iterates, same bits); off: every box runs on the chunk kernels.
- *     Returns 1 if a resident box now takes the persistent solve, else 0.
- *     Synchronous; re-captures the graph.
+ *   pf_batch_set_persistent: on > 0, a resident box whose persistent solve
+ *     fills the GPU (pf_box_set_persistent; 128^3 FP32 and FP64) solves each
+ *     step's Poisson problem as that one cooperative launch instead of on the
+ *     chunk kernels (same iterates, same bits), at most `on` such solves per
+ *     pf_batch_advance call (default 1): a box over the cap waits for a later
+ *     call, first come first served, and the other boxes get a chunk between
+ *     persistent solves.  A larger cap raises throughput and the other
+ *     boxes' step latency.  on = 0: every box runs on the chunk kernels.
+ *     Returns 1 if a resident box now takes the persistent solve, else 0;
+ *     PF_EINVAL for on < 0.  Synchronous; re-captures the graph.
  */
 typedef struct pf_batch pf_batch;
 int pf_batch_create(pf_batch** out, int capacity, int dtype);

@@ -24,9 +24,14 @@
 //
 // A box whose persistent solve fills the GPU (pf_persist.hip: 128^3 FP32
 // takes 512 blocks) solves its Poisson problem as that one cooperative
-// launch when its step starts, right after the batched Krylov start, in
-// place of its share of the chunk kernels: its control block then says done
-// and the next chunk boundary finishes its step.  The persistent solve's
+// launch, after the batched Krylov start of its step, in place of its share
+// of the chunk kernels: its control block then says done and the next chunk
+// boundary finishes its step.  At most persist_cap such solves run per
+// pf_batch_advance call (default 1); a big box whose step starts beyond the
+// cap waits, its control block held at done so that the chunk kernels pass
+// over it, and solves at the start of a later call, first come first served
+// - so the other boxes get a chunk between persistent solves.  The
+// persistent solve's
 // iterates are the chunk kernels' bits, so results stay bit-identical.  A
 // box whose cooperative launch cannot start runs on the chunk kernels.
 //
@@ -100,7 +105,10 @@ struct pf_batch {
   std::vector<pf_box*> boxes;
   std::vector<char> running;  // per box: inside a time step
   std::vector<char> pers;     // per box: solves with the persistent kernel
-  int persist_on;             // big boxes take the persistent solve (default 1)
+  std::vector<int64_t> pend;  // per box: 0, or when its persistent solve began to wait (held done)
+  int64_t pend_seq;           // stamps of pend, in the order the waits began
+  int persist_cap;            // persistent solves per advance call (0: none)
+  int* done_vals;             // pinned {0, 1}: control-block done values to copy
   double* out_dev;            // 8 doubles per box
   double* out_host;           // pinned
   int* flags_dev;             // done flag per box
@@ -392,12 +400,19 @@ int pf_batch_create(pf_batch** out, int capacity, int dtype) {
   b->g_chunk = nullptr;
   b->maxit_max = 0;
   b->group_bytes = kGroupBytes;
-  b->persist_on = 1;
+  b->persist_cap = 1;
+  b->pend_seq = 0;
+  b->done_vals = nullptr;
   hipError_t e = hipMalloc(&b->desc, b->desc_size * capacity);
   if (e == hipSuccess) e = hipMalloc((void**)&b->out_dev, sizeof(double) * 8 * capacity);
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->out_host, sizeof(double) * 8 * capacity);
   if (e == hipSuccess) e = hipMalloc((void**)&b->flags_dev, sizeof(int) * capacity);
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->flags_host, sizeof(int) * capacity);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->done_vals, sizeof(int) * 2);
+  if (e == hipSuccess) {
+    b->done_vals[0] = 0;
+    b->done_vals[1] = 1;
+  }
   // list regions: all boxes, boxes starting a step, boxes finishing one,
   // groups, boxes starting a step on the chunk kernels
   if (e == hipSuccess) e = hipMalloc((void**)&b->list_mem, sizeof(int) * 5 * b->list_cap);
@@ -422,6 +437,7 @@ int pf_batch_destroy(pf_batch* b) {
   if (b->out_host) hipHostFree(b->out_host);
   if (b->flags_dev) hipFree(b->flags_dev);
   if (b->flags_host) hipHostFree(b->flags_host);
+  if (b->done_vals) hipHostFree(b->done_vals);
   if (b->list_mem) hipFree(b->list_mem);
   if (b->list_stage) hipHostFree(b->list_stage);
   if (b->ev) hipEventDestroy(b->ev);
@@ -440,10 +456,15 @@ int pf_batch_set(pf_batch* b, int n, pf_box* const* boxes, void* const* fields,
   PF_HIP(hipStreamSynchronize(b->st));
   // boxes keep their phase across compositions (a running solve continues)
   std::unordered_map<const pf_box*, char> was;
-  for (int i = 0; i < b->n; ++i) was[b->boxes[i]] = b->running[i];
+  std::unordered_map<const pf_box*, int64_t> wasp;
+  for (int i = 0; i < b->n; ++i) {
+    was[b->boxes[i]] = b->running[i];
+    wasp[b->boxes[i]] = b->pend[i];
+  }
   std::vector<char> host(b->desc_size * (n > 0 ? n : 1));
   std::vector<pf_box*> bx(n);
   std::vector<char> run(n, 0);
+  std::vector<int64_t> pnd(n, 0);
   int maxit = 0;
   for (int i = 0; i < n; ++i) {
     pf_box* h = boxes[i];
@@ -471,6 +492,7 @@ int pf_batch_set(pf_batch* b, int n, pf_box* const* boxes, void* const* fields,
     auto it = was.find(h);
     if (it != was.end()) {
       run[i] = it->second;
+      pnd[i] = wasp[h];
     } else {
       // a box joins idle: mark its control block done so that the chunk
       // kernels skip it until its first step starts
@@ -480,9 +502,10 @@ int pf_batch_set(pf_batch* b, int n, pf_box* const* boxes, void* const* fields,
   }
   b->boxes = bx;
   b->running = run;
+  b->pend = pnd;
   b->pers.assign(n, 0);
   for (int i = 0; i < n; ++i)
-    b->pers[i] = b->persist_on && persist_blocks(bx[i]) >= kPersistMinBlocks;
+    b->pers[i] = b->persist_cap > 0 && persist_blocks(bx[i]) >= kPersistMinBlocks;
   b->n = n;
   b->maxit_max = maxit;
   if (n > 0) PF_HIP(hipMemcpy(b->desc, host.data(), b->desc_size * n, hipMemcpyHostToDevice));
@@ -513,24 +536,66 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
   std::vector<int> ids;
   for (int i = 0; i < b->n; ++i)
     if (start && start[i]) ids.push_back(i);
+  // hold (1) or release (0) a box's control block; from pinned memory, so
+  // the copy does not wait for the work queued before it
+  auto set_done = [&](int i, int v) -> int {
+    PF_HIP(hipMemcpyAsync(&b->boxes[i]->ctl->done, b->done_vals + v, sizeof(int),
+                          hipMemcpyHostToDevice, b->st));
+    return 0;
+  };
+  // the cap lets the other boxes run between persistent solves; with no
+  // other box in a step, every waiting and starting big box solves now
+  bool others = false;
+  for (int i = 0; i < b->n; ++i)
+    others = others || (!b->pers[i] && (b->running[i] || (start && start[i])));
+  const size_t cap = others ? (size_t)b->persist_cap : (size_t)b->n;
   std::vector<int> psolved;   // boxes whose solve ran persistently in this call
+  std::vector<int> ka;        // boxes whose first KA the chunk kernels' start needs
+  // the steps, started in earlier calls, whose persistent solves have waited
+  // longest (first come, first served: a box that restarts behind the others
+  // cannot overtake them)
+  while (psolved.size() < cap) {
+    int i = -1;
+    for (int k = 0; k < b->n; ++k)
+      if (b->pend[k] && (i < 0 || b->pend[k] < b->pend[i])) i = k;
+    if (i < 0) break;
+    b->pend[i] = 0;
+    bool used = false;
+    if (b->pers[i])   // (pf_batch_set_persistent may have turned it off)
+      if (int rc = persist_start(b, i, &used)) return rc;
+    if (used) {
+      psolved.push_back(i);
+    } else {   // onto the chunk kernels
+      if (int rc = set_done(i, 0)) return rc;
+      ka.push_back(i);
+    }
+  }
   if (!ids.empty()) {
     build_lists(b, ids, b->pre);
     if (int rc = upload_lists(b, b->pre, 1)) return rc;
     if (int rc = launch_pre(b, b->pre)) return rc;
-    std::vector<int> ka;
     for (int i : ids) {
-      bool used = false;
-      if (b->pers[i])
+      if (b->pers[i]) {
+        if (psolved.size() >= cap) {   // over the cap: this one waits
+          if (int rc = set_done(i, 1)) return rc;
+          b->pend[i] = ++b->pend_seq;
+          continue;
+        }
+        bool used = false;
         if (int rc = persist_start(b, i, &used)) return rc;
-      (used ? psolved : ka).push_back(i);
-    }
-    if (!ka.empty()) {   // the first KA of the others
-      build_lists(b, ka, b->ka);
-      if (int rc = upload_lists(b, b->ka, 4)) return rc;
-      if (int rc = tiles_launch(b, b->ka, kBatchKA, 0)) return rc;
+        if (used) {
+          psolved.push_back(i);
+          continue;
+        }
+      }
+      ka.push_back(i);
     }
     for (int i : ids) b->running[i] = 1;
+  }
+  if (!ka.empty()) {   // the first KA of the boxes on the chunk kernels
+    build_lists(b, ka, b->ka);
+    if (int rc = upload_lists(b, b->ka, 4)) return rc;
+    if (int rc = tiles_launch(b, b->ka, kBatchKA, 0)) return rc;
   }
   bool any = false;
   for (int i = 0; i < b->n; ++i) any = any || b->running[i];
@@ -541,12 +606,27 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
   const int max_chunks = b->maxit_max / kChunk + 2;
   int chunks = 0;
   std::vector<int> fin;
+  // with every running box solved persistently in this call or waiting, the
+  // chunk kernels have nothing to do: read the done flags alone
+  bool chunk_work = false;
+  for (int i = 0; i < b->n; ++i)
+    chunk_work = chunk_work || (b->running[i] && !b->pend[i] &&
+                                std::find(psolved.begin(), psolved.end(), i) == psolved.end());
   for (;;) {
-    PF_HIP(hipGraphLaunch(b->g_chunk, b->st));
-    ++chunks;
+    if (chunk_work) {
+      PF_HIP(hipGraphLaunch(b->g_chunk, b->st));
+      ++chunks;
+    } else {
+      if (int rc = batch_ctl_launch(b->dtype, kBatchCtlFlags, b->desc, nullptr, b->n,
+                                    b->flags_dev, b->st))
+        return rc;
+      PF_HIP(hipMemcpyAsync(b->flags_host, b->flags_dev, sizeof(int) * b->n,
+                            hipMemcpyDeviceToHost, b->st));
+      chunk_work = true;   // (a solve that did not finish goes on in the chunks)
+    }
     if (int rc = wait_stream(b->st, b->wev)) return rc;
     for (int i = 0; i < b->n; ++i)
-      if (b->running[i] && b->flags_host[i]) fin.push_back(i);
+      if (b->running[i] && b->flags_host[i] && !b->pend[i]) fin.push_back(i);
     if (!fin.empty()) break;
     if (chunks > max_chunks)
       return fail(PF_EHIP, "pf_batch_advance: no solve finished within the iteration cap");
@@ -578,10 +658,11 @@ int pf_batch_chunk_iterations(void) { return kChunk; }
 int pf_batch_set_persistent(pf_batch* b, int on) {
   if (!b) return fail(PF_EINVAL, "pf_batch_set_persistent: NULL batch");
   PF_HIP(hipStreamSynchronize(b->st));
-  b->persist_on = on ? 1 : 0;
+  if (on < 0) return fail(PF_EINVAL, "pf_batch_set_persistent: on must be >= 0");
+  b->persist_cap = on;
   int any = 0;
   for (int i = 0; i < b->n; ++i) {
-    b->pers[i] = b->persist_on && persist_blocks(b->boxes[i]) >= kPersistMinBlocks;
+    b->pers[i] = b->persist_cap > 0 && persist_blocks(b->boxes[i]) >= kPersistMinBlocks;
     any |= b->pers[i];
   }
   if (b->n > 0) {

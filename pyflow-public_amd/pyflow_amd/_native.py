@@ -747,9 +747,14 @@ class Batch:
 
     def set_persistent(self, on):
         """Big boxes (a persistent solve that fills the GPU) solve each step as
-        one cooperative launch (on, the default) or on the chunk kernels; True
-        if a resident box now takes the persistent solve."""
-        rc = lib().pf_batch_set_persistent(self.h, 1 if on else 0)
+        one cooperative launch or on the chunk kernels (on = False or 0).  An
+        int caps the persistent solves per advance call (True: 1, the
+        default: the other boxes get a chunk between persistent solves).
+        True if a resident box now takes the persistent solve."""
+        cap = int(on)
+        if cap < 0:
+            raise ValueError('set_persistent: on must be a bool or an int >= 0')
+        rc = lib().pf_batch_set_persistent(self.h, cap)
         check(rc if rc < 0 else 0, 'pf_batch_set_persistent')
         return rc == 1
 

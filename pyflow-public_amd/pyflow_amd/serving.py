@@ -261,7 +261,8 @@ class BatchServer:
         self.batch = _native.Batch(self.capacity, dtype)
         if group_bytes is not None:
             self.batch.set_group_bytes(group_bytes)
-        # big boxes (128^3) solve each step as one persistent launch
+        # big boxes (128^3) solve each step as one persistent launch; an int
+        # caps such solves per advance call (True: one)
         self.batch.set_persistent(persistent)
         self.step_times = []
         self.batch_sizes = []

@@ -268,6 +268,27 @@ def test_serving_persistent_big_boxes_equal_chunk_kernels():
             assert torch.equal(x, y), ra.rid
 
 
+def test_serving_several_persistent_boxes_take_turns():
+    """Three 128^3 boxes start together: while the 48^3 box runs, one
+    persistent solve runs per advance call and the others wait, held out of
+    the chunk kernels, for later calls; once it is done every big box solves
+    in each call, without chunk replays.  Same iterations and bits as the
+    chunk kernels alone."""
+    from pyflow_amd.serving import BatchServer
+    specs = [(128, 5, 81, 4.0, 0.005, 0.0), (128, 4, 82, 2.0, 0.01, 0.0),
+             (128, 3, 83, 8.0, 0.005, 0.0), (48, 2, 84, 4.0, 0.01, 0.0)]
+    out = []
+    for persistent in (True, False):
+        reqs = _reqs(specs, 1e-6)
+        srv = BatchServer(capacity=4, persistent=persistent)
+        srv.serve(reqs, keep_results=True, realtime=False)
+        out.append(reqs)
+    for ra, rb in zip(*out):
+        assert ra.iters == rb.iters, ra.rid
+        for x, y in zip(ra.result, rb.result):
+            assert torch.equal(x, y), ra.rid
+
+
 def test_serving_reused_slot_keeps_results():
     """One slot of each size at a time (capacity 1): later requests reuse the
     finished slot while the previous result's copies may still be queued."""
