@@ -105,6 +105,7 @@ struct pf_batch {
   std::vector<pf_box*> boxes;
   std::vector<char> running;  // per box: inside a time step
   std::vector<char> pers;     // per box: solves with the persistent kernel
+  std::vector<char> chunked;  // per box: its current step solves on the chunk kernels
   std::vector<int64_t> pend;  // per box: 0, or when its persistent solve began to wait (held done)
   int64_t pend_seq;           // stamps of pend, in the order the waits began
   int persist_cap;            // persistent solves per advance call (0: none)
@@ -120,7 +121,9 @@ struct pf_batch {
   int* list_mem;              // device lists: full, pre, post, groups, first-KA regions
   int* list_stage;            // pinned staging of the three regions
   size_t list_cap;            // ints per region
-  hipGraphExec_t g_chunk;
+  hipGraphExec_t g_chunk;     // chunk over the groups of boxes that do not solve persistently
+  hipGraphExec_t g_chunk_all; // chunk over every group (null if no box solves persistently)
+  size_t plain_groups;        // groups before the persistent boxes' groups
   int maxit_max;
   size_t group_bytes;         // cache-residency budget of one group (0: one group)
   std::vector<Lists> groups;  // tile lists of each group (list region 3)
@@ -279,7 +282,9 @@ int build_groups(pf_batch* b) {
   int* host = b->list_stage + (size_t)3 * b->list_cap;
   int* dev = b->list_mem + (size_t)3 * b->list_cap;
   size_t off = 0;
+  b->plain_groups = 0;
   for (auto& g : ids) {
+    if (!b->pers[g[0]]) ++b->plain_groups;
     std::sort(g.begin(), g.end());
     b->groups.emplace_back();
     Lists& L = b->groups.back();
@@ -337,8 +342,9 @@ int launch_post(pf_batch* b, Lists& L) {
   return batch_ctl_launch(b->dtype, kBatchFinalize, b->desc, L.ids_dev, m, nullptr, b->st);
 }
 
-int record_chunk(pf_batch* b) {
-  for (Lists& L : b->groups) {
+int record_chunk(pf_batch* b, size_t ngroups) {
+  for (size_t k = 0; k < ngroups; ++k) {
+    Lists& L = b->groups[k];
     for (int q = 0; q < kChunk; ++q) {
       const int par = q & 1;
       if (int rc = tiles_launch(b, L, kBatchKB, par)) return rc;
@@ -354,21 +360,34 @@ int record_chunk(pf_batch* b) {
   return 0;
 }
 
-int capture_chunk(pf_batch* b) {
-  if (b->g_chunk) hipGraphExecDestroy(b->g_chunk);
-  b->g_chunk = nullptr;
+int capture_groups(pf_batch* b, size_t ngroups, hipGraphExec_t* out) {
   hipGraph_t g = nullptr;
   PF_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-  int rc = record_chunk(b);
+  int rc = record_chunk(b, ngroups);
   hipError_t e = hipStreamEndCapture(b->st, &g);
   if (rc) {
     if (g) hipGraphDestroy(g);
     return rc;
   }
   if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
-  e = hipGraphInstantiate(&b->g_chunk, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
+  return 0;
+}
+
+// The chunk graph leaves out the groups of the boxes that solve
+// persistently: their blocks would only read a done control block, and a
+// launch over a 128^3 box's 1024 tiles costs microseconds even so.  The graph
+// over every group serves the steps of such boxes whose cooperative launch
+// could not start.
+int capture_chunk(pf_batch* b) {
+  if (b->g_chunk) hipGraphExecDestroy(b->g_chunk);
+  if (b->g_chunk_all) hipGraphExecDestroy(b->g_chunk_all);
+  b->g_chunk = b->g_chunk_all = nullptr;
+  if (int rc = capture_groups(b, b->plain_groups, &b->g_chunk)) return rc;
+  if (b->plain_groups < b->groups.size())
+    return capture_groups(b, b->groups.size(), &b->g_chunk_all);
   return 0;
 }
 
@@ -398,6 +417,8 @@ int pf_batch_create(pf_batch** out, int capacity, int dtype) {
   b->list_stage = nullptr;
   b->list_cap = list_ints(capacity);
   b->g_chunk = nullptr;
+  b->g_chunk_all = nullptr;
+  b->plain_groups = 0;
   b->maxit_max = 0;
   b->group_bytes = kGroupBytes;
   b->persist_cap = 1;
@@ -432,6 +453,7 @@ int pf_batch_destroy(pf_batch* b) {
   if (!b) return 0;
   if (b->st) hipStreamSynchronize(b->st);
   if (b->g_chunk) hipGraphExecDestroy(b->g_chunk);
+  if (b->g_chunk_all) hipGraphExecDestroy(b->g_chunk_all);
   if (b->desc) hipFree(b->desc);
   if (b->out_dev) hipFree(b->out_dev);
   if (b->out_host) hipHostFree(b->out_host);
@@ -457,14 +479,17 @@ int pf_batch_set(pf_batch* b, int n, pf_box* const* boxes, void* const* fields,
   // boxes keep their phase across compositions (a running solve continues)
   std::unordered_map<const pf_box*, char> was;
   std::unordered_map<const pf_box*, int64_t> wasp;
+  std::unordered_map<const pf_box*, char> wasc;
   for (int i = 0; i < b->n; ++i) {
     was[b->boxes[i]] = b->running[i];
     wasp[b->boxes[i]] = b->pend[i];
+    wasc[b->boxes[i]] = b->chunked[i];
   }
   std::vector<char> host(b->desc_size * (n > 0 ? n : 1));
   std::vector<pf_box*> bx(n);
   std::vector<char> run(n, 0);
   std::vector<int64_t> pnd(n, 0);
+  std::vector<char> chk(n, 0);
   int maxit = 0;
   for (int i = 0; i < n; ++i) {
     pf_box* h = boxes[i];
@@ -493,6 +518,7 @@ int pf_batch_set(pf_batch* b, int n, pf_box* const* boxes, void* const* fields,
     if (it != was.end()) {
       run[i] = it->second;
       pnd[i] = wasp[h];
+      chk[i] = wasc[h];
     } else {
       // a box joins idle: mark its control block done so that the chunk
       // kernels skip it until its first step starts
@@ -503,6 +529,7 @@ int pf_batch_set(pf_batch* b, int n, pf_box* const* boxes, void* const* fields,
   b->boxes = bx;
   b->running = run;
   b->pend = pnd;
+  b->chunked = chk;
   b->pers.assign(n, 0);
   for (int i = 0; i < n; ++i)
     b->pers[i] = b->persist_cap > 0 && persist_blocks(bx[i]) >= kPersistMinBlocks;
@@ -517,7 +544,8 @@ int pf_batch_set(pf_batch* b, int n, pf_box* const* boxes, void* const* fields,
   PF_HIP(hipStreamSynchronize(b->st));
   if (n > 0) return capture_chunk(b);
   if (b->g_chunk) hipGraphExecDestroy(b->g_chunk);
-  b->g_chunk = nullptr;
+  if (b->g_chunk_all) hipGraphExecDestroy(b->g_chunk_all);
+  b->g_chunk = b->g_chunk_all = nullptr;
   return 0;
 }
 
@@ -593,6 +621,7 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
     for (int i : ids) b->running[i] = 1;
   }
   if (!ka.empty()) {   // the first KA of the boxes on the chunk kernels
+    for (int i : ka) b->chunked[i] = 1;
     build_lists(b, ka, b->ka);
     if (int rc = upload_lists(b, b->ka, 4)) return rc;
     if (int rc = tiles_launch(b, b->ka, kBatchKA, 0)) return rc;
@@ -612,9 +641,13 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
   for (int i = 0; i < b->n; ++i)
     chunk_work = chunk_work || (b->running[i] && !b->pend[i] &&
                                 std::find(psolved.begin(), psolved.end(), i) == psolved.end());
+  // a big box whose cooperative launch did not start needs its group's launches
+  bool all_groups = false;
+  for (int i = 0; i < b->n; ++i) all_groups = all_groups || (b->pers[i] && b->chunked[i]);
+  hipGraphExec_t graph = all_groups && b->g_chunk_all ? b->g_chunk_all : b->g_chunk;
   for (;;) {
     if (chunk_work) {
-      PF_HIP(hipGraphLaunch(b->g_chunk, b->st));
+      PF_HIP(hipGraphLaunch(graph, b->st));
       ++chunks;
     } else {
       if (int rc = batch_ctl_launch(b->dtype, kBatchCtlFlags, b->desc, nullptr, b->n,
@@ -646,6 +679,7 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
   PF_HIP(hipStreamSynchronize(b->st));
   for (int i : fin) {
     b->running[i] = 0;
+    b->chunked[i] = 0;
     if (finished) finished[i] = 1;
     if (out) memcpy(out + 8 * (size_t)i, b->out_host + 8 * (size_t)i, sizeof(double) * 8);
   }
