@@ -20,8 +20,10 @@
   32^3-96^3: every request's fields are the same bits as its own run on the
   single-box path (batched = single-box is exact; see test_gpu_serving.py);
   the batch solves a 128^3 box's steps as persistent launches, with the same
-  bits as on its chunk kernels; a slot reused by later requests with results
-  kept gives the same bits too.
+  bits as on its chunk kernels, also with several big boxes taking turns
+  under the per-call cap and when a big box's persistent solve does not start
+  and its steps fall back to the chunk kernels; a slot reused by later
+  requests with results kept gives the same bits too.
 """
 
 import math
@@ -287,6 +289,52 @@ def test_serving_several_persistent_boxes_take_turns():
         assert ra.iters == rb.iters, ra.rid
         for x, y in zip(ra.result, rb.result):
             assert torch.equal(x, y), ra.rid
+
+
+def _drive_batch(cap, refuse_big):
+    """A 128^3 and a 48^3 request through pf_batch directly.  refuse_big: the
+    128^3 box refuses its persistent solve after the batch counted it as a
+    persistent box, so its steps fall back to the chunk graph over every
+    group."""
+    from pyflow_amd import _native, serving
+    dev = torch.device('cuda')
+    reqs = _reqs([(128, 3, 91, 4.0, 0.005, 0.0), (48, 5, 92, 4.0, 0.01, 0.0)], 1e-6)
+    slots = []
+    for r in reqs:
+        s = serving.Slot(r.N, torch.float32, dev)
+        s.load(r)
+        slots.append(s)
+    b = _native.Batch(4, torch.float32)
+    b.set([(s.box, s.f, s.params(r)) for s, r in zip(slots, reqs)])
+    assert b.set_persistent(cap) == (cap > 0)
+    if refuse_big:
+        slots[0].box.set_persistent(False)
+    left = [r.steps for r in reqs]
+    running = [False] * len(reqs)
+    rows_by_box = {i: [] for i in range(len(reqs))}
+    while any(left):
+        start = [not running[i] and left[i] > 0 for i in range(len(reqs))]
+        running = [x or y for x, y in zip(running, start)]
+        fin, rows, _ = b.advance(start, dev)
+        for i in fin:
+            running[i] = False
+            left[i] -= 1
+            rows_by_box[i].append(list(rows[i]))
+    torch.cuda.synchronize()
+    return rows_by_box, [s.f[n].clone() for s in slots for n in ('u', 'v', 'w', 'p')]
+
+
+def test_serving_refused_persistent_solve_falls_back_to_chunks():
+    """A big box whose persistent solve does not start runs its steps on the
+    chunk kernels of its own group (left out of the default chunk graph):
+    same monitor rows and bits as with the persistent solve switched off, and
+    as with it on."""
+    ref_rows, ref = _drive_batch(0, False)
+    for cap, refuse in ((1, True), (1, False)):
+        rows, got = _drive_batch(cap, refuse)
+        assert rows == ref_rows, (cap, refuse)
+        for x, y in zip(got, ref):
+            assert torch.equal(x, y), (cap, refuse)
 
 
 def test_serving_reused_slot_keeps_results():
