@@ -54,7 +54,13 @@
 
 namespace {
 
-constexpr int kChunk = 16;    // BiCGStab iterations per graph replay (even: buffer parity)
+// BiCGStab iterations per graph replay (even: buffer parity).  With one
+// persistent solve per advance call, a smaller box's step spans fewer calls,
+// each holding a big solve, when chunks are longer; a big box then waits for
+// a longer chunk.  On the 24-request serving stream (FP32) 8 gave p50 / p99
+// step latency 35.9 / 146 ms, 16 35.3 / 85.7, 32 25.6 / 69.4, 48 20.8 / 82.4
+// and 64 21.3 / 94.5 ms, at 344-361 Mcell-steps/s (64 lowest).
+constexpr int kChunk = 32;
 // Default cache-residency budget of a group's Krylov vectors.  The nine
 // vectors per box overstate what one kernel streams (KA, KB and KC touch six,
 // two and seven of them), so the best budget exceeds the 256 MB Infinity
