@@ -31,9 +31,9 @@
 // cap waits, its control block held at done so that the chunk kernels pass
 // over it, and solves at the start of a later call, first come first served
 // - so the other boxes get a chunk between persistent solves.  The
-// persistent solve's
-// iterates are the chunk kernels' bits, so results stay bit-identical.  A
-// box whose cooperative launch cannot start runs on the chunk kernels.
+// persistent solve's iterates are the chunk kernels' bits, so results stay
+// bit-identical.  A box whose cooperative launch cannot start runs on the
+// chunk kernels.
 //
 // Inside a chunk the boxes run in cache-residency groups: boxes are packed,
 // by tile shape and then size, into groups whose Krylov vectors together fit
@@ -585,6 +585,14 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
   const size_t cap = others ? (size_t)b->persist_cap : (size_t)b->n;
   std::vector<int> psolved;   // boxes whose solve ran persistently in this call
   std::vector<int> ka;        // boxes whose first KA the chunk kernels' start needs
+  // waiting boxes that pf_batch_set_persistent has since moved to the chunk
+  // kernels (with the cap at 0 the loop below would never release them)
+  for (int i = 0; i < b->n; ++i) {
+    if (!b->pend[i] || b->pers[i]) continue;
+    b->pend[i] = 0;
+    if (int rc = set_done(i, 0)) return rc;
+    ka.push_back(i);
+  }
   // the steps, started in earlier calls, whose persistent solves have waited
   // longest (first come, first served: a box that restarts behind the others
   // cannot overtake them)
@@ -595,8 +603,7 @@ int pf_batch_advance(pf_batch* b, const int* start, int* finished, double* out, 
     if (i < 0) break;
     b->pend[i] = 0;
     bool used = false;
-    if (b->pers[i])   // (pf_batch_set_persistent may have turned it off)
-      if (int rc = persist_start(b, i, &used)) return rc;
+    if (int rc = persist_start(b, i, &used)) return rc;
     if (used) {
       psolved.push_back(i);
     } else {   // onto the chunk kernels

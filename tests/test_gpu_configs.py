@@ -291,14 +291,16 @@ def test_serving_several_persistent_boxes_take_turns():
             assert torch.equal(x, y), ra.rid
 
 
-def _drive_batch(cap, refuse_big):
-    """A 128^3 and a 48^3 request through pf_batch directly.  refuse_big: the
-    128^3 box refuses its persistent solve after the batch counted it as a
-    persistent box, so its steps fall back to the chunk graph over every
-    group."""
+def _drive_batch(cap, refuse_big, off_after=None):
+    """Two 128^3 requests and a 48^3 one through pf_batch directly.
+    refuse_big: the first 128^3 box refuses its persistent solve after the
+    batch counted it as a persistent box, so its steps fall back to the chunk
+    graph over every group.  off_after: switch the persistent solve off after
+    that many advance calls (a big box may be waiting then)."""
     from pyflow_amd import _native, serving
     dev = torch.device('cuda')
-    reqs = _reqs([(128, 3, 91, 4.0, 0.005, 0.0), (48, 5, 92, 4.0, 0.01, 0.0)], 1e-6)
+    reqs = _reqs([(128, 3, 91, 4.0, 0.005, 0.0), (48, 5, 92, 4.0, 0.01, 0.0),
+                  (128, 2, 93, 2.0, 0.01, 0.0)], 1e-6)
     slots = []
     for r in reqs:
         s = serving.Slot(r.N, torch.float32, dev)
@@ -312,7 +314,11 @@ def _drive_batch(cap, refuse_big):
     left = [r.steps for r in reqs]
     running = [False] * len(reqs)
     rows_by_box = {i: [] for i in range(len(reqs))}
+    calls = 0
     while any(left):
+        if calls == off_after:
+            assert not b.set_persistent(0)
+        calls += 1
         start = [not running[i] and left[i] > 0 for i in range(len(reqs))]
         running = [x or y for x, y in zip(running, start)]
         fin, rows, _ = b.advance(start, dev)
@@ -327,14 +333,15 @@ def _drive_batch(cap, refuse_big):
 def test_serving_refused_persistent_solve_falls_back_to_chunks():
     """A big box whose persistent solve does not start runs its steps on the
     chunk kernels of its own group (left out of the default chunk graph):
-    same monitor rows and bits as with the persistent solve switched off, and
-    as with it on."""
+    same monitor rows and bits as with the persistent solve switched off, as
+    with it on, and as with it switched off while a big box waits (the
+    waiting box goes to the chunk kernels)."""
     ref_rows, ref = _drive_batch(0, False)
-    for cap, refuse in ((1, True), (1, False)):
-        rows, got = _drive_batch(cap, refuse)
-        assert rows == ref_rows, (cap, refuse)
+    for cap, refuse, off in ((1, True, None), (1, False, None), (1, False, 1)):
+        rows, got = _drive_batch(cap, refuse, off)
+        assert rows == ref_rows, (cap, refuse, off)
         for x, y in zip(got, ref):
-            assert torch.equal(x, y), (cap, refuse)
+            assert torch.equal(x, y), (cap, refuse, off)
 
 
 def test_serving_reused_slot_keeps_results():
