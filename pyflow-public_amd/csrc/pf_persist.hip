@@ -53,7 +53,9 @@ constexpr int kSets = 4;              // slot sets in rotation (sync j uses set 
 // Replicas of each slot set: a block writes its partials into every replica
 // and polls replica (b mod 8) mod kReps, so that each replica's cache lines
 // carry a quarter of the polls (a bare all-to-all round at 256 blocks: 3.45
-// us with one replica, 2.43 with four; 512 blocks x 2 slots: 4.84 -> 4.29)
+// us with one replica, 2.43 with four; 512 blocks x 2 slots: 4.84 -> 4.29).
+// Eight, one per XCD, were no faster in the solver (128^3 FP64 24.85 us per
+// iteration with four and with eight; FP32 20.1-20.4 against 20.5-20.8).
 constexpr int kReps = 4;
 constexpr int kSpin = 1 << 21;        // polls before a block gives up
 constexpr int kPT = 256;              // threads per block
