@@ -92,10 +92,9 @@ def gpu_run(N, dtype, nu, tol, maxit, steps, warmup, seed, profile=True, solver=
         D.stats()
     if profile:
         box.profile_enable(profile)
-    # free the previous leg's boxes here, not inside the timed steps: a box
-    # in a reference cycle waits for the garbage collector, and its device
-    # frees synchronize the GPU (an FP32 leg after the FP64 one otherwise ran
-    # 232-245 instead of 316 Mcell-steps/s)
+    # collect the previous leg's garbage here, not inside the timed steps (an
+    # FP32 leg after the FP64 one otherwise ran 232-245 instead of 316
+    # Mcell-steps/s)
     gc.collect()
     torch.cuda.synchronize()
     its = []
